@@ -1,0 +1,16 @@
+"""Python handle on libcpt (the MI355X path tracer behind include/cpt.h).
+
+    import sys; sys.path.insert(0, "<repo>/cpp-pt_amd")
+    import cpt
+    scene = cpt.scenes.box_room()
+    with cpt.Context() as ctx:
+        ctx.upload(scene)
+        ctx.render(512, 512, spp_axis=8)
+        img = ctx.read_film()
+"""
+from . import scenes
+from ._lib import (ABI_FUNCTIONS, CPT_RENDER_ACCUMULATE, CPT_RENDER_COUNT, CPT_RENDER_KERNEL_TIMING,
+                   CPT_RENDER_NO_GRAPH, Context, CptError, LIB_PATH, load_library)
+
+__all__ = ["scenes", "Context", "CptError", "load_library", "LIB_PATH", "ABI_FUNCTIONS",
+           "CPT_RENDER_NO_GRAPH", "CPT_RENDER_KERNEL_TIMING", "CPT_RENDER_COUNT", "CPT_RENDER_ACCUMULATE"]

@@ -1,0 +1,211 @@
+"""ctypes binding of libcpt.so (the C ABI declared in include/cpt.h).
+
+The library is loaded from the package directory (cpp-pt_amd/libcpt.so).  There
+is no fallback: if the library is missing or a call fails, an exception is
+raised.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libcpt.so")
+
+CPT_OK = 0
+CPT_E_INVALID = -1
+CPT_E_HIP = -2
+CPT_E_NOSCENE = -3
+CPT_E_NODEVICE = -4
+CPT_E_LIMIT = -5
+
+CPT_SCENE_NO_TIE_RANK = 1
+CPT_RENDER_NO_GRAPH = 1
+CPT_RENDER_KERNEL_TIMING = 2
+CPT_RENDER_COUNT = 4
+CPT_RENDER_ACCUMULATE = 8
+
+# every function include/cpt.h declares
+ABI_FUNCTIONS = (
+    "cpt_version", "cpt_create", "cpt_destroy", "cpt_last_error", "cpt_upload_scene",
+    "cpt_bvh_info", "cpt_tie_rank", "cpt_trace_closest", "cpt_trace_any",
+    "cpt_trace_closest_device", "cpt_render", "cpt_read_film", "cpt_render_views",
+)
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("n_vertices", C.c_int32), ("positions", C.c_void_p), ("normals", C.c_void_p),
+        ("bitangents", C.c_void_p), ("n_triangles", C.c_int32), ("indices", C.c_void_p),
+        ("tri_material", C.c_void_p), ("n_materials", C.c_int32), ("materials", C.c_void_p),
+        ("n_lights", C.c_int32), ("light_triangles", C.c_void_p), ("camera", C.c_float * 16),
+        ("env", C.c_float * 3), ("flags", C.c_int32),
+    ]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("spp_axis", C.c_int32),
+        ("max_depth", C.c_int32), ("seed", C.c_uint64), ("flags", C.c_int32),
+        ("max_paths", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
+        ("view", C.c_int32), ("reserved", C.c_int32 * 3),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("seconds", C.c_double), ("paths", C.c_int64), ("extension_rays", C.c_int64),
+        ("shadow_rays", C.c_int64), ("node_visits", C.c_int64), ("triangle_tests", C.c_int64),
+        ("shadow_node_visits", C.c_int64), ("shadow_triangle_tests", C.c_int64),
+        ("ms_extend", C.c_double), ("ms_shade", C.c_double), ("ms_shadow", C.c_double),
+        ("ms_other", C.c_double), ("n_extend", C.c_int64), ("n_shade", C.c_int64),
+        ("n_shadow", C.c_int64), ("passes", C.c_int32), ("paths_per_pass", C.c_int32),
+        ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libcpt.so once; raise when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libcpt.so not found at {path}: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    lib.cpt_version.restype = C.c_int
+    lib.cpt_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    lib.cpt_destroy.argtypes = [C.c_void_p]
+    lib.cpt_destroy.restype = None
+    lib.cpt_last_error.argtypes = [C.c_void_p]
+    lib.cpt_last_error.restype = C.c_char_p
+    lib.cpt_upload_scene.argtypes = [C.c_void_p, C.POINTER(SceneDesc)]
+    lib.cpt_bvh_info.argtypes = [C.c_void_p, C.c_void_p]
+    lib.cpt_tie_rank.argtypes = [C.c_void_p, C.c_void_p]
+    lib.cpt_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    lib.cpt_trace_any.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    lib.cpt_trace_closest_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    lib.cpt_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.POINTER(RenderStats)]
+    lib.cpt_read_film.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.cpt_render_views.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
+                                     C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
+                                     C.POINTER(RenderStats)]
+    _lib = lib
+    return lib
+
+
+class CptError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libcpt error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Context:
+    """One libcpt context (one HIP stream on one gfx950 device)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.cpt_create(C.byref(h), device)
+        if rc != CPT_OK:
+            raise CptError(rc, f"cpt_create(device={device}) failed")
+        self.h = h
+        self._keep = None
+        self.n_triangles = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cpt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != CPT_OK:
+            raise CptError(rc, f"{what}: {self.lib.cpt_last_error(self.h).decode()}")
+
+    def upload(self, scene, tie_rank: bool = True):
+        arrs = scene.c_arrays()
+        d = SceneDesc()
+        d.n_vertices = arrs["positions"].shape[0]
+        d.positions = _ptr(arrs["positions"])
+        d.normals = _ptr(arrs["normals"])
+        d.bitangents = _ptr(arrs["bitangents"])
+        d.n_triangles = arrs["indices"].shape[0]
+        d.indices = _ptr(arrs["indices"])
+        d.tri_material = _ptr(arrs["tri_material"])
+        d.n_materials = arrs["materials"].shape[0]
+        d.materials = _ptr(arrs["materials"])
+        d.n_lights = arrs["light_tris"].shape[0]
+        d.light_triangles = _ptr(arrs["light_tris"])
+        for k in range(16):
+            d.camera[k] = float(arrs["camera"][k])
+        for k in range(3):
+            d.env[k] = float(arrs["env"][k])
+        d.flags = 0 if tie_rank else CPT_SCENE_NO_TIE_RANK
+        self._check(self.lib.cpt_upload_scene(self.h, C.byref(d)), "cpt_upload_scene")
+        self.n_triangles = d.n_triangles
+
+    def bvh_info(self):
+        out = np.zeros(4, np.int64)
+        self._check(self.lib.cpt_bvh_info(self.h, _ptr(out)), "cpt_bvh_info")
+        return dict(nodes=int(out[0]), leaves=int(out[1]), depth=int(out[2]), triangles=int(out[3]))
+
+    def tie_rank(self):
+        out = np.zeros(self.n_triangles, np.int32)
+        self._check(self.lib.cpt_tie_rank(self.h, _ptr(out)), "cpt_tie_rank")
+        return out
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        self._check(self.lib.cpt_trace_closest(self.h, _ptr(rays), n, _ptr(tri), _ptr(t)), "cpt_trace_closest")
+        return tri, t
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        occ = np.zeros(n, np.uint8)
+        self._check(self.lib.cpt_trace_any(self.h, _ptr(rays), n, _ptr(occ)), "cpt_trace_any")
+        return occ
+
+    def render(self, width, height, spp_axis, max_depth=32, seed=1, flags=0, max_paths=0,
+               block_size=0, grid_blocks=0, view=0):
+        p = RenderParams()
+        p.width, p.height, p.spp_axis, p.max_depth = width, height, spp_axis, max_depth
+        p.seed, p.flags, p.max_paths = seed, flags, max_paths
+        p.block_size, p.grid_blocks, p.view = block_size, grid_blocks, view
+        st = RenderStats()
+        self._check(self.lib.cpt_render(self.h, C.byref(p), C.byref(st)), "cpt_render")
+        self._last = (width, height)
+        return st.as_dict()
+
+    def read_film(self, sums: bool = False):
+        w, h = self._last
+        rgb = np.zeros((h, w, 3), np.float32)
+        s = np.zeros((h, w, 3), np.float64) if sums else None
+        self._check(self.lib.cpt_read_film(self.h, _ptr(rgb), _ptr(s)), "cpt_read_film")
+        return (rgb, s) if sums else rgb

@@ -1,0 +1,314 @@
+"""Scene construction for the path tracer: flattened arrays + cpp-pt file I/O.
+
+A Scene holds exactly what crosses the C ABI (include/cpt.h, cpt_scene_desc):
+per-vertex positions/normals/bitangents, triangles in load order, a material
+table (16 floats per material), emitter triangles in LightGroup order and a
+camera frame.  `write_cpppt` writes the same scene in cpp-pt's own formats
+(settings file, .ptscene, .ptobj, .ptmat: src/loader/loader.h:117-311), which the
+C++ loader (cpp-pt_amd/host/loader.cpp) reads back to identical arrays.
+
+Generators:
+  box_room()      -- the 36-triangle closed box room with two rotated boxes and a
+                     ceiling area light (512x512, FOV 39 deg)
+  terrain_scene() -- 1024x512 fBm heightfield + 200 instanced icospheres
+                     (subdivision 4), mixed Lambertian / GGX / dielectric
+                     materials, 8 emissive quads and a constant environment
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MAT_DIFFUSE, MAT_GLOSSY, MAT_EMISSIVE, MAT_DIELECTRIC = 0, 1, 2, 3
+
+f32 = np.float32
+
+
+def material(kind, albedo=(0, 0, 0), rough=0.0, emission=(0, 0, 0), ior=1.5,
+             normal_map=(0.5, 0.5, 1.0)):
+    m = np.zeros(16, np.float32)
+    m[0] = kind
+    m[1:4] = albedo
+    m[4] = rough
+    m[5:8] = emission
+    m[8] = ior
+    m[9:12] = normal_map
+    m[12] = 1.0 if kind in (MAT_DIFFUSE, MAT_GLOSSY) else 0.0
+    return m
+
+
+def _normalized(v):
+    """cpp-pt normalized() on float32: v * float(1.0/double(float(sqrt(dot))))."""
+    v = np.asarray(v, np.float32)
+    d = f32(f32(v[0] * v[0]) + f32(v[1] * v[1]))
+    d = f32(d + f32(v[2] * v[2]))
+    ln = f32(math.sqrt(float(d)))
+    inv = f32(1.0 / float(ln))
+    return (v * inv).astype(np.float32)
+
+
+def _cross(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return np.array([f32(a[1] * b[2]) - f32(a[2] * b[1]),
+                     f32(a[2] * b[0]) - f32(a[0] * b[2]),
+                     f32(a[0] * b[1]) - f32(a[1] * b[0])], np.float32)
+
+
+def camera_frame(origin, at, up, tan_fovy, res_x, res_y):
+    """CameraPerspective set-up (camera_perspective.h:19-27, loader.h:117-128)."""
+    o = np.asarray(origin, np.float32)
+    fwd = _normalized(np.asarray(at, np.float32) - o)
+    z = _normalized(fwd * f32(-1.0))
+    x = _normalized(_cross(np.asarray(up, np.float32), z))
+    y = _normalized(_cross(z, x))
+    cam = np.zeros(16, np.float32)
+    cam[0:3] = o
+    cam[3:6] = x
+    cam[6:9] = y
+    cam[9:12] = z
+    cam[12] = f32(tan_fovy)
+    cam[13] = f32(f32(res_x) / f32(res_y))
+    cam[14] = res_x
+    cam[15] = res_y
+    return cam
+
+
+@dataclass
+class Mesh:
+    positions: np.ndarray   # (nv,3) float32
+    normals: np.ndarray     # (nv,3)
+    tangents: np.ndarray    # (nv,3); bitangent = cross(normal, tangent) (loader.h:237)
+    faces: np.ndarray       # (nf,3) int32
+    material: int           # index into Scene.material_list
+
+
+@dataclass
+class Scene:
+    meshes: list = field(default_factory=list)
+    material_list: list = field(default_factory=list)   # 16-float records
+    camera_def: tuple = ((0, 0, 3.9), (0, 0, 0), (0, 1, 0), 0.35411, 512, 512)
+    env: tuple = (0.0, 0.0, 0.0)
+    _flat: dict = None
+
+    def add_material(self, rec):
+        self.material_list.append(np.asarray(rec, np.float32))
+        self._flat = None
+        return len(self.material_list) - 1
+
+    def add_mesh(self, positions, normals, tangents, faces, mat):
+        self.meshes.append(Mesh(np.ascontiguousarray(positions, np.float32),
+                                np.ascontiguousarray(normals, np.float32),
+                                np.ascontiguousarray(tangents, np.float32),
+                                np.ascontiguousarray(faces, np.int32), mat))
+        self._flat = None
+
+    def set_camera(self, origin, at, up, tan_fovy, res_x, res_y):
+        self.camera_def = (tuple(origin), tuple(at), tuple(up), float(tan_fovy), int(res_x), int(res_y))
+        self._flat = None
+
+    @property
+    def resolution(self):
+        return self.camera_def[4], self.camera_def[5]
+
+    def c_arrays(self):
+        """Flattened arrays in load order (loader.h:353-367)."""
+        if self._flat is not None:
+            return self._flat
+        pos, nrm, bit, idx, tmat, lights = [], [], [], [], [], []
+        voff = 0
+        toff = 0
+        for m in self.meshes:
+            pos.append(m.positions)
+            nrm.append(m.normals)
+            n, t = m.normals, m.tangents
+            b = np.stack([n[:, 1] * t[:, 2] - n[:, 2] * t[:, 1],
+                          n[:, 2] * t[:, 0] - n[:, 0] * t[:, 2],
+                          n[:, 0] * t[:, 1] - n[:, 1] * t[:, 0]], axis=1).astype(np.float32)
+            bit.append(b)
+            idx.append(m.faces + voff)
+            tmat.append(np.full(len(m.faces), m.material, np.int32))
+            if self.material_list[m.material][0] == MAT_EMISSIVE:
+                lights.append(np.arange(toff, toff + len(m.faces), dtype=np.int32))
+            voff += len(m.positions)
+            toff += len(m.faces)
+        o, at, up, tf, rx, ry = self.camera_def
+        self._flat = dict(
+            positions=np.ascontiguousarray(np.concatenate(pos), np.float32),
+            normals=np.ascontiguousarray(np.concatenate(nrm), np.float32),
+            bitangents=np.ascontiguousarray(np.concatenate(bit), np.float32),
+            indices=np.ascontiguousarray(np.concatenate(idx), np.int32),
+            tri_material=np.ascontiguousarray(np.concatenate(tmat), np.int32),
+            materials=np.ascontiguousarray(np.stack(self.material_list), np.float32),
+            light_tris=np.ascontiguousarray(np.concatenate(lights) if lights else np.zeros(0, np.int32), np.int32),
+            camera=camera_frame(o, at, up, tf, rx, ry),
+            env=np.asarray(self.env, np.float32),
+        )
+        return self._flat
+
+    @property
+    def n_triangles(self):
+        return sum(len(m.faces) for m in self.meshes)
+
+    # ------------------------------------------------------------- cpp-pt files
+    def write_cpppt(self, directory, name="scene", spp=1, output="out", renderer="pathtracer_mis",
+                    max_depth=None, seed=None):
+        """Write <dir>/<name>.settings, <name>.ptscene, mesh_*.ptobj, mat_*.ptmat."""
+        os.makedirs(directory, exist_ok=True)
+        o, at, up, tf, rx, ry = self.camera_def
+        for k, rec in enumerate(self.material_list):
+            with open(os.path.join(directory, f"mat_{k}.ptmat"), "w") as f:
+                kind = int(rec[0])
+                if kind == MAT_EMISSIVE:
+                    f.write(f"EMISSION rgb({_n(rec[5])},{_n(rec[6])},{_n(rec[7])}) float(1.0)\n")
+                elif kind == MAT_DIELECTRIC:
+                    f.write(f"DIELECTRIC float({_n(rec[8])})\n")
+                else:
+                    metal = "1.0" if kind == MAT_GLOSSY else "0.0"
+                    f.write(f"DISNEY rgb({_n(rec[1])},{_n(rec[2])},{_n(rec[3])}) float(0.5) float({_n(rec[4])}) "
+                            f"float({metal}) float(0.0) float(1.5) rgb({_n(rec[9])},{_n(rec[10])},{_n(rec[11])}) "
+                            f"float(1.0)\n")
+        lines = [f"camera {_fmt(o)} {_fmt(at)} {_fmt(up)} {_n(tf)}"]
+        if any(self.env):
+            lines.append(f"environment {_fmt(self.env)}")
+        for i, m in enumerate(self.meshes):
+            fn = f"mesh_{i}.ptobj"
+            with open(os.path.join(directory, fn), "w") as f:
+                f.write(f"{len(m.positions)} {len(m.faces)} TRIANGLE_MESH\n")
+                for p, n, t in zip(m.positions, m.normals, m.tangents):
+                    f.write(f"{_fmt(p)} 0 0 {_fmt(n)} {_fmt(t)}\n")
+                for fc in m.faces:
+                    f.write(f"{fc[0]} {fc[1]} {fc[2]}\n")
+            lines.append(f"obj {fn} mat_{m.material}.ptmat")
+        scene_path = os.path.join(directory, f"{name}.ptscene")
+        with open(scene_path, "w") as f:
+            f.write("\n".join(lines) + "\n")
+        settings = os.path.join(directory, f"{name}.settings")
+        with open(settings, "w") as f:
+            f.write(f"renderer {renderer}\nspp {spp}\nresolution {rx} {ry}\nscene {scene_path}\n"
+                    f"output {os.path.join(directory, output)}\n")
+            if max_depth is not None:
+                f.write(f"max_depth {max_depth}\n")
+            if seed is not None:
+                f.write(f"seed {seed}\n")
+        return settings
+
+
+def _n(x):
+    return repr(float(np.float32(x)))
+
+
+def _fmt(v):
+    return " ".join(_n(x) for x in v)
+
+
+# --------------------------------------------------------------------- geometry helpers
+def _quad(p0, p1, p2, p3, normal, tangent):
+    """Quad p0..p3 (counter-clockwise seen from the normal side) as 2 triangles."""
+    P = np.array([p0, p1, p2, p3], np.float32)
+    N = np.tile(np.asarray(normal, np.float32), (4, 1))
+    T = np.tile(np.asarray(tangent, np.float32), (4, 1))
+    F = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return P, N, T, F
+
+
+def _box(center, size, angle_deg):
+    """Axis-aligned box rotated about y; outward normals; 6 faces, 12 triangles."""
+    cx, cy, cz = center
+    sx, sy, sz = (s * 0.5 for s in size)
+    a = math.radians(angle_deg)
+    ca, sa = math.cos(a), math.sin(a)
+
+    def rot(p):
+        x, y, z = p
+        return (cx + ca * x + sa * z, cy + y, cz - sa * x + ca * z)
+
+    def rotn(n):
+        x, y, z = n
+        return (ca * x + sa * z, y, -sa * x + ca * z)
+
+    faces = [
+        # (normal, tangent, corners ccw seen from outside)
+        ((0, 1, 0), (1, 0, 0), [(-sx, sy, sz), (sx, sy, sz), (sx, sy, -sz), (-sx, sy, -sz)]),
+        ((0, -1, 0), (1, 0, 0), [(-sx, -sy, -sz), (sx, -sy, -sz), (sx, -sy, sz), (-sx, -sy, sz)]),
+        ((1, 0, 0), (0, 0, -1), [(sx, -sy, sz), (sx, -sy, -sz), (sx, sy, -sz), (sx, sy, sz)]),
+        ((-1, 0, 0), (0, 0, 1), [(-sx, -sy, -sz), (-sx, -sy, sz), (-sx, sy, sz), (-sx, sy, -sz)]),
+        ((0, 0, 1), (1, 0, 0), [(-sx, -sy, sz), (sx, -sy, sz), (sx, sy, sz), (-sx, sy, sz)]),
+        ((0, 0, -1), (-1, 0, 0), [(sx, -sy, -sz), (-sx, -sy, -sz), (-sx, sy, -sz), (sx, sy, -sz)]),
+    ]
+    Ps, Ns, Ts, Fs = [], [], [], []
+    for k, (n, t, cs) in enumerate(faces):
+        P, N, T, F = _quad(*[rot(c) for c in cs], rotn(n), rotn(t))
+        Ps.append(P)
+        Ns.append(N)
+        Ts.append(T)
+        Fs.append(F + 4 * k)
+    return np.concatenate(Ps), np.concatenate(Ns), np.concatenate(Ts), np.concatenate(Fs)
+
+
+def box_room(res=(512, 512)):
+    """Closed 2x2x2 box room (open front), 5 Lambertian quads, two boxes rotated
+    +/-17 deg (heights 0.6 and 1.2), one 0.47x0.38 ceiling light (17,12,4)."""
+    s = Scene()
+    white = s.add_material(material(MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73)))
+    red = s.add_material(material(MAT_DIFFUSE, albedo=(0.63, 0.07, 0.05)))
+    green = s.add_material(material(MAT_DIFFUSE, albedo=(0.14, 0.45, 0.09)))
+    light = s.add_material(material(MAT_EMISSIVE, emission=(17.0, 12.0, 4.0)))
+    walls = [
+        # floor y=-1 (normal +y)
+        ((-1, -1, 1), (1, -1, 1), (1, -1, -1), (-1, -1, -1), (0, 1, 0), (1, 0, 0), white),
+        # ceiling y=+1 (normal -y)
+        ((-1, 1, -1), (1, 1, -1), (1, 1, 1), (-1, 1, 1), (0, -1, 0), (1, 0, 0), white),
+        # back wall z=-1 (normal +z)
+        ((-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (0, 0, 1), (1, 0, 0), white),
+        # left wall x=-1 (normal +x)
+        ((-1, -1, 1), (-1, -1, -1), (-1, 1, -1), (-1, 1, 1), (1, 0, 0), (0, 0, -1), red),
+        # right wall x=+1 (normal -x)
+        ((1, -1, -1), (1, -1, 1), (1, 1, 1), (1, 1, -1), (-1, 0, 0), (0, 0, 1), green),
+    ]
+    for p0, p1, p2, p3, n, t, m in walls:
+        P, N, T, F = _quad(p0, p1, p2, p3, n, t)
+        s.add_mesh(P, N, T, F, m)
+    P, N, T, F = _box((0.33, -1.0 + 0.3, 0.35), (0.6, 0.6, 0.6), -17.0)
+    s.add_mesh(P, N, T, F, white)
+    P, N, T, F = _box((-0.33, -1.0 + 0.6, -0.3), (0.6, 1.2, 0.6), 17.0)
+    s.add_mesh(P, N, T, F, white)
+    y = 1.0 - 1e-3
+    hx, hz = 0.47 / 2, 0.38 / 2
+    P, N, T, F = _quad((-hx, y, -hz), (hx, y, -hz), (hx, y, hz), (-hx, y, hz), (0, -1, 0), (1, 0, 0))
+    s.add_mesh(P, N, T, F, light)
+    s.set_camera((0.0, 0.0, 3.9), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.tan(math.radians(19.5)), *res)
+    return s
+
+
+def emissive_box(res=(64, 64), le=(2.0, 3.0, 5.0)):
+    """Camera inside a closed box whose 6 faces all emit `le`: every camera ray
+    hits an emitter first, so the image equals le exactly (emitters do not
+    reflect, pathtracerMIS.h:67)."""
+    s = Scene()
+    em = s.add_material(material(MAT_EMISSIVE, emission=le))
+    P, N, T, F = _box((0.0, 0.0, 0.0), (4.0, 4.0, 4.0), 0.0)
+    N = -N  # inward
+    F = F[:, ::-1].copy()
+    s.add_mesh(P, N, T, F, em)
+    s.set_camera((0.1, 0.2, 0.3), (1.0, 0.5, -2.0), (0.0, 1.0, 0.0), 0.5, *res)
+    return s
+
+
+def single_quad_light(res=(32, 32), size=0.5, height=1.0, le=(1.0, 1.0, 1.0), albedo=0.5):
+    """Diffuse floor (large) under a square emitter of side `size` at `height`,
+    facing down; used for the one-bounce NEE closed form."""
+    s = Scene()
+    floor = s.add_material(material(MAT_DIFFUSE, albedo=(albedo, albedo, albedo)))
+    em = s.add_material(material(MAT_EMISSIVE, emission=le))
+    L = 50.0
+    P, N, T, F = _quad((-L, 0, L), (L, 0, L), (L, 0, -L), (-L, 0, -L), (0, 1, 0), (1, 0, 0))
+    s.add_mesh(P, N, T, F, floor)
+    h = size / 2
+    P, N, T, F = _quad((-h, height, -h), (h, height, -h), (h, height, h), (-h, height, h), (0, -1, 0), (1, 0, 0))
+    s.add_mesh(P, N, T, F, em)
+    s.set_camera((0.0, 0.5, 0.0), (0.0, -1.0, 0.0), (0.0, 0.0, 1.0), 0.001, *res)
+    return s

@@ -1,0 +1,1032 @@
+// libcpt: wavefront PT-NEE-MIS on one MI355X (gfx950) behind the C ABI of include/cpt.h.
+//
+// Design (one GPU, one HIP stream per context):
+//  * Paths live in fixed slots of SoA state arrays for one pass.  A pass is a
+//    rectangle (pixel range x sample range); slot = sample_local*npix + pixel_local.
+//  * Per bounce three grid-stride kernels run over compacted queues:
+//      extend  : closest hit for the extension-ray queue (BVH4, LDS stack)
+//      shade   : emitter MIS, BSDF sample, NEE light sample, RR; appends the
+//                next extension ray and one shadow ray per path with
+//                wave64 ballot + popcount and one atomic per wave
+//      shadow  : any hit for the shadow queue; unoccluded -> add the precomputed
+//                NEE contribution to the path's radiance
+//    Queue counts live in device memory; kernels read them and exit early, so
+//    the whole pass (raygen, max_depth bounces, film accumulation) is captured
+//    once in a hipGraph and replayed per pass.
+//  * The film is a W*H*3 FP64 accumulator; each pass adds its slots' radiance
+//    in sample order, so results are deterministic and identical in summation
+//    order to a sequential per-pixel loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cpt.h"
+#include "bvh_build.h"
+#include "pt_device.h"
+#include "pt_shade.h"
+
+using namespace cpt;
+
+namespace {
+
+struct PassDesc {
+    uint32_t p0;      // first pixel
+    uint32_t npix;    // pixels in the pass
+    uint32_t s0;      // first sample index
+    uint32_t ns;      // samples per pixel in the pass
+    uint32_t n_paths; // npix * ns
+    uint32_t pad[3];
+};
+
+enum {
+    ST_EXT = 0,
+    ST_SHADOW = 1,
+    ST_NODES = 2,
+    ST_TRIS = 3,
+    ST_SNODES = 4,
+    ST_STRIS = 5,
+    ST_PATHS = 6,
+    ST_N = 8
+};
+
+struct WfArgs {
+    DevScene sc;
+    float4* mul_p;     // mul.xyz, p
+    float4* col;       // radiance
+    uint64_t* rng;
+    uint32_t* meta;    // depth | sampled_delta << 8
+    uint32_t* pix;     // pixel index
+    float4* qo[2];     // o.xyz, tmax
+    float4* qd[2];     // d.xyz, slot
+    float4* hits;      // t, beta, gamma, tri
+    float4* so;        // shadow o.xyz, tmax
+    float4* sd;        // shadow d.xyz, slot
+    float4* sc_rgb;    // shadow contribution
+    uint32_t* counters;  // [0],[1] extension queues, [2] shadow queue
+    unsigned long long* stats;
+    int* gstack;
+    const PassDesc* pass;
+    double* film;
+    int W, H, S, max_depth;
+    uint64_t seed;
+    uint32_t view;
+};
+
+__device__ __forceinline__ unsigned gthreads() { return gridDim.x * blockDim.x; }
+__device__ __forceinline__ unsigned gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// wave64 queue append: one atomic per wave, order-preserving within the wave
+__device__ __forceinline__ unsigned wave_alloc(bool pred, uint32_t* counter) {
+    unsigned long long mask = __ballot(pred);
+    unsigned lane = __lane_id();
+    unsigned base = 0;
+    if (mask) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        if ((int)lane == leader) base = atomicAdd(counter, (unsigned)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    unsigned long long lt = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+    return base + (unsigned)__popcll(lt);
+}
+
+__device__ __forceinline__ unsigned wave_sum(unsigned v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// ------------------------------------------------------------------ raygen
+__global__ void k_raygen(WfArgs a) {
+    const PassDesc pd = *a.pass;
+    if (gtid() == 0) {
+        a.counters[0] = pd.n_paths;
+        a.counters[1] = 0;
+        a.counters[2] = 0;
+        atomicAdd(&a.stats[ST_PATHS], (unsigned long long)pd.n_paths);
+    }
+    const int S = a.S, S2 = a.S * a.S;
+    for (unsigned i = gtid(); i < pd.n_paths; i += gthreads()) {
+        unsigned sl = i / pd.npix, pl = i - sl * pd.npix;
+        unsigned pixel = pd.p0 + pl;
+        unsigned sample = pd.s0 + sl;
+        int px = (int)(pixel % (unsigned)a.W), py = (int)(pixel / (unsigned)a.W);
+        int k = (int)sample / S, l = (int)sample % S;
+        uint64_t path_id = ((uint64_t)a.view << 40) + (uint64_t)pixel * (uint64_t)S2 + sample;
+        uint64_t rng = pcg_init(a.seed, path_id);
+        float r1 = pcg_float(rng);
+        float r2 = pcg_float(rng);
+        float cx, cy;
+        film_coords(px, py, k, l, S, a.W, a.H, r1, r2, cx, cy);
+        V3 o, d;
+        camera_ray(a.sc.cam, cx, cy, o, d);
+        a.qo[0][i] = make_float4(o.x, o.y, o.z, INFINITY);
+        a.qd[0][i] = make_float4(d.x, d.y, d.z, __int_as_float((int)i));
+        a.mul_p[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        a.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.rng[i] = rng;
+        a.meta[i] = 1u << 8;  // depth 0, sampled_delta = true (camera ray)
+        a.pix[i] = pixel;
+    }
+}
+
+// ------------------------------------------------------------------ extend
+template <int B, bool COUNT>
+__global__ __launch_bounds__(B) void k_extend(WfArgs a, int cur) {
+    __shared__ int lds[CPT_LDS_SS * B];
+    const unsigned count = a.counters[cur];
+    if (gtid() == 0) {
+        a.counters[cur ^ 1] = 0;
+        a.counters[2] = 0;
+        atomicAdd(&a.stats[ST_EXT], (unsigned long long)count);
+    }
+    TravStack<B> st{lds, a.gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
+    unsigned nv = 0, nt = 0;
+    for (unsigned i = gtid(); i < count; i += gthreads()) {
+        float4 O = a.qo[cur][i], D = a.qd[cur][i];
+        Hit h;
+        traverse<B, false, COUNT>(a.sc, v3(O.x, O.y, O.z), v3(D.x, D.y, D.z), O.w, h, st, nv, nt);
+        a.hits[i] = make_float4(h.t, h.beta, h.gamma, __int_as_float(h.tri));
+    }
+    if (COUNT) {
+        nv = wave_sum(nv);
+        nt = wave_sum(nt);
+        if (__lane_id() == 0) {
+            atomicAdd(&a.stats[ST_NODES], (unsigned long long)nv);
+            atomicAdd(&a.stats[ST_TRIS], (unsigned long long)nt);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ shade
+// pathtracerMIS.h:37-126, one loop iteration per queue entry.
+template <int B>
+__global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
+    const unsigned count = a.counters[cur];
+    const unsigned lane = __lane_id();
+    const unsigned wave = gtid() >> 6, nwaves = gthreads() >> 6;
+    for (unsigned base = wave * 64; base < count; base += nwaves * 64) {
+        const unsigned i = base + lane;
+        const bool active = i < count;
+        bool push_ext = false, push_sh = false;
+        V3 no = {0, 0, 0}, nd = {0, 0, 0}, sho = {0, 0, 0}, shd = {0, 0, 0}, contrib = {0, 0, 0};
+        float shmax = 0.0f;
+        int slot = 0;
+        if (active) {
+            float4 O = a.qo[cur][i], D = a.qd[cur][i], HH = a.hits[i];
+            slot = __float_as_int(D.w);
+            V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
+            float4 MP = a.mul_p[slot], C = a.col[slot];
+            uint64_t rng = a.rng[slot];
+            uint32_t meta = a.meta[slot];
+            int depth = (int)(meta & 0xffu);
+            bool sampled_delta = (meta >> 8) & 1u;
+            V3 mulv = v3(MP.x, MP.y, MP.z), colv = v3(C.x, C.y, C.z);
+            float p = MP.w;
+            Hit h;
+            h.t = HH.x;
+            h.beta = HH.y;
+            h.gamma = HH.z;
+            h.tri = __float_as_int(HH.w);
+            bool alive = false;
+            if (h.tri < 0) {
+                // render_sky (pathtracerMIS.h:38-40), constant environment
+                colv = add(colv, mul(mulv, v3(a.sc.env[0], a.sc.env[1], a.sc.env[2])));
+            } else {
+                Isect it;
+                int matid;
+                fill_isect(a.sc, o, d, h, it, matid);
+                Mat m = load_mat(a.sc.mats + 16 * matid);
+                if (depth == 0) first_hit_frame(m, it);
+                V3 wo = mulf(d, -1.0f);
+                if (m.type == CPT_MAT_EMISSIVE) {
+                    if (sampled_delta) {
+                        colv = add(colv, mul(mulv, m.emission));
+                    } else {
+                        float pbb = p;
+                        float r = h.t;
+                        float cos_theta = (float)(fabs((double)dot(it.normal, d)) + 1e-4);
+                        float pbn = light_pdf(a.sc, h.tri) * r * r / cos_theta;
+                        float w = pbb / (pbb + pbn);
+                        colv = add(colv, mulf(mul(mulv, m.emission), w));
+                    }
+                } else {
+                    sampled_delta = false;
+                    DSample ds = bsdf_sample(m, it, wo, rng);
+                    p = ds.pdf;
+                    if (!ds.delta) {
+                        if (a.sc.n_lights > 0) {
+                            LSample ls = light_sample(a.sc, rng, it.hitpoint);
+                            V3 sdir = sub(ls.position, it.hitpoint);
+                            float len = len3(sdir);
+                            sdir = divs(sdir, len);
+                            if (bsdf_non_zero(m, it, wo, sdir)) {
+                                float pnb = bsdf_pdf(m, it, wo, sdir);
+                                float ct = (float)(fabs((double)dot(ls.normal, sdir)) + 1e-4);
+                                float pnn = ls.pdf * len * len / ct;
+                                float w = pnn / (pnb + pnn);
+                                V3 ev = bsdf_eval(m, it, wo, sdir);
+                                contrib = mulf(divs(mul(mul(mulv, ev), ls.intensity), ls.pdf * len * len), w);
+                                sho = add(it.hitpoint, mulf(sdir, CPT_EPSF));
+                                shd = sdir;
+                                shmax = (float)((double)len - 2.0 * 1e-4);
+                                push_sh = true;
+                            }
+                        }
+                    } else {
+                        sampled_delta = true;
+                    }
+                    if (bsdf_non_zero(m, it, wo, ds.wi)) {
+                        V3 ev = bsdf_eval(m, it, wo, ds.wi);
+                        bool survive = true;
+                        if (depth > 2) {
+                            float rr = russian_roulette(ev);
+                            if (pcg_float(rng) > rr) survive = false;
+                            else p = p * rr;
+                        }
+                        if (survive) {
+                            mulv = divs(mul(mulv, ev), p);
+                            if (depth + 1 < depth_cap) {
+                                no = add(it.hitpoint, mulf(ds.wi, CPT_EPSF));
+                                nd = ds.wi;
+                                alive = true;
+                            }
+                        }
+                    }
+                }
+            }
+            a.col[slot] = make_float4(colv.x, colv.y, colv.z, 0.0f);
+            if (alive || push_sh) {
+                a.mul_p[slot] = make_float4(mulv.x, mulv.y, mulv.z, p);
+                a.rng[slot] = rng;
+                a.meta[slot] = (uint32_t)(depth + 1) | ((sampled_delta ? 1u : 0u) << 8);
+            }
+            push_ext = alive;
+        }
+        unsigned qe = wave_alloc(push_ext, &a.counters[cur ^ 1]);
+        if (push_ext) {
+            a.qo[cur ^ 1][qe] = make_float4(no.x, no.y, no.z, INFINITY);
+            a.qd[cur ^ 1][qe] = make_float4(nd.x, nd.y, nd.z, __int_as_float(slot));
+        }
+        unsigned qs = wave_alloc(push_sh, &a.counters[2]);
+        if (push_sh) {
+            a.so[qs] = make_float4(sho.x, sho.y, sho.z, shmax);
+            a.sd[qs] = make_float4(shd.x, shd.y, shd.z, __int_as_float(slot));
+            a.sc_rgb[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ shadow
+template <int B, bool COUNT>
+__global__ __launch_bounds__(B) void k_shadow(WfArgs a) {
+    __shared__ int lds[CPT_LDS_SS * B];
+    const unsigned count = a.counters[2];
+    if (gtid() == 0) atomicAdd(&a.stats[ST_SHADOW], (unsigned long long)count);
+    TravStack<B> st{lds, a.gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
+    unsigned nv = 0, nt = 0;
+    for (unsigned i = gtid(); i < count; i += gthreads()) {
+        float4 O = a.so[i], D = a.sd[i];
+        Hit h;
+        bool occ = traverse<B, true, COUNT>(a.sc, v3(O.x, O.y, O.z), v3(D.x, D.y, D.z), O.w, h, st, nv, nt);
+        if (!occ) {
+            int slot = __float_as_int(D.w);
+            float4 c = a.col[slot], add_ = a.sc_rgb[i];
+            a.col[slot] = make_float4(c.x + add_.x, c.y + add_.y, c.z + add_.z, 0.0f);
+        }
+    }
+    if (COUNT) {
+        nv = wave_sum(nv);
+        nt = wave_sum(nt);
+        if (__lane_id() == 0) {
+            atomicAdd(&a.stats[ST_SNODES], (unsigned long long)nv);
+            atomicAdd(&a.stats[ST_STRIS], (unsigned long long)nt);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ film
+__global__ void k_accum(WfArgs a) {
+    const PassDesc pd = *a.pass;
+    for (unsigned pl = gtid(); pl < pd.npix; pl += gthreads()) {
+        unsigned pixel = pd.p0 + pl;
+        double* f = a.film + 3 * (size_t)pixel;
+        double x = f[0], y = f[1], z = f[2];
+        for (unsigned s = 0; s < pd.ns; s++) {
+            float4 c = a.col[(size_t)s * pd.npix + pl];
+            x += (double)c.x;
+            y += (double)c.y;
+            z += (double)c.z;
+        }
+        f[0] = x;
+        f[1] = y;
+        f[2] = z;
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// ------------------------------------------------------------------ ray-batch parity entries
+template <int B>
+__global__ __launch_bounds__(B) void k_trace_closest(DevScene sc, const float4* rays, int n, float4* hits, int* gstack) {
+    __shared__ int lds[CPT_LDS_SS * B];
+    TravStack<B> st{lds, gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
+    unsigned nv = 0, nt = 0;
+    for (unsigned i = gtid(); i < (unsigned)n; i += gthreads()) {
+        float4 O = rays[2 * i], D = rays[2 * i + 1];
+        Hit h;
+        traverse<B, false, false>(sc, v3(O.x, O.y, O.z), v3(D.x, D.y, D.z), O.w, h, st, nv, nt);
+        hits[i] = make_float4(h.t, h.beta, h.gamma, __int_as_float(h.tri));
+    }
+}
+
+template <int B>
+__global__ __launch_bounds__(B) void k_trace_any(DevScene sc, const float4* rays, int n, uint8_t* occ, int* gstack) {
+    __shared__ int lds[CPT_LDS_SS * B];
+    TravStack<B> st{lds, gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
+    unsigned nv = 0, nt = 0;
+    for (unsigned i = gtid(); i < (unsigned)n; i += gthreads()) {
+        float4 O = rays[2 * i], D = rays[2 * i + 1];
+        Hit h;
+        occ[i] = traverse<B, true, false>(sc, v3(O.x, O.y, O.z), v3(D.x, D.y, D.z), O.w, h, st, nv, nt) ? 1 : 0;
+    }
+}
+
+}  // namespace
+
+// ====================================================================== host side
+
+struct cpt_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // scene
+    bool has_scene = false;
+    int nt = 0, nv = 0, nl = 0, nm = 0;
+    int64_t bvh_nodes = 0, bvh_leaves = 0;
+    int bvh_depth = 0;
+    std::vector<int32_t> rank;
+    Node4* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    int4* d_tri_info = nullptr;
+    float4 *d_vpos = nullptr, *d_vnrm = nullptr, *d_vbit = nullptr;
+    float* d_mats = nullptr;
+    int* d_lights = nullptr;
+    float cam[16];
+    float env[3];
+
+    // wavefront buffers
+    size_t cap = 0;        // path slots
+    size_t gcap = 0;       // threads covered by the overflow stack
+    float4 *mul_p = nullptr, *col = nullptr, *qo[2] = {nullptr, nullptr}, *qd[2] = {nullptr, nullptr};
+    float4 *hits = nullptr, *so = nullptr, *sd = nullptr, *sc_rgb = nullptr;
+    uint64_t* rng = nullptr;
+    uint32_t *meta = nullptr, *pix = nullptr, *counters = nullptr;
+    unsigned long long* stats = nullptr;
+    int* gstack = nullptr;
+    PassDesc* d_pass = nullptr;
+    PassDesc* h_pass = nullptr;  // pinned
+    size_t h_pass_cap = 0;
+    double* film = nullptr;
+    size_t film_px = 0;
+    int film_w = 0, film_h = 0, film_S = 1;
+
+    // captured pass graph
+    hipGraphExec_t gexec = nullptr;
+    std::string graph_key;
+};
+
+namespace {
+
+int fail(cpt_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail(ctx, CPT_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_scene(cpt_ctx* c) {
+    dfree(c->d_nodes);
+    dfree(c->d_tris);
+    dfree(c->d_tri_info);
+    dfree(c->d_vpos);
+    dfree(c->d_vnrm);
+    dfree(c->d_vbit);
+    dfree(c->d_mats);
+    dfree(c->d_lights);
+    c->has_scene = false;
+}
+
+void free_wavefront(cpt_ctx* c) {
+    dfree(c->mul_p);
+    dfree(c->col);
+    dfree(c->qo[0]);
+    dfree(c->qo[1]);
+    dfree(c->qd[0]);
+    dfree(c->qd[1]);
+    dfree(c->hits);
+    dfree(c->so);
+    dfree(c->sd);
+    dfree(c->sc_rgb);
+    dfree(c->rng);
+    dfree(c->meta);
+    dfree(c->pix);
+    c->cap = 0;
+}
+
+void drop_graph(cpt_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->graph_key.clear();
+}
+
+DevScene dev_scene(const cpt_ctx* c) {
+    DevScene s;
+    s.nodes = c->d_nodes;
+    s.tris = c->d_tris;
+    s.tri_info = c->d_tri_info;
+    s.vpos = c->d_vpos;
+    s.vnrm = c->d_vnrm;
+    s.vbit = c->d_vbit;
+    s.mats = c->d_mats;
+    s.lights = c->d_lights;
+    s.n_lights = c->nl;
+    s.n_tris = c->nt;
+    for (int k = 0; k < 3; k++) s.env[k] = c->env[k];
+    for (int k = 0; k < 16; k++) s.cam[k] = c->cam[k];
+    return s;
+}
+
+template <typename T>
+int dalloc(cpt_ctx* c, T*& p, size_t count) {
+    dfree(p);
+    if (count == 0) count = 1;
+    HIPCHK(c, hipMalloc((void**)&p, count * sizeof(T)));
+    return CPT_OK;
+}
+
+int ensure_gstack(cpt_ctx* c, size_t threads) {
+    if (c->gcap >= threads && c->gstack) return CPT_OK;
+    drop_graph(c);
+    int r = dalloc(c, c->gstack, threads * (size_t)CPT_GSTACK);
+    if (r) return r;
+    c->gcap = threads;
+    return CPT_OK;
+}
+
+int ensure_wavefront(cpt_ctx* c, size_t n) {
+    if (c->cap >= n && c->mul_p) return CPT_OK;
+    drop_graph(c);
+    free_wavefront(c);
+    int r = 0;
+    r |= dalloc(c, c->mul_p, n);
+    r |= dalloc(c, c->col, n);
+    r |= dalloc(c, c->qo[0], n);
+    r |= dalloc(c, c->qo[1], n);
+    r |= dalloc(c, c->qd[0], n);
+    r |= dalloc(c, c->qd[1], n);
+    r |= dalloc(c, c->hits, n);
+    r |= dalloc(c, c->so, n);
+    r |= dalloc(c, c->sd, n);
+    r |= dalloc(c, c->sc_rgb, n);
+    r |= dalloc(c, c->rng, n);
+    r |= dalloc(c, c->meta, n);
+    r |= dalloc(c, c->pix, n);
+    if (r) return CPT_E_HIP;
+    c->cap = n;
+    return CPT_OK;
+}
+
+using KernelVoid = void (*)(WfArgs);
+
+struct LaunchCfg {
+    int B;
+    int G;
+    bool count;
+};
+
+// launch helpers dispatching on the block size
+template <int B>
+void launch_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) {
+    if (L.count) hipLaunchKernelGGL((k_extend<B, true>), dim3(L.G), dim3(B), 0, s, a, cur);
+    else hipLaunchKernelGGL((k_extend<B, false>), dim3(L.G), dim3(B), 0, s, a, cur);
+}
+template <int B>
+void launch_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
+    hipLaunchKernelGGL((k_shade<B>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
+}
+template <int B>
+void launch_shadow(const LaunchCfg& L, const WfArgs& a, hipStream_t s) {
+    if (L.count) hipLaunchKernelGGL((k_shadow<B, true>), dim3(L.G), dim3(B), 0, s, a);
+    else hipLaunchKernelGGL((k_shadow<B, false>), dim3(L.G), dim3(B), 0, s, a);
+}
+
+void L_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) {
+    switch (L.B) {
+        case 64: launch_extend<64>(L, a, cur, s); break;
+        case 256: launch_extend<256>(L, a, cur, s); break;
+        default: launch_extend<128>(L, a, cur, s); break;
+    }
+}
+void L_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
+    switch (L.B) {
+        case 64: launch_shade<64>(L, a, cur, cap, s); break;
+        case 256: launch_shade<256>(L, a, cur, cap, s); break;
+        default: launch_shade<128>(L, a, cur, cap, s); break;
+    }
+}
+void L_shadow(const LaunchCfg& L, const WfArgs& a, hipStream_t s) {
+    switch (L.B) {
+        case 64: launch_shadow<64>(L, a, s); break;
+        case 256: launch_shadow<256>(L, a, s); break;
+        default: launch_shadow<128>(L, a, s); break;
+    }
+}
+
+// Issue one pass worth of kernels on stream s.
+struct Timers {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;  // 0 extend, 1 shade, 2 shadow, 3 other
+};
+
+int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, Timers* tm) {
+    auto mark = [&](int kind) {
+        if (!tm) return;
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        tm->ev.push_back(e);
+        tm->kind.push_back(kind);
+    };
+    mark(-1);
+    hipLaunchKernelGGL(k_raygen, dim3(L.G), dim3(L.B), 0, s, a);
+    mark(3);
+    for (int b = 0; b < a.max_depth; b++) {
+        int cur = b & 1;
+        L_extend(L, a, cur, s);
+        mark(0);
+        L_shade(L, a, cur, a.max_depth, s);
+        mark(1);
+        L_shadow(L, a, s);
+        mark(2);
+    }
+    hipLaunchKernelGGL(k_accum, dim3(L.G), dim3(L.B), 0, s, a);
+    mark(3);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, CPT_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return CPT_OK;
+}
+
+int check_device(cpt_ctx* c) {
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(c, CPT_E_NODEVICE, "device %d is %s, libcpt is built for gfx950", c->device, prop.gcnArchName);
+    c->cus = prop.multiProcessorCount;
+    return CPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpt_version(void) { return 1000; }
+
+int cpt_create(cpt_ctx** out, int device) {
+    if (!out) return CPT_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return CPT_E_NODEVICE;
+    cpt_ctx* c = new cpt_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return CPT_E_NODEVICE;
+    }
+    int r = check_device(c);
+    if (r) {
+        delete c;
+        return r;
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&c->counters, 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&c->stats, ST_N * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void**)&c->d_pass, sizeof(PassDesc)) != hipSuccess) {
+        delete c;
+        return CPT_E_HIP;
+    }
+    *out = c;
+    return CPT_OK;
+}
+
+void cpt_destroy(cpt_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    free_scene(c);
+    free_wavefront(c);
+    dfree(c->gstack);
+    dfree(c->counters);
+    dfree(c->stats);
+    dfree(c->d_pass);
+    dfree(c->film);
+    if (c->h_pass) (void)hipHostFree(c->h_pass);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* cpt_last_error(cpt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
+    if (!c || !d) return CPT_E_INVALID;
+    if (d->n_triangles < 0 || d->n_vertices < 0 || d->n_materials <= 0 || d->n_lights < 0)
+        return fail(c, CPT_E_INVALID, "negative sizes or no materials");
+    if (d->n_triangles > 0 && (!d->positions || !d->normals || !d->bitangents || !d->indices || !d->tri_material))
+        return fail(c, CPT_E_INVALID, "missing vertex or triangle arrays");
+    if (d->n_triangles >= (1 << 27)) return fail(c, CPT_E_LIMIT, "at most 2^27-1 triangles");
+    for (int t = 0; t < d->n_triangles; t++) {
+        for (int k = 0; k < 3; k++) {
+            int v = d->indices[3 * (size_t)t + k];
+            if (v < 0 || v >= d->n_vertices) return fail(c, CPT_E_INVALID, "triangle %d: vertex %d out of range", t, v);
+        }
+        int m = d->tri_material[t];
+        if (m < 0 || m >= d->n_materials) return fail(c, CPT_E_INVALID, "triangle %d: material %d out of range", t, m);
+    }
+    for (int i = 0; i < d->n_lights; i++) {
+        int t = d->light_triangles[i];
+        if (t < 0 || t >= d->n_triangles) return fail(c, CPT_E_INVALID, "light %d: triangle %d out of range", i, t);
+    }
+    for (int m = 0; m < d->n_materials; m++) {
+        int ty = (int)d->materials[16 * (size_t)m];
+        if (ty < 0 || ty > 3) return fail(c, CPT_E_INVALID, "material %d: unknown type %d", m, ty);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    drop_graph(c);
+    free_scene(c);
+    c->nt = d->n_triangles;
+    c->nv = d->n_vertices;
+    c->nl = d->n_lights;
+    c->nm = d->n_materials;
+    std::memcpy(c->cam, d->camera, sizeof(c->cam));
+    std::memcpy(c->env, d->env, sizeof(c->env));
+
+    c->rank.clear();
+    if (!(d->flags & CPT_SCENE_NO_TIE_RANK)) reference_tie_rank(d->positions, d->indices, c->nt, c->rank);
+    Bvh4 bvh;
+    build_bvh4(d->positions, d->indices, c->nt, c->rank, bvh);
+    if (3 * bvh.max_depth + 1 > CPT_LDS_SS + CPT_GSTACK)
+        return fail(c, CPT_E_LIMIT, "BVH depth %d exceeds the traversal stack", bvh.max_depth);
+    c->bvh_nodes = (int64_t)bvh.nodes.size();
+    c->bvh_leaves = bvh.leaves;
+    c->bvh_depth = bvh.max_depth;
+
+    std::vector<float4> vp(std::max(1, c->nv)), vn(std::max(1, c->nv)), vb(std::max(1, c->nv));
+    for (int i = 0; i < c->nv; i++) {
+        vp[i] = make_float4(d->positions[3 * (size_t)i], d->positions[3 * (size_t)i + 1], d->positions[3 * (size_t)i + 2], 0.f);
+        vn[i] = make_float4(d->normals[3 * (size_t)i], d->normals[3 * (size_t)i + 1], d->normals[3 * (size_t)i + 2], 0.f);
+        vb[i] = make_float4(d->bitangents[3 * (size_t)i], d->bitangents[3 * (size_t)i + 1], d->bitangents[3 * (size_t)i + 2], 0.f);
+    }
+    std::vector<int4> ti(std::max(1, c->nt));
+    for (int t = 0; t < c->nt; t++)
+        ti[t] = make_int4(d->indices[3 * (size_t)t], d->indices[3 * (size_t)t + 1], d->indices[3 * (size_t)t + 2], d->tri_material[t]);
+    int r = 0;
+    r |= dalloc(c, c->d_nodes, bvh.nodes.size());
+    r |= dalloc(c, c->d_tris, 3 * std::max<size_t>(1, bvh.tris.size()));
+    r |= dalloc(c, c->d_tri_info, ti.size());
+    r |= dalloc(c, c->d_vpos, vp.size());
+    r |= dalloc(c, c->d_vnrm, vn.size());
+    r |= dalloc(c, c->d_vbit, vb.size());
+    r |= dalloc(c, c->d_mats, 16 * (size_t)c->nm);
+    r |= dalloc(c, c->d_lights, std::max(1, c->nl));
+    if (r) return CPT_E_HIP;
+    HIPCHK(c, hipMemcpy(c->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4), hipMemcpyHostToDevice));
+    if (!bvh.tris.empty())
+        HIPCHK(c, hipMemcpy(c->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRec), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_tri_info, ti.data(), ti.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_vpos, vp.data(), vp.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_vnrm, vn.data(), vn.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_vbit, vb.data(), vb.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_mats, d->materials, 16 * sizeof(float) * (size_t)c->nm, hipMemcpyHostToDevice));
+    if (c->nl > 0)
+        HIPCHK(c, hipMemcpy(c->d_lights, d->light_triangles, sizeof(int) * (size_t)c->nl, hipMemcpyHostToDevice));
+    c->has_scene = true;
+    return CPT_OK;
+}
+
+int cpt_bvh_info(cpt_ctx* c, int64_t* info) {
+    if (!c || !info) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    info[0] = c->bvh_nodes;
+    info[1] = c->bvh_leaves;
+    info[2] = c->bvh_depth;
+    info[3] = c->nt;
+    return CPT_OK;
+}
+
+int cpt_tie_rank(cpt_ctx* c, int32_t* out) {
+    if (!c || !out) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (c->rank.empty() && c->nt > 0) return fail(c, CPT_E_INVALID, "scene uploaded with CPT_SCENE_NO_TIE_RANK");
+    std::memcpy(out, c->rank.data(), sizeof(int32_t) * c->rank.size());
+    return CPT_OK;
+}
+
+static int default_grid(cpt_ctx* c, int B) { return c->cus * (2048 / B) ; }
+
+static int trace_common(cpt_ctx* c, const float* rays, int32_t n, bool any, int32_t* tri_out, float* t_out,
+                        uint8_t* occ_out) {
+    if (!c || (n > 0 && !rays) || n < 0) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (n == 0) return CPT_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<float4> h(2 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 7 * (size_t)i;
+        h[2 * (size_t)i] = make_float4(r[0], r[1], r[2], r[6]);
+        h[2 * (size_t)i + 1] = make_float4(r[3], r[4], r[5], 0.0f);
+    }
+    const int B = 128;
+    int G = std::min(default_grid(c, B), (n + B - 1) / B);
+    int ret = ensure_gstack(c, (size_t)G * B);
+    if (ret) return ret;
+    float4* d_rays = nullptr;
+    void* d_out = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d_rays, h.size() * sizeof(float4)));
+    size_t out_bytes = any ? (size_t)n : (size_t)n * sizeof(float4);
+    if (hipMalloc(&d_out, out_bytes) != hipSuccess) {
+        (void)hipFree(d_rays);
+        return fail(c, CPT_E_HIP, "hipMalloc of %zu bytes failed", out_bytes);
+    }
+    int rc = CPT_OK;
+    if (hipMemcpyAsync(d_rays, h.data(), h.size() * sizeof(float4), hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+        rc = fail(c, CPT_E_HIP, "ray upload failed");
+    } else {
+        DevScene sc = dev_scene(c);
+        if (any)
+            hipLaunchKernelGGL((k_trace_any<B>), dim3(G), dim3(B), 0, c->stream, sc, d_rays, n, (uint8_t*)d_out, c->gstack);
+        else
+            hipLaunchKernelGGL((k_trace_closest<B>), dim3(G), dim3(B), 0, c->stream, sc, d_rays, n, (float4*)d_out, c->gstack);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(c, CPT_E_HIP, "trace launch failed: %s", hipGetErrorString(e));
+    }
+    if (rc == CPT_OK) {
+        if (any) {
+            if (hipMemcpyAsync(occ_out, d_out, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+                rc = fail(c, CPT_E_HIP, "result download failed");
+        } else {
+            std::vector<float4> hh(n);
+            if (hipMemcpyAsync(hh.data(), d_out, n * sizeof(float4), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = fail(c, CPT_E_HIP, "result download failed");
+            } else {
+                for (int i = 0; i < n; i++) {
+                    int32_t tid;
+                    std::memcpy(&tid, &hh[i].w, sizeof(tid));
+                    tri_out[i] = tid;
+                    t_out[i] = tri_out[i] >= 0 ? hh[i].x : INFINITY;
+                }
+            }
+        }
+        if (rc == CPT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, CPT_E_HIP, "trace failed");
+    }
+    (void)hipFree(d_rays);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+int cpt_trace_closest(cpt_ctx* c, const float* rays, int32_t n, int32_t* tri_out, float* t_out) {
+    if (!tri_out || !t_out) return CPT_E_INVALID;
+    return trace_common(c, rays, n, false, tri_out, t_out, nullptr);
+}
+
+int cpt_trace_any(cpt_ctx* c, const float* rays, int32_t n, uint8_t* occ) {
+    if (!occ) return CPT_E_INVALID;
+    return trace_common(c, rays, n, true, nullptr, nullptr, occ);
+}
+
+int cpt_trace_closest_device(cpt_ctx* c, const void* d_rays, int32_t n, void* d_hits, void* stream) {
+    if (!c || !d_rays || !d_hits || n < 0) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (n == 0) return CPT_OK;
+    const int B = 128;
+    int G = std::min(default_grid(c, B), (n + B - 1) / B);
+    int ret = ensure_gstack(c, (size_t)G * B);
+    if (ret) return ret;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    hipLaunchKernelGGL((k_trace_closest<B>), dim3(G), dim3(B), 0, s, dev_scene(c), (const float4*)d_rays, n,
+                       (float4*)d_hits, c->gstack);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, CPT_E_HIP, "trace launch failed: %s", hipGetErrorString(e));
+    return CPT_OK;
+}
+
+int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) {
+    if (!c || !p) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (p->width <= 0 || p->height <= 0 || p->spp_axis <= 0 || p->max_depth <= 0 || p->max_depth > 255)
+        return fail(c, CPT_E_INVALID, "bad render size/spp/depth");
+    int64_t P = (int64_t)p->width * p->height;
+    int64_t S2 = (int64_t)p->spp_axis * p->spp_axis;
+    if (P > (1 << 30) || S2 > (1 << 20) || P * S2 >= (1ll << 40)) return fail(c, CPT_E_LIMIT, "film or spp too large");
+    HIPCHK(c, hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+
+    int B = p->block_size ? p->block_size : 128;
+    if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
+    int G = p->grid_blocks ? p->grid_blocks : default_grid(c, B);
+    int64_t want = p->max_paths ? p->max_paths : (1 << 22);
+    want = std::max<int64_t>(want, 1024);
+    // pass plan
+    std::vector<PassDesc> passes;
+    if (P >= want) {
+        for (int64_t s = 0; s < S2; s++)
+            for (int64_t p0 = 0; p0 < P; p0 += want) {
+                PassDesc pd{};
+                pd.p0 = (uint32_t)p0;
+                pd.npix = (uint32_t)std::min<int64_t>(want, P - p0);
+                pd.s0 = (uint32_t)s;
+                pd.ns = 1;
+                pd.n_paths = pd.npix;
+                passes.push_back(pd);
+            }
+    } else {
+        int64_t ns = std::max<int64_t>(1, std::min<int64_t>(S2, want / P));
+        for (int64_t s0 = 0; s0 < S2; s0 += ns) {
+            PassDesc pd{};
+            pd.p0 = 0;
+            pd.npix = (uint32_t)P;
+            pd.s0 = (uint32_t)s0;
+            pd.ns = (uint32_t)std::min<int64_t>(ns, S2 - s0);
+            pd.n_paths = pd.npix * pd.ns;
+            passes.push_back(pd);
+        }
+    }
+    size_t slots = 0;
+    for (auto& pd : passes) slots = std::max<size_t>(slots, pd.n_paths);
+    int r = ensure_wavefront(c, slots);
+    if (r) return r;
+    r = ensure_gstack(c, (size_t)G * B);
+    if (r) return r;
+    if (c->film_px < (size_t)P || !c->film) {
+        drop_graph(c);
+        r = dalloc(c, c->film, 3 * (size_t)P);
+        if (r) return r;
+        c->film_px = (size_t)P;
+    }
+    if (c->h_pass_cap < passes.size()) {
+        if (c->h_pass) (void)hipHostFree(c->h_pass);
+        c->h_pass = nullptr;
+        HIPCHK(c, hipHostMalloc((void**)&c->h_pass, passes.size() * sizeof(PassDesc), hipHostMallocDefault));
+        c->h_pass_cap = passes.size();
+    }
+    // the previous render's pass copies must have completed before the pinned buffer is rewritten
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(c->h_pass, passes.data(), passes.size() * sizeof(PassDesc));
+    c->film_w = p->width;
+    c->film_h = p->height;
+    c->film_S = p->spp_axis;
+
+    WfArgs a;
+    a.sc = dev_scene(c);
+    a.mul_p = c->mul_p;
+    a.col = c->col;
+    a.rng = c->rng;
+    a.meta = c->meta;
+    a.pix = c->pix;
+    a.qo[0] = c->qo[0];
+    a.qo[1] = c->qo[1];
+    a.qd[0] = c->qd[0];
+    a.qd[1] = c->qd[1];
+    a.hits = c->hits;
+    a.so = c->so;
+    a.sd = c->sd;
+    a.sc_rgb = c->sc_rgb;
+    a.counters = c->counters;
+    a.stats = c->stats;
+    a.gstack = c->gstack;
+    a.pass = c->d_pass;
+    a.film = c->film;
+    a.W = p->width;
+    a.H = p->height;
+    a.S = p->spp_axis;
+    a.max_depth = p->max_depth;
+    a.seed = p->seed;
+    a.view = (uint32_t)p->view;
+    LaunchCfg L{B, G, (p->flags & CPT_RENDER_COUNT) != 0};
+
+    if (!(p->flags & CPT_RENDER_ACCUMULATE)) HIPCHK(c, hipMemsetAsync(c->film, 0, 3 * sizeof(double) * (size_t)P, c->stream));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
+
+    bool timing = (p->flags & CPT_RENDER_KERNEL_TIMING) != 0;
+    bool use_graph = !(p->flags & (CPT_RENDER_NO_GRAPH | CPT_RENDER_KERNEL_TIMING));
+    Timers tm;
+    if (use_graph) {
+        char key[256];
+        snprintf(key, sizeof(key), "%d,%d,%d,%d,%llu,%u,%d,%d,%d,%zu,%p", p->width, p->height, p->spp_axis, p->max_depth,
+                 (unsigned long long)p->seed, a.view, B, G, (int)L.count, c->cap, (void*)c->d_nodes);
+        if (!c->gexec || c->graph_key != key) {
+            drop_graph(c);
+            hipGraph_t g;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int ir = issue_pass(c, L, a, c->stream, nullptr);
+            hipError_t ec = hipStreamEndCapture(c->stream, &g);
+            if (ir) return ir;
+            if (ec != hipSuccess) return fail(c, CPT_E_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+            hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return fail(c, CPT_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+            c->graph_key = key;
+        }
+    }
+    for (size_t i = 0; i < passes.size(); i++) {
+        HIPCHK(c, hipMemcpyAsync(c->d_pass, &c->h_pass[i], sizeof(PassDesc), hipMemcpyHostToDevice, c->stream));
+        if (use_graph) {
+            HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            r = issue_pass(c, L, a, c->stream, timing ? &tm : nullptr);
+            if (r) return r;
+        }
+    }
+    unsigned long long st[ST_N];
+    HIPCHK(c, hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats->paths = (int64_t)st[ST_PATHS];
+        stats->extension_rays = (int64_t)st[ST_EXT];
+        stats->shadow_rays = (int64_t)st[ST_SHADOW];
+        stats->node_visits = (int64_t)st[ST_NODES];
+        stats->triangle_tests = (int64_t)st[ST_TRIS];
+        stats->shadow_node_visits = (int64_t)st[ST_SNODES];
+        stats->shadow_triangle_tests = (int64_t)st[ST_STRIS];
+        stats->passes = (int32_t)passes.size();
+        stats->paths_per_pass = (int32_t)slots;
+        stats->block_size = B;
+        stats->grid_blocks = G;
+        for (size_t i = 1; i < tm.ev.size(); i++) {
+            float ms = 0;
+            if (tm.kind[i] >= 0 && hipEventElapsedTime(&ms, tm.ev[i - 1], tm.ev[i]) == hipSuccess) {
+                switch (tm.kind[i]) {
+                    case 0: stats->ms_extend += ms; stats->n_extend++; break;
+                    case 1: stats->ms_shade += ms; stats->n_shade++; break;
+                    case 2: stats->ms_shadow += ms; stats->n_shadow++; break;
+                    default: stats->ms_other += ms; break;
+                }
+            }
+        }
+    }
+    for (auto e : tm.ev) (void)hipEventDestroy(e);
+    return CPT_OK;
+}
+
+int cpt_read_film(cpt_ctx* c, float* rgb, double* sums) {
+    if (!c) return CPT_E_INVALID;
+    if (!c->film) return fail(c, CPT_E_INVALID, "nothing rendered yet");
+    size_t P = (size_t)c->film_w * c->film_h;
+    std::vector<double> h(3 * P);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->film, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (sums) std::memcpy(sums, h.data(), h.size() * sizeof(double));
+    if (rgb) {
+        double inv = 1.0 / ((double)c->film_S * c->film_S);
+        for (size_t i = 0; i < h.size(); i++) rgb[i] = (float)(h[i] * inv);
+    }
+    return CPT_OK;
+}
+
+int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const int32_t* wh_spp, int32_t max_depth,
+                     uint64_t seed, int32_t flags, float* films, double* view_ms, cpt_render_stats* stats) {
+    (void)cameras; (void)wh_spp; (void)max_depth; (void)seed; (void)flags; (void)films; (void)view_ms; (void)stats;
+    if (!c || n_views <= 0) return CPT_E_INVALID;
+    return fail(c, CPT_E_INVALID, "cpt_render_views: not available in this build");
+}
+
+}  // extern "C"

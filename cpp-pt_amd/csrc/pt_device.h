@@ -1,0 +1,245 @@
+// Device-side scene records, BVH4 traversal and the PT-NEE-MIS shading math.
+//
+// Float semantics: the reference mixes float and double (EPS = 1e-4 is a double
+// literal, src/math/math.h:11; Vec3::operator/(T) multiplies by 1.0/a computed
+// in double, math.h:103-105; fabs(float) returns double, ...).  Every helper
+// below reproduces those roundings, and the library is compiled with
+// -ffp-contract=off so that no multiply-add is fused.  Where the reference's
+// double result is provably equal to a float operation (1.0/x and 1.0-x rounded
+// to float, sqrt) the float form is used.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bvh_build.h"
+
+namespace cpt {
+
+// smallest float t with (double)t >= 1e-4: `t < EPS` (double compare,
+// triangle.h:55) <=> t < EPS_T
+#define CPT_EPS_T 1.00000005e-4f
+#define CPT_EPSF 9.99999975e-05f  // (float)1e-4: Vec3 * EPS
+#define CPT_PI_F 3.14159274f      // (float)M_PI
+#define CPT_M_PI 3.14159265358979323846
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float a, float b, float c) { return {a, b, c}; }
+__device__ __forceinline__ V3 v3s(float a) { return {a, a, a}; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 mul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 mulf(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+// Vec3 / float: v * float(1.0/double(a)) == v * (1.0f/a) (correctly rounded)
+__device__ __forceinline__ V3 divs(V3 a, float s) { return mulf(a, 1.0f / s); }
+__device__ __forceinline__ V3 negv(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len3(V3 a) { return (float)sqrt((double)dot(a, a)); }
+__device__ __forceinline__ V3 normalized(V3 a) { return divs(a, len3(a)); }
+__device__ __forceinline__ V3 ld3(const float4& f) { return {f.x, f.y, f.z}; }
+__device__ __forceinline__ float dabs_f(float x) { return (float)fabs((double)x); }
+
+// Mat3 with columns c0,c1,c2 times v (math.h:238-248,319-325)
+__device__ __forceinline__ V3 mat_cols_mul(V3 c0, V3 c1, V3 c2, V3 a) {
+    return {c0.x * a.x + c1.x * a.y + c2.x * a.z, c0.y * a.x + c1.y * a.y + c2.y * a.z,
+            c0.z * a.x + c1.z * a.y + c2.z * a.z};
+}
+// transpose(Mat3(c0,c1,c2)) * a: rows are the columns
+__device__ __forceinline__ V3 mat_cols_tmul(V3 c0, V3 c1, V3 c2, V3 a) {
+    return {c0.x * a.x + c0.y * a.y + c0.z * a.z, c1.x * a.x + c1.y * a.y + c1.z * a.z,
+            c2.x * a.x + c2.y * a.y + c2.z * a.z};
+}
+
+// ---------------------------------------------------------------- RNG
+// PCG32 (O'Neill 2014, XSH-RR), one stream per path: initstate =
+// splitmix64(seed + path_id), fixed increment.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+#define CPT_PCG_MULT 6364136223846793005ull
+#define CPT_PCG_INC ((0x5851F42D4C957F2Dull << 1) | 1ull)
+__device__ __forceinline__ uint32_t pcg_next(uint64_t& s) {
+    uint64_t old = s;
+    s = old * CPT_PCG_MULT + CPT_PCG_INC;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+}
+__device__ __forceinline__ uint64_t pcg_init(uint64_t seed, uint64_t path_id) {
+    uint64_t s = 0;
+    pcg_next(s);
+    s += splitmix64(seed + path_id);
+    pcg_next(s);
+    return s;
+}
+__device__ __forceinline__ float pcg_float(uint64_t& s) { return (float)(pcg_next(s) >> 8) * (1.0f / 16777216.0f); }
+
+// ---------------------------------------------------------------- scene
+struct DevScene {
+    const Node4* nodes;
+    const float4* tris;      // 3 per triangle, leaf order (TriRec)
+    const int4* tri_info;    // v0, v1, v2, material (by triangle id)
+    const float4* vpos;      // xyz
+    const float4* vnrm;
+    const float4* vbit;
+    const float* mats;       // 16 per material
+    const int* lights;       // emitter triangle ids
+    int n_lights;
+    int n_tris;
+    float env[3];
+    float cam[16];
+};
+
+struct Hit {
+    float t, beta, gamma;
+    int tri;
+};
+
+// ---------------------------------------------------------------- traversal
+// Ordered BVH4 traversal, one ray per lane.  Stack: LDS_SS entries in LDS
+// (laid out entry-major so that lanes hit distinct banks) then a per-thread
+// overflow region in global memory.
+#define CPT_LDS_SS 16
+#define CPT_GSTACK 112
+// conservative culling: a box may be entered up to this relative distance
+// beyond the current best t, so that triangles tying with the best t in other
+// leaves are still tested (ties are decided by rank, not by visit order)
+#define CPT_TIE_SLACK 1.000244140625f  // 1 + 2^-12
+#define CPT_BOX_SLACK 1.00000048f      // 1 + 2*gamma_3
+
+template <int BLOCK>
+struct TravStack {
+    int* lds;
+    int* gmem;
+    size_t gstride;
+    int tid;
+    __device__ __forceinline__ void put(int sp, int v) {
+        if (sp < CPT_LDS_SS) lds[sp * BLOCK + tid] = v;
+        else gmem[(size_t)(sp - CPT_LDS_SS) * gstride] = v;
+    }
+    __device__ __forceinline__ int get(int sp) {
+        return sp < CPT_LDS_SS ? lds[sp * BLOCK + tid] : gmem[(size_t)(sp - CPT_LDS_SS) * gstride];
+    }
+};
+
+// Reference triangle test (triangle.h:36-77).  Returns true with t, beta, gamma
+// when t in [EPS, tmax] and all barycentrics in [0,1].
+__device__ __forceinline__ bool tri_test(const float4& A, const float4& E1, const float4& E2, V3 o, V3 d, float tmax,
+                                         float& t, float& beta, float& gamma) {
+    V3 v1 = ld3(E1), v2 = ld3(E2);
+    V3 v3_ = {-d.x, -d.y, -d.z};
+    V3 vs = sub(o, ld3(A));
+    float den = v1.x * v2.y * v3_.z + v2.x * v3_.y * v1.z + v3_.x * v1.y * v2.z - v3_.x * v2.y * v1.z -
+                v3_.y * v2.z * v1.x - v3_.z * v2.x * v1.y;
+    float inv = 1.0f / den;
+    t = (v1.x * v2.y * vs.z + v2.x * vs.y * v1.z + vs.x * v1.y * v2.z - vs.x * v2.y * v1.z - vs.y * v2.z * v1.x -
+         vs.z * v2.x * v1.y) * inv;
+    if (t < CPT_EPS_T || t > tmax) return false;
+    beta = (vs.x * v2.y * v3_.z + v2.x * v3_.y * vs.z + v3_.x * vs.y * v2.z - v3_.x * v2.y * vs.z -
+            v3_.y * v2.z * vs.x - v3_.z * v2.x * vs.y) * inv;
+    gamma = (v1.x * vs.y * v3_.z + vs.x * v3_.y * v1.z + v3_.x * v1.y * vs.z - v3_.x * vs.y * v1.z -
+             v3_.y * vs.z * v1.x - v3_.z * vs.x * v1.y) * inv;
+    float alpha = 1.0f - (gamma + beta);
+    return !(alpha < 0 || alpha > 1 || beta < 0 || beta > 1 || gamma < 0 || gamma > 1);
+}
+
+template <int BLOCK, bool ANY, bool COUNT>
+__device__ bool traverse(const DevScene& sc, V3 o, V3 d, float tmax, Hit& h, TravStack<BLOCK>& st,
+                         unsigned& nvis, unsigned& ntest) {
+    // 1/d with zero components replaced by a tiny signed value so that slab
+    // products never form 0*inf
+    float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+    float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+    float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+    float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    float best = tmax;
+    int best_rank = INT_MIN;
+    h.tri = -1;
+    int sp = 0;
+    int next = 0;  // root node
+    while (true) {
+        if (next < 0) {  // leaf
+            int first = next & ((1 << 27) - 1);
+            int cnt = ((next >> 27) & 15) + 1;
+            for (int k = 0; k < cnt; k++) {
+                const float4* tp = sc.tris + 3 * (size_t)(first + k);
+                float4 A = tp[0], E1 = tp[1], E2 = tp[2];
+                float t, b, g;
+                if (COUNT) ntest++;
+                if (tri_test(A, E1, E2, o, d, best, t, b, g)) {
+                    if (ANY) return true;
+                    int rk = __float_as_int(E1.w);
+                    if (t < best || rk > best_rank) {
+                        best = t;
+                        best_rank = rk;
+                        h.t = t;
+                        h.beta = b;
+                        h.gamma = g;
+                        h.tri = __float_as_int(A.w);
+                    }
+                }
+            }
+            if (sp == 0) break;
+            next = st.get(--sp);
+            continue;
+        }
+        if (next == EMPTY_CHILD) {
+            if (sp == 0) break;
+            next = st.get(--sp);
+            continue;
+        }
+        if (COUNT) nvis++;
+        const float4* np = reinterpret_cast<const float4*>(sc.nodes + next);
+        float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        int4 ch = reinterpret_cast<const int4*>(np)[6];
+        float lim = ANY ? best * CPT_BOX_SLACK : best * CPT_TIE_SLACK;
+        float key[4];
+        int cid[4];
+#define CPT_BOX(c, LX, HX, LY, HY, LZ, HZ, CH)                                   \
+    {                                                                             \
+        float t0 = (LX - o.x) * ix, t1 = (HX - o.x) * ix;                         \
+        float t2 = (LY - o.y) * iy, t3 = (HY - o.y) * iy;                         \
+        float t4 = (LZ - o.z) * iz, t5 = (HZ - o.z) * iz;                         \
+        float tn = fmaxf(fmaxf(fminf(t0, t1), fminf(t2, t3)), fmaxf(fminf(t4, t5), 0.0f)); \
+        float tf = fminf(fminf(fmaxf(t0, t1), fmaxf(t2, t3)), fmaxf(t4, t5)) * CPT_BOX_SLACK; \
+        bool hit = tn <= tf && tn <= lim;                                         \
+        key[c] = hit ? tn : INFINITY;                                             \
+        cid[c] = hit ? CH : EMPTY_CHILD;                                          \
+    }
+        CPT_BOX(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
+        CPT_BOX(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y)
+        CPT_BOX(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z)
+        CPT_BOX(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w)
+#undef CPT_BOX
+#define CPT_CSWAP(a, b)                                  \
+    if (key[b] < key[a]) {                               \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk; \
+        int tc = cid[a]; cid[a] = cid[b]; cid[b] = tc;   \
+    }
+        CPT_CSWAP(0, 1)
+        CPT_CSWAP(2, 3)
+        CPT_CSWAP(0, 2)
+        CPT_CSWAP(1, 3)
+        CPT_CSWAP(1, 2)
+#undef CPT_CSWAP
+        if (cid[0] == EMPTY_CHILD) {
+            if (sp == 0) break;
+            next = st.get(--sp);
+            continue;
+        }
+        if (cid[3] != EMPTY_CHILD) st.put(sp++, cid[3]);
+        if (cid[2] != EMPTY_CHILD) st.put(sp++, cid[2]);
+        if (cid[1] != EMPTY_CHILD) st.put(sp++, cid[1]);
+        next = cid[0];
+    }
+    return ANY ? false : h.tri >= 0;
+}
+
+}  // namespace cpt

@@ -1,0 +1,140 @@
+// Host-side mirror of cpp-pt's plugin interface for the PT-NEE-MIS path.
+//
+// Same names, argument meaning and error behaviour as the reference:
+//   RenderSettings      src/renderer/renderer.h:11-28 (+ max_depth, seed, device)
+//   Renderer::render    src/renderer/renderer.h:30-33
+//   Loader::load_render_settings / load_scene   src/loader/loader.h:256-374
+//   Scene / SceneData   src/scene.h:16-26
+//   CameraPerspective   src/camera/camera_perspective.h:19-27
+//   RgbImage::put_pixel / save   src/image/rgb_image.h:208-210, 300-318
+// Errors are std::runtime_error, as in the reference (loader.h:94,140,203,...).
+// The scene is kept flattened (SoA arrays in load order) because that is what
+// crosses the C ABI; mesh/material/light order follows loader.h:353-367.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cpt.h"
+
+namespace cpppt {
+
+struct Vec3f {
+    float x = 0, y = 0, z = 0;
+};
+
+struct RenderSettings {
+    enum RendererType {
+        PATHTRACER,
+        PATHTRACER_MIS,
+        LIGHTTRACER,
+        DEBUG,
+        PATHTRACER_MLT,
+        LIGHTTRACER_MLT,
+        PHOTONMAPPING,
+        PURE_PT,
+        PPG,
+        VALIDATION,
+        PATHTRACER_MIS_GPU
+    };
+    RendererType renderer = RendererType::DEBUG;
+    int spp = 1;  // per axis: spp*spp samples per pixel
+    int resX = 256;
+    int resY = 256;
+    std::string scene_name = "";
+    std::string output_name = "";
+    // additions of this implementation
+    int max_depth = 32;     // the reference's hard cap (pathtracerMIS.h:36)
+    uint64_t seed = 1;
+    int device = 0;
+};
+
+// Film with the reference's layout: pixels[i + j*resX], j = 0 at the top.
+class RgbImage {
+public:
+    explicit RgbImage(int w = 0, int h = 0) : resX(w), resY(h), pixels(3 * (size_t)w * h, 0.0f) {}
+    void put_pixel(int i, int j, const Vec3f& p) {
+        size_t k = 3 * ((size_t)i + (size_t)j * resX);
+        pixels.at(k) = p.x;
+        pixels.at(k + 1) = p.y;
+        pixels.at(k + 2) = p.z;
+    }
+    // <filename>.png (sRGB 8-bit) and <filename>.hdr (RGBE) like rgb_image.h:300-318,
+    // plus <filename>.pfm (lossless float32)
+    void save(const std::string& filename) const;
+    int resX, resY;
+    std::vector<float> pixels;
+};
+
+struct Camera {
+    Vec3f origin, x, y, z;  // frame columns
+    float tan_fovy = 0.35411f;
+    float aspect = 1.0f;
+    RgbImage image;
+    // CameraPerspective(resolution, tan_fovy, origin, forward, up)
+    void setup(int resX, int resY, float tan_fovy, Vec3f origin, Vec3f forward, Vec3f up);
+    RgbImage& get_image() { return image; }
+};
+
+struct Material {
+    int type = CPT_MAT_DIFFUSE;
+    float albedo[3] = {0, 0, 0};
+    float roughness = 0;
+    float emission[3] = {0, 0, 0};
+    float ior = 1.5f;
+    float normal_map[3] = {0.5f, 0.5f, 1.0f};
+    bool has_normal_map = true;
+    bool is_emissive() const { return type == CPT_MAT_EMISSIVE; }
+};
+
+struct Mesh {
+    std::vector<float> p, n, b;          // per vertex xyz (b = bitangent)
+    std::vector<std::array<int, 3>> f;   // faces
+};
+
+struct SceneData {
+    std::vector<std::shared_ptr<Mesh>> meshes;
+    std::vector<std::shared_ptr<Material>> materials;  // one entry per mesh, load order
+};
+
+struct Scene {
+    std::shared_ptr<Camera> camera;
+    float env[3] = {0, 0, 0};
+    // flattened
+    std::vector<float> positions, normals, bitangents;
+    std::vector<int32_t> indices, tri_material, light_triangles;
+    std::vector<float> materials;  // 16 floats per material
+    int n_vertices = 0, n_triangles = 0, n_materials = 0;
+    void flatten(const SceneData& sd);
+    cpt_scene_desc desc() const;
+};
+
+namespace Loader {
+void load_render_settings(RenderSettings* rs, const std::string& filename);
+void load_scene(Scene* s, SceneData* sd, RenderSettings* rs, const std::string& filename);
+}  // namespace Loader
+
+class Renderer {
+public:
+    virtual void render(Scene& sc, std::string filename) = 0;
+    virtual ~Renderer() {}
+};
+
+// GPU drop-in for PathtracerMIS (pathtracerMIS.h:17-168).
+class PathtracerMISGPU : public Renderer {
+public:
+    explicit PathtracerMISGPU(const RenderSettings& rs);
+    ~PathtracerMISGPU() override;
+    void render(Scene& sc, std::string filename) override;
+    const cpt_render_stats& last_stats() const { return stats_; }
+
+private:
+    RenderSettings rs_;
+    cpt_ctx* ctx_ = nullptr;
+    cpt_render_stats stats_{};
+};
+
+}  // namespace cpppt

@@ -1,0 +1,175 @@
+/*
+ * cpt.h -- C ABI of the MI355X path tracer (libcpt.so).
+ *
+ * This is the drop-in boundary for cpp-pt's PathtracerMIS hot path.  The
+ * reference has no C ABI of its own: its entry point is the C++ plugin
+ * interface `Renderer::render(Scene&, std::string)` (src/renderer/renderer.h:30-33),
+ * constructed from `RenderSettings` (renderer.h:11-28) by the factory in
+ * src/main.cpp:102-125.  The host C++ class PathtracerMISGPU (cpp-pt_amd/host)
+ * implements that interface on top of the functions below; every function
+ * here replaces one piece of what `PathtracerMIS::render` and the objects it
+ * calls do on the CPU.  Each declaration names the reference code it stands in
+ * for.
+ *
+ * Conventions: plain pointers and sizes only; the caller owns every host
+ * buffer and may free it when a call returns; the context owns all device
+ * memory.  Every function returns 0 on success or a negative CPT_E* code and
+ * never throws or aborts; cpt_last_error() returns the message of the last
+ * failure on that context.  A context is used from one thread at a time.
+ */
+#ifndef CPT_H
+#define CPT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPT_OK 0
+#define CPT_E_INVALID (-1)   /* bad argument or scene (e.g. out-of-range index) */
+#define CPT_E_HIP (-2)       /* HIP runtime error */
+#define CPT_E_NOSCENE (-3)   /* call needs cpt_upload_scene first */
+#define CPT_E_NODEVICE (-4)  /* no usable gfx950 device */
+#define CPT_E_LIMIT (-5)     /* a size limit of the implementation was exceeded */
+
+/* material types (src/bxdf/standard_material.h:46-61, emissive_material.h:28-30) */
+#define CPT_MAT_DIFFUSE 0    /* StandardMaterial with metal < 0.1 -> DiffuseBxDF */
+#define CPT_MAT_GLOSSY 1     /* StandardMaterial with metal >= 0.1 -> GlossyBxDF(roughness) */
+#define CPT_MAT_EMISSIVE 2   /* EmissiveMaterial -> EmissiveBxDF(colour*intensity) */
+#define CPT_MAT_DIELECTRIC 3 /* RefractionBxDF(ior) (refraction_bxdf.h), made reachable */
+
+#define CPT_MATERIAL_FLOATS 16
+#define CPT_CAMERA_FLOATS 16
+
+typedef struct cpt_ctx cpt_ctx;
+
+/* Flattened scene.  Replaces the Scene/SceneData object graph
+ * (src/scene.h:16-26) that Loader::load_scene builds (src/loader/loader.h:314-374).
+ *  - positions/normals/bitangents: n_vertices x 3 floats (Mesh vertex data,
+ *    src/shape/mesh.h:101-106; bitangent = cross(normal, tangent), loader.h:237)
+ *  - indices: n_triangles x 3 int32 into the vertex arrays; triangle id = its
+ *    position in load order (mesh order, then face order: loader.h:353-360)
+ *  - tri_material: n_triangles int32 into the material table
+ *  - materials: n_materials x 16 floats:
+ *      [0] type  [1..3] albedo  [4] roughness  [5..7] emission (colour*intensity)
+ *      [8] ior   [9..11] constant normal-map texel  [12] has normal map (1/0)
+ *  - light_triangles: emissive triangle ids in LightGroup order (loader.h:362-364)
+ *  - camera: o[3], frame columns x[3] y[3] z[3], tan_fovy, aspect, resX, resY
+ *    (CameraPerspective, src/camera/camera_perspective.h:19-27)
+ *  - env: constant radiance returned for rays that leave the scene
+ *    (PathtracerMIS::render_sky returns 0, pathtracerMIS.h:20-22)
+ */
+typedef struct {
+    int32_t n_vertices;
+    const float* positions;
+    const float* normals;
+    const float* bitangents;
+    int32_t n_triangles;
+    const int32_t* indices;
+    const int32_t* tri_material;
+    int32_t n_materials;
+    const float* materials;
+    int32_t n_lights;
+    const int32_t* light_triangles;
+    float camera[CPT_CAMERA_FLOATS];
+    float env[3];
+    int32_t flags; /* CPT_SCENE_* */
+} cpt_scene_desc;
+
+/* skip computing the reference BVH's test order (used only to break exact-t ties
+ * the way the reference does; without it ties go to the lowest triangle id) */
+#define CPT_SCENE_NO_TIE_RANK 1
+
+/* One render.  Replaces PathtracerMIS::render (pathtracerMIS.h:135-167):
+ * width x height film, spp_axis^2 stratified samples per pixel (the settings
+ * file's `spp` is per axis, pathtracerMIS.h:148-149), at most max_depth
+ * extension rays per path (the reference's hard cap is 32, pathtracerMIS.h:36).
+ * Random numbers: one PCG32 stream per path, seeded from (seed, path id). */
+typedef struct {
+    int32_t width, height;
+    int32_t spp_axis;
+    int32_t max_depth;
+    uint64_t seed;
+    int32_t flags;          /* CPT_RENDER_* */
+    int32_t max_paths;      /* paths in flight per pass; 0 = automatic */
+    int32_t block_size;     /* threads per block of the wavefront kernels; 0 = automatic */
+    int32_t grid_blocks;    /* blocks of the grid-stride kernels; 0 = automatic */
+    int32_t view;           /* view id mixed into the seed (multi-view batches) */
+    int32_t reserved[3];
+} cpt_render_params;
+
+#define CPT_RENDER_NO_GRAPH 1       /* launch kernels one by one instead of a captured hipGraph */
+#define CPT_RENDER_KERNEL_TIMING 2  /* hipEvents around every kernel (implies NO_GRAPH) */
+#define CPT_RENDER_COUNT 4          /* count BVH node visits and triangle tests */
+#define CPT_RENDER_ACCUMULATE 8     /* add to the film instead of clearing it first */
+
+typedef struct {
+    double seconds;           /* wall time of the render call (device synchronised) */
+    int64_t paths;
+    int64_t extension_rays;
+    int64_t shadow_rays;
+    int64_t node_visits;      /* CPT_RENDER_COUNT only */
+    int64_t triangle_tests;   /* CPT_RENDER_COUNT only */
+    int64_t shadow_node_visits;
+    int64_t shadow_triangle_tests;
+    double ms_extend, ms_shade, ms_shadow, ms_other;  /* CPT_RENDER_KERNEL_TIMING only */
+    int64_t n_extend, n_shade, n_shadow;              /* launches timed */
+    int32_t passes;
+    int32_t paths_per_pass;
+    int32_t block_size;
+    int32_t grid_blocks;
+} cpt_render_stats;
+
+/* Create a context on HIP device `device`.  Fails with CPT_E_NODEVICE when the
+ * device is not gfx950. */
+int cpt_create(cpt_ctx** out, int device);
+void cpt_destroy(cpt_ctx* ctx);
+const char* cpt_last_error(cpt_ctx* ctx);
+/* ABI version (major*1000 + minor) */
+int cpt_version(void);
+
+/* Upload a scene, build the 4-wide BVH and (unless CPT_SCENE_NO_TIE_RANK) the
+ * reference test-order table.  Replaces Loader::load_scene's BVH construction
+ * (loader.h:353-372 -> BVH::build, src/primitive/bvh.h:151-217) and the
+ * LightGroup population (loader.h:362-364). */
+int cpt_upload_scene(cpt_ctx* ctx, const cpt_scene_desc* desc);
+
+/* BVH statistics of the uploaded scene: [nodes, leaves, max depth, triangles] */
+int cpt_bvh_info(cpt_ctx* ctx, int64_t* info4);
+/* The reference BVH's test order of every triangle (n_triangles int32). */
+int cpt_tie_rank(cpt_ctx* ctx, int32_t* rank_out);
+
+/* Closest hit for n rays (n x 7 floats: ox oy oz dx dy dz tmax, host memory).
+ * Replaces BVH::intersect (bvh.h:219-221 -> BVHNode::intersect 22-37 ->
+ * Triangle::intersect, src/shape/triangle.h:18-116).  tri_out = triangle id or
+ * -1, t_out = hit distance (bit-identical to the reference's t). */
+int cpt_trace_closest(cpt_ctx* ctx, const float* rays, int32_t n, int32_t* tri_out, float* t_out);
+/* Any hit (occlusion) for n rays.  Replaces BVH::intersect_any (bvh.h:222-224,
+ * triangle.h:117-179). occluded_out: 1 if any triangle has t in [EPS, tmax]. */
+int cpt_trace_any(cpt_ctx* ctx, const float* rays, int32_t n, uint8_t* occluded_out);
+/* Same on device-resident buffers (rays: n x 8 floats ox oy oz tmax dx dy dz 0;
+ * hits: n x 4 words t, beta, gamma, triangle id) on stream `stream` (NULL = the
+ * context's stream); no host synchronisation. */
+int cpt_trace_closest_device(cpt_ctx* ctx, const void* d_rays, int32_t n, void* d_hits, void* stream);
+
+/* Render into the context's device film (FP64 accumulators).  Replaces the
+ * pixel loop of PathtracerMIS::render (pathtracerMIS.h:139-160). */
+int cpt_render(cpt_ctx* ctx, const cpt_render_params* params, cpt_render_stats* stats);
+/* Copy the film out: rgb (w*h*3 floats) = sum / spp_axis^2 of the last
+ * render, row j = 0 at the top like RgbImage::put_pixel (rgb_image.h:208-210);
+ * sums (w*h*3 doubles, may be NULL) = the raw FP64 accumulators. */
+int cpt_read_film(cpt_ctx* ctx, float* rgb, double* sums);
+
+/* Multi-view batch in one launch sequence: n_views cameras (16 floats each, as
+ * cpt_scene_desc.camera) with their own width/height/spp_axis.  Paths of all
+ * views share the wavefront queues.  films: concatenated w*h*3 float images in
+ * view order.  view_ms (may be NULL): per-view completion time in ms from the
+ * start of the batch. */
+int cpt_render_views(cpt_ctx* ctx, int32_t n_views, const float* cameras, const int32_t* wh_spp,
+                     int32_t max_depth, uint64_t seed, int32_t flags, float* films, double* view_ms,
+                     cpt_render_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product.  See oracle/oracle.h for
+what each function restates from the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    vp, i32, f32p = C.c_void_p, C.c_int32, C.c_void_p
+    L.orc_scene_create.restype = vp
+    L.orc_scene_create.argtypes = [i32, vp, vp, vp, i32, vp, vp, i32, vp, i32, vp, vp, vp]
+    L.orc_scene_destroy.argtypes = [vp]
+    L.orc_scene_destroy.restype = None
+    L.orc_bvh_nodes.argtypes = [vp]
+    L.orc_bvh_rank.argtypes = [vp, vp]
+    L.orc_bvh_rank.restype = None
+    L.orc_trace_closest.argtypes = [vp, vp, i32, i32, vp, vp, i32]
+    L.orc_trace_closest.restype = None
+    L.orc_trace_any.argtypes = [vp, vp, i32, i32, vp, i32]
+    L.orc_trace_any.restype = None
+    L.orc_camera_ray.argtypes = [vp, C.c_float, C.c_float, vp, vp]
+    L.orc_camera_ray.restype = None
+    L.orc_render.argtypes = [vp, i32, i32, i32, i32, i32, C.c_uint64, i32, i32, i32, i32, vp, vp, i32, vp]
+    L.orc_path.argtypes = [vp, i32, i32, i32, i32, C.c_uint64, i32, i32, vp]
+    L.orc_path.restype = None
+    L.orc_diffuse_sample.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp]
+    L.orc_diffuse_sample.restype = None
+    L.orc_glossy_sample.argtypes = [vp, vp, vp, C.c_float, vp, C.c_float, C.c_float, vp, vp, vp]
+    L.orc_glossy_sample.restype = None
+    L.orc_glossy_eval.argtypes = [vp, vp, vp, C.c_float, vp, vp]
+    L.orc_glossy_eval.restype = C.c_float
+    L.orc_glossy_pdf.argtypes = [vp, vp, vp, C.c_float, vp, vp]
+    L.orc_glossy_pdf.restype = C.c_float
+    L.orc_ggx_d.argtypes = [C.c_float, vp]
+    L.orc_ggx_d.restype = C.c_float
+    L.orc_ggx_pdf.argtypes = [C.c_float, vp, vp]
+    L.orc_ggx_pdf.restype = C.c_float
+    L.orc_ggx_sample.argtypes = [C.c_float, vp, C.c_float, C.c_float, vp]
+    L.orc_ggx_sample.restype = None
+    L.orc_triangle_sample.argtypes = [vp, i32, C.c_float, C.c_float, vp, vp, vp]
+    L.orc_triangle_sample.restype = None
+    L.orc_pcg_floats.argtypes = [C.c_uint64, C.c_uint64, i32, vp]
+    L.orc_pcg_floats.restype = None
+    L.orc_mt_floats.argtypes = [C.c_uint32, i32, vp]
+    L.orc_mt_floats.restype = None
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _v3(x):
+    return np.ascontiguousarray(np.asarray(x, np.float32).reshape(3))
+
+
+class OracleScene:
+    """A scene (cpt.scenes.Scene) loaded into the oracle."""
+
+    def __init__(self, scene, camera=None):
+        a = scene.c_arrays() if hasattr(scene, "c_arrays") else scene
+        self.arrays = a
+        cam = np.ascontiguousarray(a["camera"] if camera is None else camera, np.float32)
+        self._cam = cam
+        L = lib()
+        self.h = L.orc_scene_create(
+            a["positions"].shape[0], _p(a["positions"]), _p(a["normals"]), _p(a["bitangents"]),
+            a["indices"].shape[0], _p(a["indices"]), _p(a["tri_material"]),
+            a["materials"].shape[0], _p(a["materials"]), a["light_tris"].shape[0], _p(a["light_tris"]),
+            _p(cam), _p(np.ascontiguousarray(a["env"], np.float32)))
+        self.n_triangles = a["indices"].shape[0]
+
+    def close(self):
+        if self.h:
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bvh_nodes(self):
+        return lib().orc_bvh_nodes(self.h)
+
+    def rank(self):
+        r = np.zeros(self.n_triangles, np.int32)
+        lib().orc_bvh_rank(self.h, _p(r))
+        return r
+
+    def trace_closest(self, rays, brute=False, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        lib().orc_trace_closest(self.h, _p(rays), n, 1 if brute else 0, _p(tri), _p(t), threads)
+        return tri, t
+
+    def trace_any(self, rays, brute=False, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        occ = np.zeros(n, np.uint8)
+        lib().orc_trace_any(self.h, _p(rays), n, 1 if brute else 0, _p(occ), threads)
+        return occ
+
+    def camera_ray(self, cx, cy):
+        o = np.zeros(3, np.float32)
+        d = np.zeros(3, np.float32)
+        lib().orc_camera_ray(self.h, cx, cy, _p(o), _p(d))
+        return o, d
+
+    def render(self, W, H, spp_axis, max_depth=32, sampler="pcg", seed=1, cols=None, rows=None,
+               threads=0):
+        """Returns (sums (H,W,3) float64, film (H,W,3) float32, counts [paths, ext, shadow])."""
+        sums = np.zeros((H, W, 3), np.float64)
+        film = np.zeros((H, W, 3), np.float32)
+        counts = np.zeros(3, np.int64)
+        c0, c1 = cols if cols else (0, W)
+        r0, r1 = rows if rows else (0, H)
+        lib().orc_render(self.h, W, H, spp_axis, max_depth, 0 if sampler == "pcg" else 1, seed,
+                         c0, c1, r0, r1, _p(sums), _p(film), threads, _p(counts))
+        return sums, film, counts
+
+    def path(self, W, H, spp_axis, max_depth, seed, pixel, sample):
+        out = np.zeros(3, np.float32)
+        lib().orc_path(self.h, W, H, spp_axis, max_depth, seed, pixel, sample, _p(out))
+        return out
+
+    def triangle_sample(self, tri, u1, u2):
+        pos = np.zeros(3, np.float32)
+        nrm = np.zeros(3, np.float32)
+        area = np.zeros(1, np.float32)
+        lib().orc_triangle_sample(self.h, tri, u1, u2, _p(pos), _p(nrm), _p(area))
+        return pos, nrm, float(area[0])
+
+
+def diffuse_sample(n, wo, u1, u2):
+    wi = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    lib().orc_diffuse_sample(_p(_v3(n)), _p(_v3(wo)), u1, u2, _p(wi), _p(pdf))
+    return wi, float(pdf[0])
+
+
+def glossy_sample(t, b, n, rough, wo, u1, u2):
+    wi = np.zeros(3, np.float32)
+    pdf = np.zeros(1, np.float32)
+    delta = np.zeros(1, np.int32)
+    lib().orc_glossy_sample(_p(_v3(t)), _p(_v3(b)), _p(_v3(n)), rough, _p(_v3(wo)), u1, u2, _p(wi), _p(pdf), _p(delta))
+    return wi, float(pdf[0]), bool(delta[0])
+
+
+def glossy_eval(t, b, n, rough, wo, wi):
+    return lib().orc_glossy_eval(_p(_v3(t)), _p(_v3(b)), _p(_v3(n)), rough, _p(_v3(wo)), _p(_v3(wi)))
+
+
+def glossy_pdf(t, b, n, rough, wo, wi):
+    return lib().orc_glossy_pdf(_p(_v3(t)), _p(_v3(b)), _p(_v3(n)), rough, _p(_v3(wo)), _p(_v3(wi)))
+
+
+def ggx_d(alpha, wh):
+    return lib().orc_ggx_d(alpha, _p(_v3(wh)))
+
+
+def ggx_pdf(alpha, wo, wh):
+    return lib().orc_ggx_pdf(alpha, _p(_v3(wo)), _p(_v3(wh)))
+
+
+def ggx_sample(alpha, wo, u1, u2):
+    out = np.zeros(3, np.float32)
+    lib().orc_ggx_sample(alpha, _p(_v3(wo)), u1, u2, _p(out))
+    return out
+
+
+def pcg_floats(seed, path_id, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_pcg_floats(seed, path_id, n, _p(out))
+    return out
+
+
+def mt_floats(seed, n):
+    out = np.zeros(n, np.float32)
+    lib().orc_mt_floats(seed, n, _p(out))
+    return out

@@ -1,0 +1,102 @@
+/*
+ * CPU oracle for the cpp-pt PathtracerMIS hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The
+ * product (libcpt.so and everything under cpp-pt_amd/) never links, loads or
+ * calls it.
+ *
+ * It restates, in plain C++ with the reference's own float/double promotion
+ * rules, the algorithms of (paths relative to the cpp-pt source tree):
+ *   src/primitive/bvh.h:57-229        binary BVH build (5 centroid bins, area cost) + traversal
+ *   src/primitive/aabb.h:19-61        slab test (max_t effectively ignored)
+ *   src/primitive/simple_group.h:22-39, primitive_leaf.h:27-37
+ *   src/shape/triangle.h:18-269       Cramer ray/triangle test, area, uniform sampling
+ *   src/shape/intersection.h:25-33    first-hit normal mapping
+ *   src/camera/camera_perspective.h:19-39
+ *   src/bxdf/{bxdf,diffuse_bxdf,emissive_bxdf,glossy_bxdf,refraction_bxdf}.h
+ *   src/math/microfacet_distribution.h:13-165, src/math/math.h:647-750
+ *   src/light/{light_group,shape_light}.h
+ *   src/renderer/pathtracerMIS.h:20-167
+ *
+ * Parity pins (tests/test_oracle_*.py): brute-force closest hit, closed-form
+ * Lambertian and GGX normalisation, triangle area-uniform density, the
+ * all-emissive furnace (image == Le exactly), Lambert's polygon formula for
+ * one-bounce irradiance, mt19937 known answer. Functions without such a pin are
+ * marked "parity unpinned" beside their definition.
+ */
+#ifndef CPT_ORACLE_H
+#define CPT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* material record: 16 floats
+ * [0] type (0 diffuse, 1 glossy GGX, 2 emissive, 3 dielectric)
+ * [1..3] albedo rgb   [4] roughness   [5..7] emission rgb (colour*intensity)
+ * [8] ior   [9..11] constant normal-map texel (rgb, 0.5,0.5,1 = flat)
+ * [12] has_normal_map (1 for StandardMaterial, 0 for emissive)  [13..15] unused */
+void* orc_scene_create(int nv, const float* pos, const float* nrm, const float* bitan,
+                       int nt, const int32_t* idx, const int32_t* tri_mat,
+                       int nm, const float* mats,
+                       int nl, const int32_t* light_tris,
+                       const float* cam /* 16: o[3], x[3], y[3], z[3], tan_fovy, aspect, resX, resY */,
+                       const float* env /* 3 */);
+void orc_scene_destroy(void* s);
+
+/* reference BVH (bvh.h:151-217) statistics and DFS test order of every triangle */
+int orc_bvh_nodes(void* s);
+void orc_bvh_rank(void* s, int32_t* rank_out);
+
+/* rays: n x 7 floats (ox oy oz dx dy dz tmax). mode 0 = reference BVH
+ * traversal, 1 = brute force over all triangles with the reference's tie rule. */
+void orc_trace_closest(void* s, const float* rays, int n, int mode, int32_t* tri, float* t,
+                       int nthreads);
+void orc_trace_any(void* s, const float* rays, int n, int mode, uint8_t* occluded, int nthreads);
+
+/* camera ray for film coordinates in [-1,1]^2 (camera_perspective.h:30-39) */
+void orc_camera_ray(void* s, float cx, float cy, float* o3, float* d3);
+
+/* Path tracing.  sampler 0 = per-path PCG32 (the GPU's stream), 1 = per-column
+ * std::mt19937 + uniform_real_distribution<float> (the reference's RandomSampler).
+ * Renders columns [col0, col1) and rows [row0, row1) of a W x H film with spp_axis^2
+ * stratified samples per pixel.  sum_out (W*H*3 doubles, may be NULL) receives the
+ * per-pixel radiance sums, film_out (W*H*3 floats, may be NULL) the reference's
+ * float average acc/spp^2.  counts (may be NULL) receives
+ * [paths, extension rays, shadow rays].  Returns 0. */
+int orc_render(void* s, int W, int H, int spp_axis, int max_depth, int sampler, uint64_t seed,
+               int col0, int col1, int row0, int row1, double* sum_out, float* film_out,
+               int nthreads, int64_t* counts);
+
+/* One path with an explicit PCG path id; returns radiance in rgb3. */
+void orc_path(void* s, int W, int H, int spp_axis, int max_depth, uint64_t seed, int pixel,
+              int sample, float* rgb3);
+
+/* --- unit-level entry points (fixed random numbers) ---------------------- */
+/* Lambertian sample (diffuse_bxdf.h:34-47): n, wo -> wi, pdf */
+void orc_diffuse_sample(const float* n, const float* wo, float u1, float u2, float* wi, float* pdf);
+/* GGX glossy (glossy_bxdf.h:29-72): frame t,b,n (columns), roughness */
+void orc_glossy_sample(const float* t, const float* b, const float* n, float rough, const float* wo,
+                       float u1, float u2, float* wi, float* pdf, int* delta);
+float orc_glossy_eval(const float* t, const float* b, const float* n, float rough, const float* wo,
+                      const float* wi);
+float orc_glossy_pdf(const float* t, const float* b, const float* n, float rough, const float* wo,
+                     const float* wi);
+/* microfacet D(wh) and pdf(wo,wh) in the local frame (microfacet_distribution.h:121-165) */
+float orc_ggx_d(float alpha, const float* wh);
+float orc_ggx_pdf(float alpha, const float* wo, const float* wh);
+void orc_ggx_sample(float alpha, const float* wo, float u1, float u2, float* wh);
+/* Uniform triangle sample (triangle.h:209-269): returns position */
+void orc_triangle_sample(void* s, int tri, float u1, float u2, float* pos, float* nrm, float* area);
+
+/* PCG32 stream used by both the oracle and the GPU (first n floats of a path) */
+void orc_pcg_floats(uint64_t seed, uint64_t path_id, int n, float* out);
+/* std::mt19937(seed) + uniform_real_distribution<float>(0,1): first n floats */
+void orc_mt_floats(uint32_t seed, int n, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
