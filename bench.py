@@ -1,0 +1,174 @@
+"""Benchmark: GPU wavefront PT-NEE-MIS on the procedural ~2.07M-triangle terrain
+scene (1024x512 fBm heightfield + 200 icospheres, 1920x1080, max depth 12).
+
+One step = one 16-spp pass over the whole 1920x1080 frame (spp_axis 4); the
+default 16 steps accumulate the workload's 256 spp.  Inputs (scene, BVH) are
+resident in HBM before timing starts; cpt_render returns after the device has
+finished (it synchronises its stream).
+
+Prints one JSON line.  `roofline` describes the dominant kernel (closest-hit
+traversal, `extend`): algorithmic bytes per extension ray = 32 (ray) + 16 (hit)
++ 128 x BVH4 nodes visited + 48 x triangles tested, counted by a counting run of
+the same kernel on the same rays, divided by that kernel's summed launch time
+measured with HIP events on the stream it runs on.  `cpu_baseline` renders a
+row sample of the same workload with the CPU oracle (a restatement of the
+reference's PathtracerMIS with its mt19937 column sampler and binary BVH).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cpp-pt_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "samples/s + Mrays/s @1080p; equal-time relMSE guided vs plain PT (1 GPU)"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp-axis", type=int, default=4, help="per-axis samples of one step (4 -> 16 spp)")
+    ap.add_argument("--depth", type=int, default=12)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--spheres", type=int, default=200)
+    ap.add_argument("--grid", type=str, default="1024x512")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--grid-blocks", type=int, default=0)
+    ap.add_argument("--max-paths", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(scene, W, H, depth, target_s):
+    import numpy as np
+
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    o = oracle.OracleScene(scene)
+    # spread single rows over the frame; grow the sample until it takes ~target_s
+    rows_done, paths, rays, elapsed = 0, 0, 0, 0.0
+    order = np.random.default_rng(0).permutation(H)
+    while elapsed < target_s and rows_done < H:
+        r = int(order[rows_done])
+        t0 = time.perf_counter()
+        _, _, counts = o.render(W, H, 1, max_depth=depth, sampler="mt", rows=(r, r + 1), threads=threads)
+        elapsed += time.perf_counter() - t0
+        paths += int(counts[0])
+        rays += int(counts[1] + counts[2])
+        rows_done += 1
+    o.close()
+    return {
+        "value": paths / elapsed,
+        "unit": "samples/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{rows_done} random rows x {W} px x 1 spp of the same {W}x{H} scene, depth {depth}, "
+                  f"mt19937 column sampler, reference binary BVH; {elapsed:.1f} s",
+        "mrays_per_s": rays / elapsed / 1e6,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
+        sys.exit(2)
+    import cpt
+
+    W, H, S, depth = args.width, args.height, args.spp_axis, args.depth
+    gx, gz = (int(v) for v in args.grid.split("x"))
+    t0 = time.perf_counter()
+    scene = cpt.scenes.terrain_scene(res=(W, H), n_spheres=args.spheres, grid=(gx, gz))
+    ctx = cpt.Context(0)
+    ctx.upload(scene)
+    setup_s = time.perf_counter() - t0
+    bvh = ctx.bvh_info()
+    print(f"bench: scene {bvh['triangles']} triangles, {bvh['nodes']} BVH4 nodes, setup {setup_s:.1f} s",
+          file=sys.stderr, flush=True)
+    kw = dict(max_depth=depth, block_size=args.block, grid_blocks=args.grid_blocks, max_paths=args.max_paths)
+    for w in range(args.warmup):
+        ctx.render(W, H, S, seed=1000 + w, **kw)
+    t0 = time.perf_counter()
+    ext = shadow = paths = 0
+    for k in range(args.steps):
+        st = ctx.render(W, H, S, seed=1 + k, flags=cpt.CPT_RENDER_ACCUMULATE if k else 0, **kw)
+        ext += st["extension_rays"]
+        shadow += st["shadow_rays"]
+        paths += st["paths"]
+        print(f"bench: step {k + 1}/{args.steps} {st['seconds'] * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    total = time.perf_counter() - t0
+    samples_per_s = paths / total
+    out = {
+        "metric": METRIC,
+        "value": samples_per_s,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"GPU wavefront PT-NEE-MIS, procedural terrain {gx}x{gz} fBm heightfield + "
+                        f"{args.spheres} icospheres (subdiv 4), {bvh['triangles']} triangles, {W}x{H}, "
+                        f"{S * S} spp per step ({S * S * args.steps} spp total), max depth {depth}",
+            "width": W, "height": H, "spp_per_step": S * S, "max_depth": depth,
+            "triangles": bvh["triangles"],
+        },
+        "mrays_per_s": (ext + shadow) / total / 1e6,
+        "extension_mrays_per_s": ext / total / 1e6,
+        "shadow_mrays_per_s": shadow / total / 1e6,
+        "setup_s": setup_s,
+    }
+    if not args.no_roofline:
+        # same seed -> same rays: timing run (HIP events around every launch) and counting run
+        tm = ctx.render(W, H, S, seed=1, flags=cpt.CPT_RENDER_KERNEL_TIMING, **kw)
+        ct = ctx.render(W, H, S, seed=1, flags=cpt.CPT_RENDER_COUNT, **kw)
+        alg_bytes = 48 * ct["extension_rays"] + 128 * ct["node_visits"] + 48 * ct["triangle_tests"]
+        ext_s = tm["ms_extend"] / 1e3
+        achieved = alg_bytes / ext_s / 1e9
+        traffic = None
+        tp = os.path.join(ROOT, "profiles", "extend_traffic.json")
+        if os.path.exists(tp):
+            traffic = json.load(open(tp)).get("hbm_bytes_per_launch")
+        out["roofline"] = {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_extend (closest-hit BVH4 traversal)",
+            "launches": tm["n_extend"],
+            "avg_launch_us": tm["ms_extend"] / max(1, tm["n_extend"]) * 1e3,
+            "bytes_per_launch": alg_bytes / max(1, tm["n_extend"]),
+            "nodes_per_ray": ct["node_visits"] / max(1, ct["extension_rays"]),
+            "tris_per_ray": ct["triangle_tests"] / max(1, ct["extension_rays"]),
+            "ms_shade": tm["ms_shade"],
+            "ms_shadow": tm["ms_shadow"],
+            "ms_extend": tm["ms_extend"],
+        }
+    if not args.no_cpu:
+        cb = cpu_baseline(scene, W, H, depth, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = samples_per_s / cb["value"]
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -32,6 +32,7 @@ ABI_FUNCTIONS = (
     "cpt_version", "cpt_create", "cpt_destroy", "cpt_last_error", "cpt_upload_scene",
     "cpt_bvh_info", "cpt_tie_rank", "cpt_trace_closest", "cpt_trace_any",
     "cpt_trace_closest_device", "cpt_render", "cpt_read_film", "cpt_render_views",
+    "cpt_reference_tie_rank",
 )
 
 
@@ -97,8 +98,21 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_render_views.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                      C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.POINTER(RenderStats)]
+    lib.cpt_reference_tie_rank.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     _lib = lib
     return lib
+
+
+def reference_tie_rank(positions, indices):
+    """Host-only: the reference BVH test order of every triangle."""
+    lib = load_library()
+    P = np.ascontiguousarray(positions, np.float32)
+    I = np.ascontiguousarray(indices, np.int32)
+    out = np.zeros(I.shape[0], np.int32)
+    rc = lib.cpt_reference_tie_rank(_ptr(P), _ptr(I), I.shape[0], _ptr(out))
+    if rc != CPT_OK:
+        raise CptError(rc, "cpt_reference_tie_rank")
+    return out
 
 
 class CptError(RuntimeError):

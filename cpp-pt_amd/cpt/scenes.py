@@ -312,3 +312,142 @@ def single_quad_light(res=(32, 32), size=0.5, height=1.0, le=(1.0, 1.0, 1.0), al
     s.add_mesh(P, N, T, F, em)
     s.set_camera((0.0, 0.5, 0.0), (0.0, -1.0, 0.0), (0.0, 0.0, 1.0), 0.001, *res)
     return s
+
+
+# --------------------------------------------------------------------- procedural terrain scene
+def _value_noise(x, z, rng_table, period=256):
+    """2-D value noise on an integer lattice with smoothstep interpolation."""
+    xi = np.floor(x).astype(np.int64)
+    zi = np.floor(z).astype(np.int64)
+    fx = x - xi
+    fz = z - zi
+    ux = fx * fx * (3 - 2 * fx)
+    uz = fz * fz * (3 - 2 * fz)
+
+    def h(i, j):
+        return rng_table[((i % period) * period + (j % period)) % (period * period)]
+
+    a, b = h(xi, zi), h(xi + 1, zi)
+    c, d = h(xi, zi + 1), h(xi + 1, zi + 1)
+    return (a * (1 - ux) + b * ux) * (1 - uz) + (c * (1 - ux) + d * ux) * uz
+
+
+def fbm_heightfield(nx=1024, nz=512, extent=(40.0, 20.0), octaves=6, amplitude=2.5, seed=2):
+    """fBm heightfield (octaves of value noise, lacunarity 2, gain 0.5)."""
+    rng = np.random.default_rng(seed)
+    table = rng.random(256 * 256) * 2.0 - 1.0
+    xs = np.linspace(-extent[0] / 2, extent[0] / 2, nx)
+    zs = np.linspace(-extent[1] / 2, extent[1] / 2, nz)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")  # (nz, nx)
+    Hh = np.zeros_like(X)
+    freq, amp = 0.15, 1.0
+    for o in range(octaves):
+        off = rng.random(2) * 100.0
+        Hh += amp * _value_noise(X * freq + off[0], Z * freq + off[1], table)
+        freq *= 2.0
+        amp *= 0.5
+    return xs, zs, (Hh * amplitude).astype(np.float64)
+
+
+def _heightfield_mesh(xs, zs, Hh):
+    nz, nx = Hh.shape
+    X, Z = np.meshgrid(xs, zs, indexing="xy")
+    P = np.stack([X, Hh, Z], axis=-1).reshape(-1, 3)
+    dhdx = np.gradient(Hh, xs, axis=1)
+    dhdz = np.gradient(Hh, zs, axis=0)
+    N = np.stack([-dhdx, np.ones_like(Hh), -dhdz], axis=-1).reshape(-1, 3)
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    T = np.stack([np.ones_like(Hh), dhdx, np.zeros_like(Hh)], axis=-1).reshape(-1, 3)
+    T -= N * np.sum(T * N, axis=1, keepdims=True)
+    T /= np.linalg.norm(T, axis=1, keepdims=True)
+    i = np.arange(nz - 1)[:, None] * nx + np.arange(nx - 1)[None, :]
+    i = i.reshape(-1)
+    # counter-clockwise seen from +y: (i, i+nx, i+1) and (i+1, i+nx, i+nx+1)
+    F = np.concatenate([np.stack([i, i + nx, i + 1], 1), np.stack([i + 1, i + nx, i + nx + 1], 1)])
+    F = F.reshape(2, -1, 3).transpose(1, 0, 2).reshape(-1, 3)
+    return P.astype(np.float32), N.astype(np.float32), T.astype(np.float32), F.astype(np.int32)
+
+
+def icosphere(subdiv=4):
+    """Unit icosphere: 20*4^subdiv faces (5120 at subdivision 4)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [tuple(np.asarray(v) / np.linalg.norm(v)) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = (np.asarray(verts[a]) + np.asarray(verts[b])) * 0.5
+                verts.append(tuple(m / np.linalg.norm(m)))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    P = np.asarray(verts, np.float64)
+    F = np.asarray(faces, np.int32)
+    # orient faces outward
+    n = np.cross(P[F[:, 1]] - P[F[:, 0]], P[F[:, 2]] - P[F[:, 0]])
+    flip = np.sum(n * P[F].mean(axis=1), axis=1) < 0
+    F[flip] = F[flip][:, ::-1]
+    return P, F
+
+
+def _sphere_tangents(N):
+    up = np.where(np.abs(N[:, 1:2]) < 0.99, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    T = np.cross(up, N)
+    return T / np.linalg.norm(T, axis=1, keepdims=True)
+
+
+def terrain_scene(res=(1920, 1080), n_spheres=200, grid=(1024, 512), subdiv=4, seed=2, material_seed=3):
+    """Procedural ~2.07M-triangle scene: 1024x512 fBm heightfield (6 octaves),
+    `n_spheres` instanced icospheres (subdivision 4) with radius U[0.05,0.5]
+    resting on the terrain, materials 60% Lambertian / 25% GGX (roughness
+    U[0.05,0.6]) / 15% dielectric (IOR 1.5), 8 emissive quads and a constant
+    environment."""
+    s = Scene()
+    extent = (40.0, 20.0)
+    xs, zs, Hh = fbm_heightfield(grid[0], grid[1], extent, seed=seed)
+    ground = s.add_material(material(MAT_DIFFUSE, albedo=(0.55, 0.5, 0.42)))
+    P, N, T, F = _heightfield_mesh(xs, zs, Hh)
+    s.add_mesh(P, N, T, F, ground)
+    rng = np.random.default_rng(material_seed)
+    SP, SF = icosphere(subdiv)
+    SN = SP.copy()
+    ST = _sphere_tangents(SN)
+    kinds = rng.random(n_spheres)
+    for k in range(n_spheres):
+        r = rng.uniform(0.05, 0.5)
+        cx = rng.uniform(-extent[0] * 0.45, extent[0] * 0.45)
+        cz = rng.uniform(-extent[1] * 0.45, extent[1] * 0.45)
+        ix = int(np.clip(np.searchsorted(xs, cx), 0, len(xs) - 1))
+        iz = int(np.clip(np.searchsorted(zs, cz), 0, len(zs) - 1))
+        cy = Hh[iz, ix] + r * 0.8
+        if kinds[k] < 0.60:
+            m = s.add_material(material(MAT_DIFFUSE, albedo=tuple(rng.uniform(0.1, 0.9, 3))))
+        elif kinds[k] < 0.85:
+            m = s.add_material(material(MAT_GLOSSY, rough=float(rng.uniform(0.05, 0.6))))
+        else:
+            m = s.add_material(material(MAT_DIELECTRIC, ior=1.5))
+        s.add_mesh(SP * r + np.array([cx, cy, cz]), SN, ST, SF, m)
+    light = s.add_material(material(MAT_EMISSIVE, emission=(12.0, 11.0, 9.0)))
+    for k in range(8):
+        lx = -extent[0] * 0.4 + k * (extent[0] * 0.8 / 7)
+        lz = -2.0 + 4.0 * (k % 2)
+        y = 9.0
+        h = 0.75
+        P, N, T, F = _quad((lx - h, y, lz - h), (lx + h, y, lz - h), (lx + h, y, lz + h), (lx - h, y, lz + h),
+                           (0, -1, 0), (1, 0, 0))
+        s.add_mesh(P, N, T, F, light)
+    s.env = (0.25, 0.3, 0.4)
+    s.set_camera((0.0, 7.0, 17.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.tan(math.radians(25.0)), *res)
+    return s

@@ -757,6 +757,14 @@ int cpt_tie_rank(cpt_ctx* c, int32_t* out) {
     return CPT_OK;
 }
 
+int cpt_reference_tie_rank(const float* positions, const int32_t* indices, int32_t nt, int32_t* out) {
+    if (nt < 0 || (nt > 0 && (!positions || !indices || !out))) return CPT_E_INVALID;
+    std::vector<int32_t> r;
+    reference_tie_rank(positions, indices, nt, r);
+    if (nt > 0) std::memcpy(out, r.data(), sizeof(int32_t) * (size_t)nt);
+    return CPT_OK;
+}
+
 static int default_grid(cpt_ctx* c, int B) { return c->cus * (2048 / B) ; }
 
 static int trace_common(cpt_ctx* c, const float* rays, int32_t n, bool any, int32_t* tri_out, float* t_out,

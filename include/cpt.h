@@ -139,6 +139,10 @@ int cpt_bvh_info(cpt_ctx* ctx, int64_t* info4);
 /* The reference BVH's test order of every triangle (n_triangles int32). */
 int cpt_tie_rank(cpt_ctx* ctx, int32_t* rank_out);
 
+/* Host only (no device needed): the reference BVH's test order for a triangle
+ * soup, as cpt_upload_scene computes it (bvh.h:57-217 restated). */
+int cpt_reference_tie_rank(const float* positions, const int32_t* indices, int32_t n_triangles, int32_t* rank_out);
+
 /* Closest hit for n rays (n x 7 floats: ox oy oz dx dy dz tmax, host memory).
  * Replaces BVH::intersect (bvh.h:219-221 -> BVHNode::intersect 22-37 ->
  * Triangle::intersect, src/shape/triangle.h:18-116).  tri_out = triangle id or
