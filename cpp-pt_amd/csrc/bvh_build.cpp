@@ -149,6 +149,9 @@ void build_bvh4(const float* P, const int32_t* I, int nt, const std::vector<int3
         for (int v = 0; v < 3; v++) tb[t].grow(P + 3 * (size_t)I[3 * (size_t)t + v]);
         for (int k = 0; k < 3; k++) cen[3 * (size_t)t + k] = 0.5f * (tb[t].lo[k] + tb[t].hi[k]);
     }
+    float scene_max = 0.0f;
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++) scene_max = std::max(scene_max, std::max(std::fabs(tb[t].lo[k]), std::fabs(tb[t].hi[k])));
     std::vector<int> idx(nt);
     for (int t = 0; t < nt; t++) idx[t] = t;
     Builder2 b2(tb, cen, idx);
@@ -209,12 +212,24 @@ void build_bvh4(const float* P, const int32_t* I, int nt, const std::vector<int3
         int ni = 0;
         for (int k = 0; k < nk; k++) {
             const BNode2& c = b2.nodes[kids[k]];
-            node.lox[k] = c.box.lo[0];
-            node.loy[k] = c.box.lo[1];
-            node.loz[k] = c.box.lo[2];
-            node.hix[k] = c.box.hi[0];
-            node.hiy[k] = c.box.hi[1];
-            node.hiz[k] = c.box.hi[2];
+            // The reference's triangle test accepts hit points a few ulps outside
+            // the triangle (rounded barycentrics, triangle.h:59-75), i.e. outside
+            // its bounding box.  Child boxes are therefore widened by a few ulps
+            // of their own coordinates plus a scene-scale term, so that
+            // traversal never culls a leaf whose triangle the reference accepts.
+            float lo[3], hi[3];
+            for (int a = 0; a < 3; a++) {
+                float m = std::max(std::fabs(c.box.lo[a]), std::fabs(c.box.hi[a]));
+                float dl = m * 0x1p-20f + scene_max * 0x1p-22f;
+                lo[a] = c.box.lo[a] - dl;
+                hi[a] = c.box.hi[a] + dl;
+            }
+            node.lox[k] = lo[0];
+            node.loy[k] = lo[1];
+            node.loz[k] = lo[2];
+            node.hix[k] = hi[0];
+            node.hiy[k] = hi[1];
+            node.hiz[k] = hi[2];
             if (c.leaf()) {
                 int first = (int)out.tris.size();
                 for (int j = 0; j < c.count; j++) {

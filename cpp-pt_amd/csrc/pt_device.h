@@ -56,7 +56,8 @@ __device__ __forceinline__ V3 mat_cols_tmul(V3 c0, V3 c1, V3 c2, V3 a) {
 
 // ---------------------------------------------------------------- RNG
 // PCG32 (O'Neill 2014, XSH-RR), one stream per path: initstate =
-// splitmix64(seed + path_id), fixed increment.
+// splitmix64(splitmix64(seed) ^ path_id), fixed increment (distinct seeds give
+// unrelated streams for every path).
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
     uint64_t z = x + 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -75,7 +76,7 @@ __device__ __forceinline__ uint32_t pcg_next(uint64_t& s) {
 __device__ __forceinline__ uint64_t pcg_init(uint64_t seed, uint64_t path_id) {
     uint64_t s = 0;
     pcg_next(s);
-    s += splitmix64(seed + path_id);
+    s += splitmix64(splitmix64(seed) ^ path_id);
     pcg_next(s);
     return s;
 }

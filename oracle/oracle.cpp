@@ -88,14 +88,14 @@ inline uint64_t splitmix64(uint64_t x) {
 }
 
 // PCG32 (O'Neill 2014, pcg32_srandom_r / pcg32_random_r, XSH-RR output),
-// seeded per path: initstate = splitmix64(seed + path_id), fixed increment.
+// seeded per path: initstate = splitmix64(splitmix64(seed) ^ path_id), fixed increment.
 struct PcgSampler : Sampler {
     uint64_t state, inc;
     PcgSampler(uint64_t seed, uint64_t path_id) {
         state = 0;
         inc = (0x5851F42D4C957F2Dull << 1) | 1u;
         next();
-        state += splitmix64(seed + path_id);
+        state += splitmix64(splitmix64(seed) ^ path_id);
         next();
     }
     uint32_t next() {

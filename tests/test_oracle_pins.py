@@ -53,7 +53,7 @@ def _splitmix64(x):
 
 @pytest.mark.parametrize("seed,path", [(1, 0), (1, 12345), (7, 2**39 + 5), (0xDEADBEEF, 99)])
 def test_oracle_pcg_stream(seed, path):
-    state, inc = _pcg32_srandom(_splitmix64(seed + path), 0x5851F42D4C957F2D)
+    state, inc = _pcg32_srandom(_splitmix64(_splitmix64(seed) ^ path), 0x5851F42D4C957F2D)
     want = []
     for _ in range(16):
         state, v = _pcg32_ref(state, inc)
