@@ -94,7 +94,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_trace_any.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     lib.cpt_trace_closest_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     lib.cpt_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.POINTER(RenderStats)]
-    lib.cpt_read_film.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.cpt_read_film.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.cpt_render_views.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                      C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.POINTER(RenderStats)]
@@ -217,9 +217,18 @@ class Context:
         self._last = (width, height)
         return st.as_dict()
 
-    def read_film(self, sums: bool = False):
+    def read_film(self, sums: bool = False, sumsq: bool = False):
+        """rgb (h,w,3) float32; with sums/sumsq also the FP64 per-pixel sums and
+        sums of squares of path radiance."""
         w, h = self._last
         rgb = np.zeros((h, w, 3), np.float32)
         s = np.zeros((h, w, 3), np.float64) if sums else None
-        self._check(self.lib.cpt_read_film(self.h, _ptr(rgb), _ptr(s)), "cpt_read_film")
-        return (rgb, s) if sums else rgb
+        s2 = np.zeros((h, w, 3), np.float64) if sumsq else None
+        self._check(self.lib.cpt_read_film(self.h, _ptr(rgb), _ptr(s), _ptr(s2)), "cpt_read_film")
+        if sums and sumsq:
+            return rgb, s, s2
+        if sums:
+            return rgb, s
+        if sumsq:
+            return rgb, s2
+        return rgb

@@ -72,7 +72,8 @@ struct WfArgs {
     unsigned long long* stats;
     int* gstack;
     const PassDesc* pass;
-    double* film;
+    double* film;      // per-pixel radiance sums
+    double* film2;     // per-pixel sums of squared radiance
     int W, H, S, max_depth;
     uint64_t seed;
     uint32_t view;
@@ -315,16 +316,24 @@ __global__ void k_accum(WfArgs a) {
     for (unsigned pl = gtid(); pl < pd.npix; pl += gthreads()) {
         unsigned pixel = pd.p0 + pl;
         double* f = a.film + 3 * (size_t)pixel;
+        double* f2 = a.film2 + 3 * (size_t)pixel;
         double x = f[0], y = f[1], z = f[2];
+        double x2 = f2[0], y2 = f2[1], z2 = f2[2];
         for (unsigned s = 0; s < pd.ns; s++) {
             float4 c = a.col[(size_t)s * pd.npix + pl];
             x += (double)c.x;
             y += (double)c.y;
             z += (double)c.z;
+            x2 += (double)c.x * (double)c.x;
+            y2 += (double)c.y * (double)c.y;
+            z2 += (double)c.z * (double)c.z;
         }
         f[0] = x;
         f[1] = y;
         f[2] = z;
+        f2[0] = x2;
+        f2[1] = y2;
+        f2[2] = z2;
     }
 }
 
@@ -397,6 +406,7 @@ struct cpt_ctx {
     PassDesc* h_pass = nullptr;  // pinned
     size_t h_pass_cap = 0;
     double* film = nullptr;
+    double* film2 = nullptr;
     size_t film_px = 0;
     int film_w = 0, film_h = 0, film_S = 1;
 
@@ -656,6 +666,7 @@ void cpt_destroy(cpt_ctx* c) {
     dfree(c->stats);
     dfree(c->d_pass);
     dfree(c->film);
+    dfree(c->film2);
     if (c->h_pass) (void)hipHostFree(c->h_pass);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -905,6 +916,8 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
         drop_graph(c);
         r = dalloc(c, c->film, 3 * (size_t)P);
         if (r) return r;
+        r = dalloc(c, c->film2, 3 * (size_t)P);
+        if (r) return r;
         c->film_px = (size_t)P;
     }
     if (c->h_pass_cap < passes.size()) {
@@ -940,6 +953,7 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     a.gstack = c->gstack;
     a.pass = c->d_pass;
     a.film = c->film;
+    a.film2 = c->film2;
     a.W = p->width;
     a.H = p->height;
     a.S = p->spp_axis;
@@ -948,7 +962,10 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     a.view = (uint32_t)p->view;
     LaunchCfg L{B, G, (p->flags & CPT_RENDER_COUNT) != 0};
 
-    if (!(p->flags & CPT_RENDER_ACCUMULATE)) HIPCHK(c, hipMemsetAsync(c->film, 0, 3 * sizeof(double) * (size_t)P, c->stream));
+    if (!(p->flags & CPT_RENDER_ACCUMULATE)) {
+        HIPCHK(c, hipMemsetAsync(c->film, 0, 3 * sizeof(double) * (size_t)P, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->film2, 0, 3 * sizeof(double) * (size_t)P, c->stream));
+    }
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
 
     bool timing = (p->flags & CPT_RENDER_KERNEL_TIMING) != 0;
@@ -1015,12 +1032,13 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     return CPT_OK;
 }
 
-int cpt_read_film(cpt_ctx* c, float* rgb, double* sums) {
+int cpt_read_film(cpt_ctx* c, float* rgb, double* sums, double* sumsq) {
     if (!c) return CPT_E_INVALID;
     if (!c->film) return fail(c, CPT_E_INVALID, "nothing rendered yet");
     size_t P = (size_t)c->film_w * c->film_h;
     std::vector<double> h(3 * P);
     HIPCHK(c, hipMemcpyAsync(h.data(), c->film, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (sumsq) HIPCHK(c, hipMemcpyAsync(sumsq, c->film2, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (sums) std::memcpy(sums, h.data(), h.size() * sizeof(double));
     if (rgb) {

@@ -35,7 +35,7 @@ void PathtracerMISGPU::render(Scene& sc, std::string filename) {
     p.seed = rs_.seed;
     check(ctx_, cpt_render(ctx_, &p, &stats_), "cpt_render");
     std::vector<float> rgb(3 * (size_t)image.resX * image.resY);
-    check(ctx_, cpt_read_film(ctx_, rgb.data(), nullptr), "cpt_read_film");
+    check(ctx_, cpt_read_film(ctx_, rgb.data(), nullptr, nullptr), "cpt_read_film");
     for (int j = 0; j < image.resY; j++)
         for (int i = 0; i < image.resX; i++) {
             const float* q = &rgb[3 * ((size_t)i + (size_t)j * image.resX)];

@@ -161,8 +161,9 @@ int cpt_trace_closest_device(cpt_ctx* ctx, const void* d_rays, int32_t n, void* 
 int cpt_render(cpt_ctx* ctx, const cpt_render_params* params, cpt_render_stats* stats);
 /* Copy the film out: rgb (w*h*3 floats) = sum / spp_axis^2 of the last
  * render, row j = 0 at the top like RgbImage::put_pixel (rgb_image.h:208-210);
- * sums (w*h*3 doubles, may be NULL) = the raw FP64 accumulators. */
-int cpt_read_film(cpt_ctx* ctx, float* rgb, double* sums);
+ * sums (w*h*3 doubles, may be NULL) = the raw FP64 accumulators; sumsq (may be
+ * NULL) = per-pixel sums of squared path radiance (variance estimates). */
+int cpt_read_film(cpt_ctx* ctx, float* rgb, double* sums, double* sumsq);
 
 /* Multi-view batch in one launch sequence: n_views cameras (16 floats each, as
  * cpt_scene_desc.camera) with their own width/height/spp_axis.  Paths of all

@@ -43,7 +43,7 @@ def lib():
     L.orc_trace_any.restype = None
     L.orc_camera_ray.argtypes = [vp, C.c_float, C.c_float, vp, vp]
     L.orc_camera_ray.restype = None
-    L.orc_render.argtypes = [vp, i32, i32, i32, i32, i32, C.c_uint64, i32, i32, i32, i32, vp, vp, i32, vp]
+    L.orc_render.argtypes = [vp, i32, i32, i32, i32, i32, C.c_uint64, i32, i32, i32, i32, vp, vp, vp, i32, vp]
     L.orc_path.argtypes = [vp, i32, i32, i32, i32, C.c_uint64, i32, i32, vp]
     L.orc_path.restype = None
     L.orc_diffuse_sample.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp]
@@ -135,16 +135,20 @@ class OracleScene:
         return o, d
 
     def render(self, W, H, spp_axis, max_depth=32, sampler="pcg", seed=1, cols=None, rows=None,
-               threads=0):
-        """Returns (sums (H,W,3) float64, film (H,W,3) float32, counts [paths, ext, shadow])."""
+               threads=0, sumsq=False):
+        """Returns (sums (H,W,3) float64, film (H,W,3) float32, counts [paths, ext, shadow])
+        and, with sumsq=True, the per-pixel sums of squares as a fourth element.
+        sampler "pcg" = the GPU's per-path stream; "mt" = the reference's per-column
+        mt19937 (seed 0 reproduces the reference's seeding)."""
         sums = np.zeros((H, W, 3), np.float64)
+        sq = np.zeros((H, W, 3), np.float64)
         film = np.zeros((H, W, 3), np.float32)
         counts = np.zeros(3, np.int64)
         c0, c1 = cols if cols else (0, W)
         r0, r1 = rows if rows else (0, H)
         lib().orc_render(self.h, W, H, spp_axis, max_depth, 0 if sampler == "pcg" else 1, seed,
-                         c0, c1, r0, r1, _p(sums), _p(film), threads, _p(counts))
-        return sums, film, counts
+                         c0, c1, r0, r1, _p(sums), _p(sq), _p(film), threads, _p(counts))
+        return (sums, film, counts, sq) if sumsq else (sums, film, counts)
 
     def path(self, W, H, spp_axis, max_depth, seed, pixel, sample):
         out = np.zeros(3, np.float32)

@@ -915,18 +915,20 @@ void orc_camera_ray(void* sp, float cx, float cy, float* o3, float* d3) {
 }
 
 int orc_render(void* sp, int W, int H, int S, int max_depth, int sampler, uint64_t seed, int col0, int col1,
-               int row0, int row1, double* sum_out, float* film_out, int nthreads, int64_t* counts) {
+               int row0, int row1, double* sum_out, double* sumsq_out, float* film_out, int nthreads,
+               int64_t* counts) {
     const Scene* s = (const Scene*)sp;
     if (nthreads > 0) omp_set_num_threads(nthreads);
     int64_t paths = 0, ext = 0, shadow = 0;
     // pathtracerMIS.h:139-160: OpenMP over columns, one sampler per column
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : paths, ext, shadow)
     for (int i = col0; i < col1; i++) {
-        MtSampler mt(i);
+        // seed 0 is the reference's RandomSampler(i); other seeds shift the column seeds
+        MtSampler mt(i + (int)(seed & 0x7fff) * W);
         Counts cnt;
         for (int j = row0; j < row1; j++) {
             V3 acc(0.0f);
-            double ax = 0, ay = 0, az = 0;
+            double ax = 0, ay = 0, az = 0, qx = 0, qy = 0, qz = 0;
             for (int k = 0; k < S; k++)
                 for (int l = 0; l < S; l++) {
                     V3 c;
@@ -948,6 +950,9 @@ int orc_render(void* sp, int W, int H, int S, int max_depth, int sampler, uint64
                     ax += c.x;
                     ay += c.y;
                     az += c.z;
+                    qx += (double)c.x * (double)c.x;
+                    qy += (double)c.y * (double)c.y;
+                    qz += (double)c.z * (double)c.z;
                     paths++;
                 }
             size_t pix = (size_t)i + (size_t)j * W;
@@ -955,6 +960,11 @@ int orc_render(void* sp, int W, int H, int S, int max_depth, int sampler, uint64
                 sum_out[3 * pix] = ax;
                 sum_out[3 * pix + 1] = ay;
                 sum_out[3 * pix + 2] = az;
+            }
+            if (sumsq_out) {
+                sumsq_out[3 * pix] = qx;
+                sumsq_out[3 * pix + 1] = qy;
+                sumsq_out[3 * pix + 2] = qz;
             }
             if (film_out) {
                 V3 f = divs(acc, float(S * S));

@@ -60,15 +60,16 @@ void orc_trace_any(void* s, const float* rays, int n, int mode, uint8_t* occlude
 void orc_camera_ray(void* s, float cx, float cy, float* o3, float* d3);
 
 /* Path tracing.  sampler 0 = per-path PCG32 (the GPU's stream), 1 = per-column
- * std::mt19937 + uniform_real_distribution<float> (the reference's RandomSampler).
+ * std::mt19937 + uniform_real_distribution<float> (the reference's RandomSampler;
+ * column i uses seed i + (seed & 0x7fff)*W, so seed 0 is the reference exactly).
  * Renders columns [col0, col1) and rows [row0, row1) of a W x H film with spp_axis^2
  * stratified samples per pixel.  sum_out (W*H*3 doubles, may be NULL) receives the
- * per-pixel radiance sums, film_out (W*H*3 floats, may be NULL) the reference's
+ * per-pixel radiance sums, sumsq_out the sums of squares, film_out (W*H*3 floats, may be NULL) the reference's
  * float average acc/spp^2.  counts (may be NULL) receives
  * [paths, extension rays, shadow rays].  Returns 0. */
 int orc_render(void* s, int W, int H, int spp_axis, int max_depth, int sampler, uint64_t seed,
-               int col0, int col1, int row0, int row1, double* sum_out, float* film_out,
-               int nthreads, int64_t* counts);
+               int col0, int col1, int row0, int row1, double* sum_out, double* sumsq_out,
+               float* film_out, int nthreads, int64_t* counts);
 
 /* One path with an explicit PCG path id; returns radiance in rgb3. */
 void orc_path(void* s, int W, int H, int spp_axis, int max_depth, uint64_t seed, int pixel,

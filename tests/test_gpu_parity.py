@@ -135,23 +135,29 @@ def test_graph_and_eager_identical(ctx):
     assert np.array_equal(a, d) and np.array_equal(a, e)
 
 
+def _zscores(s_a, q_a, n_a, s_b, q_b, n_b):
+    ma, mb = s_a / n_a, s_b / n_b
+    va = np.maximum(q_a / n_a - ma * ma, 0) * n_a / (n_a - 1)
+    vb = np.maximum(q_b / n_b - mb * mb, 0) * n_b / (n_b - 1)
+    se = np.sqrt(va / n_a + vb / n_b)
+    ok = se > 0
+    return (ma - mb)[ok] / se[ok]
+
+
 def test_statistical_equivalence_with_reference_sampler(ctx):
-    """GPU (PCG) vs the oracle with the reference's own mt19937 column sampler:
-    per-pixel means agree within 3 sigma on 4x4 blocks."""
+    """GPU (PCG per path) vs the oracle with the reference's own per-column
+    mt19937 sampler (seed = column index): per-pixel, per-channel means agree
+    within 3 sigma, sigma from both sides' sums of squares (256 spp each)."""
     W = H = 64
     s = cpt.scenes.box_room(res=(W, H))
     ctx.upload(s)
-    reps = []
-    for seed in range(1, 17):
-        ctx.render(W, H, 4, seed=seed)
-        reps.append(ctx.read_film())
-    reps = np.stack(reps)  # 16 x (16 spp)
-    blk = reps.reshape(16, H // 4, 4, W // 4, 4, 3).mean(axis=(2, 4))
-    g_mean = blk.mean(0)
-    g_se = blk.std(0, ddof=1) / np.sqrt(16)
-    _, film, _ = oracle.OracleScene(s).render(W, H, 16, sampler="mt")  # 256 spp
-    o_mean = film.reshape(H // 4, 4, W // 4, 4, 3).mean(axis=(1, 3))
-    o_se = g_se / 1.0  # same per-sample variance, equal sample count (256 vs 16x16)
-    z = (g_mean - o_mean) / np.sqrt(g_se ** 2 + o_se ** 2 + 1e-12)
+    ctx.render(W, H, 16, seed=21)
+    _, gs, gq = ctx.read_film(sums=True, sumsq=True)
+    os_, _, _, oq = oracle.OracleScene(s).render(W, H, 16, sampler="mt", seed=0, sumsq=True)
+    z = _zscores(gs, gq, 256, os_, oq, 256)
     frac = (np.abs(z) > 3).mean()
-    assert frac < 0.02, frac
+    print(f"3-sigma exceedance {frac * 100:.3f}% of {z.size} pixel-channels, mean z {z.mean():+.4f}")
+    # Gaussian: 0.27%; heavy-tailed light paths add a little.  The mean z would
+    # shift visibly (>> 0.05) under any bias of the estimator.
+    assert frac < 0.01
+    assert abs(z.mean()) < 0.05
