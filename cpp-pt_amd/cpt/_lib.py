@@ -159,7 +159,8 @@ class Context:
         if rc != CPT_OK:
             raise CptError(rc, f"{what}: {self.lib.cpt_last_error(self.h).decode()}")
 
-    def upload(self, scene, tie_rank: bool = True):
+    def upload(self, scene, tie_rank: bool = True, camera=None):
+        """Upload a cpt.scenes.Scene; `camera` (16 floats) overrides its camera."""
         arrs = scene.c_arrays()
         d = SceneDesc()
         d.n_vertices = arrs["positions"].shape[0]
@@ -173,8 +174,9 @@ class Context:
         d.materials = _ptr(arrs["materials"])
         d.n_lights = arrs["light_tris"].shape[0]
         d.light_triangles = _ptr(arrs["light_tris"])
+        cam = arrs["camera"] if camera is None else np.asarray(camera, np.float32).reshape(16)
         for k in range(16):
-            d.camera[k] = float(arrs["camera"][k])
+            d.camera[k] = float(cam[k])
         for k in range(3):
             d.env[k] = float(arrs["env"][k])
         d.flags = 0 if tie_rank else CPT_SCENE_NO_TIE_RANK
@@ -232,3 +234,28 @@ class Context:
         if sumsq:
             return rgb, s2
         return rgb
+
+
+def _render_views(self, cameras, wh_spp, max_depth=32, seed=1, flags=0):
+    """Render a batch of views in one streaming launch sequence.
+
+    cameras: (n, 16) float32 camera records; wh_spp: (n, 3) int32 (width,
+    height, spp_axis).  Returns (list of (h, w, 3) float32 images, per-view
+    completion times in ms from the batch start, stats dict)."""
+    cams = np.ascontiguousarray(cameras, np.float32).reshape(-1, 16)
+    whs = np.ascontiguousarray(wh_spp, np.int32).reshape(-1, 3)
+    n = cams.shape[0]
+    px = int(np.sum(whs[:, 0].astype(np.int64) * whs[:, 1]))
+    films = np.zeros(3 * px, np.float32)
+    ms = np.zeros(n, np.float64)
+    st = RenderStats()
+    self._check(self.lib.cpt_render_views(self.h, n, _ptr(cams), _ptr(whs), max_depth, seed, flags,
+                                          _ptr(films), _ptr(ms), C.byref(st)), "cpt_render_views")
+    out, off = [], 0
+    for w, h, _ in whs:
+        out.append(films[off:off + 3 * w * h].reshape(h, w, 3))
+        off += 3 * w * h
+    return out, ms, st.as_dict()
+
+
+Context.render_views = _render_views

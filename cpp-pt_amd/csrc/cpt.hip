@@ -61,14 +61,16 @@ struct WfArgs {
     float4* col;       // radiance
     uint64_t* rng;
     uint32_t* meta;    // depth | sampled_delta << 8
-    uint32_t* pix;     // pixel index
+    uint32_t* pix;     // pixel index (streaming mode: index into the concatenated films)
+    uint32_t* vid;     // streaming mode: view of the slot's path
+    uint32_t* done;    // streaming mode: slots whose path ended this bounce
     float4* qo[2];     // o.xyz, tmax
     float4* qd[2];     // d.xyz, slot
     float4* hits;      // t, beta, gamma, tri
     float4* so;        // shadow o.xyz, tmax
     float4* sd;        // shadow d.xyz, slot
     float4* sc_rgb;    // shadow contribution
-    uint32_t* counters;  // [0],[1] extension queues, [2] shadow queue
+    uint32_t* counters;  // [0],[1] extension queues, [2] shadow queue, [3] ended-path queue
     unsigned long long* stats;
     int* gstack;
     const PassDesc* pass;
@@ -143,6 +145,7 @@ __global__ __launch_bounds__(B) void k_extend(WfArgs a, int cur) {
     if (gtid() == 0) {
         a.counters[cur ^ 1] = 0;
         a.counters[2] = 0;
+        a.counters[3] = 0;
         atomicAdd(&a.stats[ST_EXT], (unsigned long long)count);
     }
     TravStack<B> st{lds, a.gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(B) void k_extend(WfArgs a, int cur) {
 
 // ------------------------------------------------------------------ shade
 // pathtracerMIS.h:37-126, one loop iteration per queue entry.
-template <int B>
+template <int B, bool STREAM>
 __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
     const unsigned count = a.counters[cur];
     const unsigned lane = __lane_id();
@@ -273,6 +276,11 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
             a.qo[cur ^ 1][qe] = make_float4(no.x, no.y, no.z, INFINITY);
             a.qd[cur ^ 1][qe] = make_float4(nd.x, nd.y, nd.z, __int_as_float(slot));
         }
+        if (STREAM) {
+            bool ended = active && !push_ext;
+            unsigned qd = wave_alloc(ended, &a.counters[3]);
+            if (ended) a.done[qd] = (uint32_t)slot;
+        }
         unsigned qs = wave_alloc(push_sh, &a.counters[2]);
         if (push_sh) {
             a.so[qs] = make_float4(sho.x, sho.y, sho.z, shmax);
@@ -334,6 +342,128 @@ __global__ void k_accum(WfArgs a) {
         f2[0] = x2;
         f2[1] = y2;
         f2[2] = z2;
+    }
+}
+
+
+// ------------------------------------------------------------------ streaming (multi-view) mode
+// Paths of all views form one index space [0, total): view v owns
+// [path_begin, path_begin + W*H*S*S), sample-major within the view.  A slot
+// whose path ended is refilled with the next path index by k_regen, so the
+// queues stay full until the last path of the last view has been started.
+struct ViewDesc {
+    float cam[16];
+    int W, H, S, pad;
+    unsigned long long path_begin;
+    unsigned long long n_paths;
+    unsigned long long film_off;
+    unsigned long long pad2;
+};
+
+struct StreamArgs {
+    const ViewDesc* views;
+    int n_views;
+    unsigned long long total;
+    unsigned long long* next_path;     // global path counter
+    unsigned long long* view_done;     // accumulated paths per view
+    unsigned long long* view_tend;     // s_memrealtime when the view completed
+    unsigned long long* t_start;
+};
+
+#define CPT_NO_PATH 0xffffffffu
+
+__global__ void k_stream_init(WfArgs a, StreamArgs sa, unsigned cap) {
+    if (gtid() == 0) {
+        *sa.next_path = 0;
+        *sa.t_start = __builtin_amdgcn_s_memrealtime();
+        a.counters[0] = a.counters[1] = a.counters[2] = 0;
+        a.counters[3] = cap;
+    }
+    for (unsigned i = gtid(); i < cap; i += gthreads()) {
+        a.done[i] = i;
+        a.pix[i] = CPT_NO_PATH;
+    }
+    for (unsigned v = gtid(); v < (unsigned)sa.n_views; v += gthreads()) {
+        sa.view_done[v] = 0;
+        sa.view_tend[v] = 0;
+    }
+}
+
+// Accumulate the radiance of ended paths (col already holds this bounce's NEE)
+// and start new camera paths in their slots.
+template <int B>
+__global__ __launch_bounds__(B) void k_regen(WfArgs a, StreamArgs sa, int nxt) {
+    const unsigned count = a.counters[3];
+    const unsigned lane = __lane_id();
+    const unsigned wave = gtid() >> 6, nwaves = gthreads() >> 6;
+    for (unsigned base = wave * 64; base < count; base += nwaves * 64) {
+        const unsigned i = base + lane;
+        const bool active = i < count;
+        unsigned slot = active ? a.done[i] : 0u;
+        if (active) {
+            unsigned fp = a.pix[slot];
+            if (fp != CPT_NO_PATH) {
+                float4 c = a.col[slot];
+                double* f = a.film + 3 * (size_t)fp;
+                double* f2 = a.film2 + 3 * (size_t)fp;
+                unsafeAtomicAdd(f, (double)c.x);
+                unsafeAtomicAdd(f + 1, (double)c.y);
+                unsafeAtomicAdd(f + 2, (double)c.z);
+                unsafeAtomicAdd(f2, (double)c.x * (double)c.x);
+                unsafeAtomicAdd(f2 + 1, (double)c.y * (double)c.y);
+                unsafeAtomicAdd(f2 + 2, (double)c.z * (double)c.z);
+                unsigned v = a.vid[slot];
+                unsigned long long old = atomicAdd(&sa.view_done[v], 1ull);
+                if (old + 1 == sa.views[v].n_paths) sa.view_tend[v] = __builtin_amdgcn_s_memrealtime();
+            }
+        }
+        // next path indices for the active lanes: one atomic per wave
+        unsigned long long mask = __ballot(active);
+        unsigned long long g0 = 0;
+        if (mask) {
+            int leader = __ffsll(mask) - 1;
+            if ((int)lane == leader) g0 = atomicAdd(sa.next_path, (unsigned long long)__popcll(mask));
+            g0 = __shfl(g0, leader);
+        }
+        unsigned long long lt = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+        unsigned long long g = g0 + (unsigned long long)__popcll(lt);
+        bool start = active && g < sa.total;
+        V3 o = {0, 0, 0}, d = {0, 0, 0};
+        if (start) {
+            int lo = 0, hi = sa.n_views - 1;
+            while (lo < hi) {  // last view with path_begin <= g
+                int mid = (lo + hi + 1) >> 1;
+                if (sa.views[mid].path_begin <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            const ViewDesc& vd = sa.views[lo];
+            unsigned long long local = g - vd.path_begin;
+            unsigned long long P = (unsigned long long)vd.W * (unsigned long long)vd.H;
+            unsigned sample = (unsigned)(local / P);
+            unsigned pixel = (unsigned)(local - (unsigned long long)sample * P);
+            int S = vd.S;
+            int px = (int)(pixel % (unsigned)vd.W), py = (int)(pixel / (unsigned)vd.W);
+            uint64_t path_id = ((uint64_t)lo << 40) + (uint64_t)pixel * (uint64_t)(S * S) + sample;
+            uint64_t rng = pcg_init(a.seed, path_id);
+            float r1 = pcg_float(rng);
+            float r2 = pcg_float(rng);
+            float cx, cy;
+            film_coords(px, py, (int)sample / S, (int)sample % S, S, vd.W, vd.H, r1, r2, cx, cy);
+            camera_ray(vd.cam, cx, cy, o, d);
+            a.mul_p[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+            a.col[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            a.rng[slot] = rng;
+            a.meta[slot] = 1u << 8;
+            a.pix[slot] = (unsigned)(vd.film_off + pixel);
+            a.vid[slot] = (unsigned)lo;
+        } else if (active) {
+            a.pix[slot] = CPT_NO_PATH;
+        }
+        unsigned qe = wave_alloc(start, &a.counters[nxt]);
+        if (start) {
+            a.qo[nxt][qe] = make_float4(o.x, o.y, o.z, INFINITY);
+            a.qd[nxt][qe] = make_float4(d.x, d.y, d.z, __int_as_float((int)slot));
+        }
     }
 }
 
@@ -399,7 +529,7 @@ struct cpt_ctx {
     float4 *mul_p = nullptr, *col = nullptr, *qo[2] = {nullptr, nullptr}, *qd[2] = {nullptr, nullptr};
     float4 *hits = nullptr, *so = nullptr, *sd = nullptr, *sc_rgb = nullptr;
     uint64_t* rng = nullptr;
-    uint32_t *meta = nullptr, *pix = nullptr, *counters = nullptr;
+    uint32_t *meta = nullptr, *pix = nullptr, *vid = nullptr, *done = nullptr, *counters = nullptr;
     unsigned long long* stats = nullptr;
     int* gstack = nullptr;
     PassDesc* d_pass = nullptr;
@@ -466,6 +596,8 @@ void free_wavefront(cpt_ctx* c) {
     dfree(c->rng);
     dfree(c->meta);
     dfree(c->pix);
+    dfree(c->vid);
+    dfree(c->done);
     c->cap = 0;
 }
 
@@ -527,9 +659,44 @@ int ensure_wavefront(cpt_ctx* c, size_t n) {
     r |= dalloc(c, c->rng, n);
     r |= dalloc(c, c->meta, n);
     r |= dalloc(c, c->pix, n);
+    r |= dalloc(c, c->vid, n);
+    r |= dalloc(c, c->done, n);
     if (r) return CPT_E_HIP;
     c->cap = n;
     return CPT_OK;
+}
+
+WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint32_t view) {
+    WfArgs a;
+    a.sc = dev_scene(c);
+    a.mul_p = c->mul_p;
+    a.col = c->col;
+    a.rng = c->rng;
+    a.meta = c->meta;
+    a.pix = c->pix;
+    a.vid = c->vid;
+    a.done = c->done;
+    a.qo[0] = c->qo[0];
+    a.qo[1] = c->qo[1];
+    a.qd[0] = c->qd[0];
+    a.qd[1] = c->qd[1];
+    a.hits = c->hits;
+    a.so = c->so;
+    a.sd = c->sd;
+    a.sc_rgb = c->sc_rgb;
+    a.counters = c->counters;
+    a.stats = c->stats;
+    a.gstack = c->gstack;
+    a.pass = c->d_pass;
+    a.film = c->film;
+    a.film2 = c->film2;
+    a.W = W;
+    a.H = H;
+    a.S = S;
+    a.max_depth = depth;
+    a.seed = seed;
+    a.view = view;
+    return a;
 }
 
 using KernelVoid = void (*)(WfArgs);
@@ -538,6 +705,7 @@ struct LaunchCfg {
     int B;
     int G;
     bool count;
+    bool stream = false;
 };
 
 // launch helpers dispatching on the block size
@@ -548,7 +716,8 @@ void launch_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) 
 }
 template <int B>
 void launch_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
-    hipLaunchKernelGGL((k_shade<B>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
+    if (L.stream) hipLaunchKernelGGL((k_shade<B, true>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
+    else hipLaunchKernelGGL((k_shade<B, false>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
 }
 template <int B>
 void launch_shadow(const LaunchCfg& L, const WfArgs& a, hipStream_t s) {
@@ -933,33 +1102,7 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     c->film_h = p->height;
     c->film_S = p->spp_axis;
 
-    WfArgs a;
-    a.sc = dev_scene(c);
-    a.mul_p = c->mul_p;
-    a.col = c->col;
-    a.rng = c->rng;
-    a.meta = c->meta;
-    a.pix = c->pix;
-    a.qo[0] = c->qo[0];
-    a.qo[1] = c->qo[1];
-    a.qd[0] = c->qd[0];
-    a.qd[1] = c->qd[1];
-    a.hits = c->hits;
-    a.so = c->so;
-    a.sd = c->sd;
-    a.sc_rgb = c->sc_rgb;
-    a.counters = c->counters;
-    a.stats = c->stats;
-    a.gstack = c->gstack;
-    a.pass = c->d_pass;
-    a.film = c->film;
-    a.film2 = c->film2;
-    a.W = p->width;
-    a.H = p->height;
-    a.S = p->spp_axis;
-    a.max_depth = p->max_depth;
-    a.seed = p->seed;
-    a.view = (uint32_t)p->view;
+    WfArgs a = make_args(c, p->width, p->height, p->spp_axis, p->max_depth, p->seed, (uint32_t)p->view);
     LaunchCfg L{B, G, (p->flags & CPT_RENDER_COUNT) != 0};
 
     if (!(p->flags & CPT_RENDER_ACCUMULATE)) {
@@ -1050,9 +1193,193 @@ int cpt_read_film(cpt_ctx* c, float* rgb, double* sums, double* sumsq) {
 
 int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const int32_t* wh_spp, int32_t max_depth,
                      uint64_t seed, int32_t flags, float* films, double* view_ms, cpt_render_stats* stats) {
-    (void)cameras; (void)wh_spp; (void)max_depth; (void)seed; (void)flags; (void)films; (void)view_ms; (void)stats;
-    if (!c || n_views <= 0) return CPT_E_INVALID;
-    return fail(c, CPT_E_INVALID, "cpt_render_views: not available in this build");
+    if (!c || n_views <= 0 || !cameras || !wh_spp) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (max_depth <= 0 || max_depth > 255) return fail(c, CPT_E_INVALID, "bad max_depth");
+    if (n_views > 65536) return fail(c, CPT_E_LIMIT, "at most 65536 views");
+    HIPCHK(c, hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<ViewDesc> vd(n_views);
+    unsigned long long total = 0, film_px = 0;
+    for (int v = 0; v < n_views; v++) {
+        int W = wh_spp[3 * v], H = wh_spp[3 * v + 1], S = wh_spp[3 * v + 2];
+        if (W <= 0 || H <= 0 || S <= 0 || (int64_t)W * H > (1 << 26) || S > 1024)
+            return fail(c, CPT_E_INVALID, "view %d: bad size %dx%d spp_axis %d", v, W, H, S);
+        std::memcpy(vd[v].cam, cameras + 16 * (size_t)v, 16 * sizeof(float));
+        vd[v].W = W;
+        vd[v].H = H;
+        vd[v].S = S;
+        vd[v].pad = 0;
+        vd[v].path_begin = total;
+        vd[v].n_paths = (unsigned long long)W * H * S * S;
+        vd[v].film_off = film_px;
+        vd[v].pad2 = 0;
+        total += vd[v].n_paths;
+        film_px += (unsigned long long)W * H;
+    }
+    if (film_px >= CPT_NO_PATH) return fail(c, CPT_E_LIMIT, "films too large");
+    const int B = 128;
+    int G = default_grid(c, B);
+    size_t cap = (size_t)std::min<unsigned long long>(total, 1ull << 22);
+    cap = std::max<size_t>(cap, 64);
+    int r = ensure_wavefront(c, cap);
+    if (r) return r;
+    r = ensure_gstack(c, (size_t)G * B);
+    if (r) return r;
+    if (c->film_px < film_px || !c->film) {
+        drop_graph(c);
+        r = dalloc(c, c->film, 3 * (size_t)film_px);
+        if (r) return r;
+        r = dalloc(c, c->film2, 3 * (size_t)film_px);
+        if (r) return r;
+        c->film_px = film_px;
+    }
+    ViewDesc* d_views = nullptr;
+    unsigned long long* d_sc = nullptr;  // next_path, t_start, view_done[n], view_tend[n]
+    HIPCHK(c, hipMalloc((void**)&d_views, sizeof(ViewDesc) * n_views));
+    if (hipMalloc((void**)&d_sc, sizeof(unsigned long long) * (2 + 2 * (size_t)n_views)) != hipSuccess) {
+        (void)hipFree(d_views);
+        return fail(c, CPT_E_HIP, "hipMalloc failed");
+    }
+    unsigned long long* h_poll = nullptr;
+    if (hipHostMalloc((void**)&h_poll, 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        (void)hipFree(d_views);
+        (void)hipFree(d_sc);
+        return fail(c, CPT_E_HIP, "hipHostMalloc failed");
+    }
+    int rc = CPT_OK;
+    hipGraphExec_t gx = nullptr;
+    do {
+        if (hipMemcpyAsync(d_views, vd.data(), sizeof(ViewDesc) * n_views, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->film, 0, 3 * sizeof(double) * film_px, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->film2, 0, 3 * sizeof(double) * film_px, c->stream) != hipSuccess) {
+            rc = fail(c, CPT_E_HIP, "view upload failed");
+            break;
+        }
+        StreamArgs sa;
+        sa.views = d_views;
+        sa.n_views = n_views;
+        sa.total = total;
+        sa.next_path = d_sc;
+        sa.t_start = d_sc + 1;
+        sa.view_done = d_sc + 2;
+        sa.view_tend = d_sc + 2 + n_views;
+        WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
+        LaunchCfg L{B, G, false, true};
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
+        hipLaunchKernelGGL(k_stream_init, dim3(G), dim3(B), 0, c->stream, a, sa, (unsigned)cap);
+        hipLaunchKernelGGL((k_regen<B>), dim3(G), dim3(B), 0, c->stream, a, sa, 0);
+        // one chunk = 8 bounce iterations (even, so the live queue is 0 again afterwards)
+        const int K = 8;
+        auto chunk = [&](hipStream_t st) {
+            for (int it = 0; it < K; it++) {
+                int cur = it & 1;
+                L_extend(L, a, cur, st);
+                L_shade(L, a, cur, max_depth, st);
+                L_shadow(L, a, st);
+                hipLaunchKernelGGL((k_regen<B>), dim3(G), dim3(B), 0, st, a, sa, cur ^ 1);
+            }
+        };
+        bool use_graph = !(flags & CPT_RENDER_NO_GRAPH);
+        if (use_graph) {
+            hipGraph_t g;
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                rc = fail(c, CPT_E_HIP, "capture begin failed");
+                break;
+            }
+            chunk(c->stream);
+            hipError_t ec = hipStreamEndCapture(c->stream, &g);
+            if (ec != hipSuccess) {
+                rc = fail(c, CPT_E_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+                break;
+            }
+            hipError_t ei = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                rc = fail(c, CPT_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
+                break;
+            }
+        }
+        // Every path ends within max_depth bounces after it starts and a new
+        // path starts whenever one ends, so the loop is bounded by
+        // (total / cap + 2) * max_depth iterations.
+        unsigned long long max_chunks = (total / cap + 2) * (unsigned long long)max_depth / K + 2;
+        bool finished = false;
+        for (unsigned long long ch = 0; ch < max_chunks; ch++) {
+            if (use_graph) {
+                if (hipGraphLaunch(gx, c->stream) != hipSuccess) {
+                    rc = fail(c, CPT_E_HIP, "graph launch failed");
+                    break;
+                }
+            } else {
+                chunk(c->stream);
+            }
+            if (hipMemcpyAsync(h_poll, d_sc, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipMemcpyAsync(h_poll + 1, c->counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = fail(c, CPT_E_HIP, "progress poll failed");
+                break;
+            }
+            const uint32_t* cnt = reinterpret_cast<const uint32_t*>(h_poll + 1);
+            if (h_poll[0] >= total && cnt[0] == 0) {
+                finished = true;
+                break;
+            }
+        }
+        if (rc) break;
+        if (!finished) {
+            rc = fail(c, CPT_E_INVALID, "multi-view render did not drain its queues");
+            break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            rc = fail(c, CPT_E_HIP, "kernel failure: %s", hipGetErrorString(e));
+            break;
+        }
+        std::vector<unsigned long long> hs(2 + 2 * (size_t)n_views);
+        unsigned long long st[ST_N];
+        std::vector<double> h(3 * (size_t)film_px);
+        if (hipMemcpyAsync(hs.data(), d_sc, hs.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipMemcpyAsync(h.data(), c->film, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = fail(c, CPT_E_HIP, "result download failed");
+            break;
+        }
+        for (int v = 0; v < n_views; v++) {
+            if (hs[2 + v] != vd[v].n_paths) {
+                rc = fail(c, CPT_E_INVALID, "view %d accumulated %llu of %llu paths", v, hs[2 + v], vd[v].n_paths);
+                break;
+            }
+            if (view_ms) view_ms[v] = (double)(hs[2 + n_views + v] - hs[1]) / 100e3;  // 100 MHz clock
+            if (films) {
+                double inv = 1.0 / ((double)vd[v].S * vd[v].S);
+                for (size_t k = 0; k < 3 * (size_t)vd[v].W * vd[v].H; k++)
+                    films[3 * vd[v].film_off + k] = (float)(h[3 * vd[v].film_off + k] * inv);
+            }
+        }
+        if (rc) break;
+        if (stats) {
+            std::memset(stats, 0, sizeof(*stats));
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            stats->paths = (int64_t)total;
+            stats->extension_rays = (int64_t)st[ST_EXT];
+            stats->shadow_rays = (int64_t)st[ST_SHADOW];
+            stats->paths_per_pass = (int32_t)cap;
+            stats->block_size = B;
+            stats->grid_blocks = G;
+        }
+    } while (false);
+    (void)hipStreamSynchronize(c->stream);
+    if (gx) (void)hipGraphExecDestroy(gx);
+    (void)hipFree(d_views);
+    (void)hipFree(d_sc);
+    (void)hipHostFree(h_poll);
+    // the film now holds the batch; a later cpt_read_film reads view 0's layout only
+    c->film_w = vd[0].W;
+    c->film_h = vd[0].H;
+    c->film_S = vd[0].S;
+    return rc;
 }
 
 }  // extern "C"
