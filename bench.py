@@ -45,7 +45,71 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--workload", choices=["terrain", "views"], default="terrain",
+                    help="terrain: the headline 1080p line; views: 64-view batch of the box room")
+    ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (scales HBM counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """Plain copy kernel of known size: under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    it reads and writes mib MiB per dispatch, which scales the counters."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, device="cuda", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "checksum": float(y[:16].sum())}))
+
+
+def bench_views(args):
+    """64 camera views of the box room, resolutions from {256^2, 512x384, 1024x768,
+    1920x1080}, spp from {16, 64, 256}, rendered as one streaming batch."""
+    import numpy as np
+
+    import cpt
+
+    rng = np.random.default_rng(7)
+    sizes = [(256, 256), (512, 384), (1024, 768), (1920, 1080)]
+    cams, whs = [], []
+    for v in range(args.views):
+        w, h = sizes[rng.integers(4)]
+        sa = int(rng.choice([4, 8, 16]))
+        ang = rng.uniform(-0.35, 0.35)
+        o = (3.9 * np.sin(ang), rng.uniform(-0.25, 0.25), 3.9 * np.cos(ang))
+        cams.append(cpt.scenes.camera_frame(o, (0, 0, 0), (0, 1, 0), 0.35411, w, h))
+        whs.append((w, h, sa))
+    cams, whs = np.stack(cams), np.array(whs, np.int32)
+    ctx = cpt.Context(0)
+    ctx.upload(cpt.scenes.box_room())
+    ctx.render_views(cams[:2], np.minimum(whs[:2], [256, 256, 2]), max_depth=args.depth)  # warm-up
+    t0 = time.perf_counter()
+    _, ms, st = ctx.render_views(cams, whs, max_depth=args.depth, seed=1)
+    total = time.perf_counter() - t0
+    out = {
+        "metric": "aggregate samples/s, multi-view batch",
+        "value": st["paths"] / total,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "data": "synthetic",
+        "dtype": "f32",
+        "config": {"workload": f"{args.views} views of the 36-triangle box room, uneven resolution/spp, "
+                               f"max depth {args.depth}, one streaming batch",
+                   "paths": st["paths"]},
+        "seconds": total,
+        "mrays_per_s": (st["extension_rays"] + st["shadow_rays"]) / total / 1e6,
+        "view_ms_p50": float(np.percentile(ms, 50)),
+        "view_ms_p99": float(np.percentile(ms, 99)),
+        "view_ms_max": float(ms.max()),
+    }
+    ctx.close()
+    print(json.dumps(out))
 
 
 def cpu_baseline(scene, W, H, depth, target_s):
@@ -84,6 +148,12 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "views":
+        bench_views(args)
+        return
     import cpt
 
     W, H, S, depth = args.width, args.height, args.spp_axis, args.depth

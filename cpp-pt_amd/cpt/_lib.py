@@ -26,6 +26,7 @@ CPT_RENDER_NO_GRAPH = 1
 CPT_RENDER_KERNEL_TIMING = 2
 CPT_RENDER_COUNT = 4
 CPT_RENDER_ACCUMULATE = 8
+CPT_RENDER_STREAM = 16
 
 # every function include/cpt.h declares
 ABI_FUNCTIONS = (

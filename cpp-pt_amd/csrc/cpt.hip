@@ -1037,6 +1037,11 @@ int cpt_trace_closest_device(cpt_ctx* c, const void* d_rays, int32_t n, void* d_
 int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) {
     if (!c || !p) return CPT_E_INVALID;
     if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (p->flags & CPT_RENDER_STREAM) {
+        int32_t whs[3] = {p->width, p->height, p->spp_axis};
+        return cpt_render_views(c, 1, c->cam, whs, p->max_depth, p->seed, p->flags & CPT_RENDER_NO_GRAPH, nullptr,
+                                nullptr, stats);
+    }
     if (p->width <= 0 || p->height <= 0 || p->spp_axis <= 0 || p->max_depth <= 0 || p->max_depth > 255)
         return fail(c, CPT_E_INVALID, "bad render size/spp/depth");
     int64_t P = (int64_t)p->width * p->height;

@@ -102,6 +102,8 @@ typedef struct {
 #define CPT_RENDER_KERNEL_TIMING 2  /* hipEvents around every kernel (implies NO_GRAPH) */
 #define CPT_RENDER_COUNT 4          /* count BVH node visits and triangle tests */
 #define CPT_RENDER_ACCUMULATE 8     /* add to the film instead of clearing it first */
+#define CPT_RENDER_STREAM 16        /* streaming wavefront (path regeneration) instead of passes;
+                                       the film is summed with FP64 atomics (order not fixed) */
 
 typedef struct {
     double seconds;           /* wall time of the render call (device synchronised) */
