@@ -100,7 +100,7 @@ def bench_views(args):
         "data": "synthetic",
         "dtype": "f32",
         "config": {"workload": f"{args.views} views of the 36-triangle box room, uneven resolution/spp, "
-                               f"max depth {args.depth}, one streaming batch",
+                               f"max depth {args.depth}, one batch of multi-view passes",
                    "paths": st["paths"]},
         "seconds": total,
         "mrays_per_s": (st["extension_rays"] + st["shadow_rays"]) / total / 1e6,

@@ -35,13 +35,28 @@ using namespace cpt;
 
 namespace {
 
+// A camera view of a batch (cpt_render renders a batch of one view).
+struct ViewDesc {
+    float cam[16];
+    int W, H, S;
+    unsigned id;                     // mixed into the path id: (id << 40) + pixel*S*S + sample
+    unsigned long long path_begin;   // streaming mode: first global path index of the view
+    unsigned long long n_paths;      // W*H*S*S
+    unsigned long long film_off;     // first pixel of the view in the concatenated films
+    unsigned long long pad;
+};
+
+// One pass = up to CPT_MAX_SEG rectangles (view, pixel range x sample range).
+// Slots of a segment: slot0 + sample_local*npix + pixel_local.  A pass never
+// holds two segments of one view that share pixels, so film accumulation is
+// race-free and adds every pixel's samples in ascending sample order.
+#define CPT_MAX_SEG 256
+struct Segment {
+    uint32_t view, p0, npix, s0, ns, slot0, pix0, pad;
+};
 struct PassDesc {
-    uint32_t p0;      // first pixel
-    uint32_t npix;    // pixels in the pass
-    uint32_t s0;      // first sample index
-    uint32_t ns;      // samples per pixel in the pass
-    uint32_t n_paths; // npix * ns
-    uint32_t pad[3];
+    uint32_t n_seg, n_paths, n_pix, pass_id;
+    Segment seg[CPT_MAX_SEG];
 };
 
 enum {
@@ -74,6 +89,8 @@ struct WfArgs {
     unsigned long long* stats;
     int* gstack;
     const PassDesc* pass;
+    const ViewDesc* views;
+    unsigned long long* pass_tend;  // s_memrealtime at the end of each pass
     double* film;      // per-pixel radiance sums
     double* film2;     // per-pixel sums of squared radiance
     int W, H, S, max_depth;
@@ -104,36 +121,51 @@ __device__ __forceinline__ unsigned wave_sum(unsigned v) {
 }
 
 // ------------------------------------------------------------------ raygen
+__device__ __forceinline__ int find_segment(const PassDesc* pd, unsigned i, bool by_pixel) {
+    int lo = 0, hi = (int)pd->n_seg - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        unsigned start = by_pixel ? pd->seg[mid].pix0 : pd->seg[mid].slot0;
+        if (start <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ void k_raygen(WfArgs a) {
-    const PassDesc pd = *a.pass;
+    const PassDesc* pd = a.pass;
+    const unsigned n_paths = pd->n_paths;
     if (gtid() == 0) {
-        a.counters[0] = pd.n_paths;
+        a.counters[0] = n_paths;
         a.counters[1] = 0;
         a.counters[2] = 0;
-        atomicAdd(&a.stats[ST_PATHS], (unsigned long long)pd.n_paths);
+        a.counters[3] = 0;
+        atomicAdd(&a.stats[ST_PATHS], (unsigned long long)n_paths);
     }
-    const int S = a.S, S2 = a.S * a.S;
-    for (unsigned i = gtid(); i < pd.n_paths; i += gthreads()) {
-        unsigned sl = i / pd.npix, pl = i - sl * pd.npix;
-        unsigned pixel = pd.p0 + pl;
-        unsigned sample = pd.s0 + sl;
-        int px = (int)(pixel % (unsigned)a.W), py = (int)(pixel / (unsigned)a.W);
+    for (unsigned i = gtid(); i < n_paths; i += gthreads()) {
+        const Segment sg = pd->seg[find_segment(pd, i, false)];
+        const ViewDesc& vd = a.views[sg.view];
+        const int S = vd.S;
+        unsigned local = i - sg.slot0;
+        unsigned sl = local / sg.npix, pl = local - sl * sg.npix;
+        unsigned pixel = sg.p0 + pl;
+        unsigned sample = sg.s0 + sl;
+        int px = (int)(pixel % (unsigned)vd.W), py = (int)(pixel / (unsigned)vd.W);
         int k = (int)sample / S, l = (int)sample % S;
-        uint64_t path_id = ((uint64_t)a.view << 40) + (uint64_t)pixel * (uint64_t)S2 + sample;
+        uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)(S * S) + sample;
         uint64_t rng = pcg_init(a.seed, path_id);
         float r1 = pcg_float(rng);
         float r2 = pcg_float(rng);
         float cx, cy;
-        film_coords(px, py, k, l, S, a.W, a.H, r1, r2, cx, cy);
+        film_coords(px, py, k, l, S, vd.W, vd.H, r1, r2, cx, cy);
         V3 o, d;
-        camera_ray(a.sc.cam, cx, cy, o, d);
+        camera_ray(vd.cam, cx, cy, o, d);
         a.qo[0][i] = make_float4(o.x, o.y, o.z, INFINITY);
         a.qd[0][i] = make_float4(d.x, d.y, d.z, __int_as_float((int)i));
         a.mul_p[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
         a.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.rng[i] = rng;
         a.meta[i] = 1u << 8;  // depth 0, sampled_delta = true (camera ray)
-        a.pix[i] = pixel;
     }
 }
 
@@ -320,15 +352,17 @@ __global__ __launch_bounds__(B) void k_shadow(WfArgs a) {
 
 // ------------------------------------------------------------------ film
 __global__ void k_accum(WfArgs a) {
-    const PassDesc pd = *a.pass;
-    for (unsigned pl = gtid(); pl < pd.npix; pl += gthreads()) {
-        unsigned pixel = pd.p0 + pl;
-        double* f = a.film + 3 * (size_t)pixel;
-        double* f2 = a.film2 + 3 * (size_t)pixel;
+    const PassDesc* pd = a.pass;
+    for (unsigned w = gtid(); w < pd->n_pix; w += gthreads()) {
+        const Segment sg = pd->seg[find_segment(pd, w, true)];
+        unsigned pl = w - sg.pix0;
+        size_t fp = (size_t)a.views[sg.view].film_off + sg.p0 + pl;
+        double* f = a.film + 3 * fp;
+        double* f2 = a.film2 + 3 * fp;
         double x = f[0], y = f[1], z = f[2];
         double x2 = f2[0], y2 = f2[1], z2 = f2[2];
-        for (unsigned s = 0; s < pd.ns; s++) {
-            float4 c = a.col[(size_t)s * pd.npix + pl];
+        for (unsigned s = 0; s < sg.ns; s++) {
+            float4 c = a.col[(size_t)sg.slot0 + (size_t)s * sg.npix + pl];
             x += (double)c.x;
             y += (double)c.y;
             z += (double)c.z;
@@ -345,21 +379,16 @@ __global__ void k_accum(WfArgs a) {
     }
 }
 
+// timestamp (100 MHz constant clock) after the pass's last kernel
+__global__ void k_stamp(WfArgs a, int fixed_slot) {
+    if (gtid() == 0) a.pass_tend[fixed_slot >= 0 ? (unsigned)fixed_slot : a.pass->pass_id] = __builtin_amdgcn_s_memrealtime();
+}
 
 // ------------------------------------------------------------------ streaming (multi-view) mode
 // Paths of all views form one index space [0, total): view v owns
 // [path_begin, path_begin + W*H*S*S), sample-major within the view.  A slot
 // whose path ended is refilled with the next path index by k_regen, so the
 // queues stay full until the last path of the last view has been started.
-struct ViewDesc {
-    float cam[16];
-    int W, H, S, pad;
-    unsigned long long path_begin;
-    unsigned long long n_paths;
-    unsigned long long film_off;
-    unsigned long long pad2;
-};
-
 struct StreamArgs {
     const ViewDesc* views;
     int n_views;
@@ -443,7 +472,7 @@ __global__ __launch_bounds__(B) void k_regen(WfArgs a, StreamArgs sa, int nxt) {
             unsigned pixel = (unsigned)(local - (unsigned long long)sample * P);
             int S = vd.S;
             int px = (int)(pixel % (unsigned)vd.W), py = (int)(pixel / (unsigned)vd.W);
-            uint64_t path_id = ((uint64_t)lo << 40) + (uint64_t)pixel * (uint64_t)(S * S) + sample;
+            uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)(S * S) + sample;
             uint64_t rng = pcg_init(a.seed, path_id);
             float r1 = pcg_float(rng);
             float r2 = pcg_float(rng);
@@ -535,6 +564,10 @@ struct cpt_ctx {
     PassDesc* d_pass = nullptr;
     PassDesc* h_pass = nullptr;  // pinned
     size_t h_pass_cap = 0;
+    ViewDesc* d_views = nullptr;
+    size_t views_cap = 0;
+    unsigned long long* pass_tend = nullptr;
+    size_t tend_cap = 0;
     double* film = nullptr;
     double* film2 = nullptr;
     size_t film_px = 0;
@@ -688,6 +721,8 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.stats = c->stats;
     a.gstack = c->gstack;
     a.pass = c->d_pass;
+    a.views = c->d_views;
+    a.pass_tend = c->pass_tend;
     a.film = c->film;
     a.film2 = c->film2;
     a.W = W;
@@ -775,6 +810,7 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
         mark(2);
     }
     hipLaunchKernelGGL(k_accum, dim3(L.G), dim3(L.B), 0, s, a);
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, a, -1);
     mark(3);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, CPT_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
@@ -834,6 +870,8 @@ void cpt_destroy(cpt_ctx* c) {
     dfree(c->counters);
     dfree(c->stats);
     dfree(c->d_pass);
+    dfree(c->d_views);
+    dfree(c->pass_tend);
     dfree(c->film);
     dfree(c->film2);
     if (c->h_pass) (void)hipHostFree(c->h_pass);
@@ -1034,95 +1072,150 @@ int cpt_trace_closest_device(cpt_ctx* c, const void* d_rays, int32_t n, void* d_
     return CPT_OK;
 }
 
-int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) {
-    if (!c || !p) return CPT_E_INVALID;
-    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
-    if (p->flags & CPT_RENDER_STREAM) {
-        int32_t whs[3] = {p->width, p->height, p->spp_axis};
-        return cpt_render_views(c, 1, c->cam, whs, p->max_depth, p->seed, p->flags & CPT_RENDER_NO_GRAPH, nullptr,
-                                nullptr, stats);
-    }
-    if (p->width <= 0 || p->height <= 0 || p->spp_axis <= 0 || p->max_depth <= 0 || p->max_depth > 255)
-        return fail(c, CPT_E_INVALID, "bad render size/spp/depth");
-    int64_t P = (int64_t)p->width * p->height;
-    int64_t S2 = (int64_t)p->spp_axis * p->spp_axis;
-    if (P > (1 << 30) || S2 > (1 << 20) || P * S2 >= (1ll << 40)) return fail(c, CPT_E_LIMIT, "film or spp too large");
-    HIPCHK(c, hipSetDevice(c->device));
-    auto t0 = std::chrono::steady_clock::now();
+static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras, const int32_t* wh_spp,
+                               int32_t max_depth, uint64_t seed, int32_t flags, float* films, double* view_ms,
+                               cpt_render_stats* stats);
 
-    int B = p->block_size ? p->block_size : 128;
-    if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
-    int G = p->grid_blocks ? p->grid_blocks : default_grid(c, B);
-    int64_t want = p->max_paths ? p->max_paths : (1 << 22);
-    want = std::max<int64_t>(want, 1024);
-    // pass plan
-    std::vector<PassDesc> passes;
-    if (P >= want) {
-        for (int64_t s = 0; s < S2; s++)
-            for (int64_t p0 = 0; p0 < P; p0 += want) {
-                PassDesc pd{};
-                pd.p0 = (uint32_t)p0;
-                pd.npix = (uint32_t)std::min<int64_t>(want, P - p0);
-                pd.s0 = (uint32_t)s;
-                pd.ns = 1;
-                pd.n_paths = pd.npix;
-                passes.push_back(pd);
+// Pack views into passes of at most `cap` paths (see PassDesc).
+static void plan_passes(const std::vector<ViewDesc>& vd, int64_t cap, std::vector<PassDesc>& passes,
+                        std::vector<int>& last_pass_of_view) {
+    passes.clear();
+    last_pass_of_view.assign(vd.size(), 0);
+    auto open_pass = [&]() {
+        passes.emplace_back();
+        PassDesc& pd = passes.back();
+        std::memset(&pd, 0, sizeof(PassDesc));
+        pd.pass_id = (uint32_t)(passes.size() - 1);
+    };
+    open_pass();
+    auto add = [&](uint32_t v, uint32_t p0, uint32_t npix, uint32_t s0, uint32_t ns) {
+        PassDesc& pd = passes.back();
+        Segment& sg = pd.seg[pd.n_seg++];
+        sg.view = v;
+        sg.p0 = p0;
+        sg.npix = npix;
+        sg.s0 = s0;
+        sg.ns = ns;
+        sg.slot0 = pd.n_paths;
+        sg.pix0 = pd.n_pix;
+        sg.pad = 0;
+        pd.n_paths += npix * ns;
+        pd.n_pix += npix;
+        last_pass_of_view[v] = (int)passes.size() - 1;
+    };
+    for (size_t v = 0; v < vd.size(); v++) {
+        int64_t P = (int64_t)vd[v].W * vd[v].H;
+        int64_t S2 = (int64_t)vd[v].S * vd[v].S;
+        if (P <= cap) {
+            int64_t s0 = 0;
+            while (s0 < S2) {
+                PassDesc* pd = &passes.back();
+                int64_t room = cap - pd->n_paths;
+                int64_t take = std::min<int64_t>(S2 - s0, room / P);
+                if (take <= 0 || pd->n_seg == CPT_MAX_SEG) {
+                    open_pass();
+                    continue;
+                }
+                add((uint32_t)v, 0, (uint32_t)P, (uint32_t)s0, (uint32_t)take);
+                s0 += take;
             }
-    } else {
-        int64_t ns = std::max<int64_t>(1, std::min<int64_t>(S2, want / P));
-        for (int64_t s0 = 0; s0 < S2; s0 += ns) {
-            PassDesc pd{};
-            pd.p0 = 0;
-            pd.npix = (uint32_t)P;
-            pd.s0 = (uint32_t)s0;
-            pd.ns = (uint32_t)std::min<int64_t>(ns, S2 - s0);
-            pd.n_paths = pd.npix * pd.ns;
-            passes.push_back(pd);
+        } else {
+            for (int64_t s = 0; s < S2; s++) {
+                int64_t p0 = 0;
+                while (p0 < P) {
+                    PassDesc* pd = &passes.back();
+                    int64_t room = cap - pd->n_paths;
+                    if (room <= 0 || pd->n_seg == CPT_MAX_SEG) {
+                        open_pass();
+                        continue;
+                    }
+                    int64_t n = std::min<int64_t>(room, P - p0);
+                    add((uint32_t)v, (uint32_t)p0, (uint32_t)n, (uint32_t)s, 1);
+                    p0 += n;
+                }
+            }
         }
     }
+    if (passes.back().n_seg == 0) passes.pop_back();
+}
+
+// Render a batch of views into the concatenated FP64 films.
+static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, uint64_t seed, int flags, int B, int G,
+                        int64_t max_paths, double* view_ms, cpt_render_stats* stats) {
+    if (max_depth <= 0 || max_depth > 255) return fail(c, CPT_E_INVALID, "max_depth must be in [1, 255]");
+    if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
+    if (G <= 0) return fail(c, CPT_E_INVALID, "grid_blocks must be positive");
+    unsigned long long film_px = 0;
+    for (size_t v = 0; v < vd.size(); v++) {
+        if (vd[v].W <= 0 || vd[v].H <= 0 || vd[v].S <= 0) return fail(c, CPT_E_INVALID, "view %zu: bad size or spp", v);
+        int64_t P = (int64_t)vd[v].W * vd[v].H, S2 = (int64_t)vd[v].S * vd[v].S;
+        if (P > (1 << 28) || S2 > (1 << 20) || P * S2 >= (1ll << 40)) return fail(c, CPT_E_LIMIT, "view %zu too large", v);
+        vd[v].n_paths = (unsigned long long)(P * S2);
+        vd[v].film_off = film_px;
+        vd[v].path_begin = 0;
+        vd[v].pad = 0;
+        film_px += (unsigned long long)P;
+    }
+    if (film_px >= (1ull << 31)) return fail(c, CPT_E_LIMIT, "films too large");
+    HIPCHK(c, hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    int64_t cap = std::max<int64_t>(max_paths, 1024);
+    cap = std::min<int64_t>(cap, 1ll << 30);
+    std::vector<PassDesc> passes;
+    std::vector<int> last_pass;
+    plan_passes(vd, cap, passes, last_pass);
     size_t slots = 0;
     for (auto& pd : passes) slots = std::max<size_t>(slots, pd.n_paths);
     int r = ensure_wavefront(c, slots);
     if (r) return r;
     r = ensure_gstack(c, (size_t)G * B);
     if (r) return r;
-    if (c->film_px < (size_t)P || !c->film) {
+    if (c->film_px < film_px || !c->film) {
         drop_graph(c);
-        r = dalloc(c, c->film, 3 * (size_t)P);
+        r = dalloc(c, c->film, 3 * (size_t)film_px);
         if (r) return r;
-        r = dalloc(c, c->film2, 3 * (size_t)P);
+        r = dalloc(c, c->film2, 3 * (size_t)film_px);
         if (r) return r;
-        c->film_px = (size_t)P;
+        c->film_px = film_px;
     }
+    if (c->views_cap < vd.size() || !c->d_views) {
+        drop_graph(c);
+        r = dalloc(c, c->d_views, vd.size());
+        if (r) return r;
+        c->views_cap = vd.size();
+    }
+    if (c->tend_cap < passes.size() + 1 || !c->pass_tend) {
+        drop_graph(c);
+        r = dalloc(c, c->pass_tend, passes.size() + 1);
+        if (r) return r;
+        c->tend_cap = passes.size() + 1;
+    }
+    // the previous render's copies from the pinned pass array must be done before it is rewritten
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->h_pass_cap < passes.size()) {
         if (c->h_pass) (void)hipHostFree(c->h_pass);
         c->h_pass = nullptr;
+        c->h_pass_cap = 0;
         HIPCHK(c, hipHostMalloc((void**)&c->h_pass, passes.size() * sizeof(PassDesc), hipHostMallocDefault));
         c->h_pass_cap = passes.size();
     }
-    // the previous render's pass copies must have completed before the pinned buffer is rewritten
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     std::memcpy(c->h_pass, passes.data(), passes.size() * sizeof(PassDesc));
-    c->film_w = p->width;
-    c->film_h = p->height;
-    c->film_S = p->spp_axis;
-
-    WfArgs a = make_args(c, p->width, p->height, p->spp_axis, p->max_depth, p->seed, (uint32_t)p->view);
-    LaunchCfg L{B, G, (p->flags & CPT_RENDER_COUNT) != 0};
-
-    if (!(p->flags & CPT_RENDER_ACCUMULATE)) {
-        HIPCHK(c, hipMemsetAsync(c->film, 0, 3 * sizeof(double) * (size_t)P, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->film2, 0, 3 * sizeof(double) * (size_t)P, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_views, vd.data(), vd.size() * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+    if (!(flags & CPT_RENDER_ACCUMULATE)) {
+        HIPCHK(c, hipMemsetAsync(c->film, 0, 3 * sizeof(double) * film_px, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->film2, 0, 3 * sizeof(double) * film_px, c->stream));
     }
+    WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
+    LaunchCfg L{B, G, (flags & CPT_RENDER_COUNT) != 0};
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
-
-    bool timing = (p->flags & CPT_RENDER_KERNEL_TIMING) != 0;
-    bool use_graph = !(p->flags & (CPT_RENDER_NO_GRAPH | CPT_RENDER_KERNEL_TIMING));
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, a, (int)passes.size());
+    bool timing = (flags & CPT_RENDER_KERNEL_TIMING) != 0;
+    bool use_graph = !(flags & (CPT_RENDER_NO_GRAPH | CPT_RENDER_KERNEL_TIMING));
     Timers tm;
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof(key), "%d,%d,%d,%d,%llu,%u,%d,%d,%d,%zu,%p", p->width, p->height, p->spp_axis, p->max_depth,
-                 (unsigned long long)p->seed, a.view, B, G, (int)L.count, c->cap, (void*)c->d_nodes);
+        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%zu,%p,%p,%p,%p", max_depth, (unsigned long long)seed, B, G,
+                 (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film, (void*)c->d_views, (void*)c->pass_tend);
         if (!c->gexec || c->graph_key != key) {
             drop_graph(c);
             hipGraph_t g;
@@ -1147,9 +1240,15 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
         }
     }
     unsigned long long st[ST_N];
+    std::vector<unsigned long long> tend(passes.size() + 1);
     HIPCHK(c, hipMemcpyAsync(st, c->stats, sizeof(st), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(tend.data(), c->pass_tend, tend.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     auto t1 = std::chrono::steady_clock::now();
+    if (view_ms)
+        for (size_t v = 0; v < vd.size(); v++)
+            view_ms[v] = (double)(tend[last_pass[v]] - tend[passes.size()]) / 100e3;  // 100 MHz clock
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1177,7 +1276,31 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
         }
     }
     for (auto e : tm.ev) (void)hipEventDestroy(e);
+    c->film_w = vd[0].W;
+    c->film_h = vd[0].H;
+    c->film_S = vd[0].S;
     return CPT_OK;
+}
+
+int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) {
+    if (!c || !p) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (p->width <= 0 || p->height <= 0 || p->spp_axis <= 0)
+        return fail(c, CPT_E_INVALID, "bad render size or spp");
+    if (p->flags & CPT_RENDER_STREAM) {
+        int32_t whs[3] = {p->width, p->height, p->spp_axis};
+        return render_views_stream(c, 1, c->cam, whs, p->max_depth, p->seed, p->flags, nullptr, nullptr, stats);
+    }
+    std::vector<ViewDesc> vd(1);
+    std::memcpy(vd[0].cam, c->cam, sizeof(c->cam));
+    vd[0].W = p->width;
+    vd[0].H = p->height;
+    vd[0].S = p->spp_axis;
+    vd[0].id = (unsigned)p->view;
+    int B = p->block_size ? p->block_size : 128;
+    int G = p->grid_blocks ? p->grid_blocks : default_grid(c, B);
+    int64_t cap = p->max_paths ? p->max_paths : (1 << 25);
+    return render_batch(c, vd, p->max_depth, p->seed, p->flags, B, G, cap, nullptr, stats);
 }
 
 int cpt_read_film(cpt_ctx* c, float* rgb, double* sums, double* sumsq) {
@@ -1196,8 +1319,9 @@ int cpt_read_film(cpt_ctx* c, float* rgb, double* sums, double* sumsq) {
     return CPT_OK;
 }
 
-int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const int32_t* wh_spp, int32_t max_depth,
-                     uint64_t seed, int32_t flags, float* films, double* view_ms, cpt_render_stats* stats) {
+static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras, const int32_t* wh_spp,
+                               int32_t max_depth, uint64_t seed, int32_t flags, float* films, double* view_ms,
+                               cpt_render_stats* stats) {
     if (!c || n_views <= 0 || !cameras || !wh_spp) return CPT_E_INVALID;
     if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
     if (max_depth <= 0 || max_depth > 255) return fail(c, CPT_E_INVALID, "bad max_depth");
@@ -1214,11 +1338,11 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
         vd[v].W = W;
         vd[v].H = H;
         vd[v].S = S;
+        vd[v].id = (unsigned)v;
         vd[v].pad = 0;
         vd[v].path_begin = total;
         vd[v].n_paths = (unsigned long long)W * H * S * S;
         vd[v].film_off = film_px;
-        vd[v].pad2 = 0;
         total += vd[v].n_paths;
         film_px += (unsigned long long)W * H;
     }
@@ -1385,6 +1509,39 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
     c->film_h = vd[0].H;
     c->film_S = vd[0].S;
     return rc;
+}
+
+int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const int32_t* wh_spp, int32_t max_depth,
+                     uint64_t seed, int32_t flags, float* films, double* view_ms, cpt_render_stats* stats) {
+    if (!c || n_views <= 0 || !cameras || !wh_spp) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (n_views > (1 << 20)) return fail(c, CPT_E_LIMIT, "too many views");
+    if (flags & CPT_RENDER_STREAM)
+        return render_views_stream(c, n_views, cameras, wh_spp, max_depth, seed, flags, films, view_ms, stats);
+    std::vector<ViewDesc> vd(n_views);
+    for (int v = 0; v < n_views; v++) {
+        std::memcpy(vd[v].cam, cameras + 16 * (size_t)v, 16 * sizeof(float));
+        vd[v].W = wh_spp[3 * v];
+        vd[v].H = wh_spp[3 * v + 1];
+        vd[v].S = wh_spp[3 * v + 2];
+        vd[v].id = (unsigned)v;
+    }
+    const int B = 128;
+    int r = render_batch(c, vd, max_depth, seed, flags & ~CPT_RENDER_ACCUMULATE, B, default_grid(c, B), 1 << 25, view_ms,
+                         stats);
+    if (r) return r;
+    if (films) {
+        size_t px = 0;
+        for (auto& v : vd) px += (size_t)v.W * v.H;
+        std::vector<double> h(3 * px);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->film, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (auto& v : vd) {
+            double inv = 1.0 / ((double)v.S * v.S);
+            for (size_t k = 0; k < 3 * (size_t)v.W * v.H; k++) films[3 * v.film_off + k] = (float)(h[3 * v.film_off + k] * inv);
+        }
+    }
+    return CPT_OK;
 }
 
 }  // extern "C"
