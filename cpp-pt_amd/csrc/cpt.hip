@@ -30,6 +30,7 @@
 #include "bvh_build.h"
 #include "pt_device.h"
 #include "pt_shade.h"
+#include "guide_mlp.h"
 
 using namespace cpt;
 
@@ -573,6 +574,23 @@ struct cpt_ctx {
     size_t film_px = 0;
     int film_w = 0, film_h = 0, film_S = 1;
 
+    // learned guide
+    bool has_guide = false;
+    float* g_params = nullptr;   // FP32 master weights
+    float* g_m = nullptr;        // Adam moments
+    float* g_v = nullptr;
+    float* g_grad = nullptr;
+    uint16_t* g_wt = nullptr;    // transposed BF16 weights
+    float* g_samples = nullptr;  // training samples (11 floats each)
+    size_t g_samples_cap = 0;
+    uint16_t* g_rows = nullptr;  // per-sample activation / gradient rows
+    size_t g_rows_cap = 0;
+    float* g_partial = nullptr;  // per-block dW partials
+    float* g_loss_part = nullptr;
+    float* g_wscale = nullptr;
+    int g_step = 0;
+    cpt_guide_params g_cfg{};
+
     // captured pass graph
     hipGraphExec_t gexec = nullptr;
     std::string graph_key;
@@ -874,6 +892,16 @@ void cpt_destroy(cpt_ctx* c) {
     dfree(c->pass_tend);
     dfree(c->film);
     dfree(c->film2);
+    dfree(c->g_params);
+    dfree(c->g_m);
+    dfree(c->g_v);
+    dfree(c->g_grad);
+    dfree(c->g_wt);
+    dfree(c->g_samples);
+    dfree(c->g_rows);
+    dfree(c->g_partial);
+    dfree(c->g_loss_part);
+    dfree(c->g_wscale);
     if (c->h_pass) (void)hipHostFree(c->h_pass);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1540,6 +1568,169 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
             double inv = 1.0 / ((double)v.S * v.S);
             for (size_t k = 0; k < 3 * (size_t)v.W * v.H; k++) films[3 * v.film_off + k] = (float)(h[3 * v.film_off + k] * inv);
         }
+    }
+    return CPT_OK;
+}
+
+// ------------------------------------------------------------------ learned guide
+static const int GUIDE_TRAIN_BLOCKS = 256;
+static const int GUIDE_DW_BLOCKS = 256;
+
+int cpt_guide_num_params(void) { return guide::NPARAM; }
+
+int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
+    using namespace guide;
+    if (!c || !gp) return CPT_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    int r = 0;
+    if (!c->g_params) {
+        r |= dalloc(c, c->g_params, NPARAM);
+        r |= dalloc(c, c->g_m, NPARAM);
+        r |= dalloc(c, c->g_v, NPARAM);
+        r |= dalloc(c, c->g_grad, NPARAM);
+        r |= dalloc(c, c->g_wt, NWT);
+        r |= dalloc(c, c->g_partial, (size_t)GUIDE_DW_BLOCKS * NPARAM);
+        r |= dalloc(c, c->g_loss_part, GUIDE_TRAIN_BLOCKS);
+        r |= dalloc(c, c->g_wscale, 1);
+        if (r) return CPT_E_HIP;
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_train_tile, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
+    }
+    c->g_cfg = *gp;
+    if (c->g_cfg.lr <= 0) c->g_cfg.lr = 1e-3f;
+    if (c->g_cfg.beta1 <= 0) c->g_cfg.beta1 = 0.9f;
+    if (c->g_cfg.beta2 <= 0) c->g_cfg.beta2 = 0.99f;
+    if (c->g_cfg.eps <= 0) c->g_cfg.eps = 1e-8f;
+    if (c->g_cfg.guide_fraction <= 0 || c->g_cfg.guide_fraction >= 1) c->g_cfg.guide_fraction = 0.5f;
+    // Glorot-uniform weights from a seeded splitmix64 stream; zero biases except the
+    // lobe means, which start at the 8 cube diagonals (a near-uniform mixture)
+    std::vector<float> h(NPARAM, 0.0f);
+    uint64_t st = gp->seed ^ 0x243F6A8885A308D3ull;
+    auto rnd = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (float)((z >> 40) * (1.0 / 16777216.0)) * 2.0f - 1.0f;
+    };
+    const int offs[5] = {W0, W1, W2, W3, W4}, ks[5] = {K0, HID, HID, HID, HID}, ns[5] = {HID, HID, HID, HID, NOUT};
+    for (int l = 0; l < 5; l++) {
+        float a = std::sqrt(6.0f / (float)(ks[l] + ns[l]));
+        if (l == 4) a *= 0.1f;
+        for (int i = 0; i < ks[l] * ns[l]; i++) h[offs[l] + i] = a * rnd();
+    }
+    for (int k = 0; k < NLOBES; k++) {
+        h[B4 + 5 * k + 2] = (k & 1) ? 1.0f : -1.0f;
+        h[B4 + 5 * k + 3] = (k & 2) ? 1.0f : -1.0f;
+        h[B4 + 5 * k + 4] = (k & 4) ? 1.0f : -1.0f;
+    }
+    HIPCHK(c, hipMemcpy(c->g_params, h.data(), NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->g_m, 0, NPARAM * sizeof(float)));
+    HIPCHK(c, hipMemset(c->g_v, 0, NPARAM * sizeof(float)));
+    hipLaunchKernelGGL(k_pack_weights, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
+    hipLaunchKernelGGL(k_pack_weights2, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->g_step = 0;
+    c->has_guide = true;
+    return CPT_OK;
+}
+
+int cpt_guide_get_params(cpt_ctx* c, float* out) {
+    if (!c || !out) return CPT_E_INVALID;
+    if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, c->g_params, guide::NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    return CPT_OK;
+}
+
+int cpt_guide_set_params(cpt_ctx* c, const float* in) {
+    if (!c || !in) return CPT_E_INVALID;
+    if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->g_params, in, guide::NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->g_m, 0, guide::NPARAM * sizeof(float)));
+    HIPCHK(c, hipMemset(c->g_v, 0, guide::NPARAM * sizeof(float)));
+    hipLaunchKernelGGL(guide::k_pack_weights, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
+    hipLaunchKernelGGL(guide::k_pack_weights2, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->g_step = 0;
+    return CPT_OK;
+}
+
+int cpt_guide_forward(cpt_ctx* c, const float* inputs, int32_t n, float* outputs) {
+    using namespace guide;
+    if (!c || n < 0 || (n > 0 && (!inputs || !outputs))) return CPT_E_INVALID;
+    if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
+    if (n == 0) return CPT_OK;
+    float *d_in = nullptr, *d_out = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d_in, sizeof(float) * NIN * (size_t)n));
+    if (hipMalloc((void**)&d_out, sizeof(float) * NOUT * (size_t)n) != hipSuccess) {
+        (void)hipFree(d_in);
+        return fail(c, CPT_E_HIP, "hipMalloc failed");
+    }
+    int rc = CPT_OK;
+    if (hipMemcpyAsync(d_in, inputs, sizeof(float) * NIN * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(c, CPT_E_HIP, "upload failed");
+    if (!rc) {
+        int G = std::min(c->cus, (n + TILE - 1) / TILE);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(G), dim3(256), sizeof(SmemFwd), c->stream, c->g_wt, c->g_params, d_in, n, d_out);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(outputs, d_out, sizeof(float) * NOUT * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(c, CPT_E_HIP, "guide forward failed");
+    }
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+// One Adam step on device-resident samples (n x 11 floats at c->g_samples).
+static int guide_train_device(cpt_ctx* c, int n, bool apply) {
+    using namespace guide;
+    if (c->g_rows_cap < (size_t)n) {
+        int r = dalloc(c, c->g_rows, (size_t)n * ROW);
+        if (r) return r;
+        c->g_rows_cap = n;
+    }
+    int per_block = ((n + GUIDE_DW_BLOCKS - 1) / GUIDE_DW_BLOCKS + CH - 1) / CH * CH;
+    hipLaunchKernelGGL(k_weight_scale, dim3(1), dim3(256), 0, c->stream, c->g_samples, n, c->g_wscale);
+    hipLaunchKernelGGL(k_train_tile, dim3(GUIDE_TRAIN_BLOCKS), dim3(256), sizeof(SmemFwd), c->stream, c->g_wt,
+                       c->g_params, c->g_samples, n, c->g_wscale, c->g_rows, c->g_loss_part);
+    hipLaunchKernelGGL(k_train_dw, dim3(GUIDE_DW_BLOCKS), dim3(256), 0, c->stream, c->g_rows, n, per_block, c->g_partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(64), dim3(256), 0, c->stream, c->g_partial, GUIDE_DW_BLOCKS, NPARAM,
+                       NPARAM, c->g_grad);
+    if (apply) {
+        c->g_step++;
+        hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_m, c->g_v, c->g_grad, c->g_step,
+                           c->g_cfg.lr, c->g_cfg.beta1, c->g_cfg.beta2, c->g_cfg.eps, c->g_wt);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, CPT_E_HIP, "guide training launch failed: %s", hipGetErrorString(e));
+    return CPT_OK;
+}
+
+int cpt_guide_train_step(cpt_ctx* c, const float* samples, int32_t n, int32_t apply, float* loss, float* grad) {
+    using namespace guide;
+    if (!c || n <= 0 || !samples) return CPT_E_INVALID;
+    if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->g_samples_cap < (size_t)n) {
+        int r = dalloc(c, c->g_samples, (size_t)n * SAMPLE_FLOATS);
+        if (r) return r;
+        c->g_samples_cap = n;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->g_samples, samples, sizeof(float) * SAMPLE_FLOATS * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    int r = guide_train_device(c, n, apply != 0);
+    if (r) return r;
+    std::vector<float> lp(GUIDE_TRAIN_BLOCKS);
+    HIPCHK(c, hipMemcpyAsync(lp.data(), c->g_loss_part, sizeof(float) * GUIDE_TRAIN_BLOCKS, hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->g_grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (loss) {
+        double s = 0;
+        for (float v : lp) s += v;
+        *loss = (float)s;
     }
     return CPT_OK;
 }

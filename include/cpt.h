@@ -176,6 +176,40 @@ int cpt_render_views(cpt_ctx* ctx, int32_t n_views, const float* cameras, const 
                      int32_t max_depth, uint64_t seed, int32_t flags, float* films, double* view_ms,
                      cpt_render_stats* stats);
 
+/* ---- learned path guiding -------------------------------------------------
+ * Replaces the SD-tree of the reference's practical-path-guiding renderer
+ * (src/renderer/ppg.h:41-214, src/algorithms/sdtree.h:445-538): a directional
+ * distribution per path vertex, here an MLP (one-blob encoding of position,
+ * normal and outgoing direction, 7 x 32 bins -> 4 x 64 ReLU -> 8 von Mises-Fisher
+ * lobes) with BF16 weights on MFMA and FP32 accumulation.  SDTree::sample/pdf ->
+ * the mixture's sample/pdf inside the guided shade kernel; SDTree::splat/update ->
+ * training samples and cpt_guide_train_step (forward, backward, Adam on the GPU).
+ * Parameters: 30000 floats; layer l weights W_l (K_l x N_l row-major, K = 224, 64,
+ * 64, 64, 64; N = 64, 64, 64, 64, 48) each followed by its bias (N_l). */
+typedef struct {
+    float lr;              /* Adam step size (0 = 1e-3) */
+    float beta1, beta2;    /* 0 = 0.9, 0.99 */
+    float eps;             /* 0 = 1e-8 */
+    uint64_t seed;         /* weight initialisation */
+    float guide_fraction;  /* probability of sampling the guide (0 = 0.5) */
+    int32_t train_paths;   /* guided renders: paths per pass that record training samples (0 = 65536) */
+    int32_t reserved[4];
+} cpt_guide_params;
+
+int cpt_guide_num_params(void);
+/* Allocate (first call) and initialise the guide: Glorot-uniform weights from `seed`. */
+int cpt_guide_init(cpt_ctx* ctx, const cpt_guide_params* params);
+int cpt_guide_get_params(cpt_ctx* ctx, float* params_out);
+int cpt_guide_set_params(cpt_ctx* ctx, const float* params_in);
+/* Raw network outputs (n x 48) for n input vectors (n x 7 floats in [0,1]). */
+int cpt_guide_forward(cpt_ctx* ctx, const float* inputs, int32_t n, float* outputs);
+/* One training step on n samples (n x 11 floats: 7 inputs, direction xyz, weight =
+ * target radiance / sampling pdf).  Loss = sum_i w_i (-log q(dir_i)) / sum_i w_i.
+ * loss_out (may be NULL) = that loss; grad_out (may be NULL, 30000 floats) = its
+ * gradient; apply != 0 also takes the Adam step. */
+int cpt_guide_train_step(cpt_ctx* ctx, const float* samples, int32_t n, int32_t apply, float* loss_out,
+                         float* grad_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,15 @@ def lib():
     L.orc_pcg_floats.restype = None
     L.orc_mt_floats.argtypes = [C.c_uint32, i32, vp]
     L.orc_mt_floats.restype = None
+    L.orc_mlp_num_params.restype = i32
+    L.orc_mlp_forward.argtypes = [vp, vp, i32, i32, vp]
+    L.orc_mlp_forward.restype = None
+    L.orc_mlp_mixture_pdf_batch.argtypes = [vp, vp, i32, vp]
+    L.orc_mlp_mixture_pdf_batch.restype = None
+    L.orc_mlp_loss_grad.argtypes = [vp, vp, i32, i32, vp]
+    L.orc_mlp_loss_grad.restype = C.c_double
+    L.orc_mlp_adam.argtypes = [vp, vp, vp, vp, i32, C.c_double, C.c_double, C.c_double, C.c_double]
+    L.orc_mlp_adam.restype = None
     _lib = L
     return L
 
@@ -210,3 +219,41 @@ def mt_floats(seed, n):
     out = np.zeros(n, np.float32)
     lib().orc_mt_floats(seed, n, _p(out))
     return out
+
+
+# ---------------------------------------------------------------- guide MLP oracle
+def mlp_num_params():
+    return lib().orc_mlp_num_params()
+
+
+def mlp_forward(params, inputs, bf16=True):
+    P = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(inputs, np.float32).reshape(-1, 7)
+    out = np.zeros((X.shape[0], 48), np.float64)
+    lib().orc_mlp_forward(_p(P), _p(X), X.shape[0], 1 if bf16 else 0, _p(out))
+    return out
+
+
+def mlp_mixture_pdf(outputs48, dirs):
+    o = np.ascontiguousarray(outputs48, np.float64).reshape(48)
+    d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    out = np.zeros(d.shape[0], np.float64)
+    lib().orc_mlp_mixture_pdf_batch(_p(o), _p(d), d.shape[0], _p(out))
+    return out
+
+
+def mlp_loss_grad(params, samples, bf16=True):
+    P = np.ascontiguousarray(params, np.float32)
+    S = np.ascontiguousarray(samples, np.float32).reshape(-1, 11)
+    g = np.zeros(mlp_num_params(), np.float64)
+    loss = lib().orc_mlp_loss_grad(_p(P), _p(S), S.shape[0], 1 if bf16 else 0, _p(g))
+    return loss, g
+
+
+def mlp_adam(params, m, v, grad, step, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8):
+    P = np.array(params, np.float32)
+    m = np.array(m, np.float64)
+    v = np.array(v, np.float64)
+    g = np.ascontiguousarray(grad, np.float64)
+    lib().orc_mlp_adam(_p(P), _p(m), _p(v), _p(g), step, lr, b1, b2, eps)
+    return P, m, v

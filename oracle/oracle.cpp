@@ -1074,3 +1074,188 @@ void orc_mt_floats(uint32_t seed, int n, float* out) {
 }
 
 }  // extern "C"
+
+// ============================================================================
+// Guide MLP oracle (FP64): forward, loss gradient, backprop, Adam.  With
+// bf16 != 0 the inputs of every product are rounded to BF16 where the GPU
+// rounds them (features, weights, hidden activations, output and hidden
+// gradients); otherwise everything is exact FP64 (used against finite
+// differences).  Layout and loss as documented in include/cpt.h
+// (cpt_guide_train_step): parameters W_l (K_l x N_l row-major) then b_l.
+namespace {
+namespace mlp {
+const int NIN = 7, NB = 32, K0 = NIN * NB, H = 64, NOUT = 48, NL = 8;
+const int W0 = 0, B0 = W0 + K0 * H, W1 = B0 + H, B1 = W1 + H * H, W2 = B1 + H, B2 = W2 + H * H, W3 = B2 + H,
+          B3 = W3 + H * H, W4 = B3 + H, B4 = W4 + H * NOUT, NP = B4 + NOUT;
+const int KS[5] = {K0, H, H, H, H}, NS[5] = {H, H, H, H, NOUT}, WO[5] = {W0, W1, W2, W3, W4},
+          BO[5] = {B0, B1, B2, B3, B4};
+
+inline double bf(double x, int on) {
+    if (!on) return x;
+    float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+struct Acts {
+    double a[5][K0];   // inputs of layer l (a[0] = features)
+    double o[NOUT];
+};
+
+void forward(const float* P, const float* in, int bfon, Acts& A) {
+    for (int d = 0; d < NIN; d++)
+        for (int b = 0; b < NB; b++) {
+            double x = (double)in[d] * NB - (b + 0.5);
+            A.a[0][d * NB + b] = bf(std::exp(-0.5 * x * x), bfon);
+        }
+    for (int l = 0; l < 5; l++) {
+        for (int n = 0; n < NS[l]; n++) {
+            double z = 0;
+            for (int k = 0; k < KS[l]; k++) z += A.a[l][k] * bf(P[WO[l] + k * NS[l] + n], bfon);
+            z += P[BO[l] + n];
+            if (l < 4) A.a[l + 1][n] = bf(z > 0 ? z : 0.0, bfon);
+            else A.o[n] = z;
+        }
+    }
+}
+
+struct Mix {
+    double pi[NL], kap[NL], m[NL][3], un[NL], ls[NL];
+};
+
+void decode(const double* o, Mix& M) {
+    double mx = -1e300;
+    for (int k = 0; k < NL; k++) mx = std::max(mx, o[5 * k]);
+    double s = 0;
+    for (int k = 0; k < NL; k++) s += (M.pi[k] = std::exp(o[5 * k] - mx));
+    for (int k = 0; k < NL; k++) {
+        M.pi[k] /= s;
+        M.ls[k] = o[5 * k + 1];
+        M.kap[k] = std::exp(std::min(std::max(o[5 * k + 1], -3.0), 6.0));
+        double x = o[5 * k + 2], y = o[5 * k + 3], z = o[5 * k + 4];
+        double n = std::sqrt(x * x + y * y + z * z);
+        M.un[k] = n;
+        if (n < 1e-12) { x = 0; y = 0; z = 1; n = 1; }
+        M.m[k][0] = x / n;
+        M.m[k][1] = y / n;
+        M.m[k][2] = z / n;
+    }
+}
+
+inline double vmf(double kap, double c) {
+    return kap / (2 * M_PI * (1 - std::exp(-2 * kap))) * std::exp(kap * (c - 1));
+}
+
+double mix_pdf(const Mix& M, const double* w) {
+    double p = 0;
+    for (int k = 0; k < NL; k++) p += M.pi[k] * vmf(M.kap[k], M.m[k][0] * w[0] + M.m[k][1] * w[1] + M.m[k][2] * w[2]);
+    return p;
+}
+
+// -wgt * log q(w) and its gradient w.r.t. the raw outputs (analytic)
+double loss_grad(const double* o, const double* w, double wgt, double* g) {
+    Mix M;
+    decode(o, M);
+    double v[NL], q = 0;
+    for (int k = 0; k < NL; k++) {
+        v[k] = vmf(M.kap[k], M.m[k][0] * w[0] + M.m[k][1] * w[1] + M.m[k][2] * w[2]);
+        q += M.pi[k] * v[k];
+    }
+    q = std::max(q, 1e-300);
+    for (int k = 0; k < NL; k++) {
+        double r = M.pi[k] * v[k] / q;
+        g[5 * k] = -wgt * (r - M.pi[k]);
+        double kap = M.kap[k], e2 = std::exp(-2 * kap);
+        double c = M.m[k][0] * w[0] + M.m[k][1] * w[1] + M.m[k][2] * w[2];
+        double dk = 1 / kap - 2 * e2 / (1 - e2) + (c - 1);
+        g[5 * k + 1] = (M.ls[k] > -3 && M.ls[k] < 6) ? -wgt * r * dk * kap : 0.0;
+        double f = M.un[k] < 1e-12 ? 0.0 : -wgt * r * kap / M.un[k];
+        for (int j = 0; j < 3; j++) g[5 * k + 2 + j] = f * (w[j] - M.m[k][j] * c);
+    }
+    for (int k = 5 * NL; k < NOUT; k++) g[k] = 0;
+    return -wgt * std::log(q);
+}
+
+}  // namespace mlp
+}  // namespace
+
+extern "C" {
+
+int orc_mlp_num_params(void) { return mlp::NP; }
+
+void orc_mlp_forward(const float* P, const float* in, int n, int bfon, double* out) {
+#pragma omp parallel for
+    for (int s = 0; s < n; s++) {
+        mlp::Acts A;
+        mlp::forward(P, in + (size_t)s * mlp::NIN, bfon, A);
+        for (int k = 0; k < mlp::NOUT; k++) out[(size_t)s * mlp::NOUT + k] = A.o[k];
+    }
+}
+
+double orc_mlp_mixture_pdf(const double* o, const double* w) {
+    mlp::Mix M;
+    mlp::decode(o, M);
+    return mlp::mix_pdf(M, w);
+}
+
+void orc_mlp_mixture_pdf_batch(const double* o, const double* w, int n, double* out) {
+    mlp::Mix M;
+    mlp::decode(o, M);
+    for (int i = 0; i < n; i++) out[i] = mlp::mix_pdf(M, w + 3 * (size_t)i);
+}
+
+// loss = sum_s w_s (-log q_s) / sum_s w_s and its gradient (NP doubles)
+double orc_mlp_loss_grad(const float* P, const float* samples, int n, int bfon, double* grad) {
+    using namespace mlp;
+    double wsum = 0;
+    for (int s = 0; s < n; s++) wsum += samples[(size_t)s * 11 + 10];
+    // the GPU forms 1/sum in FP32 and multiplies per sample in FP32
+    double ws = bfon ? (double)(1.0f / (float)wsum) : 1.0 / wsum;
+    for (int p = 0; p < NP; p++) grad[p] = 0;
+    double loss = 0;
+    for (int s = 0; s < n; s++) {
+        const float* sp = samples + (size_t)s * 11;
+        Acts A;
+        forward(P, sp, bfon, A);
+        double w[3] = {sp[7], sp[8], sp[9]};
+        double wgt = bfon ? (double)(sp[10] * (float)ws) : sp[10] * ws;
+        double dz[5][K0];
+        double g[NOUT];
+        loss += loss_grad(A.o, w, wgt, g);
+        for (int k = 0; k < NOUT; k++) dz[4][k] = bf(g[k], bfon);
+        for (int l = 4; l >= 1; l--) {
+            // dz_{l-1} = (dz_l W_l^T) .* (a_l > 0)
+            for (int k = 0; k < KS[l]; k++) {
+                double acc = 0;
+                for (int nn = 0; nn < NS[l]; nn++) acc += dz[l][nn] * bf(P[WO[l] + k * NS[l] + nn], bfon);
+                dz[l - 1][k] = bf(A.a[l][k] > 0 ? acc : 0.0, bfon);
+            }
+        }
+        for (int l = 0; l < 5; l++) {
+            for (int k = 0; k < KS[l]; k++) {
+                double a = A.a[l][k];
+                if (a == 0) continue;
+                for (int nn = 0; nn < NS[l]; nn++) grad[WO[l] + k * NS[l] + nn] += a * dz[l][nn];
+            }
+            for (int nn = 0; nn < NS[l]; nn++) grad[BO[l] + nn] += dz[l][nn];
+        }
+    }
+    return loss;
+}
+
+// one Adam step in FP64 on FP32 parameters (in place)
+void orc_mlp_adam(float* P, double* m, double* v, const double* g, int step, double lr, double b1, double b2,
+                  double eps) {
+    double c1 = 1 / (1 - std::pow(b1, step)), c2 = 1 / (1 - std::pow(b2, step));
+    for (int p = 0; p < mlp::NP; p++) {
+        m[p] = b1 * m[p] + (1 - b1) * g[p];
+        v[p] = b2 * v[p] + (1 - b2) * g[p] * g[p];
+        P[p] = (float)(P[p] - lr * (m[p] * c1) / (std::sqrt(v[p] * c2) + eps));
+    }
+}
+
+}  // extern "C"

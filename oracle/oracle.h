@@ -101,3 +101,18 @@ void orc_mt_floats(uint32_t seed, int n, float* out);
 }
 #endif
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* Guide MLP (FP64; bf16 != 0 rounds to BF16 where the GPU does). */
+int orc_mlp_num_params(void);
+void orc_mlp_forward(const float* params, const float* inputs, int n, int bf16, double* outputs);
+double orc_mlp_mixture_pdf(const double* outputs48, const double* dir3);
+void orc_mlp_mixture_pdf_batch(const double* outputs48, const double* dirs, int n, double* out);
+double orc_mlp_loss_grad(const float* params, const float* samples, int n, int bf16, double* grad);
+void orc_mlp_adam(float* params, double* m, double* v, const double* grad, int step, double lr, double b1, double b2,
+                  double eps);
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,84 @@
+"""Guide MLP on the GPU (BF16 MFMA, FP32 accumulation) against the FP64 oracle
+with BF16 rounding emulated at the same points."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from test_guide_oracle import random_params, random_samples
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_forward_matches_oracle(ctx):
+    ctx.guide_init(seed=3)
+    P = random_params(11)
+    ctx.guide_set_params(P)
+    assert np.array_equal(ctx.guide_params(), P)
+    X = np.random.default_rng(12).random((1000, 7)).astype(np.float32)  # 15 full tiles + a partial one
+    g = ctx.guide_forward(X)
+    r = oracle.mlp_forward(P, X, bf16=True)
+    print("forward rel err", _rel(g, r))
+    assert _rel(g, r) < 1e-3  # observed 5e-5: FP32 vs FP64 accumulation only
+    assert np.abs(g - r).max() < 0.01 * np.abs(r).max()
+
+
+def test_gradient_matches_oracle(ctx):
+    ctx.guide_init(seed=3)
+    P = random_params(21)
+    ctx.guide_set_params(P)
+    S = random_samples(777, 22)
+    loss, g = ctx.guide_train_step(S, apply=False, want_grad=True)
+    rl, rg = oracle.mlp_loss_grad(P, S, bf16=True)
+    print("loss", loss, rl)
+    assert abs(loss - rl) < 1e-3 * abs(rl) + 1e-6
+    offs = [0, 14336, 14400, 18496, 18560, 22656, 22720, 26816, 26880, 29952, 30000]
+    for i in range(len(offs) - 1):
+        e = _rel(g[offs[i]:offs[i + 1]], rg[offs[i]:offs[i + 1]])
+        print("block", i, "rel err", e)
+        assert e < 1e-3, (i, e)  # observed <= 5e-5
+    # and against exact FP64 (no BF16): the BF16 path stays close
+    _, eg = oracle.mlp_loss_grad(P, S, bf16=False)
+    assert _rel(g, eg) < 0.1
+
+
+def test_adam_step_matches_oracle(ctx):
+    ctx.guide_init(seed=5, lr=2e-3)
+    P = random_params(31)
+    ctx.guide_set_params(P)
+    S = random_samples(300, 32)
+    _, g = ctx.guide_train_step(S, apply=True, want_grad=True)
+    P1 = ctx.guide_params()
+    R1, _, _ = oracle.mlp_adam(P, np.zeros(P.size), np.zeros(P.size), g.astype(np.float64), 1, lr=2e-3)
+    assert np.abs(P1 - R1).max() < 1e-6
+
+
+def test_training_learns_a_target_distribution(ctx):
+    """Samples drawn uniformly on the sphere, weighted by a fixed vMF target:
+    minimising the weighted -log q moves the mixture toward the target."""
+    ctx.guide_init(seed=7, lr=5e-3)
+    rng = np.random.default_rng(40)
+    mu = np.array([0.3, 0.5, 0.81])
+    mu /= np.linalg.norm(mu)
+    kappa = 20.0
+    losses = []
+    for it in range(150):
+        n = 4096
+        d = rng.standard_normal((n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        S = np.zeros((n, 11), np.float32)
+        S[:, :7] = rng.random((n, 7))
+        S[:, 7:10] = d
+        S[:, 10] = kappa / (2 * math.pi * (1 - math.exp(-2 * kappa))) * np.exp(kappa * (d @ mu - 1)) * 4 * math.pi
+        losses.append(ctx.guide_train_step(S))
+    X = rng.random((8, 7)).astype(np.float32)
+    out = ctx.guide_forward(X).astype(np.float64)
+    q_mu = np.array([oracle.mlp_mixture_pdf(o, mu[None])[0] for o in out])
+    print("loss", losses[0], "->", losses[-1], "q(mu)", q_mu.mean(), "uniform", 1 / (4 * math.pi))
+    assert np.mean(losses[-10:]) < np.mean(losses[:10]) - 0.5
+    assert q_mu.min() > 10 / (4 * math.pi)
