@@ -27,6 +27,8 @@ CPT_RENDER_KERNEL_TIMING = 2
 CPT_RENDER_COUNT = 4
 CPT_RENDER_ACCUMULATE = 8
 CPT_RENDER_STREAM = 16
+CPT_RENDER_GUIDED = 32
+CPT_RENDER_GUIDE_TRAIN = 64
 
 # every function include/cpt.h declares
 ABI_FUNCTIONS = (
@@ -41,7 +43,7 @@ ABI_FUNCTIONS = (
 class GuideParams(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("seed", C.c_uint64), ("guide_fraction", C.c_float), ("train_paths", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("train_steps", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class SceneDesc(C.Structure):
@@ -71,7 +73,8 @@ class RenderStats(C.Structure):
         ("ms_extend", C.c_double), ("ms_shade", C.c_double), ("ms_shadow", C.c_double),
         ("ms_other", C.c_double), ("n_extend", C.c_int64), ("n_shade", C.c_int64),
         ("n_shadow", C.c_int64), ("passes", C.c_int32), ("paths_per_pass", C.c_int32),
-        ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
+        ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("guide_samples", C.c_int64),
+        ("guide_steps", C.c_int32), ("reserved_", C.c_int32),
     ]
 
     def as_dict(self):
@@ -275,10 +278,11 @@ def _render_views(self, cameras, wh_spp, max_depth=32, seed=1, flags=0):
 Context.render_views = _render_views
 
 
-def _guide_init(self, lr=1e-3, seed=1, guide_fraction=0.5, train_paths=0, beta1=0.9, beta2=0.99, eps=1e-8):
+def _guide_init(self, lr=1e-3, seed=1, guide_fraction=0.5, train_paths=0, train_steps=1, beta1=0.9, beta2=0.99,
+                eps=1e-8):
     gp = GuideParams()
     gp.lr, gp.beta1, gp.beta2, gp.eps = lr, beta1, beta2, eps
-    gp.seed, gp.guide_fraction, gp.train_paths = seed, guide_fraction, train_paths
+    gp.seed, gp.guide_fraction, gp.train_paths, gp.train_steps = seed, guide_fraction, train_paths, train_steps
     self._check(self.lib.cpt_guide_init(self.h, C.byref(gp)), "cpt_guide_init")
 
 

@@ -451,3 +451,112 @@ def terrain_scene(res=(1920, 1080), n_spheres=200, grid=(1024, 512), subdiv=4, s
     s.env = (0.25, 0.3, 0.4)
     s.set_camera((0.0, 7.0, 17.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), math.tan(math.radians(25.0)), *res)
     return s
+
+
+# --------------------------------------------------------------------- hard-lighting interior
+def _subdivided_boxes(centers, sizes, angles, sub=2):
+    """Boxes resting on their centres' footprint, rotated about y; every face split
+    into sub x sub quads (8 triangles per face for sub=2, 48 per box)."""
+    g = np.linspace(0.0, 1.0, sub + 1)
+    U, V = np.meshgrid(g, g, indexing="xy")
+    U, V = U.ravel(), V.ravel()
+    # unit cube faces: corner, u axis, v axis, outward normal (u x v = normal)
+    faces = [((0, 1, 0), (1, 0, 0), (0, 0, 1), (0, 1, 0)),
+             ((0, 0, 1), (1, 0, 0), (0, 0, -1), (0, -1, 0)),
+             ((1, 0, 0), (0, 0, 1), (0, 1, 0), (1, 0, 0)),
+             ((0, 0, 0), (0, 1, 0), (0, 0, 1), (-1, 0, 0)),
+             ((0, 0, 1), (0, 1, 0), (1, 0, 0), (0, 0, 1)),
+             ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, -1))]
+    P, N, T = [], [], []
+    for c0, u, v, n in faces:
+        c0, u, v, n = (np.asarray(x, float) for x in (c0, u, v, n))
+        if np.dot(np.cross(u, v), n) < 0:
+            u, v = v, u
+        P.append(c0[None] + U[:, None] * u[None] + V[:, None] * v[None])
+        N.append(np.tile(n, (len(U), 1)))
+        T.append(np.tile(u, (len(U), 1)))
+    P = np.concatenate(P) - np.array([0.5, 0.0, 0.5])  # (54,3), base centred on the floor point
+    N = np.concatenate(N)
+    T = np.concatenate(T)
+    nv = (sub + 1) ** 2
+    quads = []
+    for j in range(sub):
+        for i in range(sub):
+            a = j * (sub + 1) + i
+            quads += [(a, a + 1, a + sub + 2), (a, a + sub + 2, a + sub + 1)]
+    F1 = np.array(quads)
+    F = np.concatenate([F1 + f * nv for f in range(6)])
+    nb = len(centers)
+    ca, sa = np.cos(angles), np.sin(angles)
+    R = np.zeros((nb, 3, 3))
+    R[:, 0, 0], R[:, 0, 2], R[:, 1, 1], R[:, 2, 0], R[:, 2, 2] = ca, sa, 1.0, -sa, ca
+    Pw = np.einsum("bij,bvj->bvi", R, P[None] * sizes[:, None, :]) + centers[:, None, :]
+    Nw = np.einsum("bij,vj->bvi", R, N)
+    Tw = np.einsum("bij,vj->bvi", R, T)
+    Fw = F[None] + (np.arange(nb) * len(P))[:, None, None]
+    return Pw.reshape(-1, 3), Nw.reshape(-1, 3), Tw.reshape(-1, 3), Fw.reshape(-1, 3)
+
+
+def interior_scene(res=(1280, 720), n_boxes=40000, seed=5, light_radiance=3000.0):
+    """Hard lighting: a 10 x 3 x 10 room cluttered with `n_boxes` furniture boxes
+    (48 triangles each), lit only by a 0.2 x 0.2 light in a corridor outside,
+    through a 0.3-wide, 2-high door gap."""
+    s = Scene()
+    rng = np.random.default_rng(seed)
+    wall = s.add_material(material(MAT_DIFFUSE, albedo=(0.7, 0.7, 0.7)))
+    X0, X1, Y1, Z = 5.0, 8.0, 3.0, 5.0
+    gap, gh = 0.15, 2.0
+    quads = [
+        # room: floor, ceiling, back wall x=-5, side walls z=+-5
+        ((-X0, 0, Z), (X0, 0, Z), (X0, 0, -Z), (-X0, 0, -Z), (0, 1, 0)),
+        ((-X0, Y1, -Z), (X0, Y1, -Z), (X0, Y1, Z), (-X0, Y1, Z), (0, -1, 0)),
+        ((-X0, 0, Z), (-X0, 0, -Z), (-X0, Y1, -Z), (-X0, Y1, Z), (1, 0, 0)),
+        ((X0, 0, -Z), (-X0, 0, -Z), (-X0, Y1, -Z), (X0, Y1, -Z), (0, 0, 1)),
+        ((-X0, 0, Z), (X0, 0, Z), (X0, Y1, Z), (-X0, Y1, Z), (0, 0, -1)),
+        # wall x=5 with the door gap (three pieces)
+        ((X0, 0, -Z), (X0, 0, -gap), (X0, Y1, -gap), (X0, Y1, -Z), (-1, 0, 0)),
+        ((X0, 0, gap), (X0, 0, Z), (X0, Y1, Z), (X0, Y1, gap), (-1, 0, 0)),
+        ((X0, gh, -gap), (X0, gh, gap), (X0, Y1, gap), (X0, Y1, -gap), (-1, 0, 0)),
+        # corridor outside: floor, ceiling, far wall x=8, ends z=+-5
+        ((X0, 0, Z), (X1, 0, Z), (X1, 0, -Z), (X0, 0, -Z), (0, 1, 0)),
+        ((X0, Y1, -Z), (X1, Y1, -Z), (X1, Y1, Z), (X0, Y1, Z), (0, -1, 0)),
+        ((X1, 0, -Z), (X1, 0, Z), (X1, Y1, Z), (X1, Y1, -Z), (-1, 0, 0)),
+        ((X1, 0, -Z), (X0, 0, -Z), (X0, Y1, -Z), (X1, Y1, -Z), (0, 0, 1)),
+        ((X0, 0, Z), (X1, 0, Z), (X1, Y1, Z), (X0, Y1, Z), (0, 0, -1)),
+    ]
+    for p0, p1, p2, p3, n in quads:
+        P = np.array([p0, p1, p2, p3], float)
+        nn = np.asarray(n, float)
+        g = np.cross(P[1] - P[0], P[2] - P[0])
+        if np.dot(g, nn) < 0:
+            P = P[[0, 3, 2, 1]]
+        t = P[1] - P[0]
+        t /= np.linalg.norm(t)
+        s.add_mesh(P, np.tile(nn, (4, 1)), np.tile(t, (4, 1)), np.array([[0, 1, 2], [0, 2, 3]]), wall)
+    # furniture
+    cam = np.array([-4.0, 1.5, -3.0])
+    cx = rng.uniform(-4.8, 4.6, 4 * n_boxes)
+    cz = rng.uniform(-4.8, 4.8, 4 * n_boxes)
+    keep = (np.hypot(cx - cam[0], cz - cam[2]) > 0.8) & ~((cx > 4.0) & (np.abs(cz) < 0.8))
+    cx, cz = cx[keep][:n_boxes], cz[keep][:n_boxes]
+    nb = len(cx)
+    sizes = np.stack([rng.uniform(0.05, 0.25, nb), rng.uniform(0.05, 0.6, nb), rng.uniform(0.05, 0.25, nb)], 1)
+    centers = np.stack([cx, np.zeros(nb), cz], 1)
+    angles = rng.uniform(0, np.pi, nb)
+    mats = [s.add_material(material(MAT_DIFFUSE, albedo=tuple(rng.uniform(0.2, 0.8, 3)))) for _ in range(6)]
+    which = rng.integers(0, len(mats), nb)
+    for k, m in enumerate(mats):
+        sel = which == k
+        if sel.any():
+            P, N, T, F = _subdivided_boxes(centers[sel], sizes[sel], angles[sel])
+            s.add_mesh(P, N, T, F, m)
+    light = s.add_material(material(MAT_EMISSIVE, emission=(light_radiance,) * 3))
+    h = 0.1
+    P, N, T, F = _quad((X1 - 0.1, 1.5 - h, -h), (X1 - 0.1, 1.5 - h, h), (X1 - 0.1, 1.5 + h, h), (X1 - 0.1, 1.5 + h, -h),
+                       (-1, 0, 0), (0, 0, 1))
+    g = np.cross(P[1] - P[0], P[2] - P[0])
+    if g[0] > 0:
+        P = P[[0, 3, 2, 1]]
+    s.add_mesh(P, N, T, F, light)
+    s.set_camera(tuple(cam), (5.0, 1.0, 0.0), (0.0, 1.0, 0.0), math.tan(math.radians(30.0)), *res)
+    return s

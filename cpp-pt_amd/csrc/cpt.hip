@@ -94,10 +94,24 @@ struct WfArgs {
     unsigned long long* pass_tend;  // s_memrealtime at the end of each pass
     double* film;      // per-pixel radiance sums
     double* film2;     // per-pixel sums of squared radiance
+    // learned guide (guided renders only)
+    float* lobes;            // per queue entry: pi[8], kappa[8], mu.x[8], mu.y[8], mu.z[8]
+    float* rec;              // training records: [slot][depth][REC_FLOATS]
+    uint32_t n_rec;          // slots that record
+    int rec_depth;
+    float guide_frac;
+    float* gsamples;         // training samples out (11 floats)
+    uint32_t* gcount;
+    const uint16_t* gwt;
+    const float* gparams;
     int W, H, S, max_depth;
     uint64_t seed;
     uint32_t view;
 };
+
+// training record of one guided path vertex: inputs[7], dir[3], pdf, mul_next[3],
+// radiance snapshot[3], state (0 empty, 1 direction traced, 2 complete)
+#define REC_FLOATS 18
 
 __device__ __forceinline__ unsigned gthreads() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ unsigned gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -167,6 +181,8 @@ __global__ void k_raygen(WfArgs a) {
         a.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.rng[i] = rng;
         a.meta[i] = 1u << 8;  // depth 0, sampled_delta = true (camera ray)
+        if (i < a.n_rec)
+            for (int k = 0; k < a.rec_depth; k++) a.rec[((size_t)i * a.rec_depth + k) * REC_FLOATS + 17] = 0.0f;
     }
 }
 
@@ -199,9 +215,145 @@ __global__ __launch_bounds__(B) void k_extend(WfArgs a, int cur) {
     }
 }
 
+
+// ------------------------------------------------------------------ learned guide in the path
+
+__device__ __forceinline__ float gmix_pdf(const float* lob, V3 w) {
+    float p = 0.0f;
+#pragma unroll
+    for (int k = 0; k < guide::NLOBES; k++) {
+        float c = lob[16 + k] * w.x + lob[24 + k] * w.y + lob[32 + k] * w.z;
+        p += lob[k] * guide::vmf_pdf(lob[8 + k], c);
+    }
+    return p;
+}
+
+__device__ __forceinline__ V3 gmix_sample(const float* lob, float u0, float u1, float u2) {
+    // lobe by its weight, then a vMF sample around its mean (read from memory:
+    // a local array indexed by the chosen lobe would live in scratch)
+    int k = guide::NLOBES - 1;
+    float c = 0.0f;
+    for (int j = 0; j < guide::NLOBES; j++) {
+        c += lob[j];
+        if (u0 < c) {
+            k = j;
+            break;
+        }
+    }
+    float kap = lob[8 + k];
+    float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kap)) / kap;
+    w = fminf(fmaxf(w, -1.0f), 1.0f);
+    float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+    float sn, co;
+    __sincosf(guide::TWO_PI * u2, &sn, &co);
+    float mx = lob[16 + k], my = lob[24 + k], mz = lob[32 + k];
+    float sign = copysignf(1.0f, mz);
+    float a = -1.0f / (sign + mz);
+    float b = mx * my * a;
+    V3 t1 = {1.0f + sign * mx * mx * a, sign * b, -sign * mx};
+    V3 t2 = {b, sign + my * my * a, -my};
+    V3 d = {w * mx + r * (co * t1.x + sn * t2.x), w * my + r * (co * t1.y + sn * t2.y), w * mz + r * (co * t1.z + sn * t2.z)};
+    float n = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    return {d.x / n, d.y / n, d.z / n};
+}
+
+__device__ __forceinline__ bool guidable(const Mat& m) {
+    return m.type == CPT_MAT_DIFFUSE || (m.type == CPT_MAT_GLOSSY && m.rough >= 0.01f);
+}
+
+// Per bounce: network inputs of every queued hit, one MLP forward per tile of 64,
+// decoded mixture written per queue entry.
+__global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    guide::SmemFwd& sm = *reinterpret_cast<guide::SmemFwd*>(smem_raw);
+    __shared__ float o_tile[guide::TILE * guide::NOUT];
+    __shared__ float in_s[guide::TILE * guide::NIN];
+    const unsigned count = a.counters[cur];
+    if (blockIdx.x * guide::TILE >= count) return;
+    guide::load_weights(sm, a.gwt, a.gparams);
+    for (unsigned t = blockIdx.x; t * guide::TILE < count; t += gridDim.x) {
+        const unsigned e0 = t * guide::TILE;
+        const int nt = (int)min((unsigned)guide::TILE, count - e0);
+        if ((int)threadIdx.x < guide::TILE) {
+            float* in = in_s + threadIdx.x * guide::NIN;
+            for (int k = 0; k < guide::NIN; k++) in[k] = 0.0f;
+            if ((int)threadIdx.x < nt) {
+                unsigned e = e0 + threadIdx.x;
+                float4 O = a.qo[cur][e], D = a.qd[cur][e], HH = a.hits[e];
+                Hit h;
+                h.t = HH.x;
+                h.beta = HH.y;
+                h.gamma = HH.z;
+                h.tri = __float_as_int(HH.w);
+                if (h.tri >= 0) {
+                    V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
+                    Isect it;
+                    int matid;
+                    fill_isect(a.sc, o, d, h, it, matid);
+                    int slot = __float_as_int(D.w);
+                    if ((a.meta[slot] & 0xffu) == 0) first_hit_frame(load_mat(a.sc.mats + 16 * matid), it);
+                    guide::features(a.sc.bmin, a.sc.bext, it.hitpoint.x, it.hitpoint.y, it.hitpoint.z, it.normal.x,
+                                    it.normal.y, it.normal.z, -d.x, -d.y, -d.z, in);
+                }
+            }
+        }
+        __syncthreads();
+        guide::encode_tile(sm, in_s, nt);
+        __syncthreads();
+        guide::forward_tile(sm, o_tile);
+        if ((int)threadIdx.x < nt) {
+            guide::Lobes L;
+            guide::decode(o_tile + threadIdx.x * guide::NOUT, L);
+            float* dst = a.lobes + (size_t)(e0 + threadIdx.x) * 40;
+#pragma unroll
+            for (int k = 0; k < guide::NLOBES; k++) {
+                dst[k] = L.pi[k];
+                dst[8 + k] = L.kappa[k];
+                dst[16 + k] = L.mx[k];
+                dst[24 + k] = L.my[k];
+                dst[32 + k] = L.mz[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// After a pass: every complete record of a recording slot becomes a training
+// sample (inputs, direction, weight = luminance of the radiance that came back
+// along the direction / the pdf it was sampled with).
+__global__ void k_guide_collect(WfArgs a) {
+    const unsigned n_slots = min(a.n_rec, a.pass->n_paths);
+    const unsigned total = n_slots * (unsigned)a.rec_depth;
+    for (unsigned base = (gtid() >> 6) * 64; base < total; base += (gthreads() >> 6) * 64) {
+        unsigned idx = base + __lane_id();
+        bool emit = false;
+        float smp[guide::SAMPLE_FLOATS];
+        if (idx < total) {
+            const float* r = a.rec + (size_t)idx * REC_FLOATS;
+            if (r[17] == 2.0f) {
+                unsigned slot = idx / (unsigned)a.rec_depth;
+                float4 c = a.col[slot];
+                float lr = r[11] > 0.0f ? (c.x - r[14]) / r[11] : 0.0f;
+                float lg = r[12] > 0.0f ? (c.y - r[15]) / r[12] : 0.0f;
+                float lb = r[13] > 0.0f ? (c.z - r[16]) / r[13] : 0.0f;
+                float F = fmaxf(0.2126f * lr + 0.7152f * lg + 0.0722f * lb, 0.0f);
+                float w = r[10] > 0.0f ? F / r[10] : 0.0f;
+                if (w > 0.0f && isfinite(w)) {
+                    emit = true;
+                    for (int k = 0; k < 10; k++) smp[k] = r[k];
+                    smp[10] = w;
+                }
+            }
+        }
+        unsigned q = wave_alloc(emit, a.gcount);
+        if (emit)
+            for (int k = 0; k < guide::SAMPLE_FLOATS; k++) a.gsamples[(size_t)q * guide::SAMPLE_FLOATS + k] = smp[k];
+    }
+}
+
 // ------------------------------------------------------------------ shade
 // pathtracerMIS.h:37-126, one loop iteration per queue entry.
-template <int B, bool STREAM>
+template <int B, bool STREAM, bool GUIDED>
 __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
     const unsigned count = a.counters[cur];
     const unsigned lane = __lane_id();
@@ -230,6 +382,17 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
             h.gamma = HH.z;
             h.tri = __float_as_int(HH.w);
             bool alive = false;
+            float* rec = nullptr;
+            if (GUIDED && (unsigned)slot < a.n_rec) {
+                rec = a.rec + (size_t)slot * a.rec_depth * REC_FLOATS;
+                if (depth > 0 && rec[(depth - 1) * REC_FLOATS + 17] == 1.0f) {
+                    float* r = rec + (depth - 1) * REC_FLOATS;
+                    r[14] = colv.x;
+                    r[15] = colv.y;
+                    r[16] = colv.z;
+                    r[17] = 2.0f;
+                }
+            }
             if (h.tri < 0) {
                 // render_sky (pathtracerMIS.h:38-40), constant environment
                 colv = add(colv, mul(mulv, v3(a.sc.env[0], a.sc.env[1], a.sc.env[2])));
@@ -253,8 +416,28 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                     }
                 } else {
                     sampled_delta = false;
-                    DSample ds = bsdf_sample(m, it, wo, rng);
+                    const float* lob = GUIDED ? a.lobes + (size_t)i * 40 : nullptr;
+                    const bool gv = GUIDED && guidable(m);
+                    const float alpha = a.guide_frac;
+                    DSample ds;
+                    if (gv) {
+                        // one-sample MIS of guide and BSDF (balance heuristic, ppg.h:74-103)
+                        float u = pcg_float(rng);
+                        if (u < alpha) {
+                            float u1 = pcg_float(rng);
+                            float u2 = pcg_float(rng);
+                            ds.wi = gmix_sample(lob, u / alpha, u1, u2);
+                            ds.delta = false;
+                            ds.pdf = bsdf_pdf(m, it, wo, ds.wi);
+                        } else {
+                            ds = bsdf_sample(m, it, wo, rng);
+                        }
+                        ds.pdf = alpha * gmix_pdf(lob, ds.wi) + (1.0f - alpha) * ds.pdf;
+                    } else {
+                        ds = bsdf_sample(m, it, wo, rng);
+                    }
                     p = ds.pdf;
+                    const float p_sampled = p;
                     if (!ds.delta) {
                         if (a.sc.n_lights > 0) {
                             LSample ls = light_sample(a.sc, rng, it.hitpoint);
@@ -263,6 +446,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                             sdir = divs(sdir, len);
                             if (bsdf_non_zero(m, it, wo, sdir)) {
                                 float pnb = bsdf_pdf(m, it, wo, sdir);
+                                if (gv) pnb = alpha * gmix_pdf(lob, sdir) + (1.0f - alpha) * pnb;
                                 float ct = (float)(fabs((double)dot(ls.normal, sdir)) + 1e-4);
                                 float pnn = ls.pdf * len * len / ct;
                                 float w = pnn / (pnb + pnn);
@@ -291,6 +475,19 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                                 no = add(it.hitpoint, mulf(ds.wi, CPT_EPSF));
                                 nd = ds.wi;
                                 alive = true;
+                                if (GUIDED && rec && gv) {
+                                    float* r = rec + depth * REC_FLOATS;
+                                    guide::features(a.sc.bmin, a.sc.bext, it.hitpoint.x, it.hitpoint.y, it.hitpoint.z,
+                                                    it.normal.x, it.normal.y, it.normal.z, wo.x, wo.y, wo.z, r);
+                                    r[7] = ds.wi.x;
+                                    r[8] = ds.wi.y;
+                                    r[9] = ds.wi.z;
+                                    r[10] = p_sampled;
+                                    r[11] = mulv.x;
+                                    r[12] = mulv.y;
+                                    r[13] = mulv.z;
+                                    r[17] = 1.0f;
+                                }
                             }
                         }
                     }
@@ -552,6 +749,7 @@ struct cpt_ctx {
     int* d_lights = nullptr;
     float cam[16];
     float env[3];
+    float bmin[3], bext[3];
 
     // wavefront buffers
     size_t cap = 0;        // path slots
@@ -590,6 +788,14 @@ struct cpt_ctx {
     float* g_wscale = nullptr;
     int g_step = 0;
     cpt_guide_params g_cfg{};
+    float* g_lobes = nullptr;    // per queue entry mixture (40 floats)
+    size_t g_lobes_cap = 0;
+    float* g_rec = nullptr;      // training records
+    size_t g_rec_cap = 0;
+    uint32_t* g_count = nullptr;
+    uint32_t* h_count = nullptr; // pinned
+    int64_t g_samples_total = 0; // training samples used by the last guided render
+    int g_steps_total = 0;
 
     // captured pass graph
     hipGraphExec_t gexec = nullptr;
@@ -672,6 +878,10 @@ DevScene dev_scene(const cpt_ctx* c) {
     s.n_tris = c->nt;
     for (int k = 0; k < 3; k++) s.env[k] = c->env[k];
     for (int k = 0; k < 16; k++) s.cam[k] = c->cam[k];
+    for (int k = 0; k < 3; k++) {
+        s.bmin[k] = c->bmin[k];
+        s.bext[k] = c->bext[k];
+    }
     return s;
 }
 
@@ -743,6 +953,15 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.pass_tend = c->pass_tend;
     a.film = c->film;
     a.film2 = c->film2;
+    a.lobes = c->g_lobes;
+    a.rec = c->g_rec;
+    a.n_rec = 0;
+    a.rec_depth = depth;
+    a.guide_frac = c->g_cfg.guide_fraction;
+    a.gsamples = c->g_samples;
+    a.gcount = c->g_count;
+    a.gwt = c->g_wt;
+    a.gparams = c->g_params;
     a.W = W;
     a.H = H;
     a.S = S;
@@ -759,6 +978,8 @@ struct LaunchCfg {
     int G;
     bool count;
     bool stream = false;
+    bool guided = false;
+    int infer_blocks = 0;
 };
 
 // launch helpers dispatching on the block size
@@ -769,8 +990,9 @@ void launch_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) 
 }
 template <int B>
 void launch_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
-    if (L.stream) hipLaunchKernelGGL((k_shade<B, true>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
-    else hipLaunchKernelGGL((k_shade<B, false>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
+    if (L.stream) hipLaunchKernelGGL((k_shade<B, true, false>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
+    else if (L.guided) hipLaunchKernelGGL((k_shade<B, false, true>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
+    else hipLaunchKernelGGL((k_shade<B, false, false>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
 }
 template <int B>
 void launch_shadow(const LaunchCfg& L, const WfArgs& a, hipStream_t s) {
@@ -822,6 +1044,8 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
         int cur = b & 1;
         L_extend(L, a, cur, s);
         mark(0);
+        if (L.guided)
+            hipLaunchKernelGGL(k_guide_infer, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a, cur);
         L_shade(L, a, cur, a.max_depth, s);
         mark(1);
         L_shadow(L, a, s);
@@ -902,6 +1126,10 @@ void cpt_destroy(cpt_ctx* c) {
     dfree(c->g_partial);
     dfree(c->g_loss_part);
     dfree(c->g_wscale);
+    dfree(c->g_lobes);
+    dfree(c->g_rec);
+    dfree(c->g_count);
+    if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->h_pass) (void)hipHostFree(c->h_pass);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -952,6 +1180,19 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     c->bvh_leaves = bvh.leaves;
     c->bvh_depth = bvh.max_depth;
 
+    for (int k = 0; k < 3; k++) {
+        c->bmin[k] = INFINITY;
+        c->bext[k] = -INFINITY;
+    }
+    for (int i = 0; i < c->nv; i++)
+        for (int k = 0; k < 3; k++) {
+            c->bmin[k] = std::min(c->bmin[k], d->positions[3 * (size_t)i + k]);
+            c->bext[k] = std::max(c->bext[k], d->positions[3 * (size_t)i + k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        if (c->nv == 0) c->bmin[k] = c->bext[k] = 0.0f;
+        c->bext[k] = std::max(c->bext[k] - c->bmin[k], 1e-6f);
+    }
     std::vector<float4> vp(std::max(1, c->nv)), vn(std::max(1, c->nv)), vb(std::max(1, c->nv));
     for (int i = 0; i < c->nv; i++) {
         vp[i] = make_float4(d->positions[3 * (size_t)i], d->positions[3 * (size_t)i + 1], d->positions[3 * (size_t)i + 2], 0.f);
@@ -1103,6 +1344,7 @@ int cpt_trace_closest_device(cpt_ctx* c, const void* d_rays, int32_t n, void* d_
 static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras, const int32_t* wh_spp,
                                int32_t max_depth, uint64_t seed, int32_t flags, float* films, double* view_ms,
                                cpt_render_stats* stats);
+static int guide_train_device(cpt_ctx* c, int n, bool apply);
 
 // Pack views into passes of at most `cap` paths (see PassDesc).
 static void plan_passes(const std::vector<ViewDesc>& vd, int64_t cap, std::vector<PassDesc>& passes,
@@ -1206,6 +1448,34 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         if (r) return r;
         c->film_px = film_px;
     }
+    const bool guided = (flags & CPT_RENDER_GUIDED) != 0;
+    const bool train = guided && (flags & CPT_RENDER_GUIDE_TRAIN) != 0;
+    size_t nrec = 0;
+    if (guided) {
+        if (!c->has_guide) return fail(c, CPT_E_INVALID, "guided render needs cpt_guide_init");
+        if (c->g_lobes_cap < slots || !c->g_lobes) {
+            drop_graph(c);
+            r = dalloc(c, c->g_lobes, slots * 40);
+            if (r) return r;
+            c->g_lobes_cap = slots;
+        }
+        if (train) {
+            nrec = std::min<size_t>(slots, (size_t)c->g_cfg.train_paths);
+            size_t nr = nrec * (size_t)max_depth;
+            if (c->g_rec_cap < nr || !c->g_rec) {
+                drop_graph(c);
+                r = dalloc(c, c->g_rec, nr * REC_FLOATS);
+                if (r) return r;
+                c->g_rec_cap = nr;
+            }
+            if (c->g_samples_cap < nr || !c->g_samples) {
+                drop_graph(c);
+                r = dalloc(c, c->g_samples, nr * guide::SAMPLE_FLOATS);
+                if (r) return r;
+                c->g_samples_cap = nr;
+            }
+        }
+    }
     if (c->views_cap < vd.size() || !c->d_views) {
         drop_graph(c);
         r = dalloc(c, c->d_views, vd.size());
@@ -1234,16 +1504,22 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         HIPCHK(c, hipMemsetAsync(c->film2, 0, 3 * sizeof(double) * film_px, c->stream));
     }
     WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
+    a.n_rec = (uint32_t)nrec;
     LaunchCfg L{B, G, (flags & CPT_RENDER_COUNT) != 0};
+    L.guided = guided;
+    L.infer_blocks = c->cus;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
     hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, a, (int)passes.size());
+    c->g_samples_total = 0;
+    c->g_steps_total = 0;
     bool timing = (flags & CPT_RENDER_KERNEL_TIMING) != 0;
     bool use_graph = !(flags & (CPT_RENDER_NO_GRAPH | CPT_RENDER_KERNEL_TIMING));
     Timers tm;
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%zu,%p,%p,%p,%p", max_depth, (unsigned long long)seed, B, G,
-                 (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film, (void*)c->d_views, (void*)c->pass_tend);
+        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g", max_depth, (unsigned long long)seed,
+                 B, G, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film, (void*)c->d_views, (void*)c->pass_tend,
+                 (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec, (void*)c->g_samples, (double)a.guide_frac);
         if (!c->gexec || c->graph_key != key) {
             drop_graph(c);
             hipGraph_t g;
@@ -1265,6 +1541,21 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         } else {
             r = issue_pass(c, L, a, c->stream, timing ? &tm : nullptr);
             if (r) return r;
+        }
+        if (train) {
+            HIPCHK(c, hipMemsetAsync(c->g_count, 0, sizeof(uint32_t), c->stream));
+            hipLaunchKernelGGL(k_guide_collect, dim3(G), dim3(B), 0, c->stream, a);
+            HIPCHK(c, hipMemcpyAsync(c->h_count, c->g_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            int n = (int)c->h_count[0];
+            if (n >= 64) {
+                for (int k = 0; k < c->g_cfg.train_steps; k++) {
+                    r = guide_train_device(c, n, true);
+                    if (r) return r;
+                    c->g_steps_total++;
+                }
+                c->g_samples_total += n;
+            }
         }
     }
     unsigned long long st[ST_N];
@@ -1291,6 +1582,8 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         stats->paths_per_pass = (int32_t)slots;
         stats->block_size = B;
         stats->grid_blocks = G;
+        stats->guide_samples = c->g_samples_total;
+        stats->guide_steps = c->g_steps_total;
         for (size_t i = 1; i < tm.ev.size(); i++) {
             float ms = 0;
             if (tm.kind[i] >= 0 && hipEventElapsedTime(&ms, tm.ev[i - 1], tm.ev[i]) == hipSuccess) {
@@ -1328,6 +1621,8 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     int B = p->block_size ? p->block_size : 128;
     int G = p->grid_blocks ? p->grid_blocks : default_grid(c, B);
     int64_t cap = p->max_paths ? p->max_paths : (1 << 25);
+    // online training updates the guide between passes: one sample per pixel per pass
+    if ((p->flags & CPT_RENDER_GUIDE_TRAIN) && !p->max_paths) cap = (int64_t)p->width * p->height;
     return render_batch(c, vd, p->max_depth, p->seed, p->flags, B, G, cap, nullptr, stats);
 }
 
@@ -1595,6 +1890,10 @@ int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
         if (r) return CPT_E_HIP;
         HIPCHK(c, hipFuncSetAttribute((const void*)k_mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
         HIPCHK(c, hipFuncSetAttribute((const void*)k_train_tile, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
+        r |= dalloc(c, c->g_count, 4);
+        if (r) return CPT_E_HIP;
+        HIPCHK(c, hipHostMalloc((void**)&c->h_count, 4 * sizeof(uint32_t), hipHostMallocDefault));
     }
     c->g_cfg = *gp;
     if (c->g_cfg.lr <= 0) c->g_cfg.lr = 1e-3f;
@@ -1602,6 +1901,8 @@ int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
     if (c->g_cfg.beta2 <= 0) c->g_cfg.beta2 = 0.99f;
     if (c->g_cfg.eps <= 0) c->g_cfg.eps = 1e-8f;
     if (c->g_cfg.guide_fraction <= 0 || c->g_cfg.guide_fraction >= 1) c->g_cfg.guide_fraction = 0.5f;
+    if (c->g_cfg.train_paths <= 0) c->g_cfg.train_paths = 65536;
+    if (c->g_cfg.train_steps <= 0) c->g_cfg.train_steps = 1;
     // Glorot-uniform weights from a seeded splitmix64 stream; zero biases except the
     // lobe means, which start at the 8 cube diagonals (a near-uniform mixture)
     std::vector<float> h(NPARAM, 0.0f);

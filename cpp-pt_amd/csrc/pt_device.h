@@ -96,6 +96,7 @@ struct DevScene {
     int n_tris;
     float env[3];
     float cam[16];
+    float bmin[3], bext[3];  // scene bounds (guide features)
 };
 
 struct Hit {

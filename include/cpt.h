@@ -104,6 +104,10 @@ typedef struct {
 #define CPT_RENDER_ACCUMULATE 8     /* add to the film instead of clearing it first */
 #define CPT_RENDER_STREAM 16        /* streaming wavefront (path regeneration) instead of passes;
                                        the film is summed with FP64 atomics (order not fixed) */
+#define CPT_RENDER_GUIDED 32        /* sample directions from the learned guide mixed with the BSDF
+                                       (needs cpt_guide_init) */
+#define CPT_RENDER_GUIDE_TRAIN 64   /* with GUIDED: record training samples and train after every pass
+                                       (passes default to one sample per pixel) */
 
 typedef struct {
     double seconds;           /* wall time of the render call (device synchronised) */
@@ -120,6 +124,9 @@ typedef struct {
     int32_t paths_per_pass;
     int32_t block_size;
     int32_t grid_blocks;
+    int64_t guide_samples;    /* CPT_RENDER_GUIDE_TRAIN: training samples used */
+    int32_t guide_steps;      /* CPT_RENDER_GUIDE_TRAIN: Adam steps taken */
+    int32_t reserved;
 } cpt_render_stats;
 
 /* Create a context on HIP device `device`.  Fails with CPT_E_NODEVICE when the
@@ -193,7 +200,8 @@ typedef struct {
     uint64_t seed;         /* weight initialisation */
     float guide_fraction;  /* probability of sampling the guide (0 = 0.5) */
     int32_t train_paths;   /* guided renders: paths per pass that record training samples (0 = 65536) */
-    int32_t reserved[4];
+    int32_t train_steps;   /* guided renders: Adam steps after each pass (0 = 1) */
+    int32_t reserved[3];
 } cpt_guide_params;
 
 int cpt_guide_num_params(void);
