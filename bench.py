@@ -45,8 +45,12 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--workload", choices=["terrain", "views"], default="terrain",
-                    help="terrain: the headline 1080p line; views: 64-view batch of the box room")
+    ap.add_argument("--workload", choices=["terrain", "views", "guided"], default="terrain",
+                    help="terrain: the headline 1080p line; views: 64-view batch of the box room; "
+                         "guided: equal-time relMSE of guided vs plain PT on the hard-lighting interior")
+    ap.add_argument("--budgets", type=str, default="10,30,60", help="guided: equal-time budgets in seconds")
+    ap.add_argument("--ref-spp-axis", type=int, default=64, help="guided: reference spp per axis (64 -> 4096 spp)")
+    ap.add_argument("--boxes", type=int, default=40000, help="guided: furniture boxes (48 triangles each)")
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales HBM counters)")
@@ -65,6 +69,94 @@ def copy_calibration(mib):
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_mib": mib, "checksum": float(y[:16].sum())}))
+
+
+def bench_guided(args):
+    """Equal-time comparison on the hard-lighting interior (one 0.2x0.2 light outside
+    a cluttered room, light entering through a 0.3-wide door gap): plain PT-NEE-MIS
+    vs guided PT (learned MLP guide trained online, training time included), both
+    accumulated one sample per pixel at a time; relMSE = mean((I-R)^2/(R^2+0.01))
+    against an FP64-accumulated plain reference."""
+    import numpy as np
+
+    import cpt
+
+    W, H = args.width, args.height
+    budgets = [float(b) for b in args.budgets.split(",")]
+    scene = cpt.scenes.interior_scene(res=(W, H), n_boxes=args.boxes)
+    ctx = cpt.Context(0)
+    ctx.upload(scene)
+    depth = args.depth
+    t0 = time.perf_counter()
+    # reference in chunks of at most 64x64 spp (progress lines; distinct seeds per chunk)
+    chunk = min(args.ref_spp_axis, 64)
+    n_chunks = max(1, (args.ref_spp_axis // chunk) ** 2)
+    for k in range(n_chunks):
+        ctx.render(W, H, chunk, max_depth=depth, seed=99991 + 7919 * k,
+                   flags=cpt.CPT_RENDER_ACCUMULATE if k else 0)
+        print(f"bench: reference chunk {k + 1}/{n_chunks}", file=sys.stderr, flush=True)
+    _, ref_sums = ctx.read_film(sums=True)
+    ref_spp = n_chunks * chunk * chunk
+    ref = ref_sums / float(ref_spp)
+    ref_s = time.perf_counter() - t0
+    print(f"bench: reference {ref_spp} spp in {ref_s:.1f} s", file=sys.stderr, flush=True)
+
+    def relmse(img):
+        return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+    def run(guided):
+        if guided:
+            ctx.guide_init(seed=11, lr=3e-3, train_paths=min(W * H, 1 << 18), train_steps=1)
+        flags = (cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN) if guided else 0
+        curve = []
+        elapsed, spp, k = 0.0, 0, 0
+        next_check = 0.25
+        while elapsed < budgets[-1]:
+            t = time.perf_counter()
+            ctx.render(W, H, 1, max_depth=depth, seed=1 + k, flags=flags | (cpt.CPT_RENDER_ACCUMULATE if k else 0))
+            elapsed += time.perf_counter() - t
+            spp += 1
+            k += 1
+            if elapsed >= next_check or elapsed >= budgets[-1]:
+                _, sums = ctx.read_film(sums=True)
+                curve.append((elapsed, spp, relmse(sums / spp)))
+                next_check = elapsed * 1.25
+                print(f"bench: {'guided' if guided else 'plain'} t={elapsed:.2f}s spp={spp} relMSE={curve[-1][2]:.4f}",
+                      file=sys.stderr, flush=True)
+        return curve
+
+    plain = run(False)
+    guided = run(True)
+
+    def at(curve, T):
+        best = curve[0]
+        for c in curve:
+            if c[0] <= T * 1.0001:
+                best = c
+        return best
+
+    rows = []
+    for T in budgets:
+        p_ = at(plain, T)
+        g_ = at(guided, T)
+        reach = next((c[0] for c in guided if c[2] <= p_[2]), None)
+        rows.append({"budget_s": T, "plain_relmse": p_[2], "plain_spp": p_[1], "guided_relmse": g_[2],
+                     "guided_spp": g_[1], "guided_time_to_plain_relmse_s": reach,
+                     "time_ratio": (reach / p_[0]) if reach else None})
+        print(f"bench: T={T:.0f}s plain {p_[2]:.4f} ({p_[1]} spp) guided {g_[2]:.4f} ({g_[1]} spp) "
+              f"guided reaches plain's error at {reach}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "equal-time relMSE guided vs plain PT (1 GPU)",
+        "value": rows[-1]["plain_relmse"] / max(rows[-1]["guided_relmse"], 1e-30),
+        "unit": "relMSE ratio plain/guided at the largest budget",
+        "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "f32 (guide: bf16 MFMA, fp32 acc)",
+        "config": {"workload": f"hard-lighting interior, {scene.n_triangles} triangles, {W}x{H}, max depth {depth}, "
+                               f"reference {ref_spp} spp FP64", "budgets_s": budgets},
+        "budgets": rows,
+        "reference_seconds": ref_s,
+    }
+    ctx.close()
+    print(json.dumps(out))
 
 
 def bench_views(args):
@@ -153,6 +245,9 @@ def main():
         return
     if args.workload == "views":
         bench_views(args)
+        return
+    if args.workload == "guided":
+        bench_guided(args)
         return
     import cpt
 
