@@ -86,7 +86,11 @@ struct WfArgs {
     float4* so;        // shadow o.xyz, tmax
     float4* sd;        // shadow d.xyz, slot
     float4* sc_rgb;    // shadow contribution
-    uint32_t* counters;  // [0],[1] extension queues, [2] shadow queue, [3] ended-path queue
+    // queue counters.  Words 2k and 2k+1 form one 64-bit word per extension queue
+    // k: its ray count (low) and the count of the shadow queue filled by the same
+    // shading launch (high), so that one atomic reserves space in both.  Word 4
+    // counts the ended-path queue (streaming mode).
+    uint32_t* counters;
     unsigned long long* stats;
     int* gstack;
     const PassDesc* pass;
@@ -130,6 +134,43 @@ __device__ __forceinline__ unsigned wave_alloc(bool pred, uint32_t* counter) {
     return base + (unsigned)__popcll(lt);
 }
 
+// Block-wide append to the two queues whose counts share the 64-bit word *ctr
+// (low: q0, high: q1): one atomic per block, order-preserving within a wave.
+// Every thread of the block must call it (it holds two barriers); sh is a
+// 2*B/64+2 word LDS buffer that callers alternate between consecutive calls.
+template <int B>
+__device__ __forceinline__ void block_alloc2(bool p0, bool p1, unsigned long long* ctr, unsigned* sh, unsigned& o0,
+                                             unsigned& o1) {
+    constexpr int NW = B / 64;
+    const unsigned lane = __lane_id(), w = threadIdx.x >> 6;
+    const unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
+    if (lane == 0) {
+        sh[2 * w] = (unsigned)__popcll(m0);
+        sh[2 * w + 1] = (unsigned)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t0 = 0, t1 = 0;
+        for (int k = 0; k < NW; k++) {
+            t0 += sh[2 * k];
+            t1 += sh[2 * k + 1];
+        }
+        unsigned long long old = 0;
+        if (t0 | t1) old = atomicAdd(ctr, ((unsigned long long)t1 << 32) | t0);
+        sh[2 * NW] = (unsigned)old;
+        sh[2 * NW + 1] = (unsigned)(old >> 32);
+    }
+    __syncthreads();
+    unsigned b0 = sh[2 * NW], b1 = sh[2 * NW + 1];
+    for (unsigned k = 0; k < w; k++) {
+        b0 += sh[2 * k];
+        b1 += sh[2 * k + 1];
+    }
+    const unsigned long long lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    o0 = b0 + (unsigned)__popcll(m0 & lt);
+    o1 = b1 + (unsigned)__popcll(m1 & lt);
+}
+
 __device__ __forceinline__ unsigned wave_sum(unsigned v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
@@ -155,6 +196,7 @@ __global__ void k_raygen(WfArgs a) {
         a.counters[1] = 0;
         a.counters[2] = 0;
         a.counters[3] = 0;
+        a.counters[4] = 0;
         atomicAdd(&a.stats[ST_PATHS], (unsigned long long)n_paths);
     }
     for (unsigned i = gtid(); i < n_paths; i += gthreads()) {
@@ -190,11 +232,11 @@ __global__ void k_raygen(WfArgs a) {
 template <int B, bool COUNT>
 __global__ __launch_bounds__(B) void k_extend(WfArgs a, int cur) {
     __shared__ int lds[CPT_LDS_SS * B];
-    const unsigned count = a.counters[cur];
+    const unsigned count = a.counters[2 * cur];
     if (gtid() == 0) {
-        a.counters[cur ^ 1] = 0;
-        a.counters[2] = 0;
-        a.counters[3] = 0;
+        a.counters[2 * (cur ^ 1)] = 0;
+        a.counters[2 * (cur ^ 1) + 1] = 0;
+        a.counters[4] = 0;
         atomicAdd(&a.stats[ST_EXT], (unsigned long long)count);
     }
     TravStack<B> st{lds, a.gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
@@ -268,7 +310,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     guide::SmemFwd& sm = *reinterpret_cast<guide::SmemFwd*>(smem_raw);
     __shared__ float o_tile[guide::TILE * guide::NOUT];
     __shared__ float in_s[guide::TILE * guide::NIN];
-    const unsigned count = a.counters[cur];
+    const unsigned count = a.counters[2 * cur];
     if (blockIdx.x * guide::TILE >= count) return;
     guide::load_weights(sm, a.gwt, a.gparams);
     for (unsigned t = blockIdx.x; t * guide::TILE < count; t += gridDim.x) {
@@ -355,11 +397,13 @@ __global__ void k_guide_collect(WfArgs a) {
 // pathtracerMIS.h:37-126, one loop iteration per queue entry.
 template <int B, bool STREAM, bool GUIDED>
 __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
-    const unsigned count = a.counters[cur];
-    const unsigned lane = __lane_id();
-    const unsigned wave = gtid() >> 6, nwaves = gthreads() >> 6;
-    for (unsigned base = wave * 64; base < count; base += nwaves * 64) {
-        const unsigned i = base + lane;
+    __shared__ unsigned qsh[2][2 * (B / 64) + 2];
+    const unsigned count = a.counters[2 * cur];
+    unsigned long long* const out_ctr = reinterpret_cast<unsigned long long*>(a.counters + 2 * (cur ^ 1));
+    int par = 0;
+    // block-uniform trip count: block_alloc2 holds barriers
+    for (unsigned base = blockIdx.x * B; base < count; base += gridDim.x * B, par ^= 1) {
+        const unsigned i = base + threadIdx.x;
         const bool active = i < count;
         bool push_ext = false, push_sh = false;
         V3 no = {0, 0, 0}, nd = {0, 0, 0}, sho = {0, 0, 0}, shd = {0, 0, 0}, contrib = {0, 0, 0};
@@ -501,17 +545,17 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
             }
             push_ext = alive;
         }
-        unsigned qe = wave_alloc(push_ext, &a.counters[cur ^ 1]);
+        unsigned qe, qs;
+        block_alloc2<B>(push_ext, push_sh, out_ctr, qsh[par], qe, qs);
         if (push_ext) {
             a.qo[cur ^ 1][qe] = make_float4(no.x, no.y, no.z, INFINITY);
             a.qd[cur ^ 1][qe] = make_float4(nd.x, nd.y, nd.z, __int_as_float(slot));
         }
         if (STREAM) {
             bool ended = active && !push_ext;
-            unsigned qd = wave_alloc(ended, &a.counters[3]);
+            unsigned qd = wave_alloc(ended, &a.counters[4]);
             if (ended) a.done[qd] = (uint32_t)slot;
         }
-        unsigned qs = wave_alloc(push_sh, &a.counters[2]);
         if (push_sh) {
             a.so[qs] = make_float4(sho.x, sho.y, sho.z, shmax);
             a.sd[qs] = make_float4(shd.x, shd.y, shd.z, __int_as_float(slot));
@@ -522,9 +566,9 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
 
 // ------------------------------------------------------------------ shadow
 template <int B, bool COUNT>
-__global__ __launch_bounds__(B) void k_shadow(WfArgs a) {
+__global__ __launch_bounds__(B) void k_shadow(WfArgs a, int nxt) {
     __shared__ int lds[CPT_LDS_SS * B];
-    const unsigned count = a.counters[2];
+    const unsigned count = a.counters[2 * nxt + 1];
     if (gtid() == 0) atomicAdd(&a.stats[ST_SHADOW], (unsigned long long)count);
     TravStack<B> st{lds, a.gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
     unsigned nv = 0, nt = 0;
@@ -603,8 +647,8 @@ __global__ void k_stream_init(WfArgs a, StreamArgs sa, unsigned cap) {
     if (gtid() == 0) {
         *sa.next_path = 0;
         *sa.t_start = __builtin_amdgcn_s_memrealtime();
-        a.counters[0] = a.counters[1] = a.counters[2] = 0;
-        a.counters[3] = cap;
+        a.counters[0] = a.counters[1] = a.counters[2] = a.counters[3] = 0;
+        a.counters[4] = cap;
     }
     for (unsigned i = gtid(); i < cap; i += gthreads()) {
         a.done[i] = i;
@@ -620,7 +664,7 @@ __global__ void k_stream_init(WfArgs a, StreamArgs sa, unsigned cap) {
 // and start new camera paths in their slots.
 template <int B>
 __global__ __launch_bounds__(B) void k_regen(WfArgs a, StreamArgs sa, int nxt) {
-    const unsigned count = a.counters[3];
+    const unsigned count = a.counters[4];
     const unsigned lane = __lane_id();
     const unsigned wave = gtid() >> 6, nwaves = gthreads() >> 6;
     for (unsigned base = wave * 64; base < count; base += nwaves * 64) {
@@ -686,7 +730,7 @@ __global__ __launch_bounds__(B) void k_regen(WfArgs a, StreamArgs sa, int nxt) {
         } else if (active) {
             a.pix[slot] = CPT_NO_PATH;
         }
-        unsigned qe = wave_alloc(start, &a.counters[nxt]);
+        unsigned qe = wave_alloc(start, &a.counters[2 * nxt]);
         if (start) {
             a.qo[nxt][qe] = make_float4(o.x, o.y, o.z, INFINITY);
             a.qd[nxt][qe] = make_float4(d.x, d.y, d.z, __int_as_float((int)slot));
@@ -744,6 +788,7 @@ struct cpt_ctx {
     Node4* d_nodes = nullptr;
     float4* d_tris = nullptr;
     int4* d_tri_info = nullptr;
+    float4* d_shade = nullptr;
     float4 *d_vpos = nullptr, *d_vnrm = nullptr, *d_vbit = nullptr;
     float* d_mats = nullptr;
     int* d_lights = nullptr;
@@ -831,6 +876,7 @@ void free_scene(cpt_ctx* c) {
     dfree(c->d_nodes);
     dfree(c->d_tris);
     dfree(c->d_tri_info);
+    dfree(c->d_shade);
     dfree(c->d_vpos);
     dfree(c->d_vnrm);
     dfree(c->d_vbit);
@@ -869,6 +915,7 @@ DevScene dev_scene(const cpt_ctx* c) {
     s.nodes = c->d_nodes;
     s.tris = c->d_tris;
     s.tri_info = c->d_tri_info;
+    s.shade = c->d_shade;
     s.vpos = c->d_vpos;
     s.vnrm = c->d_vnrm;
     s.vbit = c->d_vbit;
@@ -980,6 +1027,7 @@ struct LaunchCfg {
     bool stream = false;
     bool guided = false;
     int infer_blocks = 0;
+    int SB = 256;  // shading kernel block
 };
 
 // launch helpers dispatching on the block size
@@ -990,14 +1038,16 @@ void launch_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) 
 }
 template <int B>
 void launch_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
-    if (L.stream) hipLaunchKernelGGL((k_shade<B, true, false>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
-    else if (L.guided) hipLaunchKernelGGL((k_shade<B, false, true>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
-    else hipLaunchKernelGGL((k_shade<B, false, false>), dim3(L.G), dim3(B), 0, s, a, cur, cap);
+    // same thread count as the other grid-stride kernels
+    const unsigned G = (unsigned)std::max(1, (int)(((long long)L.G * L.B) / B));
+    if (L.stream) hipLaunchKernelGGL((k_shade<B, true, false>), dim3(G), dim3(B), 0, s, a, cur, cap);
+    else if (L.guided) hipLaunchKernelGGL((k_shade<B, false, true>), dim3(G), dim3(B), 0, s, a, cur, cap);
+    else hipLaunchKernelGGL((k_shade<B, false, false>), dim3(G), dim3(B), 0, s, a, cur, cap);
 }
 template <int B>
-void launch_shadow(const LaunchCfg& L, const WfArgs& a, hipStream_t s) {
-    if (L.count) hipLaunchKernelGGL((k_shadow<B, true>), dim3(L.G), dim3(B), 0, s, a);
-    else hipLaunchKernelGGL((k_shadow<B, false>), dim3(L.G), dim3(B), 0, s, a);
+void launch_shadow(const LaunchCfg& L, const WfArgs& a, int nxt, hipStream_t s) {
+    if (L.count) hipLaunchKernelGGL((k_shadow<B, true>), dim3(L.G), dim3(B), 0, s, a, nxt);
+    else hipLaunchKernelGGL((k_shadow<B, false>), dim3(L.G), dim3(B), 0, s, a, nxt);
 }
 
 void L_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) {
@@ -1008,17 +1058,19 @@ void L_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) {
     }
 }
 void L_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
-    switch (L.B) {
+    switch (L.SB) {
         case 64: launch_shade<64>(L, a, cur, cap, s); break;
-        case 256: launch_shade<256>(L, a, cur, cap, s); break;
-        default: launch_shade<128>(L, a, cur, cap, s); break;
+        case 128: launch_shade<128>(L, a, cur, cap, s); break;
+        case 512: launch_shade<512>(L, a, cur, cap, s); break;
+        case 1024: launch_shade<1024>(L, a, cur, cap, s); break;
+        default: launch_shade<256>(L, a, cur, cap, s); break;
     }
 }
-void L_shadow(const LaunchCfg& L, const WfArgs& a, hipStream_t s) {
+void L_shadow(const LaunchCfg& L, const WfArgs& a, int nxt, hipStream_t s) {
     switch (L.B) {
-        case 64: launch_shadow<64>(L, a, s); break;
-        case 256: launch_shadow<256>(L, a, s); break;
-        default: launch_shadow<128>(L, a, s); break;
+        case 64: launch_shadow<64>(L, a, nxt, s); break;
+        case 256: launch_shadow<256>(L, a, nxt, s); break;
+        default: launch_shadow<128>(L, a, nxt, s); break;
     }
 }
 
@@ -1048,7 +1100,7 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
             hipLaunchKernelGGL(k_guide_infer, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a, cur);
         L_shade(L, a, cur, a.max_depth, s);
         mark(1);
-        L_shadow(L, a, s);
+        L_shadow(L, a, cur ^ 1, s);
         mark(2);
     }
     hipLaunchKernelGGL(k_accum, dim3(L.G), dim3(L.B), 0, s, a);
@@ -1091,7 +1143,7 @@ int cpt_create(cpt_ctx** out, int device) {
         return r;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc((void**)&c->counters, 4 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&c->counters, 8 * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void**)&c->stats, ST_N * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc((void**)&c->d_pass, sizeof(PassDesc)) != hipSuccess) {
         delete c;
@@ -1202,10 +1254,42 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     std::vector<int4> ti(std::max(1, c->nt));
     for (int t = 0; t < c->nt; t++)
         ti[t] = make_int4(d->indices[3 * (size_t)t], d->indices[3 * (size_t)t + 1], d->indices[3 * (size_t)t + 2], d->tri_material[t]);
+    // shading records: geometric normal with the reference's float operations
+    // (normalized(cross(b-a, c-a)), triangle.h:109; math.h:132-140), then vertex
+    // normals and bitangents
+    std::vector<float4> sh(6 * (size_t)std::max(1, c->nt), make_float4(0, 0, 0, 0));
+    for (int t = 0; t < c->nt; t++) {
+        const float* P = d->positions;
+        int i0 = d->indices[3 * (size_t)t], i1 = d->indices[3 * (size_t)t + 1], i2 = d->indices[3 * (size_t)t + 2];
+        float e1[3], e2[3];
+        for (int k = 0; k < 3; k++) {
+            e1[k] = P[3 * (size_t)i1 + k] - P[3 * (size_t)i0 + k];
+            e2[k] = P[3 * (size_t)i2 + k] - P[3 * (size_t)i0 + k];
+        }
+        float cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+        float dd = cx * cx + cy * cy + cz * cz;
+        float len = (float)std::sqrt((double)dd);
+        float inv = (float)(1.0 / (double)len);
+        float v[24];
+        v[0] = cx * inv;
+        v[1] = cy * inv;
+        v[2] = cz * inv;
+        int32_t m = d->tri_material[t];
+        std::memcpy(&v[3], &m, 4);
+        const int vi[3] = {i0, i1, i2};
+        for (int j = 0; j < 3; j++)
+            for (int k = 0; k < 3; k++) {
+                v[4 + 3 * j + k] = d->normals[3 * (size_t)vi[j] + k];
+                v[13 + 3 * j + k] = d->bitangents[3 * (size_t)vi[j] + k];
+            }
+        v[22] = v[23] = 0.0f;
+        std::memcpy(&sh[6 * (size_t)t], v, sizeof(v));
+    }
     int r = 0;
     r |= dalloc(c, c->d_nodes, bvh.nodes.size());
     r |= dalloc(c, c->d_tris, 3 * std::max<size_t>(1, bvh.tris.size()));
     r |= dalloc(c, c->d_tri_info, ti.size());
+    r |= dalloc(c, c->d_shade, sh.size());
     r |= dalloc(c, c->d_vpos, vp.size());
     r |= dalloc(c, c->d_vnrm, vn.size());
     r |= dalloc(c, c->d_vbit, vb.size());
@@ -1216,6 +1300,7 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     if (!bvh.tris.empty())
         HIPCHK(c, hipMemcpy(c->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRec), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_tri_info, ti.data(), ti.size() * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_shade, sh.data(), sh.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_vpos, vp.data(), vp.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_vnrm, vn.data(), vn.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_vbit, vb.data(), vb.size() * sizeof(float4), hipMemcpyHostToDevice));
@@ -1411,9 +1496,11 @@ static void plan_passes(const std::vector<ViewDesc>& vd, int64_t cap, std::vecto
 
 // Render a batch of views into the concatenated FP64 films.
 static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, uint64_t seed, int flags, int B, int G,
-                        int64_t max_paths, double* view_ms, cpt_render_stats* stats) {
+                        int SB, int64_t max_paths, double* view_ms, cpt_render_stats* stats) {
     if (max_depth <= 0 || max_depth > 255) return fail(c, CPT_E_INVALID, "max_depth must be in [1, 255]");
     if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
+    if (SB != 64 && SB != 128 && SB != 256 && SB != 512 && SB != 1024)
+        return fail(c, CPT_E_INVALID, "shade_block must be 64, 128, 256, 512 or 1024");
     if (G <= 0) return fail(c, CPT_E_INVALID, "grid_blocks must be positive");
     unsigned long long film_px = 0;
     for (size_t v = 0; v < vd.size(); v++) {
@@ -1506,6 +1593,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
     a.n_rec = (uint32_t)nrec;
     LaunchCfg L{B, G, (flags & CPT_RENDER_COUNT) != 0};
+    L.SB = SB;
     L.guided = guided;
     L.infer_blocks = c->cus;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
@@ -1517,8 +1605,8 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     Timers tm;
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g", max_depth, (unsigned long long)seed,
-                 B, G, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film, (void*)c->d_views, (void*)c->pass_tend,
+        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g", max_depth,
+                 (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film, (void*)c->d_views, (void*)c->pass_tend,
                  (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec, (void*)c->g_samples, (double)a.guide_frac);
         if (!c->gexec || c->graph_key != key) {
             drop_graph(c);
@@ -1623,7 +1711,8 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     int64_t cap = p->max_paths ? p->max_paths : (1 << 25);
     // online training updates the guide between passes: one sample per pixel per pass
     if ((p->flags & CPT_RENDER_GUIDE_TRAIN) && !p->max_paths) cap = (int64_t)p->width * p->height;
-    return render_batch(c, vd, p->max_depth, p->seed, p->flags, B, G, cap, nullptr, stats);
+    int SB = p->shade_block ? p->shade_block : 256;
+    return render_batch(c, vd, p->max_depth, p->seed, p->flags, B, G, SB, cap, nullptr, stats);
 }
 
 int cpt_read_film(cpt_ctx* c, float* rgb, double* sums, double* sumsq) {
@@ -1728,7 +1817,7 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
                 int cur = it & 1;
                 L_extend(L, a, cur, st);
                 L_shade(L, a, cur, max_depth, st);
-                L_shadow(L, a, st);
+                L_shadow(L, a, cur ^ 1, st);
                 hipLaunchKernelGGL((k_regen<B>), dim3(G), dim3(B), 0, st, a, sa, cur ^ 1);
             }
         };
@@ -1850,8 +1939,8 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
         vd[v].id = (unsigned)v;
     }
     const int B = 128;
-    int r = render_batch(c, vd, max_depth, seed, flags & ~CPT_RENDER_ACCUMULATE, B, default_grid(c, B), 1 << 25, view_ms,
-                         stats);
+    int r = render_batch(c, vd, max_depth, seed, flags & ~CPT_RENDER_ACCUMULATE, B, default_grid(c, B), 256, 1 << 25,
+                         view_ms, stats);
     if (r) return r;
     if (films) {
         size_t px = 0;

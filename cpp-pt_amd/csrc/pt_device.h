@@ -87,6 +87,7 @@ struct DevScene {
     const Node4* nodes;
     const float4* tris;      // 3 per triangle, leaf order (TriRec)
     const int4* tri_info;    // v0, v1, v2, material (by triangle id)
+    const float4* shade;     // 6 per triangle id: g_normal + material, then n0 n1 n2 b0 b1 b2 packed
     const float4* vpos;      // xyz
     const float4* vnrm;
     const float4* vbit;
