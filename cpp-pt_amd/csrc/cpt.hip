@@ -104,6 +104,7 @@ struct WfArgs {
     uint32_t n_rec;          // slots that record
     int rec_depth;
     float guide_frac;
+    int guide_depth;         // vertices at depth >= this are not guided
     float* gsamples;         // training samples out (11 floats)
     uint32_t* gcount;
     const uint16_t* gwt;
@@ -113,8 +114,9 @@ struct WfArgs {
     uint32_t view;
 };
 
-// training record of one guided path vertex: inputs[7], dir[3], pdf, mul_next[3],
-// radiance snapshot[3], state (0 empty, 1 direction traced, 2 complete)
+// training record of one guided path vertex: inputs[7], dir[3], pdf, path throughput
+// before the vertex [3], radiance snapshot[3], state (0 empty, 1 direction traced,
+// 2 complete)
 #define REC_FLOATS 18
 
 __device__ __forceinline__ unsigned gthreads() { return gridDim.x * blockDim.x; }
@@ -375,11 +377,13 @@ __global__ void k_guide_collect(WfArgs a) {
             if (r[17] == 2.0f) {
                 unsigned slot = idx / (unsigned)a.rec_depth;
                 float4 c = a.col[slot];
+                // radiance gathered through the sampled direction, divided by the
+                // throughput up to the vertex: L_i f cos / p, the product
+                // integrand over the density the direction was drawn from
                 float lr = r[11] > 0.0f ? (c.x - r[14]) / r[11] : 0.0f;
                 float lg = r[12] > 0.0f ? (c.y - r[15]) / r[12] : 0.0f;
                 float lb = r[13] > 0.0f ? (c.z - r[16]) / r[13] : 0.0f;
-                float F = fmaxf(0.2126f * lr + 0.7152f * lg + 0.0722f * lb, 0.0f);
-                float w = r[10] > 0.0f ? F / r[10] : 0.0f;
+                float w = fmaxf(0.2126f * lr + 0.7152f * lg + 0.0722f * lb, 0.0f);
                 if (w > 0.0f && isfinite(w)) {
                     emit = true;
                     for (int k = 0; k < 10; k++) smp[k] = r[k];
@@ -461,7 +465,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                 } else {
                     sampled_delta = false;
                     const float* lob = GUIDED ? a.lobes + (size_t)i * 40 : nullptr;
-                    const bool gv = GUIDED && guidable(m);
+                    const bool gv = GUIDED && depth < a.guide_depth && guidable(m);
                     const float alpha = a.guide_frac;
                     DSample ds;
                     if (gv) {
@@ -514,6 +518,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                             else p = p * rr;
                         }
                         if (survive) {
+                            const V3 mul_prev = mulv;
                             mulv = divs(mul(mulv, ev), p);
                             if (depth + 1 < depth_cap) {
                                 no = add(it.hitpoint, mulf(ds.wi, CPT_EPSF));
@@ -527,9 +532,9 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                                     r[8] = ds.wi.y;
                                     r[9] = ds.wi.z;
                                     r[10] = p_sampled;
-                                    r[11] = mulv.x;
-                                    r[12] = mulv.y;
-                                    r[13] = mulv.z;
+                                    r[11] = mul_prev.x;
+                                    r[12] = mul_prev.y;
+                                    r[13] = mul_prev.z;
                                     r[17] = 1.0f;
                                 }
                             }
@@ -1005,6 +1010,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.n_rec = 0;
     a.rec_depth = depth;
     a.guide_frac = c->g_cfg.guide_fraction;
+    a.guide_depth = c->g_cfg.guide_depth > 0 ? c->g_cfg.guide_depth : 3;
     a.gsamples = c->g_samples;
     a.gcount = c->g_count;
     a.gwt = c->g_wt;
@@ -1096,7 +1102,7 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
         int cur = b & 1;
         L_extend(L, a, cur, s);
         mark(0);
-        if (L.guided)
+        if (L.guided && b < a.guide_depth)
             hipLaunchKernelGGL(k_guide_infer, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a, cur);
         L_shade(L, a, cur, a.max_depth, s);
         mark(1);
@@ -1605,9 +1611,10 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     Timers tm;
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g", max_depth,
-                 (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film, (void*)c->d_views, (void*)c->pass_tend,
-                 (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec, (void*)c->g_samples, (double)a.guide_frac);
+        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d", max_depth,
+                 (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
+                 (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
+                 (void*)c->g_samples, (double)a.guide_frac, a.guide_depth);
         if (!c->gexec || c->graph_key != key) {
             drop_graph(c);
             hipGraph_t g;

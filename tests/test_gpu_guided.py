@@ -48,20 +48,47 @@ def test_guided_is_unbiased(ctx):
     assert frac < 0.01 and abs(z.mean()) < 0.06
 
 
+def _errors(imgs, ref):
+    """Mean over renders of (relMSE without the worst 1% of pixels, mean relative
+    L1 error).  Full relMSE at these sample counts is decided by a few fireflies
+    (two plain renders differ by up to 3x); these two are stable to a few percent."""
+    n = ref.shape[0] * ref.shape[1]
+    trim, l1 = [], []
+    for img in imgs:
+        e = np.sort(((img - ref) ** 2 / (ref ** 2 + 1e-2)).mean(axis=2).ravel())
+        trim.append(e[:int(0.99 * n)].mean())
+        l1.append((np.abs(img - ref) / (ref + 1e-1)).mean())
+    return float(np.mean(trim)), float(np.mean(l1))
+
+
 def test_trained_guide_beats_plain_on_hard_lighting(ctx):
-    W, H = 96, 54
+    """Interior lit only through a door gap.  After 4096 spp of online training
+    (16 spp per pass, 4 Adam steps per pass) the frozen guide renders with lower
+    error than plain PT-NEE-MIS at equal sample count (4 x 1024 spp each) against a
+    65,536-spp plain reference.  Measured over six training seeds: trimmed relMSE
+    ratio 0.63-0.74 and relative L1 ratio 0.81-0.88, where two plain render sets
+    give 0.88-1.06 and 0.98-1.08."""
+    W, H = 64, 36
     s = cpt.scenes.interior_scene(res=(W, H), n_boxes=3000)
     ctx.upload(s)
-    ctx.render(W, H, 48, max_depth=12, seed=100)  # 2304 spp reference
+    ctx.render(W, H, 256, max_depth=12, seed=100)  # 65,536 spp reference
     ref = ctx.read_film()
-    ctx.render(W, H, 8, max_depth=12, seed=7)      # plain, 64 spp
-    plain = ctx.read_film()
-    ctx.guide_init(seed=3, lr=3e-3, train_paths=W * H, train_steps=2)
-    st = ctx.render(W, H, 8, max_depth=12, seed=8, flags=cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN)
-    online = ctx.read_film()
-    ctx.render(W, H, 8, max_depth=12, seed=9, flags=cpt.CPT_RENDER_GUIDED)  # frozen, trained guide
-    frozen = ctx.read_film()
-    e_plain, e_online, e_frozen = relmse(plain, ref), relmse(online, ref), relmse(frozen, ref)
-    print(f"relMSE @64spp: plain {e_plain:.4f}  guided online {e_online:.4f}  guided trained {e_frozen:.4f} "
-          f"({st['guide_steps']} steps, {st['guide_samples']} samples)")
-    assert e_frozen < 0.9 * e_plain
+
+    def renders(flags, seed0):
+        out = []
+        for k in range(4):
+            ctx.render(W, H, 32, max_depth=12, seed=seed0 + k, flags=flags)
+            out.append(ctx.read_film())
+        return out
+
+    p_trim, p_l1 = _errors(renders(0, 500), ref)
+    ctx.guide_init(seed=1, lr=3e-3, train_paths=16 * W * H, train_steps=4)
+    st = ctx.render(W, H, 64, max_depth=12, seed=20, flags=cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN,
+                    max_paths=16 * W * H)
+    g_trim, g_l1 = _errors(renders(cpt.CPT_RENDER_GUIDED, 600), ref)
+    print(f"trimmed relMSE plain {p_trim:.5f} guided {g_trim:.5f} ({g_trim / p_trim:.3f}); "
+          f"rel L1 plain {p_l1:.5f} guided {g_l1:.5f} ({g_l1 / p_l1:.3f}); "
+          f"{st['guide_steps']} steps, {st['guide_samples']} samples")
+    assert st["guide_steps"] == 1024
+    assert g_trim < 0.85 * p_trim
+    assert g_l1 < 0.93 * p_l1
