@@ -61,7 +61,7 @@ class RenderParams(C.Structure):
         ("width", C.c_int32), ("height", C.c_int32), ("spp_axis", C.c_int32),
         ("max_depth", C.c_int32), ("seed", C.c_uint64), ("flags", C.c_int32),
         ("max_paths", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
-        ("view", C.c_int32), ("shade_block", C.c_int32), ("reserved", C.c_int32 * 2),
+        ("view", C.c_int32), ("shade_block", C.c_int32), ("refill", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -226,12 +226,13 @@ class Context:
         return occ
 
     def render(self, width, height, spp_axis, max_depth=32, seed=1, flags=0, max_paths=0,
-               block_size=0, grid_blocks=0, view=0, shade_block=0):
+               block_size=0, grid_blocks=0, view=0, shade_block=0, refill=0):
         p = RenderParams()
         p.width, p.height, p.spp_axis, p.max_depth = width, height, spp_axis, max_depth
         p.seed, p.flags, p.max_paths = seed, flags, max_paths
         p.block_size, p.grid_blocks, p.view = block_size, grid_blocks, view
         p.shade_block = shade_block
+        p.refill = refill
         st = RenderStats()
         self._check(self.lib.cpt_render(self.h, C.byref(p), C.byref(st)), "cpt_render")
         self._last = (width, height)

@@ -104,7 +104,8 @@ struct WfArgs {
     uint32_t n_rec;          // slots that record
     int rec_depth;
     float guide_frac;
-    int guide_depth;         // vertices at depth >= this are not guided
+    int guide_depth;
+    unsigned refill;         // idle lanes that make a traversal wave fetch new rays         // vertices at depth >= this are not guided
     float* gsamples;         // training samples out (11 floats)
     uint32_t* gcount;
     const uint16_t* gwt;
@@ -118,6 +119,9 @@ struct WfArgs {
 // before the vertex [3], radiance snapshot[3], state (0 empty, 1 direction traced,
 // 2 complete)
 #define REC_FLOATS 18
+
+// idle lanes (of 64) that make a traversal wave fetch new rays of its chunk
+#define CPT_DEFAULT_REFILL 32u
 
 __device__ __forceinline__ unsigned gthreads() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ unsigned gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -243,12 +247,18 @@ __global__ __launch_bounds__(B) void k_extend(WfArgs a, int cur) {
     }
     TravStack<B> st{lds, a.gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
     unsigned nv = 0, nt = 0;
-    for (unsigned i = gtid(); i < count; i += gthreads()) {
-        float4 O = a.qo[cur][i], D = a.qd[cur][i];
-        Hit h;
-        traverse<B, false, COUNT>(a.sc, v3(O.x, O.y, O.z), v3(D.x, D.y, D.z), O.w, h, st, nv, nt);
-        a.hits[i] = make_float4(h.t, h.beta, h.gamma, __int_as_float(h.tri));
-    }
+    const float4* qo = a.qo[cur];
+    const float4* qd = a.qd[cur];
+    float4* hits = a.hits;
+    trace_chunked<B, false, COUNT>(
+        a.sc, count, a.refill, st,
+        [&](unsigned i, V3& o, V3& d, float& tmax) {
+            float4 O = qo[i], D = qd[i];
+            o = v3(O.x, O.y, O.z);
+            d = v3(D.x, D.y, D.z);
+            tmax = O.w;
+        },
+        [&](unsigned i, const Hit& h) { hits[i] = make_float4(h.t, h.beta, h.gamma, __int_as_float(h.tri)); }, nv, nt);
     if (COUNT) {
         nv = wave_sum(nv);
         nt = wave_sum(nt);
@@ -577,16 +587,22 @@ __global__ __launch_bounds__(B) void k_shadow(WfArgs a, int nxt) {
     if (gtid() == 0) atomicAdd(&a.stats[ST_SHADOW], (unsigned long long)count);
     TravStack<B> st{lds, a.gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
     unsigned nv = 0, nt = 0;
-    for (unsigned i = gtid(); i < count; i += gthreads()) {
-        float4 O = a.so[i], D = a.sd[i];
-        Hit h;
-        bool occ = traverse<B, true, COUNT>(a.sc, v3(O.x, O.y, O.z), v3(D.x, D.y, D.z), O.w, h, st, nv, nt);
-        if (!occ) {
-            int slot = __float_as_int(D.w);
-            float4 c = a.col[slot], add_ = a.sc_rgb[i];
-            a.col[slot] = make_float4(c.x + add_.x, c.y + add_.y, c.z + add_.z, 0.0f);
-        }
-    }
+    trace_chunked<B, true, COUNT>(
+        a.sc, count, a.refill, st,
+        [&](unsigned i, V3& o, V3& d, float& tmax) {
+            float4 O = a.so[i], D = a.sd[i];
+            o = v3(O.x, O.y, O.z);
+            d = v3(D.x, D.y, D.z);
+            tmax = O.w;
+        },
+        [&](unsigned i, const Hit& h) {
+            if (h.tri < 0) {
+                int slot = __float_as_int(a.sd[i].w);
+                float4 c = a.col[slot], add_ = a.sc_rgb[i];
+                a.col[slot] = make_float4(c.x + add_.x, c.y + add_.y, c.z + add_.z, 0.0f);
+            }
+        },
+        nv, nt);
     if (COUNT) {
         nv = wave_sum(nv);
         nt = wave_sum(nt);
@@ -750,28 +766,37 @@ __global__ void k_fill_u64(unsigned long long* p, int n) {
 
 // ------------------------------------------------------------------ ray-batch parity entries
 template <int B>
-__global__ __launch_bounds__(B) void k_trace_closest(DevScene sc, const float4* rays, int n, float4* hits, int* gstack) {
+__global__ __launch_bounds__(B) void k_trace_closest(DevScene sc, const float4* rays, int n, float4* hits, int* gstack,
+                                                     unsigned refill) {
     __shared__ int lds[CPT_LDS_SS * B];
     TravStack<B> st{lds, gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
     unsigned nv = 0, nt = 0;
-    for (unsigned i = gtid(); i < (unsigned)n; i += gthreads()) {
-        float4 O = rays[2 * i], D = rays[2 * i + 1];
-        Hit h;
-        traverse<B, false, false>(sc, v3(O.x, O.y, O.z), v3(D.x, D.y, D.z), O.w, h, st, nv, nt);
-        hits[i] = make_float4(h.t, h.beta, h.gamma, __int_as_float(h.tri));
-    }
+    trace_chunked<B, false, false>(
+        sc, (unsigned)n, refill, st,
+        [&](unsigned i, V3& o, V3& d, float& tmax) {
+            float4 O = rays[2 * i], D = rays[2 * i + 1];
+            o = v3(O.x, O.y, O.z);
+            d = v3(D.x, D.y, D.z);
+            tmax = O.w;
+        },
+        [&](unsigned i, const Hit& h) { hits[i] = make_float4(h.t, h.beta, h.gamma, __int_as_float(h.tri)); }, nv, nt);
 }
 
 template <int B>
-__global__ __launch_bounds__(B) void k_trace_any(DevScene sc, const float4* rays, int n, uint8_t* occ, int* gstack) {
+__global__ __launch_bounds__(B) void k_trace_any(DevScene sc, const float4* rays, int n, uint8_t* occ, int* gstack,
+                                                 unsigned refill) {
     __shared__ int lds[CPT_LDS_SS * B];
     TravStack<B> st{lds, gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
     unsigned nv = 0, nt = 0;
-    for (unsigned i = gtid(); i < (unsigned)n; i += gthreads()) {
-        float4 O = rays[2 * i], D = rays[2 * i + 1];
-        Hit h;
-        occ[i] = traverse<B, true, false>(sc, v3(O.x, O.y, O.z), v3(D.x, D.y, D.z), O.w, h, st, nv, nt) ? 1 : 0;
-    }
+    trace_chunked<B, true, false>(
+        sc, (unsigned)n, refill, st,
+        [&](unsigned i, V3& o, V3& d, float& tmax) {
+            float4 O = rays[2 * i], D = rays[2 * i + 1];
+            o = v3(O.x, O.y, O.z);
+            d = v3(D.x, D.y, D.z);
+            tmax = O.w;
+        },
+        [&](unsigned i, const Hit& h) { occ[i] = h.tri >= 0 ? 1 : 0; }, nv, nt);
 }
 
 }  // namespace
@@ -1011,6 +1036,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.rec_depth = depth;
     a.guide_frac = c->g_cfg.guide_fraction;
     a.guide_depth = c->g_cfg.guide_depth > 0 ? c->g_cfg.guide_depth : 3;
+    a.refill = CPT_DEFAULT_REFILL;
     a.gsamples = c->g_samples;
     a.gcount = c->g_count;
     a.gwt = c->g_wt;
@@ -1375,9 +1401,11 @@ static int trace_common(cpt_ctx* c, const float* rays, int32_t n, bool any, int3
     } else {
         DevScene sc = dev_scene(c);
         if (any)
-            hipLaunchKernelGGL((k_trace_any<B>), dim3(G), dim3(B), 0, c->stream, sc, d_rays, n, (uint8_t*)d_out, c->gstack);
+            hipLaunchKernelGGL((k_trace_any<B>), dim3(G), dim3(B), 0, c->stream, sc, d_rays, n, (uint8_t*)d_out, c->gstack,
+                               CPT_DEFAULT_REFILL);
         else
-            hipLaunchKernelGGL((k_trace_closest<B>), dim3(G), dim3(B), 0, c->stream, sc, d_rays, n, (float4*)d_out, c->gstack);
+            hipLaunchKernelGGL((k_trace_closest<B>), dim3(G), dim3(B), 0, c->stream, sc, d_rays, n, (float4*)d_out, c->gstack,
+                               CPT_DEFAULT_REFILL);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(c, CPT_E_HIP, "trace launch failed: %s", hipGetErrorString(e));
     }
@@ -1426,7 +1454,7 @@ int cpt_trace_closest_device(cpt_ctx* c, const void* d_rays, int32_t n, void* d_
     if (ret) return ret;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     hipLaunchKernelGGL((k_trace_closest<B>), dim3(G), dim3(B), 0, s, dev_scene(c), (const float4*)d_rays, n,
-                       (float4*)d_hits, c->gstack);
+                       (float4*)d_hits, c->gstack, CPT_DEFAULT_REFILL);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, CPT_E_HIP, "trace launch failed: %s", hipGetErrorString(e));
     return CPT_OK;
@@ -1502,8 +1530,9 @@ static void plan_passes(const std::vector<ViewDesc>& vd, int64_t cap, std::vecto
 
 // Render a batch of views into the concatenated FP64 films.
 static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, uint64_t seed, int flags, int B, int G,
-                        int SB, int64_t max_paths, double* view_ms, cpt_render_stats* stats) {
+                        int SB, int refill, int64_t max_paths, double* view_ms, cpt_render_stats* stats) {
     if (max_depth <= 0 || max_depth > 255) return fail(c, CPT_E_INVALID, "max_depth must be in [1, 255]");
+    if (refill < 1 || refill > 64) return fail(c, CPT_E_INVALID, "refill must be in [1, 64]");
     if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
     if (SB != 64 && SB != 128 && SB != 256 && SB != 512 && SB != 1024)
         return fail(c, CPT_E_INVALID, "shade_block must be 64, 128, 256, 512 or 1024");
@@ -1598,6 +1627,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     }
     WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
     a.n_rec = (uint32_t)nrec;
+    a.refill = (unsigned)refill;
     LaunchCfg L{B, G, (flags & CPT_RENDER_COUNT) != 0};
     L.SB = SB;
     L.guided = guided;
@@ -1611,10 +1641,10 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     Timers tm;
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d", max_depth,
+        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d", max_depth,
                  (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
-                 (void*)c->g_samples, (double)a.guide_frac, a.guide_depth);
+                 (void*)c->g_samples, (double)a.guide_frac, a.guide_depth, refill);
         if (!c->gexec || c->graph_key != key) {
             drop_graph(c);
             hipGraph_t g;
@@ -1719,7 +1749,8 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     // online training updates the guide between passes: one sample per pixel per pass
     if ((p->flags & CPT_RENDER_GUIDE_TRAIN) && !p->max_paths) cap = (int64_t)p->width * p->height;
     int SB = p->shade_block ? p->shade_block : 256;
-    return render_batch(c, vd, p->max_depth, p->seed, p->flags, B, G, SB, cap, nullptr, stats);
+    int refill = p->refill ? p->refill : (int)CPT_DEFAULT_REFILL;
+    return render_batch(c, vd, p->max_depth, p->seed, p->flags, B, G, SB, refill, cap, nullptr, stats);
 }
 
 int cpt_read_film(cpt_ctx* c, float* rgb, double* sums, double* sumsq) {
@@ -1946,7 +1977,7 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
         vd[v].id = (unsigned)v;
     }
     const int B = 128;
-    int r = render_batch(c, vd, max_depth, seed, flags & ~CPT_RENDER_ACCUMULATE, B, default_grid(c, B), 256, 1 << 25,
+    int r = render_batch(c, vd, max_depth, seed, flags & ~CPT_RENDER_ACCUMULATE, B, default_grid(c, B), 256, (int)CPT_DEFAULT_REFILL, 1 << 25,
                          view_ms, stats);
     if (r) return r;
     if (films) {

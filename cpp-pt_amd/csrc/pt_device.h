@@ -117,6 +117,9 @@ struct Hit {
 #define CPT_TIE_SLACK 1.000244140625f  // 1 + 2^-12
 #define CPT_BOX_SLACK 1.00000048f      // 1 + 2*gamma_3
 
+__device__ __forceinline__ unsigned gthreads_() { return gridDim.x * blockDim.x; }
+__device__ __forceinline__ unsigned gtid_() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
 template <int BLOCK>
 struct TravStack {
     int* lds;
@@ -243,6 +246,155 @@ __device__ bool traverse(const DevScene& sc, V3 o, V3 d, float tmax, Hit& h, Tra
         next = cid[0];
     }
     return ANY ? false : h.tri >= 0;
+}
+
+// Chunked persistent traversal.  Wave w of the grid owns rays
+// [w*per, (w+1)*per) of the queue and keeps its 64 lanes busy: whenever at
+// least `refill` lanes (or all of them) are idle it hands them the next rays of
+// its chunk (refill: a launch parameter), so a wave no longer waits for its slowest ray before starting new
+// ones, and no atomics are needed.  Each iteration a lane may do one node step
+// (four slab tests, near-first push) and one triangle test of a pending leaf:
+// a leaf met during the node step is parked in `leaf` until the previous leaf's
+// triangles are done.  Culling stays conservative and ties are decided by rank,
+// so the result does not depend on this visiting order.
+//   load(i, o, d, tmax)   fetches ray i of the queue
+//   done(i, hit)          receives the closest hit (ANY: h.tri >= 0 iff occluded)
+// no node pending.  As a leaf reference -2 would be 16 triangles from index
+// 2^27-2, which no scene of fewer than 2^27 triangles has; EMPTY_CHILD is never
+// stored in `node`.
+#define CPT_NONE (-2)
+template <int BLOCK, bool ANY, bool COUNT, class Load, class Done>
+__device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count, unsigned refill, TravStack<BLOCK>& st,
+                                              Load load, Done done, unsigned& nvis, unsigned& ntest) {
+    const unsigned lane = __lane_id();
+    const unsigned nw = gthreads_() >> 6, w = gtid_() >> 6;
+    const unsigned per = (count + nw - 1) / nw;
+    unsigned nextr = min(count, w * per);
+    const unsigned endr = min(count, nextr + per);
+    const unsigned long long lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    bool has = false;
+    unsigned ray = 0;
+    V3 o = {0, 0, 0}, d = {0, 0, 0};
+    float ix = 0, iy = 0, iz = 0, best = 0;
+    int best_rank = INT_MIN;
+    Hit h;
+    h.t = 0.0f;
+    h.beta = 0.0f;
+    h.gamma = 0.0f;
+    h.tri = -1;
+    int sp = 0, node = CPT_NONE, leaf = CPT_NONE;
+    int tcur = 0, tend = 0;
+    while (true) {
+        const unsigned long long idle = __ballot(!has);
+        const unsigned nidle = (unsigned)__popcll(idle);
+        if (nextr < endr && (nidle >= refill || nidle == 64u)) {
+            const unsigned r = nextr + (unsigned)__popcll(idle & lt);
+            if (!has && r < endr) {
+                float tmax;
+                load(r, o, d, tmax);
+                float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+                float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+                float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+                ix = 1.0f / dx;
+                iy = 1.0f / dy;
+                iz = 1.0f / dz;
+                best = tmax;
+                best_rank = INT_MIN;
+                h.tri = -1;
+                sp = 0;
+                node = 0;  // root
+                leaf = CPT_NONE;
+                tcur = tend = 0;
+                ray = r;
+                has = true;
+            }
+            nextr = min(endr, nextr + nidle);
+        }
+        if (__ballot(has) == 0ull) break;
+        if (!has) continue;
+        // ---- node step
+        if (node >= 0) {
+            if (COUNT) nvis++;
+            const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
+            float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            int4 ch = reinterpret_cast<const int4*>(np)[6];
+            float lim = ANY ? best * CPT_BOX_SLACK : best * CPT_TIE_SLACK;
+            float key[4];
+            int cid[4];
+#define CPT_BOX(c, LX, HX, LY, HY, LZ, HZ, CH)                                   \
+    {                                                                             \
+        float t0 = (LX - o.x) * ix, t1 = (HX - o.x) * ix;                         \
+        float t2 = (LY - o.y) * iy, t3 = (HY - o.y) * iy;                         \
+        float t4 = (LZ - o.z) * iz, t5 = (HZ - o.z) * iz;                         \
+        float tn = fmaxf(fmaxf(fminf(t0, t1), fminf(t2, t3)), fmaxf(fminf(t4, t5), 0.0f)); \
+        float tf = fminf(fminf(fmaxf(t0, t1), fmaxf(t2, t3)), fmaxf(t4, t5)) * CPT_BOX_SLACK; \
+        bool hit = tn <= tf && tn <= lim;                                         \
+        key[c] = hit ? tn : INFINITY;                                             \
+        cid[c] = hit ? CH : EMPTY_CHILD;                                          \
+    }
+            CPT_BOX(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
+            CPT_BOX(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y)
+            CPT_BOX(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z)
+            CPT_BOX(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w)
+#undef CPT_BOX
+#define CPT_CSWAP(a, b)                                  \
+    if (key[b] < key[a]) {                               \
+        float tk = key[a]; key[a] = key[b]; key[b] = tk; \
+        int tc = cid[a]; cid[a] = cid[b]; cid[b] = tc;   \
+    }
+            CPT_CSWAP(0, 1)
+            CPT_CSWAP(2, 3)
+            CPT_CSWAP(0, 2)
+            CPT_CSWAP(1, 3)
+            CPT_CSWAP(1, 2)
+#undef CPT_CSWAP
+            if (cid[3] != EMPTY_CHILD) st.put(sp++, cid[3]);
+            if (cid[2] != EMPTY_CHILD) st.put(sp++, cid[2]);
+            if (cid[1] != EMPTY_CHILD) st.put(sp++, cid[1]);
+            node = cid[0] != EMPTY_CHILD ? cid[0] : CPT_NONE;
+        }
+        // ---- park a leaf, keep the node stream going
+        if (node != CPT_NONE && node < 0 && leaf == CPT_NONE) {
+            leaf = node;
+            node = CPT_NONE;
+        }
+        if (leaf != CPT_NONE && tcur >= tend) {
+            tcur = leaf & ((1 << 27) - 1);
+            tend = tcur + ((leaf >> 27) & 15) + 1;
+            leaf = CPT_NONE;
+        }
+        if (node == CPT_NONE && sp > 0) node = st.get(--sp);
+        // ---- one triangle of the pending leaf
+        bool finished = false;
+        if (tcur < tend) {
+            const float4* tp = sc.tris + 3 * (size_t)tcur;
+            float4 A = tp[0], E1 = tp[1], E2 = tp[2];
+            tcur++;
+            float t, b, g;
+            if (COUNT) ntest++;
+            if (tri_test(A, E1, E2, o, d, best, t, b, g)) {
+                if (ANY) {
+                    h.tri = __float_as_int(A.w);
+                    finished = true;
+                } else {
+                    int rk = __float_as_int(E1.w);
+                    if (t < best || rk > best_rank) {
+                        best = t;
+                        best_rank = rk;
+                        h.t = t;
+                        h.beta = b;
+                        h.gamma = g;
+                        h.tri = __float_as_int(A.w);
+                    }
+                }
+            }
+        }
+        if (!finished && tcur >= tend && leaf == CPT_NONE && node == CPT_NONE) finished = true;
+        if (finished) {
+            done(ray, h);
+            has = false;
+        }
+    }
 }
 
 }  // namespace cpt

@@ -96,7 +96,8 @@ typedef struct {
     int32_t grid_blocks;    /* blocks of the grid-stride kernels; 0 = automatic */
     int32_t view;           /* view id mixed into the seed (multi-view batches) */
     int32_t shade_block;    /* threads per block of the shading kernel (64..1024); 0 = automatic (256) */
-    int32_t reserved[2];
+    int32_t refill;         /* traversal: idle lanes of a wave (1..64) that make it fetch new rays; 0 = automatic */
+    int32_t reserved;
 } cpt_render_params;
 
 #define CPT_RENDER_NO_GRAPH 1       /* launch kernels one by one instead of a captured hipGraph */
