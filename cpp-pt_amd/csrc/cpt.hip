@@ -120,6 +120,11 @@ struct WfArgs {
 // 2 complete)
 #define REC_FLOATS 18
 
+// waves per SIMD the traversal kernels are compiled for (caps their VGPRs)
+#ifndef CPT_TRAV_WAVES
+#define CPT_TRAV_WAVES 1
+#endif
+
 // idle lanes (of 64) that make a traversal wave fetch new rays of its chunk
 #define CPT_DEFAULT_REFILL 32u
 
@@ -236,7 +241,7 @@ __global__ void k_raygen(WfArgs a) {
 
 // ------------------------------------------------------------------ extend
 template <int B, bool COUNT>
-__global__ __launch_bounds__(B) void k_extend(WfArgs a, int cur) {
+__global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_extend(WfArgs a, int cur) {
     __shared__ int lds[CPT_LDS_SS * B];
     const unsigned count = a.counters[2 * cur];
     if (gtid() == 0) {
@@ -581,7 +586,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
 
 // ------------------------------------------------------------------ shadow
 template <int B, bool COUNT>
-__global__ __launch_bounds__(B) void k_shadow(WfArgs a, int nxt) {
+__global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_shadow(WfArgs a, int nxt) {
     __shared__ int lds[CPT_LDS_SS * B];
     const unsigned count = a.counters[2 * nxt + 1];
     if (gtid() == 0) atomicAdd(&a.stats[ST_SHADOW], (unsigned long long)count);

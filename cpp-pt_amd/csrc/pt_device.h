@@ -109,8 +109,10 @@ struct Hit {
 // Ordered BVH4 traversal, one ray per lane.  Stack: LDS_SS entries in LDS
 // (laid out entry-major so that lanes hit distinct banks) then a per-thread
 // overflow region in global memory.
+#ifndef CPT_LDS_SS
 #define CPT_LDS_SS 16
-#define CPT_GSTACK 112
+#endif
+#define CPT_GSTACK (128 - CPT_LDS_SS)
 // conservative culling: a box may be entered up to this relative distance
 // beyond the current best t, so that triangles tying with the best t in other
 // leaves are still tested (ties are decided by rank, not by visit order)
@@ -272,7 +274,7 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
     unsigned nextr = min(count, w * per);
     const unsigned endr = min(count, nextr + per);
     const unsigned long long lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-    bool has = false;
+    bool has = false, fresh = false;
     unsigned ray = 0;
     V3 o = {0, 0, 0}, d = {0, 0, 0};
     float ix = 0, iy = 0, iz = 0, best = 0;
@@ -285,20 +287,15 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
     int sp = 0, node = CPT_NONE, leaf = CPT_NONE;
     int tcur = 0, tend = 0;
     while (true) {
+        // The reciprocal direction of a fresh ray is formed inside its first
+        // node step, after the root's loads are issued, so that the ray loads
+        // and the node loads are in flight together.
         const unsigned long long idle = __ballot(!has);
         const unsigned nidle = (unsigned)__popcll(idle);
         if (nextr < endr && (nidle >= refill || nidle == 64u)) {
             const unsigned r = nextr + (unsigned)__popcll(idle & lt);
             if (!has && r < endr) {
-                float tmax;
-                load(r, o, d, tmax);
-                float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
-                float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
-                float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
-                ix = 1.0f / dx;
-                iy = 1.0f / dy;
-                iz = 1.0f / dz;
-                best = tmax;
+                load(r, o, d, best);
                 best_rank = INT_MIN;
                 h.tri = -1;
                 sp = 0;
@@ -307,17 +304,71 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
                 tcur = tend = 0;
                 ray = r;
                 has = true;
+                fresh = true;
             }
             nextr = min(endr, nextr + nidle);
         }
         if (__ballot(has) == 0ull) break;
         if (!has) continue;
-        // ---- node step
-        if (node >= 0) {
-            if (COUNT) nvis++;
+        // This iteration's loads are issued together (the pending triangle and
+        // the node), then the triangle is tested and the node's boxes culled
+        // against the possibly improved best.
+        const bool do_tri = tcur < tend, do_node = node >= 0;
+        float4 A, E1, E2, lx, hx, ly, hy, lz, hz;
+        int4 ch;
+        if (do_tri) {
+            const float4* tp = sc.tris + 3 * (size_t)tcur;
+            A = tp[0];
+            E1 = tp[1];
+            E2 = tp[2];
+        }
+        if (do_node) {
             const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
-            float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            int4 ch = reinterpret_cast<const int4*>(np)[6];
+            lx = np[0];
+            hx = np[1];
+            ly = np[2];
+            hy = np[3];
+            lz = np[4];
+            hz = np[5];
+            ch = reinterpret_cast<const int4*>(np)[6];
+        }
+        if (fresh) {
+            // 1/d with zero components replaced by a tiny signed value so that
+            // slab products never form 0*inf
+            float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
+            float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
+            float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
+            ix = 1.0f / dx;
+            iy = 1.0f / dy;
+            iz = 1.0f / dz;
+            fresh = false;
+        }
+        bool finished = false;
+        // ---- one triangle of the pending leaf
+        if (do_tri) {
+            tcur++;
+            float t, b, g;
+            if (COUNT) ntest++;
+            if (tri_test(A, E1, E2, o, d, best, t, b, g)) {
+                if (ANY) {
+                    h.tri = __float_as_int(A.w);
+                    finished = true;
+                } else {
+                    int rk = __float_as_int(E1.w);
+                    if (t < best || rk > best_rank) {
+                        best = t;
+                        best_rank = rk;
+                        h.t = t;
+                        h.beta = b;
+                        h.gamma = g;
+                        h.tri = __float_as_int(A.w);
+                    }
+                }
+            }
+        }
+        // ---- node step
+        if (do_node && !finished) {
+            if (COUNT) nvis++;
             float lim = ANY ? best * CPT_BOX_SLACK : best * CPT_TIE_SLACK;
             float key[4];
             int cid[4];
@@ -342,54 +393,37 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
         float tk = key[a]; key[a] = key[b]; key[b] = tk; \
         int tc = cid[a]; cid[a] = cid[b]; cid[b] = tc;   \
     }
-            CPT_CSWAP(0, 1)
-            CPT_CSWAP(2, 3)
-            CPT_CSWAP(0, 2)
-            CPT_CSWAP(1, 3)
-            CPT_CSWAP(1, 2)
+#ifndef CPT_ANY_UNSORTED
+#define CPT_ANY_UNSORTED 0
+#endif
+            if (!(ANY && CPT_ANY_UNSORTED)) {
+                CPT_CSWAP(0, 1)
+                CPT_CSWAP(2, 3)
+                CPT_CSWAP(0, 2)
+                CPT_CSWAP(1, 3)
+                CPT_CSWAP(1, 2)
+            }
 #undef CPT_CSWAP
             if (cid[3] != EMPTY_CHILD) st.put(sp++, cid[3]);
             if (cid[2] != EMPTY_CHILD) st.put(sp++, cid[2]);
             if (cid[1] != EMPTY_CHILD) st.put(sp++, cid[1]);
             node = cid[0] != EMPTY_CHILD ? cid[0] : CPT_NONE;
         }
-        // ---- park a leaf, keep the node stream going
-        if (node != CPT_NONE && node < 0 && leaf == CPT_NONE) {
-            leaf = node;
-            node = CPT_NONE;
-        }
-        if (leaf != CPT_NONE && tcur >= tend) {
-            tcur = leaf & ((1 << 27) - 1);
-            tend = tcur + ((leaf >> 27) & 15) + 1;
-            leaf = CPT_NONE;
-        }
-        if (node == CPT_NONE && sp > 0) node = st.get(--sp);
-        // ---- one triangle of the pending leaf
-        bool finished = false;
-        if (tcur < tend) {
-            const float4* tp = sc.tris + 3 * (size_t)tcur;
-            float4 A = tp[0], E1 = tp[1], E2 = tp[2];
-            tcur++;
-            float t, b, g;
-            if (COUNT) ntest++;
-            if (tri_test(A, E1, E2, o, d, best, t, b, g)) {
-                if (ANY) {
-                    h.tri = __float_as_int(A.w);
-                    finished = true;
-                } else {
-                    int rk = __float_as_int(E1.w);
-                    if (t < best || rk > best_rank) {
-                        best = t;
-                        best_rank = rk;
-                        h.t = t;
-                        h.beta = b;
-                        h.gamma = g;
-                        h.tri = __float_as_int(A.w);
-                    }
-                }
+        if (!finished) {
+            // park a leaf met by the node step; start it once the previous
+            // leaf's triangles are done; keep the node stream going
+            if (node != CPT_NONE && node < 0 && leaf == CPT_NONE) {
+                leaf = node;
+                node = CPT_NONE;
             }
+            if (leaf != CPT_NONE && tcur >= tend) {
+                tcur = leaf & ((1 << 27) - 1);
+                tend = tcur + ((leaf >> 27) & 15) + 1;
+                leaf = CPT_NONE;
+            }
+            if (node == CPT_NONE && sp > 0) node = st.get(--sp);
+            finished = tcur >= tend && leaf == CPT_NONE && node == CPT_NONE;
         }
-        if (!finished && tcur >= tend && leaf == CPT_NONE && node == CPT_NONE) finished = true;
         if (finished) {
             done(ray, h);
             has = false;
