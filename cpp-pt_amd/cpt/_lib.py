@@ -36,7 +36,7 @@ ABI_FUNCTIONS = (
     "cpt_bvh_info", "cpt_tie_rank", "cpt_trace_closest", "cpt_trace_any",
     "cpt_trace_closest_device", "cpt_render", "cpt_read_film", "cpt_render_views",
     "cpt_reference_tie_rank", "cpt_guide_num_params", "cpt_guide_init", "cpt_guide_get_params",
-    "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step",
+    "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
 )
 
 
@@ -81,6 +81,17 @@ class RenderStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneResult(C.Structure):
+    _fields_ = [
+        ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("shade_block", C.c_int32),
+        ("refill", C.c_int32), ("trials", C.c_int32), ("reserved0", C.c_int32),
+        ("samples_per_s", C.c_double), ("seconds", C.c_double), ("reserved", C.c_int32 * 4),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+
+
 _lib = None
 
 
@@ -105,6 +116,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_trace_any.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     lib.cpt_trace_closest_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     lib.cpt_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.POINTER(RenderStats)]
+    lib.cpt_autotune.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.POINTER(TuneResult)]
     lib.cpt_read_film.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.cpt_render_views.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                      C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
@@ -237,6 +249,17 @@ class Context:
         self._check(self.lib.cpt_render(self.h, C.byref(p), C.byref(st)), "cpt_render")
         self._last = (width, height)
         return st.as_dict()
+
+    def autotune(self, width, height, max_depth=32, seed=1, flags=0, max_paths=0, view=0):
+        """Search launch configurations for this scene/size/depth (cpt_autotune);
+        later renders with those parameters left 0 use the winner."""
+        p = RenderParams()
+        p.width, p.height, p.spp_axis, p.max_depth = width, height, 1, max_depth
+        p.seed, p.flags, p.max_paths, p.view = seed, flags, max_paths, view
+        r = TuneResult()
+        self._check(self.lib.cpt_autotune(self.h, C.byref(p), C.byref(r)), "cpt_autotune")
+        self._last = (width, height)
+        return r.as_dict()
 
     def read_film(self, sums: bool = False, sumsq: bool = False):
         """rgb (h,w,3) float32; with sums/sumsq also the FP64 per-pixel sums and

@@ -816,6 +816,14 @@ struct cpt_ctx {
 
     // scene
     bool has_scene = false;
+    uint64_t scene_gen = 0;  // bumped by every upload
+    // launch configuration chosen by cpt_autotune, valid for one key
+    struct Tuned {
+        bool valid = false;
+        uint64_t scene_gen;
+        int W, H, depth, guided;
+        int B, G, SB, refill;
+    } tuned;
     int nt = 0, nv = 0, nl = 0, nm = 0;
     int64_t bvh_nodes = 0, bvh_leaves = 0;
     int bvh_depth = 0;
@@ -1065,13 +1073,29 @@ struct LaunchCfg {
     bool guided = false;
     int infer_blocks = 0;
     int SB = 256;  // shading kernel block
+    // grids of the traversal kernels (0: G).  Their waves split the queue into
+    // equal chunks, so the grid should be what the device holds at once: a
+    // second round of waves would double the time of the CUs that run it.
+    int GE = 0, GS = 0;
 };
+
+template <int B>
+void resident_grids(int cus, int& ge, int& gs) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_extend<B, false>, B, 0) != hipSuccess || nb <= 0)
+        nb = 2048 / B;
+    ge = nb * cus;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_shadow<B, false>, B, 0) != hipSuccess || nb <= 0)
+        nb = 2048 / B;
+    gs = nb * cus;
+}
 
 // launch helpers dispatching on the block size
 template <int B>
 void launch_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) {
-    if (L.count) hipLaunchKernelGGL((k_extend<B, true>), dim3(L.G), dim3(B), 0, s, a, cur);
-    else hipLaunchKernelGGL((k_extend<B, false>), dim3(L.G), dim3(B), 0, s, a, cur);
+    const unsigned g = (unsigned)(L.GE ? L.GE : L.G);
+    if (L.count) hipLaunchKernelGGL((k_extend<B, true>), dim3(g), dim3(B), 0, s, a, cur);
+    else hipLaunchKernelGGL((k_extend<B, false>), dim3(g), dim3(B), 0, s, a, cur);
 }
 template <int B>
 void launch_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
@@ -1083,8 +1107,9 @@ void launch_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStre
 }
 template <int B>
 void launch_shadow(const LaunchCfg& L, const WfArgs& a, int nxt, hipStream_t s) {
-    if (L.count) hipLaunchKernelGGL((k_shadow<B, true>), dim3(L.G), dim3(B), 0, s, a, nxt);
-    else hipLaunchKernelGGL((k_shadow<B, false>), dim3(L.G), dim3(B), 0, s, a, nxt);
+    const unsigned g = (unsigned)(L.GS ? L.GS : L.G);
+    if (L.count) hipLaunchKernelGGL((k_shadow<B, true>), dim3(g), dim3(B), 0, s, a, nxt);
+    else hipLaunchKernelGGL((k_shadow<B, false>), dim3(g), dim3(B), 0, s, a, nxt);
 }
 
 void L_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) {
@@ -1345,6 +1370,7 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     if (c->nl > 0)
         HIPCHK(c, hipMemcpy(c->d_lights, d->light_triangles, sizeof(int) * (size_t)c->nl, hipMemcpyHostToDevice));
     c->has_scene = true;
+    c->scene_gen++;
     return CPT_OK;
 }
 
@@ -1541,7 +1567,14 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
     if (SB != 64 && SB != 128 && SB != 256 && SB != 512 && SB != 1024)
         return fail(c, CPT_E_INVALID, "shade_block must be 64, 128, 256, 512 or 1024");
-    if (G <= 0) return fail(c, CPT_E_INVALID, "grid_blocks must be positive");
+    if (G < 0) return fail(c, CPT_E_INVALID, "grid_blocks must not be negative");
+    int GE = 0, GS = 0;
+    if (G == 0) {  // automatic
+        G = default_grid(c, B);
+        if (B == 64) resident_grids<64>(c->cus, GE, GS);
+        else if (B == 256) resident_grids<256>(c->cus, GE, GS);
+        else resident_grids<128>(c->cus, GE, GS);
+    }
     unsigned long long film_px = 0;
     for (size_t v = 0; v < vd.size(); v++) {
         if (vd[v].W <= 0 || vd[v].H <= 0 || vd[v].S <= 0) return fail(c, CPT_E_INVALID, "view %zu: bad size or spp", v);
@@ -1635,6 +1668,8 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     a.refill = (unsigned)refill;
     LaunchCfg L{B, G, (flags & CPT_RENDER_COUNT) != 0};
     L.SB = SB;
+    L.GE = GE;
+    L.GS = GS;
     L.guided = guided;
     L.infer_blocks = c->cus;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
@@ -1748,14 +1783,97 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     vd[0].H = p->height;
     vd[0].S = p->spp_axis;
     vd[0].id = (unsigned)p->view;
-    int B = p->block_size ? p->block_size : 128;
-    int G = p->grid_blocks ? p->grid_blocks : default_grid(c, B);
+    // parameters left 0: the autotuned configuration for this key, else defaults
+    const cpt_ctx::Tuned& T = c->tuned;
+    const bool tuned = T.valid && T.scene_gen == c->scene_gen && T.W == p->width && T.H == p->height &&
+                       T.depth == p->max_depth && T.guided == ((p->flags & CPT_RENDER_GUIDED) ? 1 : 0);
+    int B = p->block_size ? p->block_size : (tuned ? T.B : 128);
+    int G = p->grid_blocks ? p->grid_blocks : (tuned && !p->block_size ? T.G : 0);  // 0: resident grid
     int64_t cap = p->max_paths ? p->max_paths : (1 << 25);
     // online training updates the guide between passes: one sample per pixel per pass
     if ((p->flags & CPT_RENDER_GUIDE_TRAIN) && !p->max_paths) cap = (int64_t)p->width * p->height;
-    int SB = p->shade_block ? p->shade_block : 256;
-    int refill = p->refill ? p->refill : (int)CPT_DEFAULT_REFILL;
+    int SB = p->shade_block ? p->shade_block : (tuned ? T.SB : 256);
+    int refill = p->refill ? p->refill : (tuned ? T.refill : (int)CPT_DEFAULT_REFILL);
     return render_batch(c, vd, p->max_depth, p->seed, p->flags, B, G, SB, refill, cap, nullptr, stats);
+}
+
+int cpt_autotune(cpt_ctx* c, const cpt_render_params* p, cpt_tune_result* out) {
+    if (!c || !p) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (p->width <= 0 || p->height <= 0) return fail(c, CPT_E_INVALID, "bad render size");
+    if (p->flags & (CPT_RENDER_STREAM | CPT_RENDER_GUIDE_TRAIN))
+        return fail(c, CPT_E_INVALID, "autotune covers pass renders without guide training");
+    auto t0 = std::chrono::steady_clock::now();
+    c->tuned.valid = false;
+    // trial size: at least 2^23 paths, so that every bounce fills the device
+    const int64_t px = (int64_t)p->width * p->height;
+    int S = 1;
+    while (px * S * S < (1 << 23)) S++;
+    const int flags = p->flags & (CPT_RENDER_GUIDED | CPT_RENDER_NO_GRAPH);
+    const int64_t cap = p->max_paths ? p->max_paths : (1 << 25);
+    int trials = 0;
+    auto time_cfg = [&](int B, int G, int SB, int rf, double& sps) -> int {
+        std::vector<ViewDesc> vd(1);
+        std::memcpy(vd[0].cam, c->cam, sizeof(c->cam));
+        vd[0].W = p->width;
+        vd[0].H = p->height;
+        vd[0].S = S;
+        vd[0].id = (unsigned)p->view;
+        cpt_render_stats st;
+        // first call captures the graph of this configuration; the second is timed
+        int r = render_batch(c, vd, p->max_depth, p->seed, flags, B, G, SB, rf, cap, nullptr, &st);
+        if (!r) r = render_batch(c, vd, p->max_depth, p->seed, flags, B, G, SB, rf, cap, nullptr, &st);
+        if (r) return r;
+        trials++;
+        sps = (double)st.paths / st.seconds;
+        return CPT_OK;
+    };
+    int B = 128, G = 0, SB = 256, rf = (int)CPT_DEFAULT_REFILL;
+    double best = 0;
+    int r = time_cfg(B, G, SB, rf, best);
+    if (r) return r;
+    auto try_one = [&](int b, int g, int sb, int f) -> int {
+        double sps = 0;
+        int rr = time_cfg(b, g, sb, f, sps);
+        if (rr) return rr;
+        if (sps > best * 1.01) {  // keep the incumbent unless clearly faster
+            best = sps;
+            B = b;
+            G = g;
+            SB = sb;
+            rf = f;
+        }
+        return CPT_OK;
+    };
+    for (int f : {16, 48, 64})
+        if ((r = try_one(B, G, SB, f))) return r;
+    for (int b : {64, 256})
+        if ((r = try_one(b, G, SB, rf))) return r;
+    for (int g : {default_grid(c, B), default_grid(c, B) / 2})
+        if ((r = try_one(B, g, SB, rf))) return r;
+    for (int sb : {128, 512})
+        if ((r = try_one(B, G, sb, rf))) return r;
+    c->tuned.valid = true;
+    c->tuned.scene_gen = c->scene_gen;
+    c->tuned.W = p->width;
+    c->tuned.H = p->height;
+    c->tuned.depth = p->max_depth;
+    c->tuned.guided = (p->flags & CPT_RENDER_GUIDED) ? 1 : 0;
+    c->tuned.B = B;
+    c->tuned.G = G;
+    c->tuned.SB = SB;
+    c->tuned.refill = rf;
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->block_size = B;
+        out->grid_blocks = G;
+        out->shade_block = SB;
+        out->refill = rf;
+        out->trials = trials;
+        out->samples_per_s = best;
+        out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return CPT_OK;
 }
 
 int cpt_read_film(cpt_ctx* c, float* rgb, double* sums, double* sumsq) {
@@ -1982,7 +2100,7 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
         vd[v].id = (unsigned)v;
     }
     const int B = 128;
-    int r = render_batch(c, vd, max_depth, seed, flags & ~CPT_RENDER_ACCUMULATE, B, default_grid(c, B), 256, (int)CPT_DEFAULT_REFILL, 1 << 25,
+    int r = render_batch(c, vd, max_depth, seed, flags & ~CPT_RENDER_ACCUMULATE, B, 0, 256, (int)CPT_DEFAULT_REFILL, 1 << 25,
                          view_ms, stats);
     if (r) return r;
     if (films) {

@@ -170,6 +170,24 @@ int cpt_trace_closest_device(cpt_ctx* ctx, const void* d_rays, int32_t n, void* 
 /* Render into the context's device film (FP64 accumulators).  Replaces the
  * pixel loop of PathtracerMIS::render (pathtracerMIS.h:139-160). */
 int cpt_render(cpt_ctx* ctx, const cpt_render_params* params, cpt_render_stats* stats);
+
+/* Launch configuration found by cpt_autotune. */
+typedef struct {
+    int32_t block_size, grid_blocks, shade_block, refill;  /* grid_blocks 0 = resident grid */
+    int32_t trials;          /* configurations timed */
+    int32_t reserved0;
+    double samples_per_s;    /* best trial */
+    double seconds;          /* wall time of the search */
+    int32_t reserved[4];
+} cpt_tune_result;
+
+/* Time short renders of the uploaded scene at params' size, depth and flags over
+ * candidate launch configurations (block size, traversal grid, refill threshold,
+ * shading block; coordinate descent from the defaults) and keep the fastest: later
+ * cpt_render calls on the same scene, size, depth and guided flag use it for every
+ * launch parameter they leave 0.  The film holds the last trial afterwards.
+ * No counterpart in the reference (its renderer has no launch parameters). */
+int cpt_autotune(cpt_ctx* ctx, const cpt_render_params* params, cpt_tune_result* out);
 /* Copy the film out: rgb (w*h*3 floats) = sum / spp_axis^2 of the last
  * render, row j = 0 at the top like RgbImage::put_pixel (rgb_image.h:208-210);
  * sums (w*h*3 doubles, may be NULL) = the raw FP64 accumulators; sumsq (may be
