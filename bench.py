@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip cpt_autotune (default launch configuration)")
     ap.add_argument("--workload", choices=["terrain", "views", "guided"], default="terrain",
                     help="terrain: the headline 1080p line; views: 64-view batch of the box room; "
                          "guided: equal-time relMSE of guided vs plain PT on the hard-lighting interior")
@@ -262,6 +263,11 @@ def main():
     print(f"bench: scene {bvh['triangles']} triangles, {bvh['nodes']} BVH4 nodes, setup {setup_s:.1f} s",
           file=sys.stderr, flush=True)
     kw = dict(max_depth=depth, block_size=args.block, grid_blocks=args.grid_blocks, max_paths=args.max_paths)
+    tune = None
+    if not args.no_tune:
+        # launch configuration search (part of setup, outside the timed steps)
+        tune = ctx.autotune(W, H, max_depth=depth, max_paths=args.max_paths)
+        print(f"bench: autotune {tune}", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         ctx.render(W, H, S, seed=1000 + w, **kw)
     t0 = time.perf_counter()
@@ -293,6 +299,7 @@ def main():
                         f"{S * S} spp per step ({S * S * args.steps} spp total), max depth {depth}",
             "width": W, "height": H, "spp_per_step": S * S, "max_depth": depth,
             "triangles": bvh["triangles"],
+            "launch": tune,
         },
         "mrays_per_s": (ext + shadow) / total / 1e6,
         "extension_mrays_per_s": ext / total / 1e6,
