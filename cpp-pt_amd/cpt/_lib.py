@@ -29,6 +29,8 @@ CPT_RENDER_ACCUMULATE = 8
 CPT_RENDER_STREAM = 16
 CPT_RENDER_GUIDED = 32
 CPT_RENDER_GUIDE_TRAIN = 64
+CPT_RENDER_NEE_ONLY = 128
+CPT_RENDER_PURE_PT = 256
 
 # every function include/cpt.h declares
 ABI_FUNCTIONS = (
@@ -43,7 +45,8 @@ ABI_FUNCTIONS = (
 class GuideParams(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("seed", C.c_uint64), ("guide_fraction", C.c_float), ("train_paths", C.c_int32),
-                ("train_steps", C.c_int32), ("guide_depth", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("train_steps", C.c_int32), ("guide_depth", C.c_int32), ("guide_target", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class SceneDesc(C.Structure):
@@ -304,11 +307,12 @@ Context.render_views = _render_views
 
 
 def _guide_init(self, lr=1e-3, seed=1, guide_fraction=0.5, train_paths=0, train_steps=1, beta1=0.9, beta2=0.99,
-                eps=1e-8, guide_depth=0):
+                eps=1e-8, guide_depth=0, guide_target=0):
     gp = GuideParams()
     gp.lr, gp.beta1, gp.beta2, gp.eps = lr, beta1, beta2, eps
     gp.seed, gp.guide_fraction, gp.train_paths, gp.train_steps = seed, guide_fraction, train_paths, train_steps
     gp.guide_depth = guide_depth
+    gp.guide_target = guide_target
     self._check(self.lib.cpt_guide_init(self.h, C.byref(gp)), "cpt_guide_init")
 
 

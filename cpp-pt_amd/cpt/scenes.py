@@ -249,9 +249,10 @@ def _box(center, size, angle_deg):
     return np.concatenate(Ps), np.concatenate(Ns), np.concatenate(Ts), np.concatenate(Fs)
 
 
-def box_room(res=(512, 512)):
+def box_room(res=(512, 512), light_gap=1e-3):
     """Closed 2x2x2 box room (open front), 5 Lambertian quads, two boxes rotated
-    +/-17 deg (heights 0.6 and 1.2), one 0.47x0.38 ceiling light (17,12,4)."""
+    +/-17 deg (heights 0.6 and 1.2), one 0.47x0.38 ceiling light (17,12,4)
+    `light_gap` below the ceiling."""
     s = Scene()
     white = s.add_material(material(MAT_DIFFUSE, albedo=(0.73, 0.73, 0.73)))
     red = s.add_material(material(MAT_DIFFUSE, albedo=(0.63, 0.07, 0.05)))
@@ -276,7 +277,7 @@ def box_room(res=(512, 512)):
     s.add_mesh(P, N, T, F, white)
     P, N, T, F = _box((-0.33, -1.0 + 0.6, -0.3), (0.6, 1.2, 0.6), 17.0)
     s.add_mesh(P, N, T, F, white)
-    y = 1.0 - 1e-3
+    y = 1.0 - light_gap
     hx, hz = 0.47 / 2, 0.38 / 2
     P, N, T, F = _quad((-hx, y, -hz), (hx, y, -hz), (hx, y, hz), (-hx, y, hz), (0, -1, 0), (1, 0, 0))
     s.add_mesh(P, N, T, F, light)

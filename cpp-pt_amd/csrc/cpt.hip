@@ -104,8 +104,10 @@ struct WfArgs {
     uint32_t n_rec;          // slots that record
     int rec_depth;
     float guide_frac;
-    int guide_depth;
-    unsigned refill;         // idle lanes that make a traversal wave fetch new rays         // vertices at depth >= this are not guided
+    int guide_depth;         // vertices at depth >= this are not guided
+    int guide_target;        // 0: train on L_i f cos, 1: on L_i
+    unsigned refill;         // idle lanes that make a traversal wave fetch new rays
+    int integrator;          // 0 PathtracerMIS, 1 Pathtracer (NEE only), 2 PurePt
     float* gsamples;         // training samples out (11 floats)
     uint32_t* gcount;
     const uint16_t* gwt;
@@ -117,8 +119,8 @@ struct WfArgs {
 
 // training record of one guided path vertex: inputs[7], dir[3], pdf, path throughput
 // before the vertex [3], radiance snapshot[3], state (0 empty, 1 direction traced,
-// 2 complete)
-#define REC_FLOATS 18
+// 2 complete), path throughput after the vertex [3]
+#define REC_FLOATS 21
 
 // waves per SIMD the traversal kernels are compiled for (caps their VGPRs)
 #ifndef CPT_TRAV_WAVES
@@ -392,13 +394,15 @@ __global__ void k_guide_collect(WfArgs a) {
             if (r[17] == 2.0f) {
                 unsigned slot = idx / (unsigned)a.rec_depth;
                 float4 c = a.col[slot];
-                // radiance gathered through the sampled direction, divided by the
-                // throughput up to the vertex: L_i f cos / p, the product
-                // integrand over the density the direction was drawn from
-                float lr = r[11] > 0.0f ? (c.x - r[14]) / r[11] : 0.0f;
-                float lg = r[12] > 0.0f ? (c.y - r[15]) / r[12] : 0.0f;
-                float lb = r[13] > 0.0f ? (c.z - r[16]) / r[13] : 0.0f;
+                // radiance gathered through the sampled direction divided by the
+                // throughput after the vertex is L_i; divided by the throughput
+                // before it, L_i f cos / p.  Weight = target / sampling density.
+                const int m = a.guide_target ? 18 : 11;
+                float lr = r[m] > 0.0f ? (c.x - r[14]) / r[m] : 0.0f;
+                float lg = r[m + 1] > 0.0f ? (c.y - r[15]) / r[m + 1] : 0.0f;
+                float lb = r[m + 2] > 0.0f ? (c.z - r[16]) / r[m + 2] : 0.0f;
                 float w = fmaxf(0.2126f * lr + 0.7152f * lg + 0.0722f * lb, 0.0f);
+                if (a.guide_target) w = r[10] > 0.0f ? w / r[10] : 0.0f;
                 if (w > 0.0f && isfinite(w)) {
                     emit = true;
                     for (int k = 0; k < 10; k++) smp[k] = r[k];
@@ -467,8 +471,12 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                 if (depth == 0) first_hit_frame(m, it);
                 V3 wo = mulf(d, -1.0f);
                 if (m.type == CPT_MAT_EMISSIVE) {
-                    if (sampled_delta) {
+                    if (a.integrator == 2 || sampled_delta) {
+                        // pure_pt.h:43-46 counts every hit, pathtracer.h:44-48 and
+                        // pathtracerMIS.h:53-57 those after delta or camera bounces
                         colv = add(colv, mul(mulv, m.emission));
+                    } else if (a.integrator == 1) {
+                        // pathtracer.h: NEE already counted this light
                     } else {
                         float pbb = p;
                         float r = h.t;
@@ -502,7 +510,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                     p = ds.pdf;
                     const float p_sampled = p;
                     if (!ds.delta) {
-                        if (a.sc.n_lights > 0) {
+                        if (a.sc.n_lights > 0 && a.integrator != 2) {
                             LSample ls = light_sample(a.sc, rng, it.hitpoint);
                             V3 sdir = sub(ls.position, it.hitpoint);
                             float len = len3(sdir);
@@ -514,6 +522,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                                 float pnn = ls.pdf * len * len / ct;
                                 float w = pnn / (pnb + pnn);
                                 V3 ev = bsdf_eval(m, it, wo, sdir);
+                                if (a.integrator == 1) w = 1.0f;  // pathtracer.h:70-76: no MIS weight
                                 contrib = mulf(divs(mul(mul(mulv, ev), ls.intensity), ls.pdf * len * len), w);
                                 sho = add(it.hitpoint, mulf(sdir, CPT_EPSF));
                                 shd = sdir;
@@ -550,6 +559,9 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                                     r[11] = mul_prev.x;
                                     r[12] = mul_prev.y;
                                     r[13] = mul_prev.z;
+                                    r[18] = mulv.x;
+                                    r[19] = mulv.y;
+                                    r[20] = mulv.z;
                                     r[17] = 1.0f;
                                 }
                             }
@@ -1048,8 +1060,10 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.n_rec = 0;
     a.rec_depth = depth;
     a.guide_frac = c->g_cfg.guide_fraction;
-    a.guide_depth = c->g_cfg.guide_depth > 0 ? c->g_cfg.guide_depth : 3;
+    a.guide_depth = c->g_cfg.guide_depth > 0 ? c->g_cfg.guide_depth : 256;
+    a.guide_target = c->g_cfg.guide_target;
     a.refill = CPT_DEFAULT_REFILL;
+    a.integrator = 0;
     a.gsamples = c->g_samples;
     a.gcount = c->g_count;
     a.gwt = c->g_wt;
@@ -1564,6 +1578,11 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
                         int SB, int refill, int64_t max_paths, double* view_ms, cpt_render_stats* stats) {
     if (max_depth <= 0 || max_depth > 255) return fail(c, CPT_E_INVALID, "max_depth must be in [1, 255]");
     if (refill < 1 || refill > 64) return fail(c, CPT_E_INVALID, "refill must be in [1, 64]");
+    const int integrator = (flags & CPT_RENDER_PURE_PT) ? 2 : (flags & CPT_RENDER_NEE_ONLY) ? 1 : 0;
+    if ((flags & CPT_RENDER_PURE_PT) && (flags & CPT_RENDER_NEE_ONLY))
+        return fail(c, CPT_E_INVALID, "CPT_RENDER_NEE_ONLY and CPT_RENDER_PURE_PT exclude each other");
+    if (integrator && (flags & CPT_RENDER_GUIDED))
+        return fail(c, CPT_E_INVALID, "guided rendering uses the MIS integrator");
     if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
     if (SB != 64 && SB != 128 && SB != 256 && SB != 512 && SB != 1024)
         return fail(c, CPT_E_INVALID, "shade_block must be 64, 128, 256, 512 or 1024");
@@ -1666,6 +1685,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
     a.n_rec = (uint32_t)nrec;
     a.refill = (unsigned)refill;
+    a.integrator = integrator;
     LaunchCfg L{B, G, (flags & CPT_RENDER_COUNT) != 0};
     L.SB = SB;
     L.GE = GE;
@@ -1681,10 +1701,10 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     Timers tm;
     if (use_graph) {
         char key[256];
-        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d", max_depth,
+        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d,%d", max_depth,
                  (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
-                 (void*)c->g_samples, (double)a.guide_frac, a.guide_depth, refill);
+                 (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator);
         if (!c->gexec || c->graph_key != key) {
             drop_graph(c);
             hipGraph_t g;
@@ -1967,6 +1987,7 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
         sa.view_done = d_sc + 2;
         sa.view_tend = d_sc + 2 + n_views;
         WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
+        a.integrator = (flags & CPT_RENDER_PURE_PT) ? 2 : (flags & CPT_RENDER_NEE_ONLY) ? 1 : 0;
         LaunchCfg L{B, G, false, true};
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
         hipLaunchKernelGGL(k_stream_init, dim3(G), dim3(B), 0, c->stream, a, sa, (unsigned)cap);

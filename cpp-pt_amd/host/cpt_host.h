@@ -38,7 +38,9 @@ struct RenderSettings {
         PURE_PT,
         PPG,
         VALIDATION,
-        PATHTRACER_MIS_GPU
+        PATHTRACER_MIS_GPU,
+        PATHTRACER_GPU,  // Pathtracer (pathtracer.h) on the GPU kernels
+        PURE_PT_GPU      // PurePt (pure_pt.h) on the GPU kernels
     };
     RendererType renderer = RendererType::DEBUG;
     int spp = 1;  // per axis: spp*spp samples per pixel
@@ -126,15 +128,29 @@ public:
 // GPU drop-in for PathtracerMIS (pathtracerMIS.h:17-168).
 class PathtracerMISGPU : public Renderer {
 public:
-    explicit PathtracerMISGPU(const RenderSettings& rs);
+    // integrator_flags: 0 (PathtracerMIS), CPT_RENDER_NEE_ONLY or CPT_RENDER_PURE_PT
+    explicit PathtracerMISGPU(const RenderSettings& rs, int integrator_flags = 0);
     ~PathtracerMISGPU() override;
     void render(Scene& sc, std::string filename) override;
     const cpt_render_stats& last_stats() const { return stats_; }
 
 private:
     RenderSettings rs_;
+    int flags_;
     cpt_ctx* ctx_ = nullptr;
     cpt_render_stats stats_{};
+};
+
+// The reference's Pathtracer (src/renderer/pathtracer.h:15-137): NEE without MIS.
+class PathtracerGPU : public PathtracerMISGPU {
+public:
+    explicit PathtracerGPU(const RenderSettings& rs) : PathtracerMISGPU(rs, CPT_RENDER_NEE_ONLY) {}
+};
+
+// The reference's PurePt (src/renderer/pure_pt.h:15-106): BSDF sampling only.
+class PurePtGPU : public PathtracerMISGPU {
+public:
+    explicit PurePtGPU(const RenderSettings& rs) : PathtracerMISGPU(rs, CPT_RENDER_PURE_PT) {}
 };
 
 }  // namespace cpppt

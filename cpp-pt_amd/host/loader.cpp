@@ -332,6 +332,8 @@ void load_render_settings(RenderSettings* rs, const std::string& filename) {
             else if (rt == "lighttracer") rs->renderer = RenderSettings::LIGHTTRACER;
             else if (rt == "pure_pt") rs->renderer = RenderSettings::PURE_PT;
             else if (rt == "pathtracer") rs->renderer = RenderSettings::PATHTRACER;
+            else if (rt == "pathtracer_gpu") rs->renderer = RenderSettings::PATHTRACER_GPU;
+            else if (rt == "pure_pt_gpu") rs->renderer = RenderSettings::PURE_PT_GPU;
             else if (rt == "pathtracer_mlt") rs->renderer = RenderSettings::PATHTRACER_MLT;
             else if (rt == "lighttracer_mlt") rs->renderer = RenderSettings::LIGHTTRACER_MLT;
             else if (rt == "photonmapping") rs->renderer = RenderSettings::PHOTONMAPPING;

@@ -1,9 +1,10 @@
 // cpt_render: command-line entry with the reference's conventions
 // (src/main.cpp:89-146): argv[1] = settings file, renderer chosen by the
 // settings token, wall time of render() printed as "<seconds>s", runtime
-// errors printed.  Renderer tokens `pathtracer_mis` and `pathtracer_mis_gpu`
-// select the GPU integrator; the reference's other algorithms are not part of
-// this build and are reported as such.
+// errors printed.  Renderer tokens `pathtracer_mis` / `pathtracer_mis_gpu`,
+// `pathtracer` / `pathtracer_gpu` and `pure_pt` / `pure_pt_gpu` select the GPU
+// integrators; the reference's other algorithms are not part of this build and
+// are reported as such.
 #include <chrono>
 #include <iostream>
 #include <memory>
@@ -26,8 +27,12 @@ int main(int argc, char** argv) {
         std::unique_ptr<Renderer> renderer;
         if (rs.renderer == RenderSettings::PATHTRACER_MIS_GPU) {
             renderer = std::make_unique<PathtracerMISGPU>(rs);
+        } else if (rs.renderer == RenderSettings::PATHTRACER || rs.renderer == RenderSettings::PATHTRACER_GPU) {
+            renderer = std::make_unique<PathtracerGPU>(rs);
+        } else if (rs.renderer == RenderSettings::PURE_PT || rs.renderer == RenderSettings::PURE_PT_GPU) {
+            renderer = std::make_unique<PurePtGPU>(rs);
         } else {
-            throw std::runtime_error("renderer not provided by this build (only pathtracer_mis / pathtracer_mis_gpu)");
+            throw std::runtime_error("renderer not provided by this build (pathtracer_mis, pathtracer, pure_pt)");
         }
         auto t1 = std::chrono::high_resolution_clock::now();
         renderer->render(s, rs.output_name);

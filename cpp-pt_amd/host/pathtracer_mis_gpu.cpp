@@ -16,7 +16,8 @@ void check(cpt_ctx* ctx, int rc, const char* what) {
 }
 }  // namespace
 
-PathtracerMISGPU::PathtracerMISGPU(const RenderSettings& rs) : rs_(rs) {
+PathtracerMISGPU::PathtracerMISGPU(const RenderSettings& rs, int integrator_flags)
+    : rs_(rs), flags_(integrator_flags) {
     int rc = cpt_create(&ctx_, rs.device);
     if (rc != CPT_OK) throw std::runtime_error("cpt_create failed: no usable gfx950 device " + std::to_string(rs.device));
 }
@@ -33,6 +34,7 @@ void PathtracerMISGPU::render(Scene& sc, std::string filename) {
     p.spp_axis = rs_.spp;
     p.max_depth = rs_.max_depth;
     p.seed = rs_.seed;
+    p.flags = flags_;
     check(ctx_, cpt_render(ctx_, &p, &stats_), "cpt_render");
     std::vector<float> rgb(3 * (size_t)image.resX * image.resY);
     check(ctx_, cpt_read_film(ctx_, rgb.data(), nullptr, nullptr), "cpt_read_film");

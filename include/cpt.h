@@ -110,6 +110,11 @@ typedef struct {
                                        (needs cpt_guide_init) */
 #define CPT_RENDER_GUIDE_TRAIN 64   /* with GUIDED: record training samples and train after every pass
                                        (passes default to one sample per pixel) */
+/* Integrator (default PathtracerMIS, src/renderer/pathtracerMIS.h:29-130): */
+#define CPT_RENDER_NEE_ONLY 128     /* Pathtracer (src/renderer/pathtracer.h:27-110): NEE without MIS,
+                                       emitters count only after delta or camera bounces */
+#define CPT_RENDER_PURE_PT 256      /* PurePt (src/renderer/pure_pt.h:27-67): BSDF sampling only,
+                                       every emitter hit counts, no shadow rays */
 
 typedef struct {
     double seconds;           /* wall time of the render call (device synchronised) */
@@ -221,10 +226,10 @@ typedef struct {
     float guide_fraction;  /* probability of sampling the guide (0 = 0.5) */
     int32_t train_paths;   /* guided renders: paths per pass that record training samples (0 = 65536) */
     int32_t train_steps;   /* guided renders: Adam steps after each pass (0 = 1) */
-    int32_t guide_depth;   /* guided renders: guide the vertices of the first this many bounces (0 = 3);
-                              one-sample MIS with the guide can double the throughput of a BSDF-sampled
-                              bounce, so deep guided vertices compound into fireflies */
-    int32_t reserved[2];
+    int32_t guide_depth;   /* guided renders: guide the vertices of the first this many bounces (0 = all) */
+    int32_t guide_target;  /* training target: 0 = the product L_i f cos (default), 1 = the incident
+                              radiance L_i alone (what PPG splats, ppg.h:130-141) */
+    int32_t reserved;
 } cpt_guide_params;
 
 int cpt_guide_num_params(void);

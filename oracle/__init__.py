@@ -45,6 +45,7 @@ def lib():
     L.orc_camera_ray.restype = None
     L.orc_render.argtypes = [vp, i32, i32, i32, i32, i32, C.c_uint64, i32, i32, i32, i32, vp, vp, vp, i32, vp]
     L.orc_path.argtypes = [vp, i32, i32, i32, i32, C.c_uint64, i32, i32, vp]
+    L.orc_set_integrator.argtypes = [vp, i32]
     L.orc_path.restype = None
     L.orc_diffuse_sample.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp]
     L.orc_diffuse_sample.restype = None
@@ -144,11 +145,13 @@ class OracleScene:
         return o, d
 
     def render(self, W, H, spp_axis, max_depth=32, sampler="pcg", seed=1, cols=None, rows=None,
-               threads=0, sumsq=False):
+               threads=0, sumsq=False, integrator=0):
         """Returns (sums (H,W,3) float64, film (H,W,3) float32, counts [paths, ext, shadow])
         and, with sumsq=True, the per-pixel sums of squares as a fourth element.
         sampler "pcg" = the GPU's per-path stream; "mt" = the reference's per-column
-        mt19937 (seed 0 reproduces the reference's seeding)."""
+        mt19937 (seed 0 reproduces the reference's seeding).  integrator 0 =
+        PathtracerMIS, 1 = Pathtracer (NEE without MIS), 2 = PurePt."""
+        lib().orc_set_integrator(self.h, integrator)
         sums = np.zeros((H, W, 3), np.float64)
         sq = np.zeros((H, W, 3), np.float64)
         film = np.zeros((H, W, 3), np.float32)
@@ -159,8 +162,9 @@ class OracleScene:
                          c0, c1, r0, r1, _p(sums), _p(sq), _p(film), threads, _p(counts))
         return (sums, film, counts, sq) if sumsq else (sums, film, counts)
 
-    def path(self, W, H, spp_axis, max_depth, seed, pixel, sample):
+    def path(self, W, H, spp_axis, max_depth, seed, pixel, sample, integrator=0):
         out = np.zeros(3, np.float32)
+        lib().orc_set_integrator(self.h, integrator)
         lib().orc_path(self.h, W, H, spp_axis, max_depth, seed, pixel, sample, _p(out))
         return out
 

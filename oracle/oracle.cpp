@@ -154,6 +154,9 @@ struct Scene {
     std::vector<int> leaf_prims;
     std::vector<int> rank;
     int root = -1;
+    // 0 PathtracerMIS (pathtracerMIS.h:29-130), 1 Pathtracer: NEE without MIS
+    // (pathtracer.h:27-110), 2 PurePt: BSDF sampling only (pure_pt.h:27-67)
+    int integrator = 0;
 };
 
 struct Ray {
@@ -737,7 +740,9 @@ bool intersect(const Scene& s, Ray& r, Hit& h) {
     return h.tri >= 0;
 }
 
-// pathtracerMIS.h:29-130
+// pathtracerMIS.h:29-130; with s.integrator 1 the NEE-only Pathtracer
+// (pathtracer.h:27-110: emitters count only after delta/camera bounces, NEE
+// unweighted) and with 2 PurePt (pure_pt.h:27-67: emitters always count, no NEE)
 V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Counts* cnt) {
     Ray ray = camera_ray(s.cam, cx, cy);
     Isect it;
@@ -756,8 +761,10 @@ V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Co
         first_hit_frame(m, it);
         V3 wo = ray.d * (float)(-1.0);
         if (m.type == 2) {
-            if (sampled_delta) {
-                col = col + mul * m.emission;
+            if (s.integrator == 2 || sampled_delta) {
+                col = col + mul * m.emission;  // pure_pt.h:43-46, pathtracer.h:44-48
+            } else if (s.integrator == 1) {
+                // pathtracer.h:44-48: nothing after a non-delta bounce (NEE counted it)
             } else {
                 float pbb = p;
                 float r = ray.max_t;
@@ -772,6 +779,7 @@ V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Co
         DSample smpl = bsdf_sample(m, it, wo, smp);
         V3 sdir = smpl.wi;
         p = smpl.pdf;
+        if (s.integrator == 2) goto no_nee;  // pure_pt.h: no light sampling
         if (!smpl.delta) {
             if (s.lights.empty()) goto no_nee;  // the reference divides by zero here
             {
@@ -784,12 +792,16 @@ V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Co
                 if (cnt) cnt->shadow++;
                 if (!ref_any(s, s.root, sh)) {
                     float r = len;
-                    float pnb = bsdf_pdf(m, it, wo, sd);
-                    float ct = (float)(std::fabs((double)dot(ls.normal, sd)) + EPSD);
-                    float pnn = ls.pdf * r * r / ct;
-                    float w = pnn / (pnb + pnn);
                     V3 ev = bsdf_eval(m, it, wo, sh.d);
-                    col = col + divs(mul * ev * ls.intensity, ls.pdf * r * r) * w;
+                    if (s.integrator == 1) {
+                        col = col + divs(mul * ev * ls.intensity, ls.pdf * r * r);  // pathtracer.h:70-76
+                    } else {
+                        float pnb = bsdf_pdf(m, it, wo, sd);
+                        float ct = (float)(std::fabs((double)dot(ls.normal, sd)) + EPSD);
+                        float pnn = ls.pdf * r * r / ct;
+                        float w = pnn / (pnb + pnn);
+                        col = col + divs(mul * ev * ls.intensity, ls.pdf * r * r) * w;
+                    }
                 }
             }
             }
@@ -983,6 +995,8 @@ int orc_render(void* sp, int W, int H, int S, int max_depth, int sampler, uint64
     }
     return 0;
 }
+
+void orc_set_integrator(void* sp, int integrator) { ((Scene*)sp)->integrator = integrator; }
 
 void orc_path(void* sp, int W, int H, int S, int max_depth, uint64_t seed, int pixel, int sample, float* rgb) {
     const Scene* s = (const Scene*)sp;

@@ -71,6 +71,10 @@ int orc_render(void* s, int W, int H, int spp_axis, int max_depth, int sampler, 
                int col0, int col1, int row0, int row1, double* sum_out, double* sumsq_out,
                float* film_out, int nthreads, int64_t* counts);
 
+/* Integrator of orc_render / orc_path: 0 PathtracerMIS (pathtracerMIS.h:29-130),
+ * 1 Pathtracer, NEE without MIS (pathtracer.h:27-110), 2 PurePt (pure_pt.h:27-67). */
+void orc_set_integrator(void* s, int integrator);
+
 /* One path with an explicit PCG path id; returns radiance in rgb3. */
 void orc_path(void* s, int W, int H, int spp_axis, int max_depth, uint64_t seed, int pixel,
               int sample, float* rgb3);

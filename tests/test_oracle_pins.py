@@ -243,3 +243,21 @@ def test_one_bounce_nee_mis_matches_lambert_polygon():
     got = film[..., 0].mean()
     # 64 px x 1024 spp; per-path std is below the mean for this geometry
     assert abs(got - want) / want < 0.01, (got, want)
+
+
+@pytest.mark.parametrize("integrator,spp_axis,tol", [(1, 32, 0.01), (2, 64, 0.03)])
+def test_one_bounce_sibling_integrators_match_lambert_polygon(integrator, spp_axis, tol):
+    """The same closed form through the reference's other two integrators: NEE
+    without MIS (pathtracer.h: only the light sample counts after a diffuse
+    bounce) and pure path tracing (pure_pt.h: only BSDF rays that hit the
+    emitter count; noisier, so more samples and a 3% bound, about 4 sigma)."""
+    size, height, albedo = 0.5, 1.0, 0.5
+    s = cpt.scenes.single_quad_light(res=(8, 8), size=size, height=height, albedo=albedo)
+    sums, film, counts = oracle.OracleScene(s).render(8, 8, spp_axis, sampler="pcg", seed=3, integrator=integrator)
+    h = size / 2
+    poly = [np.array(v, float) for v in [(-h, height, -h), (h, height, -h), (h, height, h), (-h, height, h)]]
+    want = albedo / math.pi * _polygon_irradiance(poly, np.zeros(3), np.array([0.0, 1.0, 0.0]))
+    got = sums[..., 0].sum() / (64 * spp_axis ** 2)
+    if integrator == 2:
+        assert counts[2] == 0  # no shadow rays
+    assert abs(got - want) / want < tol, (got, want)
