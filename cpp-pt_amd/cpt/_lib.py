@@ -31,6 +31,8 @@ CPT_RENDER_GUIDED = 32
 CPT_RENDER_GUIDE_TRAIN = 64
 CPT_RENDER_NEE_ONLY = 128
 CPT_RENDER_PURE_PT = 256
+CPT_RENDER_PPG = 512
+CPT_RENDER_PPG_STEP = 1024
 
 # every function include/cpt.h declares
 ABI_FUNCTIONS = (
@@ -39,6 +41,7 @@ ABI_FUNCTIONS = (
     "cpt_trace_closest_device", "cpt_render", "cpt_read_film", "cpt_render_views",
     "cpt_reference_tie_rank", "cpt_guide_num_params", "cpt_guide_init", "cpt_guide_get_params",
     "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
+    "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export",
 )
 
 
@@ -120,6 +123,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_trace_closest_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     lib.cpt_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.POINTER(RenderStats)]
     lib.cpt_autotune.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.POINTER(TuneResult)]
+    lib.cpt_ppg_reset.argtypes = [C.c_void_p]
+    lib.cpt_ppg_update.argtypes = [C.c_void_p]
+    lib.cpt_ppg_export.argtypes = [C.c_void_p] + [C.c_void_p] * 8
     lib.cpt_read_film.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.cpt_render_views.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                      C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
@@ -263,6 +269,26 @@ class Context:
         self._check(self.lib.cpt_autotune(self.h, C.byref(p), C.byref(r)), "cpt_autotune")
         self._last = (width, height)
         return r.as_dict()
+
+    def ppg_reset(self):
+        self._check(self.lib.cpt_ppg_reset(self.h), "cpt_ppg_reset")
+
+    def ppg_update(self):
+        self._check(self.lib.cpt_ppg_update(self.h), "cpt_ppg_update")
+
+    def ppg_export(self):
+        """The flattened SD-tree the next PPG step uses and the current splats."""
+        sizes = np.zeros(5, np.int32)
+        nul = C.c_void_p(0)
+        self._check(self.lib.cpt_ppg_export(self.h, _ptr(sizes), nul, nul, nul, nul, nul, nul, nul), "cpt_ppg_export")
+        t = dict(snode=np.zeros(sizes[0], np.int32), dinfo=np.zeros((sizes[1], 4), np.int32),
+                 quad=np.zeros(sizes[2], np.int32), rect=np.zeros((sizes[3], 4), np.float32),
+                 sums=np.zeros(sizes[4], np.float32), counts=np.zeros(sizes[4], np.uint32),
+                 bounds=np.zeros(6, np.float32))
+        self._check(self.lib.cpt_ppg_export(self.h, _ptr(sizes), _ptr(t["snode"]), _ptr(t["dinfo"]), _ptr(t["quad"]),
+                                            _ptr(t["rect"]), _ptr(t["sums"]), _ptr(t["counts"]), _ptr(t["bounds"])),
+                    "cpt_ppg_export")
+        return t
 
     def read_film(self, sums: bool = False, sumsq: bool = False):
         """rgb (h,w,3) float32; with sums/sumsq also the FP64 per-pixel sums and

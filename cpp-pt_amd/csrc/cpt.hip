@@ -23,11 +23,14 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/cpt.h"
 #include "bvh_build.h"
+#include "sdtree.h"
+#include "sdtree_dev.h"
 #include "pt_device.h"
 #include "pt_shade.h"
 #include "guide_mlp.h"
@@ -40,12 +43,21 @@ namespace {
 struct ViewDesc {
     float cam[16];
     int W, H, S;
-    unsigned id;                     // mixed into the path id: (id << 40) + pixel*S*S + sample
+    unsigned id;                     // mixed into the path id: (id << 40) + pixel*spp + sample
     unsigned long long path_begin;   // streaming mode: first global path index of the view
-    unsigned long long n_paths;      // W*H*S*S
+    unsigned long long n_paths;      // W*H*spp
     unsigned long long film_off;     // first pixel of the view in the concatenated films
-    unsigned long long pad;
+    unsigned spp;                    // samples per pixel: S*S, or any count when not stratified
+    int strat;                       // 1: sample s covers stratum (s/S, s%S) of an SxS grid
+                                     // (pathtracerMIS.h:147-153); 0: the whole pixel (ppg.h:178-181)
 };
+
+// film coordinates of sample `sample` of pixel (px, py) of a view
+__device__ __forceinline__ void view_coords(const ViewDesc& vd, int px, int py, unsigned sample, float r1, float r2,
+                                            float& cx, float& cy) {
+    if (vd.strat) film_coords(px, py, (int)sample / vd.S, (int)sample % vd.S, vd.S, vd.W, vd.H, r1, r2, cx, cy);
+    else film_coords(px, py, 0, 0, 1, vd.W, vd.H, r1, r2, cx, cy);
+}
 
 // One pass = up to CPT_MAX_SEG rectangles (view, pixel range x sample range).
 // Slots of a segment: slot0 + sample_local*npix + pixel_local.  A pass never
@@ -107,7 +119,12 @@ struct WfArgs {
     int guide_depth;         // vertices at depth >= this are not guided
     int guide_target;        // 0: train on L_i f cos, 1: on L_i
     unsigned refill;         // idle lanes that make a traversal wave fetch new rays
-    int integrator;          // 0 PathtracerMIS, 1 Pathtracer (NEE only), 2 PurePt
+    int integrator;          // 0 PathtracerMIS, 1 Pathtracer (NEE only), 2 PurePt, 3 PPG
+    // SD-tree guided arm (PPG): the tree, per path its vertices (origin, direction,
+    // radiance factor: 9 floats each, max_depth per path) and vertex count
+    SdDev sdt;
+    float* ppg_vtx;
+    uint32_t* ppg_nv;
     float* gsamples;         // training samples out (11 floats)
     uint32_t* gcount;
     const uint16_t* gwt;
@@ -215,19 +232,17 @@ __global__ void k_raygen(WfArgs a) {
     for (unsigned i = gtid(); i < n_paths; i += gthreads()) {
         const Segment sg = pd->seg[find_segment(pd, i, false)];
         const ViewDesc& vd = a.views[sg.view];
-        const int S = vd.S;
         unsigned local = i - sg.slot0;
         unsigned sl = local / sg.npix, pl = local - sl * sg.npix;
         unsigned pixel = sg.p0 + pl;
         unsigned sample = sg.s0 + sl;
         int px = (int)(pixel % (unsigned)vd.W), py = (int)(pixel / (unsigned)vd.W);
-        int k = (int)sample / S, l = (int)sample % S;
-        uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)(S * S) + sample;
+        uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)vd.spp + sample;
         uint64_t rng = pcg_init(a.seed, path_id);
         float r1 = pcg_float(rng);
         float r2 = pcg_float(rng);
         float cx, cy;
-        film_coords(px, py, k, l, S, vd.W, vd.H, r1, r2, cx, cy);
+        view_coords(vd, px, py, sample, r1, r2, cx, cy);
         V3 o, d;
         camera_ray(vd.cam, cx, cy, o, d);
         a.qo[0][i] = make_float4(o.x, o.y, o.z, INFINITY);
@@ -596,6 +611,140 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
     }
 }
 
+// ------------------------------------------------------------------ SD-tree guided arm
+// PPG::integrate (ppg.h:41-143), one loop iteration per queue entry: no NEE;
+// with probability 1/2 the BSDF, else the SD-tree; one-sample MIS weight
+// p_s / (p_s + p_other) over p = p_s / 2.  Vertex d-1 (the ray that reached this
+// hit) records its origin, direction and radiance factor: Le or the sky if the
+// ray ended there, eval * weight / p if the path went on, 1 if it was cut by the
+// BSDF support or Russian roulette (as the reference leaves it).
+template <int B>
+__global__ __launch_bounds__(B) void k_shade_ppg(WfArgs a, int cur, int depth_cap) {
+    __shared__ unsigned qsh[2][2 * (B / 64) + 2];
+    const unsigned count = a.counters[2 * cur];
+    unsigned long long* const out_ctr = reinterpret_cast<unsigned long long*>(a.counters + 2 * (cur ^ 1));
+    int par = 0;
+    for (unsigned base = blockIdx.x * B; base < count; base += gridDim.x * B, par ^= 1) {
+        const unsigned i = base + threadIdx.x;
+        bool push_ext = false;
+        V3 no = {0, 0, 0}, nd = {0, 0, 0};
+        int slot = 0;
+        if (i < count) {
+            float4 O = a.qo[cur][i], D = a.qd[cur][i], HH = a.hits[i];
+            slot = __float_as_int(D.w);
+            V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
+            float4 MP = a.mul_p[slot], C = a.col[slot];
+            uint64_t rng = a.rng[slot];
+            const int depth = (int)(a.meta[slot] & 0xffu);
+            V3 mulv = v3(MP.x, MP.y, MP.z), colv = v3(C.x, C.y, C.z);
+            Hit h;
+            h.t = HH.x;
+            h.beta = HH.y;
+            h.gamma = HH.z;
+            h.tri = __float_as_int(HH.w);
+            V3 vrad = v3s(1.0f);
+            bool alive = false;
+            if (h.tri < 0) {
+                V3 sky = v3(a.sc.env[0], a.sc.env[1], a.sc.env[2]);
+                colv = add(colv, mul(mulv, sky));
+                vrad = sky;
+            } else {
+                Isect it;
+                int matid;
+                fill_isect(a.sc, o, d, h, it, matid);
+                Mat m = load_mat(a.sc.mats + 16 * matid);
+                if (depth == 0) first_hit_frame(m, it);
+                V3 wo = mulf(d, -1.0f);
+                if (m.type == CPT_MAT_EMISSIVE) {
+                    colv = add(colv, mul(mulv, m.emission));
+                    vrad = m.emission;
+                } else {
+                    DSample ds;
+                    float weight = 1.0f;
+                    bool ok = true;
+                    if (pcg_float(rng) > 0.5f) {
+                        ds = bsdf_sample(m, it, wo, rng);
+                        weight = ds.pdf / (ds.pdf + sd_pdf(a.sdt, it.hitpoint, ds.wi));
+                    } else {
+                        ds.wi = sd_sample(a.sdt, it.hitpoint, rng, ds.pdf);
+                        ds.delta = false;
+                        if (!bsdf_non_zero(m, it, wo, ds.wi)) ok = false;
+                        else weight = ds.pdf / (ds.pdf + bsdf_pdf(m, it, wo, ds.wi));
+                    }
+                    if (ok && bsdf_non_zero(m, it, wo, ds.wi)) {
+                        float p = ds.pdf * 0.5f;
+                        V3 ev = bsdf_eval(m, it, wo, ds.wi);
+                        bool survive = true;
+                        if (depth > 2) {
+                            float rr = russian_roulette(ev);
+                            if (pcg_float(rng) > rr) survive = false;
+                            else p = p * rr;
+                        }
+                        if (survive) {
+                            V3 nm = mulf(ev, weight / p);
+                            vrad = nm;
+                            mulv = mul(mulv, nm);
+                            if (depth + 1 < depth_cap) {
+                                no = add(it.hitpoint, mulf(ds.wi, CPT_EPSF));
+                                nd = ds.wi;
+                                alive = true;
+                            }
+                        }
+                    }
+                }
+            }
+            if (depth > 0) {
+                float* v = a.ppg_vtx + ((size_t)slot * a.max_depth + (size_t)(depth - 1)) * 9;
+                v[0] = o.x;
+                v[1] = o.y;
+                v[2] = o.z;
+                v[3] = d.x;
+                v[4] = d.y;
+                v[5] = d.z;
+                v[6] = vrad.x;
+                v[7] = vrad.y;
+                v[8] = vrad.z;
+            }
+            a.col[slot] = make_float4(colv.x, colv.y, colv.z, 0.0f);
+            if (alive) {
+                a.mul_p[slot] = make_float4(mulv.x, mulv.y, mulv.z, 1.0f);
+                a.rng[slot] = rng;
+                a.meta[slot] = (uint32_t)(depth + 1);
+            } else {
+                a.ppg_nv[slot] = (uint32_t)depth;  // vertices of the ended path
+            }
+            push_ext = alive;
+        }
+        unsigned qe, qs;
+        block_alloc2<B>(push_ext, false, out_ctr, qsh[par], qe, qs);
+        if (push_ext) {
+            a.qo[cur ^ 1][qe] = make_float4(no.x, no.y, no.z, INFINITY);
+            a.qd[cur ^ 1][qe] = make_float4(nd.x, nd.y, nd.z, __int_as_float(slot));
+        }
+    }
+}
+
+// After a pass: every path whose last vertex carries radiance splats each vertex
+// with the product of the factors from it to the end (ppg.h:130-141), its
+// largest channel (rF, ppg.h:37-39).
+__global__ void k_sd_splat(WfArgs a, unsigned n_paths) {
+    for (unsigned slot = gtid(); slot < n_paths; slot += gthreads()) {
+        const int nv = (int)a.ppg_nv[slot];
+        if (nv <= 0) continue;
+        const float* vb = a.ppg_vtx + (size_t)slot * a.max_depth * 9;
+        const float* last = vb + (size_t)(nv - 1) * 9;
+        if (!(last[6] > 0.0f || last[7] > 0.0f || last[8] > 0.0f)) continue;
+        V3 acc = v3s(1.0f);
+        for (int k = nv - 1; k >= 0; k--) {
+            const float* v = vb + (size_t)k * 9;
+            V3 r = v3(v[6], v[7], v[8]);
+            if (k < nv - 1) r = mul(r, acc);
+            acc = r;
+            sd_splat(a.sdt, fmaxf(r.x, fmaxf(r.y, r.z)), v3(v[0], v[1], v[2]), v3(v[3], v[4], v[5]));
+        }
+    }
+}
+
 // ------------------------------------------------------------------ shadow
 template <int B, bool COUNT>
 __global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_shadow(WfArgs a, int nxt) {
@@ -750,14 +899,13 @@ __global__ __launch_bounds__(B) void k_regen(WfArgs a, StreamArgs sa, int nxt) {
             unsigned long long P = (unsigned long long)vd.W * (unsigned long long)vd.H;
             unsigned sample = (unsigned)(local / P);
             unsigned pixel = (unsigned)(local - (unsigned long long)sample * P);
-            int S = vd.S;
             int px = (int)(pixel % (unsigned)vd.W), py = (int)(pixel / (unsigned)vd.W);
-            uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)(S * S) + sample;
+            uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)vd.spp + sample;
             uint64_t rng = pcg_init(a.seed, path_id);
             float r1 = pcg_float(rng);
             float r2 = pcg_float(rng);
             float cx, cy;
-            film_coords(px, py, (int)sample / S, (int)sample % S, S, vd.W, vd.H, r1, r2, cx, cy);
+            view_coords(vd, px, py, sample, r1, r2, cx, cy);
             camera_ray(vd.cam, cx, cy, o, d);
             a.mul_p[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
             a.col[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -850,6 +998,7 @@ struct cpt_ctx {
     float cam[16];
     float env[3];
     float bmin[3], bext[3];
+    float tb_lo[3], tb_hi[3];  // bounds of the triangles (the reference BVH's root box)
 
     // wavefront buffers
     size_t cap = 0;        // path slots
@@ -870,7 +1019,8 @@ struct cpt_ctx {
     double* film = nullptr;
     double* film2 = nullptr;
     size_t film_px = 0;
-    int film_w = 0, film_h = 0, film_S = 1;
+    int film_w = 0, film_h = 0;
+    unsigned film_spp = 1;  // samples per pixel of the last single-view render
 
     // learned guide
     bool has_guide = false;
@@ -896,6 +1046,21 @@ struct cpt_ctx {
     uint32_t* h_count = nullptr; // pinned
     int64_t g_samples_total = 0; // training samples used by the last guided render
     int g_steps_total = 0;
+
+    // SD-tree guided arm: host tree, its flattened device copy, splat
+    // accumulators and per-path vertex records
+    std::unique_ptr<cpt::SdTree> sd_tree;
+    cpt::SdFlat sd_flat;
+    bool sd_dirty = true;  // host tree changed since the last upload
+    int32_t *sd_snode = nullptr, *sd_quad = nullptr;
+    int4* sd_dinfo = nullptr;
+    float4* sd_rect = nullptr;
+    float* sd_sum = nullptr;
+    uint32_t* sd_cnt = nullptr;
+    size_t sd_snode_cap = 0, sd_quad_cap = 0, sd_dinfo_cap = 0, sd_rect_cap = 0, sd_slot_cap = 0;
+    float* ppg_vtx = nullptr;
+    uint32_t* ppg_nv = nullptr;
+    size_t ppg_cap = 0;  // floats of ppg_vtx
 
     // captured pass graph
     hipGraphExec_t gexec = nullptr;
@@ -1062,6 +1227,18 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.guide_frac = c->g_cfg.guide_fraction;
     a.guide_depth = c->g_cfg.guide_depth > 0 ? c->g_cfg.guide_depth : 256;
     a.guide_target = c->g_cfg.guide_target;
+    a.sdt.snode = c->sd_snode;
+    a.sdt.dinfo = c->sd_dinfo;
+    a.sdt.quad = c->sd_quad;
+    a.sdt.rect = c->sd_rect;
+    a.sdt.sum = c->sd_sum;
+    a.sdt.cnt = c->sd_cnt;
+    for (int k = 0; k < 3; k++) {
+        a.sdt.bmin[k] = c->sd_tree ? c->sd_tree->bmin[k] : 0.0f;
+        a.sdt.bmax[k] = c->sd_tree ? c->sd_tree->bmax[k] : 0.0f;
+    }
+    a.ppg_vtx = c->ppg_vtx;
+    a.ppg_nv = c->ppg_nv;
     a.refill = CPT_DEFAULT_REFILL;
     a.integrator = 0;
     a.gsamples = c->g_samples;
@@ -1091,6 +1268,7 @@ struct LaunchCfg {
     // equal chunks, so the grid should be what the device holds at once: a
     // second round of waves would double the time of the CUs that run it.
     int GE = 0, GS = 0;
+    bool ppg = false;  // SD-tree guided arm: k_shade_ppg, no shadow rays
 };
 
 template <int B>
@@ -1134,6 +1312,11 @@ void L_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) {
     }
 }
 void L_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
+    if (L.ppg) {
+        const unsigned G = (unsigned)std::max(1, (int)(((long long)L.G * L.B) / 256));
+        hipLaunchKernelGGL((k_shade_ppg<256>), dim3(G), dim3(256), 0, s, a, cur, cap);
+        return;
+    }
     switch (L.SB) {
         case 64: launch_shade<64>(L, a, cur, cap, s); break;
         case 128: launch_shade<128>(L, a, cur, cap, s); break;
@@ -1176,8 +1359,10 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
             hipLaunchKernelGGL(k_guide_infer, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a, cur);
         L_shade(L, a, cur, a.max_depth, s);
         mark(1);
-        L_shadow(L, a, cur ^ 1, s);
-        mark(2);
+        if (!L.ppg) {
+            L_shadow(L, a, cur ^ 1, s);
+            mark(2);
+        }
     }
     hipLaunchKernelGGL(k_accum, dim3(L.G), dim3(L.B), 0, s, a);
     hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, s, a, -1);
@@ -1255,6 +1440,14 @@ void cpt_destroy(cpt_ctx* c) {
     dfree(c->g_loss_part);
     dfree(c->g_wscale);
     dfree(c->g_lobes);
+    dfree(c->sd_snode);
+    dfree(c->sd_quad);
+    dfree(c->sd_dinfo);
+    dfree(c->sd_rect);
+    dfree(c->sd_sum);
+    dfree(c->sd_cnt);
+    dfree(c->ppg_vtx);
+    dfree(c->ppg_nv);
     dfree(c->g_rec);
     dfree(c->g_count);
     if (c->h_count) (void)hipHostFree(c->h_count);
@@ -1320,7 +1513,16 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     for (int k = 0; k < 3; k++) {
         if (c->nv == 0) c->bmin[k] = c->bext[k] = 0.0f;
         c->bext[k] = std::max(c->bext[k] - c->bmin[k], 1e-6f);
+        c->tb_lo[k] = INFINITY;
+        c->tb_hi[k] = -INFINITY;
     }
+    for (size_t i = 0; i < 3 * (size_t)c->nt; i++)
+        for (int k = 0; k < 3; k++) {
+            const float v = d->positions[3 * (size_t)d->indices[i] + k];
+            c->tb_lo[k] = std::min(c->tb_lo[k], v);
+            c->tb_hi[k] = std::max(c->tb_hi[k], v);
+        }
+    c->sd_tree.reset();
     std::vector<float4> vp(std::max(1, c->nv)), vn(std::max(1, c->nv)), vb(std::max(1, c->nv));
     for (int i = 0; i < c->nv; i++) {
         vp[i] = make_float4(d->positions[3 * (size_t)i], d->positions[3 * (size_t)i + 1], d->positions[3 * (size_t)i + 2], 0.f);
@@ -1505,6 +1707,7 @@ int cpt_trace_closest_device(cpt_ctx* c, const void* d_rays, int32_t n, void* d_
     return CPT_OK;
 }
 
+static int render_ppg(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats);
 static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras, const int32_t* wh_spp,
                                int32_t max_depth, uint64_t seed, int32_t flags, float* films, double* view_ms,
                                cpt_render_stats* stats);
@@ -1539,7 +1742,7 @@ static void plan_passes(const std::vector<ViewDesc>& vd, int64_t cap, std::vecto
     };
     for (size_t v = 0; v < vd.size(); v++) {
         int64_t P = (int64_t)vd[v].W * vd[v].H;
-        int64_t S2 = (int64_t)vd[v].S * vd[v].S;
+        int64_t S2 = (int64_t)vd[v].spp;
         if (P <= cap) {
             int64_t s0 = 0;
             while (s0 < S2) {
@@ -1573,16 +1776,58 @@ static void plan_passes(const std::vector<ViewDesc>& vd, int64_t cap, std::vecto
     if (passes.back().n_seg == 0) passes.pop_back();
 }
 
+// Flatten the host SD-tree into the device arrays (when it changed) and clear
+// the splat accumulators.
+static int sd_upload(cpt_ctx* c) {
+    if (!c->sd_dirty) return CPT_OK;
+    cpt::SdFlat& f = c->sd_flat;
+    c->sd_tree->flatten(f);
+    const size_t nd = f.dinfo.size() / 4, nr = f.rect.size() / 4, ns = f.slot_dtree.size();
+    int r = CPT_OK;
+    auto grow = [&](auto*& p, size_t& cap, size_t n) {
+        if (r || (cap >= n && p)) return;
+        drop_graph(c);
+        const size_t want = std::max<size_t>(n + n / 2, 64);
+        r = dalloc(c, p, want);
+        if (!r) cap = want;
+    };
+    grow(c->sd_snode, c->sd_snode_cap, f.snode.size());
+    grow(c->sd_dinfo, c->sd_dinfo_cap, nd);
+    grow(c->sd_quad, c->sd_quad_cap, f.quad.size());
+    grow(c->sd_rect, c->sd_rect_cap, nr);
+    if (!r && (c->sd_slot_cap < ns || !c->sd_sum || !c->sd_cnt)) {
+        drop_graph(c);
+        const size_t want = std::max<size_t>(ns + ns / 2, 64);
+        r = dalloc(c, c->sd_sum, want);
+        if (!r) r = dalloc(c, c->sd_cnt, want);
+        if (!r) c->sd_slot_cap = want;
+    }
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(c->sd_snode, f.snode.data(), f.snode.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->sd_dinfo, f.dinfo.data(), f.dinfo.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->sd_quad, f.quad.data(), f.quad.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->sd_rect, f.rect.data(), f.rect.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sd_sum, 0, ns * sizeof(float), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sd_cnt, 0, ns * sizeof(uint32_t), c->stream));
+    // the host copies must stay alive until the copies ran
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->sd_dirty = false;
+    return CPT_OK;
+}
+
 // Render a batch of views into the concatenated FP64 films.
 static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, uint64_t seed, int flags, int B, int G,
                         int SB, int refill, int64_t max_paths, double* view_ms, cpt_render_stats* stats) {
     if (max_depth <= 0 || max_depth > 255) return fail(c, CPT_E_INVALID, "max_depth must be in [1, 255]");
     if (refill < 1 || refill > 64) return fail(c, CPT_E_INVALID, "refill must be in [1, 64]");
-    const int integrator = (flags & CPT_RENDER_PURE_PT) ? 2 : (flags & CPT_RENDER_NEE_ONLY) ? 1 : 0;
-    if ((flags & CPT_RENDER_PURE_PT) && (flags & CPT_RENDER_NEE_ONLY))
-        return fail(c, CPT_E_INVALID, "CPT_RENDER_NEE_ONLY and CPT_RENDER_PURE_PT exclude each other");
+    const int integrator = (flags & CPT_RENDER_PPG_STEP) ? 3
+                           : (flags & CPT_RENDER_PURE_PT) ? 2
+                           : (flags & CPT_RENDER_NEE_ONLY) ? 1
+                                                            : 0;
+    if (__builtin_popcount(flags & (CPT_RENDER_PURE_PT | CPT_RENDER_NEE_ONLY | CPT_RENDER_PPG_STEP)) > 1)
+        return fail(c, CPT_E_INVALID, "CPT_RENDER_NEE_ONLY, CPT_RENDER_PURE_PT and CPT_RENDER_PPG exclude each other");
     if (integrator && (flags & CPT_RENDER_GUIDED))
-        return fail(c, CPT_E_INVALID, "guided rendering uses the MIS integrator");
+        return fail(c, CPT_E_INVALID, "the learned guide uses the MIS integrator");
     if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
     if (SB != 64 && SB != 128 && SB != 256 && SB != 512 && SB != 1024)
         return fail(c, CPT_E_INVALID, "shade_block must be 64, 128, 256, 512 or 1024");
@@ -1597,12 +1842,16 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     unsigned long long film_px = 0;
     for (size_t v = 0; v < vd.size(); v++) {
         if (vd[v].W <= 0 || vd[v].H <= 0 || vd[v].S <= 0) return fail(c, CPT_E_INVALID, "view %zu: bad size or spp", v);
-        int64_t P = (int64_t)vd[v].W * vd[v].H, S2 = (int64_t)vd[v].S * vd[v].S;
+        if (vd[v].spp == 0) {  // stratified S x S
+            if (vd[v].S > (1 << 10)) return fail(c, CPT_E_LIMIT, "view %zu too large", v);
+            vd[v].spp = (unsigned)(vd[v].S * vd[v].S);
+            vd[v].strat = 1;
+        }
+        int64_t P = (int64_t)vd[v].W * vd[v].H, S2 = (int64_t)vd[v].spp;
         if (P > (1 << 28) || S2 > (1 << 20) || P * S2 >= (1ll << 40)) return fail(c, CPT_E_LIMIT, "view %zu too large", v);
         vd[v].n_paths = (unsigned long long)(P * S2);
         vd[v].film_off = film_px;
         vd[v].path_begin = 0;
-        vd[v].pad = 0;
         film_px += (unsigned long long)P;
     }
     if (film_px >= (1ull << 31)) return fail(c, CPT_E_LIMIT, "films too large");
@@ -1682,6 +1931,20 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         HIPCHK(c, hipMemsetAsync(c->film, 0, 3 * sizeof(double) * film_px, c->stream));
         HIPCHK(c, hipMemsetAsync(c->film2, 0, 3 * sizeof(double) * film_px, c->stream));
     }
+    if (integrator == 3) {
+        if (!c->sd_tree) return fail(c, CPT_E_INVALID, "PPG step without a tree (cpt_ppg_reset)");
+        r = sd_upload(c);
+        if (r) return r;
+        const size_t need = slots * (size_t)max_depth * 9;
+        if (c->ppg_cap < need || !c->ppg_vtx) {
+            drop_graph(c);
+            dfree(c->ppg_nv);
+            r = dalloc(c, c->ppg_vtx, need);
+            if (!r) r = dalloc(c, c->ppg_nv, slots);
+            if (r) return r;
+            c->ppg_cap = need;
+        }
+    }
     WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
     a.n_rec = (uint32_t)nrec;
     a.refill = (unsigned)refill;
@@ -1691,6 +1954,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     L.GE = GE;
     L.GS = GS;
     L.guided = guided;
+    L.ppg = integrator == 3;
     L.infer_blocks = c->cus;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
     hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, a, (int)passes.size());
@@ -1700,11 +1964,15 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     bool use_graph = !(flags & (CPT_RENDER_NO_GRAPH | CPT_RENDER_KERNEL_TIMING));
     Timers tm;
     if (use_graph) {
-        char key[256];
-        snprintf(key, sizeof(key), "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d,%d", max_depth,
-                 (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
+        char key[512];
+        snprintf(key, sizeof(key),
+                 "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d,%d,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
+                 max_depth, (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
-                 (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator);
+                 (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator,
+                 (void*)c->sd_snode, (void*)c->sd_dinfo, (void*)c->sd_quad, (void*)c->sd_rect, (void*)c->sd_sum,
+                 (void*)c->ppg_vtx, (void*)c->ppg_nv, a.sdt.bmin[0], a.sdt.bmin[1], a.sdt.bmin[2], a.sdt.bmax[0],
+                 a.sdt.bmax[1], a.sdt.bmax[2]);
         if (!c->gexec || c->graph_key != key) {
             drop_graph(c);
             hipGraph_t g;
@@ -1727,6 +1995,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
             r = issue_pass(c, L, a, c->stream, timing ? &tm : nullptr);
             if (r) return r;
         }
+        if (L.ppg) hipLaunchKernelGGL(k_sd_splat, dim3(G), dim3(B), 0, c->stream, a, c->h_pass[i].n_paths);
         if (train) {
             HIPCHK(c, hipMemsetAsync(c->g_count, 0, sizeof(uint32_t), c->stream));
             hipLaunchKernelGGL(k_guide_collect, dim3(G), dim3(B), 0, c->stream, a);
@@ -1784,7 +2053,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     for (auto e : tm.ev) (void)hipEventDestroy(e);
     c->film_w = vd[0].W;
     c->film_h = vd[0].H;
-    c->film_S = vd[0].S;
+    c->film_spp = vd[0].spp;
     return CPT_OK;
 }
 
@@ -1797,12 +2066,18 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
         int32_t whs[3] = {p->width, p->height, p->spp_axis};
         return render_views_stream(c, 1, c->cam, whs, p->max_depth, p->seed, p->flags, nullptr, nullptr, stats);
     }
+    if (p->flags & CPT_RENDER_PPG) return render_ppg(c, p, stats);
     std::vector<ViewDesc> vd(1);
     std::memcpy(vd[0].cam, c->cam, sizeof(c->cam));
     vd[0].W = p->width;
     vd[0].H = p->height;
     vd[0].S = p->spp_axis;
     vd[0].id = (unsigned)p->view;
+    if (p->flags & CPT_RENDER_PPG_STEP) {  // spp_axis = samples per pixel, whole-pixel jitter
+        vd[0].S = 1;
+        vd[0].spp = (unsigned)p->spp_axis;
+        vd[0].strat = 0;
+    }
     // parameters left 0: the autotuned configuration for this key, else defaults
     const cpt_ctx::Tuned& T = c->tuned;
     const bool tuned = T.valid && T.scene_gen == c->scene_gen && T.W == p->width && T.H == p->height &&
@@ -1810,11 +2085,105 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     int B = p->block_size ? p->block_size : (tuned ? T.B : 128);
     int G = p->grid_blocks ? p->grid_blocks : (tuned && !p->block_size ? T.G : 0);  // 0: resident grid
     int64_t cap = p->max_paths ? p->max_paths : (1 << 25);
+    // PPG keeps max_depth vertices per path in flight
+    if ((p->flags & CPT_RENDER_PPG_STEP) && !p->max_paths) cap = 1 << 22;
     // online training updates the guide between passes: one sample per pixel per pass
     if ((p->flags & CPT_RENDER_GUIDE_TRAIN) && !p->max_paths) cap = (int64_t)p->width * p->height;
     int SB = p->shade_block ? p->shade_block : (tuned ? T.SB : 256);
     int refill = p->refill ? p->refill : (tuned ? T.refill : (int)CPT_DEFAULT_REFILL);
     return render_batch(c, vd, p->max_depth, p->seed, p->flags, B, G, SB, refill, cap, nullptr, stats);
+}
+
+// PPG::render (ppg.h:145-205): iterations of 1, 2, 4, ... samples per pixel
+// (the last one below spp_axis^2), each rendered with the SD-tree refined from
+// the previous ones' splats; the film keeps the last iteration only.
+static int render_ppg(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) {
+    if (p->flags & (CPT_RENDER_GUIDED | CPT_RENDER_STREAM | CPT_RENDER_ACCUMULATE | CPT_RENDER_PPG_STEP))
+        return fail(c, CPT_E_INVALID, "CPT_RENDER_PPG runs its own schedule");
+    std::vector<int> iters;
+    const int64_t total = (int64_t)p->spp_axis * p->spp_axis;
+    for (int64_t spp = 1;; spp *= 2) {
+        iters.push_back((int)spp);
+        if (spp * 2 >= total) break;
+    }
+    int r = cpt_ppg_reset(c);
+    if (r) return r;
+    cpt_render_stats acc{};
+    for (size_t l = 0; l < iters.size(); l++) {
+        cpt_render_params q = *p;
+        q.flags = (p->flags & ~CPT_RENDER_PPG) | CPT_RENDER_PPG_STEP;
+        q.spp_axis = iters[l];
+        q.view = (int32_t)l;
+        cpt_render_stats st{};
+        r = cpt_render(c, &q, &st);
+        if (r) return r;
+        if (l + 1 < iters.size()) {
+            r = cpt_ppg_update(c);
+            if (r) return r;
+        }
+        acc.seconds += st.seconds;
+        acc.paths += st.paths;
+        acc.extension_rays += st.extension_rays;
+        acc.passes += st.passes;
+        acc.paths_per_pass = st.paths_per_pass;
+        acc.block_size = st.block_size;
+        acc.grid_blocks = st.grid_blocks;
+    }
+    if (stats) *stats = acc;
+    return CPT_OK;
+}
+
+int cpt_ppg_reset(cpt_ctx* c) {
+    if (!c) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    c->sd_tree.reset(new cpt::SdTree(c->tb_lo, c->tb_hi));
+    c->sd_dirty = true;
+    return CPT_OK;
+}
+
+int cpt_ppg_update(cpt_ctx* c) {
+    if (!c) return CPT_E_INVALID;
+    if (!c->sd_tree) return fail(c, CPT_E_INVALID, "no SD-tree (cpt_ppg_reset)");
+    if (!c->sd_dirty) {  // the device holds splats made with the current flattening
+        const size_t ns = c->sd_flat.slot_dtree.size();
+        std::vector<float> sums(ns);
+        std::vector<uint32_t> cnt(ns);
+        HIPCHK(c, hipMemcpyAsync(sums.data(), c->sd_sum, ns * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(cnt.data(), c->sd_cnt, ns * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->sd_tree->add_splats(c->sd_flat, sums.data(), cnt.data());
+    }
+    c->sd_tree->update();
+    c->sd_dirty = true;
+    return CPT_OK;
+}
+
+int cpt_ppg_export(cpt_ctx* c, int32_t* sizes, int32_t* snode, int32_t* dinfo, int32_t* quad, float* rect, float* sums,
+                   uint32_t* counts, float* bounds) {
+    if (!c || !sizes) return CPT_E_INVALID;
+    if (!c->sd_tree) return fail(c, CPT_E_INVALID, "no SD-tree (cpt_ppg_reset)");
+    int r = sd_upload(c);  // the flattening the next step uses
+    if (r) return r;
+    const cpt::SdFlat& f = c->sd_flat;
+    const size_t ns = f.slot_dtree.size();
+    sizes[0] = (int32_t)f.snode.size();
+    sizes[1] = (int32_t)(f.dinfo.size() / 4);
+    sizes[2] = (int32_t)f.quad.size();
+    sizes[3] = (int32_t)(f.rect.size() / 4);
+    sizes[4] = (int32_t)ns;
+    if (snode) std::memcpy(snode, f.snode.data(), f.snode.size() * 4);
+    if (dinfo) std::memcpy(dinfo, f.dinfo.data(), f.dinfo.size() * 4);
+    if (quad) std::memcpy(quad, f.quad.data(), f.quad.size() * 4);
+    if (rect) std::memcpy(rect, f.rect.data(), f.rect.size() * 4);
+    if (sums) HIPCHK(c, hipMemcpyAsync(sums, c->sd_sum, ns * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (counts) HIPCHK(c, hipMemcpyAsync(counts, c->sd_cnt, ns * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bounds)
+        for (int k = 0; k < 3; k++) {
+            bounds[k] = c->sd_tree->bmin[k];
+            bounds[3 + k] = c->sd_tree->bmax[k];
+        }
+    return CPT_OK;
 }
 
 int cpt_autotune(cpt_ctx* c, const cpt_render_params* p, cpt_tune_result* out) {
@@ -1906,7 +2275,7 @@ int cpt_read_film(cpt_ctx* c, float* rgb, double* sums, double* sumsq) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (sums) std::memcpy(sums, h.data(), h.size() * sizeof(double));
     if (rgb) {
-        double inv = 1.0 / ((double)c->film_S * c->film_S);
+        double inv = 1.0 / (double)c->film_spp;
         for (size_t i = 0; i < h.size(); i++) rgb[i] = (float)(h[i] * inv);
     }
     return CPT_OK;
@@ -1932,7 +2301,8 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
         vd[v].H = H;
         vd[v].S = S;
         vd[v].id = (unsigned)v;
-        vd[v].pad = 0;
+        vd[v].spp = (unsigned)(S * S);
+        vd[v].strat = 1;
         vd[v].path_begin = total;
         vd[v].n_paths = (unsigned long long)W * H * S * S;
         vd[v].film_off = film_px;
@@ -2076,7 +2446,7 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
             }
             if (view_ms) view_ms[v] = (double)(hs[2 + n_views + v] - hs[1]) / 100e3;  // 100 MHz clock
             if (films) {
-                double inv = 1.0 / ((double)vd[v].S * vd[v].S);
+                double inv = 1.0 / (double)vd[v].spp;
                 for (size_t k = 0; k < 3 * (size_t)vd[v].W * vd[v].H; k++)
                     films[3 * vd[v].film_off + k] = (float)(h[3 * vd[v].film_off + k] * inv);
             }
@@ -2101,7 +2471,7 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
     // the film now holds the batch; a later cpt_read_film reads view 0's layout only
     c->film_w = vd[0].W;
     c->film_h = vd[0].H;
-    c->film_S = vd[0].S;
+    c->film_spp = vd[0].spp;
     return rc;
 }
 
@@ -2131,7 +2501,7 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
         HIPCHK(c, hipMemcpyAsync(h.data(), c->film, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (auto& v : vd) {
-            double inv = 1.0 / ((double)v.S * v.S);
+            double inv = 1.0 / (double)v.spp;
             for (size_t k = 0; k < 3 * (size_t)v.W * v.H; k++) films[3 * v.film_off + k] = (float)(h[3 * v.film_off + k] * inv);
         }
     }

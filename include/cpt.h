@@ -115,6 +115,12 @@ typedef struct {
                                        emitters count only after delta or camera bounces */
 #define CPT_RENDER_PURE_PT 256      /* PurePt (src/renderer/pure_pt.h:27-67): BSDF sampling only,
                                        every emitter hit counts, no shadow rays */
+#define CPT_RENDER_PPG 512          /* PPG (src/renderer/ppg.h:145-205): the SD-tree guided renderer's
+                                       schedule of 1, 2, 4, ... spp iterations, the tree refined between
+                                       them, the film holding the last iteration */
+#define CPT_RENDER_PPG_STEP 1024    /* one PPG iteration with the current SD-tree (ppg.h:41-143):
+                                       spp_axis = samples per pixel (whole-pixel jitter); splats
+                                       accumulate until cpt_ppg_update */
 
 typedef struct {
     double seconds;           /* wall time of the render call (device synchronised) */
@@ -175,6 +181,21 @@ int cpt_trace_closest_device(cpt_ctx* ctx, const void* d_rays, int32_t n, void* 
 /* Render into the context's device film (FP64 accumulators).  Replaces the
  * pixel loop of PathtracerMIS::render (pathtracerMIS.h:139-160). */
 int cpt_render(cpt_ctx* ctx, const cpt_render_params* params, cpt_render_stats* stats);
+
+/* SD-tree of the PPG arm (src/algorithms/sdtree.h).
+ * cpt_ppg_reset: a fresh tree over the scene's triangle bounds (SDTree ctor, sdtree.h:434-438).
+ * cpt_ppg_update: add the splats of the PPG steps since the last update, then refine
+ *   (SDTree::update / subdivide, sdtree.h:454-504; DTree::adapt_and_build_leaves 172-233).
+ * cpt_ppg_export: the flattened tree the next step uses and the current splat sums:
+ *   sizes = {spatial nodes, directional trees, quadtree nodes, sampling leaves, accumulators};
+ *   snode (>= 0: children v, v+1; < 0: dtree -1-v), dinfo (4 per dtree: quad base, leaf base,
+ *   leaf count, 0), quad (>= 0: first child; < 0: accumulator -1-v), rect (4 floats per leaf:
+ *   phi, cos theta, sizes), sums/counts per accumulator, bounds = lo[3], hi[3].  Any array
+ *   pointer may be NULL (sizes first). */
+int cpt_ppg_reset(cpt_ctx* ctx);
+int cpt_ppg_update(cpt_ctx* ctx);
+int cpt_ppg_export(cpt_ctx* ctx, int32_t* sizes, int32_t* snode, int32_t* dinfo, int32_t* quad, float* rect,
+                   float* sums, uint32_t* counts, float* bounds);
 
 /* Launch configuration found by cpt_autotune. */
 typedef struct {

@@ -46,6 +46,7 @@ def lib():
     L.orc_render.argtypes = [vp, i32, i32, i32, i32, i32, C.c_uint64, i32, i32, i32, i32, vp, vp, vp, i32, vp]
     L.orc_path.argtypes = [vp, i32, i32, i32, i32, C.c_uint64, i32, i32, vp]
     L.orc_set_integrator.argtypes = [vp, i32]
+    L.orc_ppg_render.argtypes = [vp, i32, i32, i32, i32, C.c_uint64, C.c_uint32] + [vp] * 5 + [i32, vp, vp, vp, i32]
     L.orc_path.restype = None
     L.orc_diffuse_sample.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp]
     L.orc_diffuse_sample.restype = None
@@ -161,6 +162,18 @@ class OracleScene:
         lib().orc_render(self.h, W, H, spp_axis, max_depth, 0 if sampler == "pcg" else 1, seed,
                          c0, c1, r0, r1, _p(sums), _p(sq), _p(film), threads, _p(counts))
         return (sums, film, counts, sq) if sumsq else (sums, film, counts)
+
+    def ppg_render(self, W, H, spp, max_depth, seed, view, tree, threads=0):
+        """One PPG iteration with the fixed SD-tree `tree` (a dict as returned by
+        cpt.Context.ppg_export); returns (pixel sums (H,W,3), splat sums, splat counts)."""
+        sums = np.zeros((H, W, 3), np.float64)
+        ns = len(tree["sums"])
+        ssum = np.zeros(ns, np.float64)
+        scnt = np.zeros(ns, np.int64)
+        t = {k: np.ascontiguousarray(v) for k, v in tree.items()}
+        lib().orc_ppg_render(self.h, W, H, spp, max_depth, seed, view, _p(t["snode"]), _p(t["dinfo"]), _p(t["quad"]),
+                             _p(t["rect"]), _p(t["bounds"]), ns, _p(sums), _p(ssum), _p(scnt), threads)
+        return sums, ssum, scnt
 
     def path(self, W, H, spp_axis, max_depth, seed, pixel, sample, integrator=0):
         out = np.zeros(3, np.float32)

@@ -822,6 +822,140 @@ V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Co
     return col;
 }
 
+// ---------------------------------------------------------------- PPG (SD-tree guided arm)
+// Flattened SD-tree (the layout of the library's cpt_ppg_export) and the
+// reference's lookups on it: SDTree::get_leaf(Vec3) sdtree.h:345-396, DTree
+// get_leaf/get_area 268-343, sample_direction 400-423, pdf 525-538, splat 514-524.
+struct SdTreeView {
+    const int32_t* snode;
+    const int32_t* dinfo;
+    const int32_t* quad;
+    const float* rect;
+    float lo[3], hi[3];
+};
+
+int sd_dtree_of(const SdTreeView& t, V3 p) {
+    float lo[3] = {t.lo[0], t.lo[1], t.lo[2]}, hi[3] = {t.hi[0], t.hi[1], t.hi[2]};
+    const float pc[3] = {p.x, p.y, p.z};
+    int n = 0;
+    for (int depth = 0;; depth++) {
+        const int v = t.snode[n];
+        if (v < 0) return -1 - v;
+        const int ax = depth % 3;
+        const float mid = (float)((double)(lo[ax] + hi[ax]) * 0.5);
+        if (pc[ax] > mid) {
+            lo[ax] = mid;
+            n = v + 1;
+        } else {
+            hi[ax] = mid;
+            n = v;
+        }
+    }
+}
+
+int sd_quad_of(const SdTreeView& t, int dt, float x, float y, float* ax, float* ay) {
+    float mnx = (float)-M_PI, mny = -1.0f, mxx = (float)M_PI, mxy = 1.0f;
+    int n = t.dinfo[4 * dt];
+    while (true) {
+        const float midx = (float)((double)(mnx + mxx) * 0.5), midy = (float)((double)(mny + mxy) * 0.5);
+        const int v = t.quad[n];
+        if (v < 0) {
+            *ax = mxx - mnx;
+            *ay = mxy - mny;
+            return -1 - v;
+        }
+        int child;
+        if (x < midx) {
+            if (y < midy) { child = 0; mxx = midx; mxy = midy; }
+            else { child = 2; mxx = midx; mny = midy; }
+        } else {
+            if (y < midy) { child = 1; mnx = midx; mxy = midy; }
+            else { child = 3; mnx = midx; mny = midy; }
+        }
+        n = v + child;
+    }
+}
+
+float sd_pdf_of(const SdTreeView& t, V3 p, V3 w) {
+    const int dt = sd_dtree_of(t, p);
+    const float phi = std::atan2(w.y, w.x);
+    float ax, ay;
+    sd_quad_of(t, dt, phi, w.z, &ax, &ay);
+    return (float)((1.0 / (double)t.dinfo[4 * dt + 2]) * (1.0 / (double)(ax * ay)));
+}
+
+V3 sd_sample_of(const SdTreeView& t, V3 p, Sampler& smp, float* pdf) {
+    const int dt = sd_dtree_of(t, p);
+    const int sz = t.dinfo[4 * dt + 2];
+    const int i = (int)(smp.sample() * (float)sz);
+    const float u = smp.sample();
+    const float v = smp.sample();
+    const float* r = t.rect + 4 * (size_t)(t.dinfo[4 * dt + 1] + i % sz);
+    const float area = r[2] * r[3];
+    const float phi = r[0] + u * r[2];
+    const float ct = r[1] + v * r[3];
+    const float st = (float)std::sqrt(1.0 - (double)(ct * ct));
+    *pdf = (float)(1.0 / (double)((float)sz * area));
+    return V3(std::cos(phi) * st, std::sin(phi) * st, ct);
+}
+
+struct PpgVertex {
+    V3 pos, dir, rad;
+};
+
+// PPG::integrate (ppg.h:41-143)
+V3 ppg_integrate(const Scene& s, const SdTreeView& t, float cx, float cy, Sampler& smp, int max_depth,
+                 std::vector<PpgVertex>& vs) {
+    Ray ray = camera_ray(s.cam, cx, cy);
+    Isect it;
+    V3 col(0.0f), mul(1.0f);
+    vs.clear();
+    for (int i = 0; i < max_depth; i++) {
+        Hit h;
+        const bool hit = intersect(s, ray, h);
+        if (i > 0) vs.push_back({ray.o, ray.d, V3(1.0f)});
+        if (!hit) {
+            col = col + mul * s.env;
+            if (i > 0) vs.back().rad = s.env;
+            break;
+        }
+        fill_isect(s, ray, h, it);
+        const Material& m = s.mats[s.tri_mat[h.tri]];
+        first_hit_frame(m, it);
+        V3 wo = ray.d * (float)(-1.0);
+        if (m.type == 2) {
+            if (i > 0) vs.back().rad = m.emission;
+            col = col + mul * m.emission;
+            break;
+        }
+        DSample smpl;
+        float weight = 1.0f;
+        if (smp.sample() > 0.5f) {
+            smpl = bsdf_sample(m, it, wo, smp);
+            weight = smpl.pdf / (smpl.pdf + sd_pdf_of(t, it.hitpoint, smpl.wi));
+        } else {
+            smpl.wi = sd_sample_of(t, it.hitpoint, smp, &smpl.pdf);
+            smpl.delta = false;
+            if (!bsdf_non_zero(m, it, wo, smpl.wi)) break;
+            weight = smpl.pdf / (smpl.pdf + bsdf_pdf(m, it, wo, smpl.wi));
+        }
+        V3 sdir = smpl.wi;
+        float p = (float)((double)smpl.pdf * 0.5);
+        if (!bsdf_non_zero(m, it, wo, sdir)) break;
+        V3 ev = bsdf_eval(m, it, wo, sdir);
+        if (i > 2) {
+            float rr = russian_roulette(ev);
+            if (smp.sample() > rr) break;
+            p = p * rr;
+        }
+        V3 nm = ev * (weight / p);
+        if (i > 0) vs.back().rad = nm;
+        mul = mul * nm;
+        ray = Ray{it.hitpoint + sdir * EPSF, sdir, std::numeric_limits<float>::infinity()};
+    }
+    return col;
+}
+
 // pathtracerMIS.h:150-153 film coordinates of stratum (k,l) of pixel (i,j)
 inline void film_coords(int i, int j, int k, int l, int S, int W, int H, float r1, float r2, float* cx,
                         float* cy) {
@@ -992,6 +1126,63 @@ int orc_render(void* sp, int W, int H, int S, int max_depth, int sampler, uint64
         counts[0] = paths;
         counts[1] = ext;
         counts[2] = shadow;
+    }
+    return 0;
+}
+
+int orc_ppg_render(void* sp, int W, int H, int spp, int max_depth, uint64_t seed, uint32_t view, const int32_t* snode,
+                   const int32_t* dinfo, const int32_t* quad, const float* rect, const float* bounds, int n_slots,
+                   double* sum_out, double* splat_sum, int64_t* splat_cnt, int nthreads) {
+    const Scene* s = (const Scene*)sp;
+    SdTreeView t{snode, dinfo, quad, rect, {bounds[0], bounds[1], bounds[2]}, {bounds[3], bounds[4], bounds[5]}};
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    for (int k = 0; k < n_slots; k++) {
+        splat_sum[k] = 0.0;
+        splat_cnt[k] = 0;
+    }
+#pragma omp parallel
+    {
+        std::vector<double> ls((size_t)n_slots, 0.0);
+        std::vector<int64_t> lc((size_t)n_slots, 0);
+        std::vector<PpgVertex> vs;
+#pragma omp for schedule(dynamic, 1)
+        for (int j = 0; j < H; j++)
+            for (int i = 0; i < W; i++) {
+                const uint64_t pixel = (uint64_t)i + (uint64_t)j * W;
+                double ax = 0, ay = 0, az = 0;
+                for (int k = 0; k < spp; k++) {
+                    PcgSampler pcg(seed, ((uint64_t)view << 40) + pixel * (uint64_t)spp + (uint64_t)k);
+                    float r1 = pcg.sample();
+                    float r2 = pcg.sample();
+                    float cx, cy;
+                    film_coords(i, j, 0, 0, 1, W, H, r1, r2, &cx, &cy);  // ppg.h:178-181
+                    V3 c = ppg_integrate(*s, t, cx, cy, pcg, max_depth, vs);
+                    ax += c.x;
+                    ay += c.y;
+                    az += c.z;
+                    // ppg.h:130-141
+                    if (!vs.empty() && (vs.back().rad.x > 0 || vs.back().rad.y > 0 || vs.back().rad.z > 0)) {
+                        for (int v = (int)vs.size() - 1; v >= 0; v--) {
+                            if (v < (int)vs.size() - 1) vs[v].rad = vs[v].rad * vs[v + 1].rad;
+                            const V3 r = vs[v].rad;
+                            const float f = std::max(r.x, std::max(r.y, r.z));
+                            const int dt = sd_dtree_of(t, vs[v].pos);
+                            float qx, qy;
+                            const int slot = sd_quad_of(t, dt, std::atan2(vs[v].dir.y, vs[v].dir.x), vs[v].dir.z, &qx, &qy);
+                            ls[slot] += f;
+                            lc[slot] += 1;
+                        }
+                    }
+                }
+                sum_out[3 * pixel] = ax;
+                sum_out[3 * pixel + 1] = ay;
+                sum_out[3 * pixel + 2] = az;
+            }
+#pragma omp critical
+        for (int k = 0; k < n_slots; k++) {
+            splat_sum[k] += ls[k];
+            splat_cnt[k] += lc[k];
+        }
     }
     return 0;
 }

@@ -75,6 +75,14 @@ int orc_render(void* s, int W, int H, int spp_axis, int max_depth, int sampler, 
  * 1 Pathtracer, NEE without MIS (pathtracer.h:27-110), 2 PurePt (pure_pt.h:27-67). */
 void orc_set_integrator(void* s, int integrator);
 
+/* One iteration of PPG (src/renderer/ppg.h:41-143) with a fixed, flattened SD-tree (the
+ * layout of cpt_ppg_export; bounds = lo[3], hi[3]): spp whole-pixel samples per pixel, PCG
+ * path id (view << 40) + pixel*spp + sample.  sum_out receives the per-pixel radiance sums,
+ * splat_sum / splat_cnt (n_slots each) what the paths splat into each quadtree leaf. */
+int orc_ppg_render(void* s, int W, int H, int spp, int max_depth, uint64_t seed, uint32_t view, const int32_t* snode,
+                   const int32_t* dinfo, const int32_t* quad, const float* rect, const float* bounds, int n_slots,
+                   double* sum_out, double* splat_sum, int64_t* splat_cnt, int nthreads);
+
 /* One path with an explicit PCG path id; returns radiance in rgb3. */
 void orc_path(void* s, int W, int H, int spp_axis, int max_depth, uint64_t seed, int pixel,
               int sample, float* rgb3);
