@@ -16,6 +16,7 @@ reference's PathtracerMIS with its mt19937 column sampler and binary BVH).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -53,6 +54,7 @@ def parse():
     ap.add_argument("--ref-spp-axis", type=int, default=64, help="guided: reference spp per axis (64 -> 4096 spp)")
     ap.add_argument("--boxes", type=int, default=40000, help="guided: furniture boxes (48 triangles each)")
     ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--no-ppg", action="store_true", help="guided: skip the SD-tree (PPG) arm")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales HBM counters)")
     return ap.parse_args()
@@ -135,6 +137,27 @@ def bench_guided(args):
     plain = run(False)
     guided = run(True)
 
+    # SD-tree arm: the reference's PPG schedule (iterations of 1, 2, 4, ... spp, the
+    # last one kept; no NEE), sized to each budget from a short calibration run
+    ppg = {}
+    if not args.no_ppg:
+        t = time.perf_counter()
+        ctx.render(W, H, 8, max_depth=depth, seed=7, flags=cpt.CPT_RENDER_PPG)
+        per_spp = (time.perf_counter() - t) / 63.0
+        for T in budgets:
+            # the schedule renders about 2x its last iteration: total ~ sa^2 spp
+            sa = max(2, int(math.sqrt(T / per_spp)))
+            t = time.perf_counter()
+            ctx.render(W, H, sa, max_depth=depth, seed=11, flags=cpt.CPT_RENDER_PPG)
+            el = time.perf_counter() - t
+            last = 1
+            while last * 2 < sa * sa:
+                last *= 2
+            _, sums = ctx.read_film(sums=True)
+            ppg[T] = (el, last, relmse(sums / last), relmse_trim(sums / last))
+            print(f"bench: ppg budget {T:.0f}s: {el:.1f}s, last iteration {last} spp, relMSE={ppg[T][2]:.4f}",
+                  file=sys.stderr, flush=True)
+
     def at(curve, T):
         best = curve[0]
         for c in curve:
@@ -153,6 +176,9 @@ def bench_guided(args):
                      "time_ratio": (reach / p_[0]) if reach else None,
                      "plain_relmse_trimmed": p_[3], "guided_relmse_trimmed": g_[3],
                      "time_ratio_trimmed": (reach_t / p_[0]) if reach_t else None})
+        if T in ppg:
+            rows[-1].update({"ppg_seconds": ppg[T][0], "ppg_last_spp": ppg[T][1], "ppg_relmse": ppg[T][2],
+                             "ppg_relmse_trimmed": ppg[T][3]})
         print(f"bench: T={T:.0f}s plain {p_[2]:.4f} ({p_[1]} spp) guided {g_[2]:.4f} ({g_[1]} spp) "
               f"guided reaches plain's error at {reach}", file=sys.stderr, flush=True)
     out = {
