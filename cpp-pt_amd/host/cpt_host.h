@@ -153,4 +153,11 @@ public:
     explicit PurePtGPU(const RenderSettings& rs) : PathtracerMISGPU(rs, CPT_RENDER_PURE_PT) {}
 };
 
+// The reference's PPG (src/renderer/ppg.h:17-214): SD-tree guided, iterations of
+// 1, 2, 4, ... spp, the image of the last one.
+class PpgGPU : public PathtracerMISGPU {
+public:
+    explicit PpgGPU(const RenderSettings& rs) : PathtracerMISGPU(rs, CPT_RENDER_PPG) {}
+};
+
 }  // namespace cpppt

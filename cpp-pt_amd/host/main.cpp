@@ -2,9 +2,9 @@
 // (src/main.cpp:89-146): argv[1] = settings file, renderer chosen by the
 // settings token, wall time of render() printed as "<seconds>s", runtime
 // errors printed.  Renderer tokens `pathtracer_mis` / `pathtracer_mis_gpu`,
-// `pathtracer` / `pathtracer_gpu` and `pure_pt` / `pure_pt_gpu` select the GPU
-// integrators; the reference's other algorithms are not part of this build and
-// are reported as such.
+// `pathtracer` / `pathtracer_gpu`, `pure_pt` / `pure_pt_gpu` and `ppg` select the
+// GPU integrators; the reference's other algorithms are not part of this build
+// and are reported as such.
 #include <chrono>
 #include <iostream>
 #include <memory>
@@ -31,8 +31,10 @@ int main(int argc, char** argv) {
             renderer = std::make_unique<PathtracerGPU>(rs);
         } else if (rs.renderer == RenderSettings::PURE_PT || rs.renderer == RenderSettings::PURE_PT_GPU) {
             renderer = std::make_unique<PurePtGPU>(rs);
+        } else if (rs.renderer == RenderSettings::PPG) {
+            renderer = std::make_unique<PpgGPU>(rs);
         } else {
-            throw std::runtime_error("renderer not provided by this build (pathtracer_mis, pathtracer, pure_pt)");
+            throw std::runtime_error("renderer not provided by this build (pathtracer_mis, pathtracer, pure_pt, ppg)");
         }
         auto t1 = std::chrono::high_resolution_clock::now();
         renderer->render(s, rs.output_name);
