@@ -58,7 +58,8 @@ class SceneDesc(C.Structure):
         ("bitangents", C.c_void_p), ("n_triangles", C.c_int32), ("indices", C.c_void_p),
         ("tri_material", C.c_void_p), ("n_materials", C.c_int32), ("materials", C.c_void_p),
         ("n_lights", C.c_int32), ("light_triangles", C.c_void_p), ("camera", C.c_float * 16),
-        ("env", C.c_float * 3), ("flags", C.c_int32),
+        ("env", C.c_float * 3), ("flags", C.c_int32), ("uvs", C.c_void_p), ("n_textures", C.c_int32),
+        ("textures", C.c_void_p), ("texels", C.c_void_p), ("material_textures", C.c_void_p),
     ]
 
 
@@ -218,6 +219,11 @@ class Context:
         for k in range(3):
             d.env[k] = float(arrs["env"][k])
         d.flags = 0 if tie_rank else CPT_SCENE_NO_TIE_RANK
+        d.uvs = _ptr(arrs["uvs"])
+        d.n_textures = arrs["textures"].shape[0]
+        d.textures = _ptr(arrs["textures"]) if d.n_textures else None
+        d.texels = _ptr(arrs["texels"]) if d.n_textures else None
+        d.material_textures = _ptr(arrs["material_textures"])
         self._check(self.lib.cpt_upload_scene(self.h, C.byref(d)), "cpt_upload_scene")
         self.n_triangles = d.n_triangles
 

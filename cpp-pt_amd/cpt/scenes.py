@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-MAT_DIFFUSE, MAT_GLOSSY, MAT_EMISSIVE, MAT_DIELECTRIC = 0, 1, 2, 3
+MAT_DIFFUSE, MAT_GLOSSY, MAT_EMISSIVE, MAT_DIELECTRIC, MAT_STANDARD = 0, 1, 2, 3, 4
 
 f32 = np.float32
 
@@ -77,6 +77,54 @@ def camera_frame(origin, at, up, tan_fovy, res_x, res_y):
     return cam
 
 
+_LIBM = None
+
+
+def _powf(x, y):
+    """The C library's powf (what the C++ loader calls), for bit-identical sRGB decoding."""
+    global _LIBM
+    if _LIBM is None:
+        import ctypes
+        _LIBM = ctypes.CDLL("libm.so.6")
+        _LIBM.powf.restype = ctypes.c_float
+        _LIBM.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+    return f32(_LIBM.powf(float(x), float(y)))
+
+
+def srgb2linear(a):
+    """rgb_image.h:123-129 on float32."""
+    a = float(f32(a))
+    if a > 0.04045:
+        return _powf(f32((a + 0.055) / 1.055), f32(2.4))
+    return f32(a / 12.92)
+
+
+def image_texels(pixels, bw, srgb):
+    """Image<T>(file, colour space) (rgb_image.h:161-197) from 8-bit samples (h, w, n):
+    linear float32 texels, 3 channels (Vec3) or 1 (float)."""
+    px = np.asarray(pixels, np.uint8)
+    if px.ndim == 2:
+        px = px[:, :, None]
+    h, w, n = px.shape
+    c = (px.astype(np.float64) / 255.0).astype(np.float32)  # char2float (rgb_image.h:19-21)
+    if bw:
+        out = c[:, :, :1]
+    elif n >= 3:
+        out = c[:, :, :3]
+    else:
+        out = np.repeat(c[:, :, :1], 3, axis=2)
+    out = np.ascontiguousarray(out, np.float32)
+    if srgb:
+        flat = out.reshape(-1)
+        lut = {}
+        for i, v in enumerate(flat):
+            k = float(v)
+            if k not in lut:
+                lut[k] = srgb2linear(k)
+            flat[i] = lut[k]
+    return out
+
+
 @dataclass
 class Mesh:
     positions: np.ndarray   # (nv,3) float32
@@ -84,6 +132,7 @@ class Mesh:
     tangents: np.ndarray    # (nv,3); bitangent = cross(normal, tangent) (loader.h:237)
     faces: np.ndarray       # (nf,3) int32
     material: int           # index into Scene.material_list
+    uvs: np.ndarray = None  # (nv,2) texture coordinates (zeros when None)
 
 
 @dataclass
@@ -92,18 +141,35 @@ class Scene:
     material_list: list = field(default_factory=list)   # 16-float records
     camera_def: tuple = ((0, 0, 3.9), (0, 0, 0), (0, 1, 0), 0.35411, 512, 512)
     env: tuple = (0.0, 0.0, 0.0)
+    images: list = field(default_factory=list)        # (pixels uint8 (h,w,n), bw, srgb)
+    material_tex: list = field(default_factory=list)  # per material: albedo, roughness, metal, normal image
     _flat: dict = None
 
-    def add_material(self, rec):
-        self.material_list.append(np.asarray(rec, np.float32))
+    def add_material(self, rec, tex=(-1, -1, -1, -1)):
+        """tex: image ids (add_image) for albedo, roughness, metal, normal map (-1: the
+        record's constant).  An image metal makes the material MAT_STANDARD."""
+        rec = np.asarray(rec, np.float32).copy()
+        if tex[2] >= 0:
+            rec[0] = MAT_STANDARD
+        self.material_list.append(rec)
+        self.material_tex.append(np.asarray(tex, np.int32))
         self._flat = None
         return len(self.material_list) - 1
 
-    def add_mesh(self, positions, normals, tangents, faces, mat):
+    def add_image(self, pixels, bw, srgb):
+        """An image texture from 8-bit samples (h, w) or (h, w, n); `bw` and `srgb` as the
+        material slot reads it (albedo: Vec3, sRGB; roughness, metal: float, linear;
+        normal map: Vec3, linear)."""
+        self.images.append((np.ascontiguousarray(pixels, np.uint8), bool(bw), bool(srgb)))
+        self._flat = None
+        return len(self.images) - 1
+
+    def add_mesh(self, positions, normals, tangents, faces, mat, uvs=None):
         self.meshes.append(Mesh(np.ascontiguousarray(positions, np.float32),
                                 np.ascontiguousarray(normals, np.float32),
                                 np.ascontiguousarray(tangents, np.float32),
-                                np.ascontiguousarray(faces, np.int32), mat))
+                                np.ascontiguousarray(faces, np.int32), mat,
+                                None if uvs is None else np.ascontiguousarray(uvs, np.float32)))
         self._flat = None
 
     def set_camera(self, origin, at, up, tan_fovy, res_x, res_y):
@@ -118,12 +184,13 @@ class Scene:
         """Flattened arrays in load order (loader.h:353-367)."""
         if self._flat is not None:
             return self._flat
-        pos, nrm, bit, idx, tmat, lights = [], [], [], [], [], []
+        pos, nrm, bit, idx, tmat, lights, uvs = [], [], [], [], [], [], []
         voff = 0
         toff = 0
         for m in self.meshes:
             pos.append(m.positions)
             nrm.append(m.normals)
+            uvs.append(m.uvs if m.uvs is not None else np.zeros((len(m.positions), 2), np.float32))
             n, t = m.normals, m.tangents
             b = np.stack([n[:, 1] * t[:, 2] - n[:, 2] * t[:, 1],
                           n[:, 2] * t[:, 0] - n[:, 0] * t[:, 2],
@@ -146,6 +213,20 @@ class Scene:
             light_tris=np.ascontiguousarray(np.concatenate(lights) if lights else np.zeros(0, np.int32), np.int32),
             camera=camera_frame(o, at, up, tf, rx, ry),
             env=np.asarray(self.env, np.float32),
+            uvs=np.ascontiguousarray(np.concatenate(uvs), np.float32),
+        )
+        texels, table = [], []
+        off = 0
+        for px, bw, srgb in self.images:
+            t = image_texels(px, bw, srgb)
+            table.append((t.shape[1], t.shape[0], t.shape[2], off))
+            texels.append(t.reshape(-1))
+            off += t.size
+        mt = list(self.material_tex) + [np.full(4, -1, np.int32)] * (len(self.material_list) - len(self.material_tex))
+        self._flat.update(
+            textures=np.ascontiguousarray(np.array(table, np.int32).reshape(-1, 4)),
+            texels=np.ascontiguousarray(np.concatenate(texels) if texels else np.zeros(1, np.float32), np.float32),
+            material_textures=np.ascontiguousarray(np.stack(mt), np.int32),
         )
         return self._flat
 
@@ -159,7 +240,13 @@ class Scene:
         """Write <dir>/<name>.settings, <name>.ptscene, mesh_*.ptobj, mat_*.ptmat."""
         os.makedirs(directory, exist_ok=True)
         o, at, up, tf, rx, ry = self.camera_def
+        for k, (px, bw, srgb) in enumerate(self.images):
+            write_png(os.path.join(directory, f"tex_{k}.png"), px)
         for k, rec in enumerate(self.material_list):
+            tex = self.material_tex[k] if k < len(self.material_tex) else (-1, -1, -1, -1)
+
+            def slot(i, const):
+                return f"image(tex_{tex[i]}.png)" if tex[i] >= 0 else const
             with open(os.path.join(directory, f"mat_{k}.ptmat"), "w") as f:
                 kind = int(rec[0])
                 if kind == MAT_EMISSIVE:
@@ -168,9 +255,9 @@ class Scene:
                     f.write(f"DIELECTRIC float({_n(rec[8])})\n")
                 else:
                     metal = "1.0" if kind == MAT_GLOSSY else "0.0"
-                    f.write(f"DISNEY rgb({_n(rec[1])},{_n(rec[2])},{_n(rec[3])}) float(0.5) float({_n(rec[4])}) "
-                            f"float({metal}) float(0.0) float(1.5) rgb({_n(rec[9])},{_n(rec[10])},{_n(rec[11])}) "
-                            f"float(1.0)\n")
+                    f.write(f"DISNEY {slot(0, f'rgb({_n(rec[1])},{_n(rec[2])},{_n(rec[3])})')} float(0.5) "
+                            f"{slot(1, f'float({_n(rec[4])})')} {slot(2, f'float({metal})')} float(0.0) float(1.5) "
+                            f"{slot(3, f'rgb({_n(rec[9])},{_n(rec[10])},{_n(rec[11])})')} float(1.0)\n")
         lines = [f"camera {_fmt(o)} {_fmt(at)} {_fmt(up)} {_n(tf)}"]
         if any(self.env):
             lines.append(f"environment {_fmt(self.env)}")
@@ -178,8 +265,9 @@ class Scene:
             fn = f"mesh_{i}.ptobj"
             with open(os.path.join(directory, fn), "w") as f:
                 f.write(f"{len(m.positions)} {len(m.faces)} TRIANGLE_MESH\n")
-                for p, n, t in zip(m.positions, m.normals, m.tangents):
-                    f.write(f"{_fmt(p)} 0 0 {_fmt(n)} {_fmt(t)}\n")
+                uv = m.uvs if m.uvs is not None else np.zeros((len(m.positions), 2), np.float32)
+                for p, q, n, t in zip(m.positions, uv, m.normals, m.tangents):
+                    f.write(f"{_fmt(p)} {_fmt(q)} {_fmt(n)} {_fmt(t)}\n")
                 for fc in m.faces:
                     f.write(f"{fc[0]} {fc[1]} {fc[2]}\n")
             lines.append(f"obj {fn} mat_{m.material}.ptmat")
@@ -199,6 +287,24 @@ class Scene:
 
 def _n(x):
     return repr(float(np.float32(x)))
+
+
+def write_png(path, pixels):
+    """8-bit PNG (gray, gray+alpha, RGB or RGBA by the channel count), filter 0."""
+    import struct
+    import zlib
+    px = np.asarray(pixels, np.uint8)
+    if px.ndim == 2:
+        px = px[:, :, None]
+    h, w, n = px.shape
+    ctype = {1: 0, 2: 4, 3: 2, 4: 6}[n]
+    raw = b"".join(b"\x00" + px[y].tobytes() for y in range(h))
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xffffffff)
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, ctype, 0, 0, 0)) +
+                chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
 
 
 def _fmt(v):
@@ -409,7 +515,7 @@ def _sphere_tangents(N):
     return T / np.linalg.norm(T, axis=1, keepdims=True)
 
 
-def terrain_scene(res=(1920, 1080), n_spheres=200, grid=(1024, 512), subdiv=4, seed=2, material_seed=3):
+def terrain_scene(res=(1920, 1080), n_spheres=200, grid=(1024, 512), subdiv=4, seed=2, material_seed=3, n_lights=8):
     """Procedural ~2.07M-triangle scene: 1024x512 fBm heightfield (6 octaves),
     `n_spheres` instanced icospheres (subdivision 4) with radius U[0.05,0.5]
     resting on the terrain, materials 60% Lambertian / 25% GGX (roughness
@@ -441,7 +547,7 @@ def terrain_scene(res=(1920, 1080), n_spheres=200, grid=(1024, 512), subdiv=4, s
             m = s.add_material(material(MAT_DIELECTRIC, ior=1.5))
         s.add_mesh(SP * r + np.array([cx, cy, cz]), SN, ST, SF, m)
     light = s.add_material(material(MAT_EMISSIVE, emission=(12.0, 11.0, 9.0)))
-    for k in range(8):
+    for k in range(n_lights):
         lx = -extent[0] * 0.4 + k * (extent[0] * 0.8 / 7)
         lz = -2.0 + 4.0 * (k % 2)
         y = 9.0
@@ -560,4 +666,40 @@ def interior_scene(res=(1280, 720), n_boxes=40000, seed=5, light_radiance=3000.0
         P = P[[0, 3, 2, 1]]
     s.add_mesh(P, N, T, F, light)
     s.set_camera(tuple(cam), (5.0, 1.0, 0.0), (0.0, 1.0, 0.0), math.tan(math.radians(30.0)), *res)
+    return s
+
+
+# --------------------------------------------------------------------- textured box room
+def textured_room(res=(64, 64), tex_res=16, seed=9):
+    """The box room with image textures: a checker albedo (sRGB) on the floor and
+    back wall, a noise roughness map and a striped metal map on the tall box
+    (diffuse where the stripes are dark, GGX where they are bright), a random
+    tangent-space normal map on the left wall.  Texture coordinates span [0, 1.5]
+    so that lookups wrap (REPEAT border)."""
+    rng = np.random.default_rng(seed)
+    s = box_room(res=res)
+    n = tex_res
+    yy, xx = np.mgrid[0:n, 0:n]
+    checker = np.where(((xx // 4) + (yy // 4)) % 2 == 0, 230, 40).astype(np.uint8)
+    albedo = np.stack([checker, (checker * 0.8).astype(np.uint8), (255 - checker // 2).astype(np.uint8)], axis=2)
+    rough = rng.integers(20, 200, size=(n, n), dtype=np.uint8)
+    metal = np.where((xx // 3) % 2 == 0, 0, 255).astype(np.uint8)
+    nmap = np.stack([rng.integers(90, 166, (n, n)), rng.integers(90, 166, (n, n)),
+                     np.full((n, n), 255)], axis=2).astype(np.uint8)
+    i_alb = s.add_image(albedo, bw=False, srgb=True)
+    i_rough = s.add_image(rough, bw=True, srgb=False)
+    i_metal = s.add_image(metal, bw=True, srgb=False)
+    i_nmap = s.add_image(nmap, bw=False, srgb=False)
+    m_floor = s.add_material(material(MAT_DIFFUSE, albedo=(0.7, 0.7, 0.7)), tex=(i_alb, -1, -1, -1))
+    m_box = s.add_material(material(MAT_DIFFUSE, albedo=(0.8, 0.75, 0.7), rough=0.3), tex=(-1, i_rough, i_metal, -1))
+    m_wall = s.add_material(material(MAT_DIFFUSE, albedo=(0.6, 0.2, 0.2)), tex=(-1, -1, -1, i_nmap))
+
+    def uv_of(P, axes):
+        return np.stack([(P[:, axes[0]] + 1.0) * 0.75, (P[:, axes[1]] + 1.0) * 0.75], axis=1).astype(np.float32)
+    # re-material the floor (mesh 0), back wall (2), left wall (3) and the tall box (6)
+    for mi, mat, axes in ((0, m_floor, (0, 2)), (2, m_floor, (0, 1)), (3, m_wall, (2, 1)), (6, m_box, (0, 1))):
+        m = s.meshes[mi]
+        m.material = mat
+        m.uvs = uv_of(m.positions, axes)
+    s._flat = None
     return s

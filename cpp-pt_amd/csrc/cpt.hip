@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
                     int matid;
                     fill_isect(a.sc, o, d, h, it, matid);
                     int slot = __float_as_int(D.w);
-                    if ((a.meta[slot] & 0xffu) == 0) first_hit_frame(load_mat(a.sc.mats + 16 * matid), it);
+                    if ((a.meta[slot] & 0xffu) == 0) first_hit_frame(hit_mat(a.sc, matid, h), it);
                     guide::features(a.sc.bmin, a.sc.bext, it.hitpoint.x, it.hitpoint.y, it.hitpoint.z, it.normal.x,
                                     it.normal.y, it.normal.z, -d.x, -d.y, -d.z, in);
                 }
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                 Isect it;
                 int matid;
                 fill_isect(a.sc, o, d, h, it, matid);
-                Mat m = load_mat(a.sc.mats + 16 * matid);
+                Mat m = hit_mat(a.sc, matid, h);
                 if (depth == 0) first_hit_frame(m, it);
                 V3 wo = mulf(d, -1.0f);
                 if (m.type == CPT_MAT_EMISSIVE) {
@@ -652,7 +652,7 @@ __global__ __launch_bounds__(B) void k_shade_ppg(WfArgs a, int cur, int depth_ca
                 Isect it;
                 int matid;
                 fill_isect(a.sc, o, d, h, it, matid);
-                Mat m = load_mat(a.sc.mats + 16 * matid);
+                Mat m = hit_mat(a.sc, matid, h);
                 if (depth == 0) first_hit_frame(m, it);
                 V3 wo = mulf(d, -1.0f);
                 if (m.type == CPT_MAT_EMISSIVE) {
@@ -994,6 +994,10 @@ struct cpt_ctx {
     float4* d_shade = nullptr;
     float4 *d_vpos = nullptr, *d_vnrm = nullptr, *d_vbit = nullptr;
     float* d_mats = nullptr;
+    int4* d_mtex = nullptr;     // per material: texture ids
+    float2* d_uv = nullptr;     // per vertex texture coordinates
+    int4* d_tex = nullptr;      // texture table
+    float* d_texels = nullptr;
     int* d_lights = nullptr;
     float cam[16];
     float env[3];
@@ -1101,6 +1105,10 @@ void free_scene(cpt_ctx* c) {
     dfree(c->d_vnrm);
     dfree(c->d_vbit);
     dfree(c->d_mats);
+    dfree(c->d_mtex);
+    dfree(c->d_uv);
+    dfree(c->d_tex);
+    dfree(c->d_texels);
     dfree(c->d_lights);
     c->has_scene = false;
 }
@@ -1140,6 +1148,10 @@ DevScene dev_scene(const cpt_ctx* c) {
     s.vnrm = c->d_vnrm;
     s.vbit = c->d_vbit;
     s.mats = c->d_mats;
+    s.mtex = c->d_mtex;
+    s.uv = c->d_uv;
+    s.tex = c->d_tex;
+    s.texels = c->d_texels;
     s.lights = c->d_lights;
     s.n_lights = c->nl;
     s.n_tris = c->nt;
@@ -1473,13 +1485,31 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
         int m = d->tri_material[t];
         if (m < 0 || m >= d->n_materials) return fail(c, CPT_E_INVALID, "triangle %d: material %d out of range", t, m);
     }
+    // image textures (ImageTexture, image_texture.h:24-40)
+    if (d->n_textures < 0) return fail(c, CPT_E_INVALID, "negative texture count");
+    if (d->n_textures > 0 && (!d->textures || !d->texels || !d->uvs))
+        return fail(c, CPT_E_INVALID, "textures need the texture table, texels and uvs");
+    size_t n_texels = 0;
+    for (int t = 0; t < d->n_textures; t++) {
+        const int32_t* e = d->textures + 4 * (size_t)t;
+        if (e[0] <= 0 || e[1] <= 0 || (e[2] != 1 && e[2] != 3) || e[3] < 0)
+            return fail(c, CPT_E_INVALID, "texture %d: bad size, channels or offset", t);
+        n_texels = std::max(n_texels, (size_t)e[3] + (size_t)e[0] * e[1] * e[2]);
+    }
+    for (int m = 0; m < d->n_materials; m++) {
+        const float type = d->materials[16 * (size_t)m];
+        if (!(type >= 0.0f && type <= (float)CPT_MAT_STANDARD) || type != (float)(int)type)
+            return fail(c, CPT_E_INVALID, "material %d: unknown type", m);
+        for (int k = 0; k < 4; k++) {
+            const int tx = d->material_textures ? d->material_textures[4 * (size_t)m + k] : -1;
+            if (tx < -1 || tx >= d->n_textures) return fail(c, CPT_E_INVALID, "material %d: texture %d out of range", m, tx);
+        }
+        if ((int)type == CPT_MAT_STANDARD && (!d->material_textures || d->material_textures[4 * (size_t)m + 2] < 0))
+            return fail(c, CPT_E_INVALID, "material %d: CPT_MAT_STANDARD needs a metal texture", m);
+    }
     for (int i = 0; i < d->n_lights; i++) {
         int t = d->light_triangles[i];
         if (t < 0 || t >= d->n_triangles) return fail(c, CPT_E_INVALID, "light %d: triangle %d out of range", i, t);
-    }
-    for (int m = 0; m < d->n_materials; m++) {
-        int ty = (int)d->materials[16 * (size_t)m];
-        if (ty < 0 || ty > 3) return fail(c, CPT_E_INVALID, "material %d: unknown type %d", m, ty);
     }
     HIPCHK(c, hipSetDevice(c->device));
     drop_graph(c);
@@ -1572,6 +1602,10 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     r |= dalloc(c, c->d_vnrm, vn.size());
     r |= dalloc(c, c->d_vbit, vb.size());
     r |= dalloc(c, c->d_mats, 16 * (size_t)c->nm);
+    r |= dalloc(c, c->d_mtex, (size_t)c->nm);
+    r |= dalloc(c, c->d_uv, std::max<size_t>(1, d->uvs ? (size_t)d->n_vertices : 1));
+    r |= dalloc(c, c->d_tex, std::max<size_t>(1, (size_t)d->n_textures));
+    r |= dalloc(c, c->d_texels, std::max<size_t>(1, n_texels));
     r |= dalloc(c, c->d_lights, std::max(1, c->nl));
     if (r) return CPT_E_HIP;
     HIPCHK(c, hipMemcpy(c->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4), hipMemcpyHostToDevice));
@@ -1583,6 +1617,21 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     HIPCHK(c, hipMemcpy(c->d_vnrm, vn.data(), vn.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_vbit, vb.data(), vb.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_mats, d->materials, 16 * sizeof(float) * (size_t)c->nm, hipMemcpyHostToDevice));
+    {
+        std::vector<int4> mt((size_t)c->nm, make_int4(-1, -1, -1, -1));
+        if (d->material_textures)
+            for (int m = 0; m < c->nm; m++) {
+                const int32_t* e = d->material_textures + 4 * (size_t)m;
+                mt[m] = make_int4(e[0], e[1], e[2], e[3]);
+            }
+        HIPCHK(c, hipMemcpy(c->d_mtex, mt.data(), mt.size() * sizeof(int4), hipMemcpyHostToDevice));
+        if (d->uvs && d->n_vertices > 0)
+            HIPCHK(c, hipMemcpy(c->d_uv, d->uvs, 2 * sizeof(float) * (size_t)d->n_vertices, hipMemcpyHostToDevice));
+        if (d->n_textures > 0) {
+            HIPCHK(c, hipMemcpy(c->d_tex, d->textures, 4 * sizeof(int32_t) * (size_t)d->n_textures, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(c->d_texels, d->texels, sizeof(float) * n_texels, hipMemcpyHostToDevice));
+        }
+    }
     if (c->nl > 0)
         HIPCHK(c, hipMemcpy(c->d_lights, d->light_triangles, sizeof(int) * (size_t)c->nl, hipMemcpyHostToDevice));
     c->has_scene = true;

@@ -92,6 +92,10 @@ struct DevScene {
     const float4* vnrm;
     const float4* vbit;
     const float* mats;       // 16 per material
+    const int4* mtex;        // per material: albedo, roughness, metal, normal-map texture (-1: constant)
+    const float2* uv;        // per vertex texture coordinates
+    const int4* tex;         // per texture: width, height, channels, first texel
+    const float* texels;     // linear, row 0 at the top
     const int* lights;       // emitter triangle ids
     int n_lights;
     int n_tris;

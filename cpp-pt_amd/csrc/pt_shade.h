@@ -43,6 +43,56 @@ __device__ __forceinline__ Mat load_mat(const float* m) {
     return r;
 }
 
+// Image::handle_border, REPEAT (rgb_image.h:215-280): a negative multiple of
+// the size maps to the size itself, which the reference then reads as the
+// first texel of the next row; the index is clamped only past the last texel
+// (where the reference reads outside the image)
+__device__ __forceinline__ int tex_wrap(int i, int n) {
+    if (i >= n) return i % n;
+    if (i < 0) return i % n + n;
+    return i;
+}
+
+// ImageTexture::sample with point filtering (image_texture.h:24-30)
+__device__ __forceinline__ const float* tex_texel(const DevScene& sc, int t, float u, float v) {
+    const int4 d = sc.tex[t];  // width, height, channels, first texel
+    const float x = u * (float)d.x;
+    const float y = (float)((1.0 - (double)v) * (double)d.y);
+    const int i = tex_wrap((int)floorf(x), d.x), j = tex_wrap((int)floorf(y), d.y);
+    const long long n = (long long)d.x * d.y;
+    long long idx = (long long)i + (long long)j * d.x;
+    if (idx >= n) idx = n - 1;
+    return sc.texels + d.w + idx * d.z;
+}
+
+// triangle.h:90-95: uv = uv1*alpha + uv2*beta + uv3*gamma
+__device__ __forceinline__ float2 hit_uv(const DevScene& sc, const Hit& h) {
+    const int4 ti = sc.tri_info[h.tri];
+    const float alpha = 1.0f - (h.gamma + h.beta);
+    const float2 a = sc.uv[ti.x], b = sc.uv[ti.y], c = sc.uv[ti.z];
+    return make_float2(a.x * alpha + b.x * h.beta + c.x * h.gamma, a.y * alpha + b.y * h.beta + c.y * h.gamma);
+}
+
+// StandardMaterial::get_bxdf / get_normal_mapped inputs at the hit
+// (standard_material.h:46-71): image textures replace the record's constants
+__device__ __forceinline__ Mat hit_mat(const DevScene& sc, int matid, const Hit& h) {
+    Mat m = load_mat(sc.mats + 16 * matid);
+    const int4 mt = sc.mtex[matid];
+    if (mt.x < 0 && mt.y < 0 && mt.z < 0 && mt.w < 0) return m;  // constant textures only
+    const float2 uv = hit_uv(sc, h);
+    if (mt.x >= 0) {
+        const float* t = tex_texel(sc, mt.x, uv.x, uv.y);
+        m.albedo = sc.tex[mt.x].z == 3 ? v3(t[0], t[1], t[2]) : v3s(t[0]);
+    }
+    if (mt.y >= 0) m.rough = tex_texel(sc, mt.y, uv.x, uv.y)[0];
+    if (mt.z >= 0) m.type = tex_texel(sc, mt.z, uv.x, uv.y)[0] < 0.1f ? CPT_MAT_DIFFUSE : CPT_MAT_GLOSSY;
+    if (mt.w >= 0) {
+        const float* t = tex_texel(sc, mt.w, uv.x, uv.y);
+        m.nmap = sc.tex[mt.w].z == 3 ? v3(t[0], t[1], t[2]) : v3s(t[0]);
+    }
+    return m;
+}
+
 // triangle.h:79-113.  The per-triangle shading record holds the geometric normal
 // normalized(cross(b-a, c-a)) (computed on the host with the same float
 // operations) and the three vertex normals and bitangents, so one hit costs one

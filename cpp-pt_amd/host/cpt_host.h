@@ -16,6 +16,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "cpt.h"
@@ -81,8 +82,19 @@ struct Camera {
     RgbImage& get_image() { return image; }
 };
 
+// Image<T> (rgb_image.h:161-197): linear floats, row 0 at the top, 3 channels
+// (Vec3 textures) or 1 (float textures)
+struct TexImage {
+    int w = 0, h = 0, ch = 0;
+    std::vector<float> texels;
+};
+
+// load_png: 8-bit samples with the file's channel count (stbi_load(..., 0) semantics)
+void load_png(const std::string& path, int* w, int* h, int* n, std::vector<uint8_t>* out);
+
 struct Material {
     int type = CPT_MAT_DIFFUSE;
+    int tex[4] = {-1, -1, -1, -1};  // albedo, roughness, metal, normal-map images (SceneData::images)
     float albedo[3] = {0, 0, 0};
     float roughness = 0;
     float emission[3] = {0, 0, 0};
@@ -94,12 +106,15 @@ struct Material {
 
 struct Mesh {
     std::vector<float> p, n, b;          // per vertex xyz (b = bitangent)
+    std::vector<float> uv;               // per vertex texture coordinates
     std::vector<std::array<int, 3>> f;   // faces
 };
 
 struct SceneData {
     std::vector<std::shared_ptr<Mesh>> meshes;
     std::vector<std::shared_ptr<Material>> materials;  // one entry per mesh, load order
+    std::vector<TexImage> images;                      // image textures
+    std::unordered_map<std::string, int> image_index;  // (file, colour space, bw) -> images
 };
 
 struct Scene {
@@ -109,6 +124,9 @@ struct Scene {
     std::vector<float> positions, normals, bitangents;
     std::vector<int32_t> indices, tri_material, light_triangles;
     std::vector<float> materials;  // 16 floats per material
+    std::vector<float> uvs, texels;
+    std::vector<int32_t> textures, material_textures;
+    int n_textures = 0;
     int n_vertices = 0, n_triangles = 0, n_materials = 0;
     void flatten(const SceneData& sd);
     cpt_scene_desc desc() const;

@@ -66,19 +66,62 @@ void parse_rgb(const std::string& s, float* r, float* g, float* b) {
     *b = (float)atof(s2.substr(c2 + 1, s2.length() - c2 - 2).c_str());
 }
 
-// loader.h:45-97: constant textures only; bw textures keep the first channel
-Vec3f get_texture(const std::string& tex, bool bw, bool srgb) {
+struct Tex {
+    Vec3f value;     // constant textures
+    int image = -1;  // image textures: index into SceneData::images
+};
+
+// Image<T>(filename, colour space) (rgb_image.h:161-197): 8-bit samples / 255;
+// Vec3 images take the first three channels (or the first one three times
+// when there are fewer), float images the first; sRGB images are decoded
+int load_image(SceneData* sd, const std::string& path, bool bw, bool srgb) {
+    const std::string key = path + (bw ? "|bw" : "|rgb") + (srgb ? "|srgb" : "|linear");
+    auto it = sd->image_index.find(key);
+    if (it != sd->image_index.end()) return it->second;
+    int w, h, n;
+    std::vector<uint8_t> px;
+    load_png(path, &w, &h, &n, &px);
+    TexImage im;
+    im.w = w;
+    im.h = h;
+    im.ch = bw ? 1 : 3;
+    im.texels.resize((size_t)w * h * im.ch);
+    auto c2f = [](uint8_t a) { return (float)((double)(float)a / 255.0); };  // char2float, rgb_image.h:19-21
+    for (size_t i = 0; i < (size_t)w * h; i++) {
+        const uint8_t* s = &px[i * n];
+        if (bw) {
+            im.texels[i] = c2f(s[0]);
+        } else {
+            const bool rgb = n >= 3;
+            im.texels[3 * i] = c2f(s[0]);
+            im.texels[3 * i + 1] = c2f(rgb ? s[1] : s[0]);
+            im.texels[3 * i + 2] = c2f(rgb ? s[2] : s[0]);
+        }
+    }
+    if (srgb)
+        for (float& t : im.texels) t = srgb2linear(t);
+    sd->images.push_back(std::move(im));
+    sd->image_index[key] = (int)sd->images.size() - 1;
+    return (int)sd->images.size() - 1;
+}
+
+// loader.h:45-97; bw textures keep the first channel
+Tex get_texture(SceneData* sd, const std::string& dir, const std::string& tex, bool bw, bool srgb) {
+    Tex t;
     if (tex.substr(0, 3) == "rgb") {
         float r, g, b;
         parse_rgb(tex, &r, &g, &b);
-        if (!bw) return v(r, g, b);  // rgb(...) is never sRGB-decoded (loader.h:51-55)
-        return v(r, r, r);
+        t.value = bw ? v(r, r, r) : v(r, g, b);  // rgb(...) is never sRGB-decoded (loader.h:51-55)
+        return t;
     } else if (tex.substr(0, 5) == "float") {
         float r = parse_float_tex(tex);
         if (srgb) r = srgb2linear(r);
-        return v(r, r, r);
+        t.value = v(r, r, r);
+        return t;
     } else if (tex.substr(0, 5) == "image") {
-        throw std::runtime_error("Image textures are not supported by the GPU path: " + tex);
+        t.image = load_image(sd, dir + tex.substr(6, tex.length() - 7), bw, srgb);
+        t.value = v(0, 0, 0);
+        return t;
     }
     throw std::runtime_error("Unknown type of tex: " + tex);
 }
@@ -127,29 +170,38 @@ void parse_mat(LoaderData* ld, const std::string& dir, const std::string& filena
         std::string albedo, spec, rough, metal, transparent, ior, normal, alpha;
         iss >> albedo >> spec >> rough >> metal >> transparent >> ior >> normal >> alpha;
         if (!iss) throw std::runtime_error("Malformed DISNEY material: " + filename);
-        Vec3f a = get_texture(albedo, false, true);
-        get_texture(spec, true, true);
-        Vec3f r = get_texture(rough, true, false);
-        Vec3f me = get_texture(metal, true, false);
-        get_texture(transparent, true, true);
-        get_texture(ior, true, true);
-        Vec3f nm = get_texture(normal, false, false);
-        get_texture(alpha, true, false);
-        // standard_material.h:46-61: metal < 0.1 -> Lambertian(albedo), else GGX(roughness)
-        m->type = me.x < 0.1f ? CPT_MAT_DIFFUSE : CPT_MAT_GLOSSY;
-        m->albedo[0] = a.x;
-        m->albedo[1] = a.y;
-        m->albedo[2] = a.z;
-        m->roughness = r.x;
-        m->normal_map[0] = nm.x;
-        m->normal_map[1] = nm.y;
-        m->normal_map[2] = nm.z;
+        SceneData* sd = ld->sd;
+        Tex ta = get_texture(sd, dir, albedo, false, true);
+        get_texture(sd, dir, spec, true, true);
+        Tex tr = get_texture(sd, dir, rough, true, false);
+        Tex tm = get_texture(sd, dir, metal, true, false);
+        get_texture(sd, dir, transparent, true, true);
+        get_texture(sd, dir, ior, true, true);
+        Tex tn = get_texture(sd, dir, normal, false, false);
+        get_texture(sd, dir, alpha, true, false);
+        // standard_material.h:46-61: metal < 0.1 -> Lambertian(albedo), else GGX(roughness);
+        // an image metal texture makes the choice per hit
+        if (tm.image >= 0) m->type = CPT_MAT_STANDARD;
+        else m->type = tm.value.x < 0.1f ? CPT_MAT_DIFFUSE : CPT_MAT_GLOSSY;
+        m->albedo[0] = ta.value.x;
+        m->albedo[1] = ta.value.y;
+        m->albedo[2] = ta.value.z;
+        m->roughness = tr.value.x;
+        m->normal_map[0] = tn.value.x;
+        m->normal_map[1] = tn.value.y;
+        m->normal_map[2] = tn.value.z;
         m->has_normal_map = true;
+        m->tex[0] = ta.image;
+        m->tex[1] = tr.image;
+        m->tex[2] = tm.image;
+        m->tex[3] = tn.image;
     } else if (word == "EMISSION") {
         std::string col, inten;
         iss >> col >> inten;
         if (!iss) throw std::runtime_error("Malformed EMISSION material: " + filename);
-        Vec3f c = get_texture(col, false, true);
+        Tex tc = get_texture(ld->sd, dir, col, false, true);
+        if (tc.image >= 0) throw std::runtime_error("Image emission textures are not supported: " + col);
+        Vec3f c = tc.value;
         float intensity = (float)atof(inten.substr(6, inten.length() - 7).c_str());
         m->type = CPT_MAT_EMISSIVE;
         m->emission[0] = c.x * intensity;  // albedo->sample(uv)*intensity (emissive_material.h:28-30)
@@ -162,7 +214,7 @@ void parse_mat(LoaderData* ld, const std::string& dir, const std::string& filena
         iss >> ior;
         if (!iss) throw std::runtime_error("Malformed DIELECTRIC material: " + filename);
         m->type = CPT_MAT_DIELECTRIC;
-        m->ior = get_texture(ior, true, false).x;
+        m->ior = get_texture(ld->sd, dir, ior, true, false).value.x;
         m->has_normal_map = false;
     } else {
         throw std::runtime_error("Wrong bsdf type: " + word);
@@ -187,6 +239,7 @@ void parse_obj(LoaderData* ld, const std::string& dir, const std::string& filena
     mesh->p.resize(3 * (size_t)n_v);
     mesh->n.resize(3 * (size_t)n_v);
     mesh->b.resize(3 * (size_t)n_v);
+    mesh->uv.resize(2 * (size_t)n_v);
     mesh->f.resize(n_f);
     for (int i = 0; i < n_v; i++) {
         if (!std::getline(ifs, line)) throw std::runtime_error("Insuficient number of lines when reading verts");
@@ -198,6 +251,8 @@ void parse_obj(LoaderData* ld, const std::string& dir, const std::string& filena
             mesh->p[3 * (size_t)i + k] = p[k];
             mesh->n[3 * (size_t)i + k] = n[k];
         }
+        mesh->uv[2 * (size_t)i] = uv[0];
+        mesh->uv[2 * (size_t)i + 1] = uv[1];
         mesh->b[3 * (size_t)i] = bt.x;
         mesh->b[3 * (size_t)i + 1] = bt.y;
         mesh->b[3 * (size_t)i + 2] = bt.z;
@@ -235,6 +290,10 @@ void Scene::flatten(const SceneData& sd) {
     tri_material.clear();
     light_triangles.clear();
     materials.clear();
+    uvs.clear();
+    texels.clear();
+    textures.clear();
+    material_textures.clear();
     if (sd.materials.size() < sd.meshes.size()) throw std::runtime_error("mesh without material");
     // distinct material objects -> table index
     std::vector<const Material*> table;
@@ -266,8 +325,20 @@ void Scene::flatten(const SceneData& sd) {
         rec[11] = m->normal_map[2];
         rec[12] = m->has_normal_map ? 1.0f : 0.0f;
         materials.insert(materials.end(), rec, rec + 16);
+        material_textures.insert(material_textures.end(), m->tex, m->tex + 4);
     }
-    if (table.empty()) materials.assign(16, 0.0f);
+    if (table.empty()) {
+        materials.assign(16, 0.0f);
+        material_textures.assign(4, -1);
+    }
+    for (const TexImage& im : sd.images) {
+        textures.push_back(im.w);
+        textures.push_back(im.h);
+        textures.push_back(im.ch);
+        textures.push_back((int32_t)texels.size());
+        texels.insert(texels.end(), im.texels.begin(), im.texels.end());
+    }
+    n_textures = (int)sd.images.size();
     n_materials = (int)std::max<size_t>(1, table.size());
     // loader.h:353-367: mesh order, then face order; emissive triangles become lights
     int voff = 0;
@@ -276,6 +347,8 @@ void Scene::flatten(const SceneData& sd) {
         positions.insert(positions.end(), m.p.begin(), m.p.end());
         normals.insert(normals.end(), m.n.begin(), m.n.end());
         bitangents.insert(bitangents.end(), m.b.begin(), m.b.end());
+        if (m.uv.size() == m.p.size() / 3 * 2) uvs.insert(uvs.end(), m.uv.begin(), m.uv.end());
+        else uvs.insert(uvs.end(), m.p.size() / 3 * 2, 0.0f);
         for (const auto& f : m.f) {
             int tid = (int)tri_material.size();
             indices.push_back(f[0] + voff);
@@ -310,6 +383,11 @@ cpt_scene_desc Scene::desc() const {
                      c.y.z, c.z.x, c.z.y, c.z.z, c.tan_fovy, c.aspect, (float)c.image.resX, (float)c.image.resY};
     std::memcpy(d.camera, cam, sizeof(cam));
     for (int k = 0; k < 3; k++) d.env[k] = env[k];
+    d.uvs = uvs.empty() ? nullptr : uvs.data();
+    d.n_textures = n_textures;
+    d.textures = textures.empty() ? nullptr : textures.data();
+    d.texels = texels.empty() ? nullptr : texels.data();
+    d.material_textures = material_textures.data();
     return d;
 }
 

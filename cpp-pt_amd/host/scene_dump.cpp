@@ -2,7 +2,9 @@
 // flattened scene (the arrays cpt_upload_scene receives) as raw little-endian
 // binary: header int32[5] = {nv, nt, nm, nl, spp}, then positions, normals,
 // bitangents (nv*3 f32), indices (nt*3 i32), tri_material (nt i32), materials
-// (nm*16 f32), light triangles (nl i32), camera (16 f32), env (3 f32).
+// (nm*16 f32), light triangles (nl i32), camera (16 f32), env (3 f32), uvs (nv*2 f32),
+// n_textures (i32), textures (n_textures*4 i32), n_texels (i32), texels (f32),
+// material textures (nm*4 i32).
 // No device is used.
 #include <cstdio>
 #include <iostream>
@@ -36,6 +38,13 @@ int main(int argc, char** argv) {
         fwrite(d.light_triangles, sizeof(int32_t), (size_t)d.n_lights, f);
         fwrite(d.camera, sizeof(float), 16, f);
         fwrite(d.env, sizeof(float), 3, f);
+        fwrite(s.uvs.data(), sizeof(float), s.uvs.size(), f);
+        int32_t nt = d.n_textures, nx = (int32_t)s.texels.size();
+        fwrite(&nt, sizeof(nt), 1, f);
+        fwrite(s.textures.data(), sizeof(int32_t), s.textures.size(), f);
+        fwrite(&nx, sizeof(nx), 1, f);
+        fwrite(s.texels.data(), sizeof(float), s.texels.size(), f);
+        fwrite(s.material_textures.data(), sizeof(int32_t), s.material_textures.size(), f);
         fclose(f);
     } catch (const std::runtime_error& e) {
         std::cout << e.what() << std::endl;

@@ -37,6 +37,8 @@ extern "C" {
 #define CPT_MAT_GLOSSY 1     /* StandardMaterial with metal >= 0.1 -> GlossyBxDF(roughness) */
 #define CPT_MAT_EMISSIVE 2   /* EmissiveMaterial -> EmissiveBxDF(colour*intensity) */
 #define CPT_MAT_DIELECTRIC 3 /* RefractionBxDF(ior) (refraction_bxdf.h), made reachable */
+#define CPT_MAT_STANDARD 4   /* StandardMaterial with an image metal texture: DiffuseBxDF or
+                                GlossyBxDF chosen per hit (standard_material.h:46-61) */
 
 #define CPT_MATERIAL_FLOATS 16
 #define CPT_CAMERA_FLOATS 16
@@ -58,6 +60,15 @@ typedef struct cpt_ctx cpt_ctx;
  *    (CameraPerspective, src/camera/camera_perspective.h:19-27)
  *  - env: constant radiance returned for rays that leave the scene
  *    (PathtracerMIS::render_sky returns 0, pathtracerMIS.h:20-22)
+ *  - image textures (all optional; n_textures = 0 without them), ImageTexture
+ *    (src/texture/image_texture.h:24-40) over Image (src/image/rgb_image.h:161-197):
+ *      uvs: n_vertices x 2 floats (the .ptobj texture coordinates, loader.h:236)
+ *      textures: n_textures x 4 int32: width, height, channels (1 or 3), offset of
+ *        the first texel in `texels`
+ *      texels: linear floats (sRGB already decoded where the reference decodes it),
+ *        row 0 at the top, channels interleaved
+ *      material_textures: n_materials x 4 int32: albedo, roughness, metal and
+ *        normal-map textures (-1: the material record's constant)
  */
 typedef struct {
     int32_t n_vertices;
@@ -74,6 +85,11 @@ typedef struct {
     float camera[CPT_CAMERA_FLOATS];
     float env[3];
     int32_t flags; /* CPT_SCENE_* */
+    const float* uvs;
+    int32_t n_textures;
+    const int32_t* textures;
+    const float* texels;
+    const int32_t* material_textures;
 } cpt_scene_desc;
 
 /* skip computing the reference BVH's test order (used only to break exact-t ties

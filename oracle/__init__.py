@@ -32,6 +32,12 @@ def lib():
     vp, i32, f32p = C.c_void_p, C.c_int32, C.c_void_p
     L.orc_scene_create.restype = vp
     L.orc_scene_create.argtypes = [i32, vp, vp, vp, i32, vp, vp, i32, vp, i32, vp, vp, vp]
+    L.orc_hit_material.restype = None
+    L.orc_hit_material.argtypes = [vp, vp, i32, vp]
+    L.orc_image_texel.restype = None
+    L.orc_image_texel.argtypes = [vp, i32, i32, i32, vp]
+    L.orc_scene_set_textures.restype = None
+    L.orc_scene_set_textures.argtypes = [vp, vp, i32, vp, i32, vp, vp]
     L.orc_scene_destroy.argtypes = [vp]
     L.orc_scene_destroy.restype = None
     L.orc_bvh_nodes.argtypes = [vp]
@@ -103,6 +109,9 @@ class OracleScene:
             a["indices"].shape[0], _p(a["indices"]), _p(a["tri_material"]),
             a["materials"].shape[0], _p(a["materials"]), a["light_tris"].shape[0], _p(a["light_tris"]),
             _p(cam), _p(np.ascontiguousarray(a["env"], np.float32)))
+        if "textures" in a and len(a["textures"]):
+            L.orc_scene_set_textures(self.h, _p(a["uvs"]), a["textures"].shape[0], _p(a["textures"]),
+                                     a["texels"].size, _p(a["texels"]), _p(a["material_textures"]))
         self.n_triangles = a["indices"].shape[0]
 
     def close(self):
@@ -131,6 +140,19 @@ class OracleScene:
         t = np.zeros(n, np.float32)
         lib().orc_trace_closest(self.h, _p(rays), n, 1 if brute else 0, _p(tri), _p(t), threads)
         return tri, t
+
+    def hit_material(self, rays):
+        """Closest-hit material after textures: (n, 10) = type (-1 miss), albedo,
+        roughness, normal-map texel, u, v."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((rays.shape[0], 10), np.float32)
+        lib().orc_hit_material(self.h, _p(rays), rays.shape[0], _p(out))
+        return out
+
+    def image_texel(self, img, i, j):
+        out = np.zeros(3, np.float32)
+        lib().orc_image_texel(self.h, img, i, j, _p(out))
+        return out[:self.arrays["textures"][img][2]]
 
     def trace_any(self, rays, brute=False, threads=0):
         rays = np.ascontiguousarray(rays, np.float32)

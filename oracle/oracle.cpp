@@ -157,6 +157,12 @@ struct Scene {
     // 0 PathtracerMIS (pathtracerMIS.h:29-130), 1 Pathtracer: NEE without MIS
     // (pathtracer.h:27-110), 2 PurePt: BSDF sampling only (pure_pt.h:27-67)
     int integrator = 0;
+    // image textures: per-vertex uv, images (w, h, channels, texel offset),
+    // per material the image of its albedo, roughness, metal and normal slot (-1: constant)
+    std::vector<std::array<float, 2>> UV;
+    std::vector<std::array<int, 4>> images;
+    std::vector<float> texels;
+    std::vector<std::array<int, 4>> mat_tex;
 };
 
 struct Ray {
@@ -422,6 +428,7 @@ bool brute_any(const Scene& s, Ray& r) {
 // ---------------------------------------------------------------- shading record
 struct Isect {
     V3 normal, g_normal, bitangent, tangent, hitpoint;
+    float u = 0.0f, v = 0.0f;  // texture coordinates
     int tri = -1;
     bool computed_normal = false;
 };
@@ -439,6 +446,52 @@ void fill_isect(const Scene& s, const Ray& r, const Hit& h, Isect& it) {
     it.tangent = normalized(cross(bt, it.normal));
     it.hitpoint = r.o + r.d * h.t;
     it.tri = h.tri;
+    if (!s.UV.empty()) {  // triangle.h:90-95
+        const auto &p = s.UV[f[0]], &q = s.UV[f[1]], &w = s.UV[f[2]];
+        it.u = p[0] * alpha + q[0] * beta + w[0] * gamma;
+        it.v = p[1] * alpha + q[1] * beta + w[1] * gamma;
+    }
+}
+
+// Image<T>::get_pixel with the REPEAT border (rgb_image.h:211-299): an index at or
+// past the end wraps with %, a negative one becomes i%n + n, which is n itself
+// for negative multiples of n (the reference then reads the first texel of the
+// next row; past the image's last texel this restatement takes the last one)
+const float* image_texel(const Scene& s, int img, int i, int j) {
+    const auto& d = s.images[img];
+    const int w = d[0], h = d[1], ch = d[2];
+    if (i >= w) i %= w;
+    else if (i < 0) i = i % w + w;
+    if (j >= h) j %= h;
+    else if (j < 0) j = j % h + h;
+    long long k = (long long)i + (long long)j * w;
+    k = std::min(k, (long long)w * h - 1);
+    return s.texels.data() + d[3] + k * ch;
+}
+
+// ImageTexture::sample, point filtering (image_texture.h:24-28): x in float,
+// y = (1.0 - v) * height in double stored as float, pixel (floor x, floor y);
+// a one-channel image broadcasts its value to the three components
+V3 image_sample(const Scene& s, int img, float u, float v) {
+    const float x = u * (float)s.images[img][0];
+    const float y = (float)((1.0 - (double)v) * (double)s.images[img][1]);
+    const float* t = image_texel(s, img, (int)std::floor(x), (int)std::floor(y));
+    return s.images[img][2] == 3 ? V3(t[0], t[1], t[2]) : V3(t[0]);
+}
+
+// StandardMaterial::get_bxdf / get_normal_mapped inputs (standard_material.h:45-71)
+// at the hit: albedo and normal map from their images, roughness from the first
+// component of its image, and a metal image picks Lambertian (< 0.1) or GGX
+Material hit_material(const Scene& s, const Isect& it) {
+    const int id = s.tri_mat[it.tri];
+    Material m = s.mats[id];
+    if (s.mat_tex.empty()) return m;
+    const auto& t = s.mat_tex[id];
+    if (t[0] >= 0) m.albedo = image_sample(s, t[0], it.u, it.v);
+    if (t[1] >= 0) m.rough = image_sample(s, t[1], it.u, it.v).x;
+    if (t[2] >= 0) m.type = image_sample(s, t[2], it.u, it.v).x < 0.1 ? 0 : 1;
+    if (t[3] >= 0) m.nmap = image_sample(s, t[3], it.u, it.v);
+    return m;
 }
 
 // intersection.h:25-33 + standard_material.h:64-71 (normal map applied on the
@@ -757,7 +810,7 @@ V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Co
             break;
         }
         fill_isect(s, ray, h, it);
-        const Material& m = s.mats[s.tri_mat[h.tri]];
+        const Material m = hit_material(s, it);
         first_hit_frame(m, it);
         V3 wo = ray.d * (float)(-1.0);
         if (m.type == 2) {
@@ -920,7 +973,7 @@ V3 ppg_integrate(const Scene& s, const SdTreeView& t, float cx, float cy, Sample
             break;
         }
         fill_isect(s, ray, h, it);
-        const Material& m = s.mats[s.tri_mat[h.tri]];
+        const Material m = hit_material(s, it);
         first_hit_frame(m, it);
         V3 wo = ray.d * (float)(-1.0);
         if (m.type == 2) {
@@ -1015,6 +1068,21 @@ void* orc_scene_create(int nv, const float* pos, const float* nrm, const float* 
     return s;
 }
 
+void orc_scene_set_textures(void* sp, const float* uvs, int n_images, const int32_t* images, int n_texels,
+                            const float* texels, const int32_t* material_textures) {
+    Scene* s = (Scene*)sp;
+    s->UV.resize(s->P.size());
+    for (size_t i = 0; i < s->UV.size(); i++) s->UV[i] = {uvs[2 * i], uvs[2 * i + 1]};
+    s->images.resize(n_images);
+    for (int i = 0; i < n_images; i++)
+        s->images[i] = {images[4 * i], images[4 * i + 1], images[4 * i + 2], images[4 * i + 3]};
+    s->texels.assign(texels, texels + n_texels);
+    s->mat_tex.resize(s->mats.size());
+    for (size_t i = 0; i < s->mat_tex.size(); i++)
+        s->mat_tex[i] = {material_textures[4 * i], material_textures[4 * i + 1], material_textures[4 * i + 2],
+                         material_textures[4 * i + 3]};
+}
+
 void orc_scene_destroy(void* s) { delete (Scene*)s; }
 
 int orc_bvh_nodes(void* s) { return (int)((Scene*)s)->nodes.size(); }
@@ -1039,6 +1107,32 @@ void orc_trace_closest(void* sp, const float* rays, int n, int mode, int32_t* tr
         tri[i] = h.tri;
         t[i] = h.tri >= 0 ? h.t : std::numeric_limits<float>::infinity();
     }
+}
+
+void orc_hit_material(void* sp, const float* rays, int n, float* out) {
+    const Scene* s = (const Scene*)sp;
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 7 * i;
+        Ray ray{ld(r), ld(r + 3), r[6]};
+        Hit h;
+        float* o = out + 10 * i;
+        std::fill(o, o + 10, 0.0f);
+        o[0] = -1.0f;
+        if (s->root >= 0) ref_closest(*s, s->root, ray, h, nullptr);
+        if (h.tri < 0) continue;
+        Isect it;
+        fill_isect(*s, ray, h, it);
+        const Material m = hit_material(*s, it);
+        const float v[10] = {(float)m.type, m.albedo.x, m.albedo.y, m.albedo.z, m.rough,
+                             m.nmap.x, m.nmap.y, m.nmap.z, it.u, it.v};
+        std::copy(v, v + 10, o);
+    }
+}
+
+void orc_image_texel(void* sp, int img, int i, int j, float* out) {
+    const Scene* s = (const Scene*)sp;
+    const float* t = image_texel(*s, img, i, j);
+    std::copy(t, t + s->images[img][2], out);
 }
 
 void orc_trace_any(void* sp, const float* rays, int n, int mode, uint8_t* occ, int nthreads) {

@@ -44,6 +44,17 @@ void* orc_scene_create(int nv, const float* pos, const float* nrm, const float* 
                        int nl, const int32_t* light_tris,
                        const float* cam /* 16: o[3], x[3], y[3], z[3], tan_fovy, aspect, resX, resY */,
                        const float* env /* 3 */);
+/* image textures (loader.h:45-97): per-vertex uv (nv x 2), images (n x 4: width,
+ * height, channels 1|3, first texel), linear texels, per material the image of
+ * its albedo, roughness, metal and normal-map slot (nm x 4, -1 = the record's
+ * constant).  A metal image makes the Lambertian/GGX choice per hit. */
+void orc_scene_set_textures(void* s, const float* uvs, int n_images, const int32_t* images, int n_texels,
+                            const float* texels, const int32_t* material_textures);
+/* per ray: the closest hit's material after its textures (10 floats: type or -1
+ * for a miss, albedo rgb, roughness, normal-map texel rgb, u, v) */
+void orc_hit_material(void* s, const float* rays, int n, float* out);
+/* Image<T>::get_pixel(i, j) with the REPEAT border (channels 1 or 3 floats) */
+void orc_image_texel(void* s, int img, int i, int j, float* out);
 void orc_scene_destroy(void* s);
 
 /* reference BVH (bvh.h:151-217) statistics and DFS test order of every triangle */

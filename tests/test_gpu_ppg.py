@@ -59,6 +59,16 @@ def test_ppg_iterations_and_updates_match_oracle(ctx):
     assert max(n_leaves) > 256 or min(n_leaves) < 256
 
 
+def test_ppg_step_on_textured_scene_matches_oracle(ctx):
+    """One PPG iteration on the image-textured room (per-hit albedo, roughness,
+    lobe choice and normal map) against the oracle, path by path."""
+    W, H = 48, 48
+    scene = cpt.scenes.textured_room(res=(W, H))
+    ctx.upload(scene)
+    ctx.ppg_reset()
+    _compare_step(ctx, scene, W, H, 4, 8, 0)
+
+
 def test_ppg_schedule_is_unbiased(ctx):
     """The whole PPG schedule (iterations of 1..512 spp, the last one kept) against
     NEE without MIS at 4096 spp, on 4x4-pixel blocks.  PPG as the reference builds

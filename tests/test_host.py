@@ -38,6 +38,12 @@ def _load_dump(path):
                bitangents=take(np.float32, 3 * nv).reshape(-1, 3), indices=take(np.int32, 3 * nt).reshape(-1, 3),
                tri_material=take(np.int32, nt), materials=take(np.float32, 16 * nm).reshape(-1, 16),
                light_tris=take(np.int32, nl), camera=take(np.float32, 16), env=take(np.float32, 3))
+    out["uvs"] = take(np.float32, 2 * nv).reshape(-1, 2)
+    n_tex = int(take(np.int32, 1)[0])
+    out["textures"] = take(np.int32, 4 * n_tex).reshape(-1, 4)
+    n_texels = int(take(np.int32, 1)[0])
+    out["texels"] = take(np.float32, n_texels)
+    out["material_textures"] = take(np.int32, 4 * nm).reshape(-1, 4)
     assert off == len(raw)
     return out, int(spp)
 
@@ -59,6 +65,156 @@ def test_loader_reads_generated_files_bit_exactly(tmp_path, make):
     assert np.array_equal(got["materials"][:, :13], want["materials"][:, :13])
 
 
+def test_loader_reads_image_textures(tmp_path):
+    """Image textures through the file loader (loader.h:45-97, rgb_image.h:161-197):
+    PNG decode, char2float, the sRGB decode of albedo images, the channel choice of
+    float (first channel) and Vec3 images, texture coordinates, and per-material
+    texture slots.  The loader numbers materials and images in order of first use,
+    so the comparison goes through each triangle's material."""
+    s = cpt.scenes.textured_room(res=(32, 32))
+    settings = s.write_cpppt(str(tmp_path), spp=2)
+    out = tmp_path / "dump.bin"
+    r = subprocess.run([DUMP, settings, str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got, _ = _load_dump(str(out))
+    want = s.c_arrays()
+    for k in ("positions", "normals", "bitangents", "indices", "light_tris", "camera", "env", "uvs"):
+        assert np.array_equal(got[k], want[k]), k
+    assert len(got["textures"]) == len(s.images) == 4
+
+    def image(a, m, slot):
+        t = a["material_textures"][m, slot]
+        if t < 0:
+            return None
+        w, h, ch, o = a["textures"][t]
+        return (w, h, ch), a["texels"][o:o + w * h * ch]
+    fields = {0: slice(1, 4), 1: slice(4, 5), 3: slice(9, 12)}
+    pairs = set(zip(got["tri_material"].tolist(), want["tri_material"].tolist()))
+    assert len(pairs) == len(set(want["tri_material"].tolist()))
+    n_tex = 0
+    for gm, wm in pairs:
+        assert got["materials"][gm, 0] == want["materials"][wm, 0]
+        for slot in range(4):
+            gi, wi = image(got, gm, slot), image(want, wm, slot)
+            assert (gi is None) == (wi is None), (wm, slot)
+            if gi is None:
+                if slot in fields:
+                    f = fields[slot]
+                    assert np.array_equal(got["materials"][gm, f], want["materials"][wm, f]), (wm, slot)
+                continue
+            n_tex += 1
+            assert gi[0] == wi[0] and np.array_equal(gi[1], wi[1]), (wm, slot)
+    assert n_tex == 4
+    # sRGB albedo: 230/255 decodes to the IEC 61966-2-1 value
+    alb = image(want, 4, 0)[1]
+    c = 230 / 255
+    assert abs(alb[0] - ((c + 0.055) / 1.055) ** 2.4) < 1e-6
+
+
+def test_png_reader_formats(tmp_path):
+    """Gray, gray+alpha, RGB, RGBA, palette (with tRNS), 16-bit and 1/2/4-bit
+    images, every filter type: the loader's texels equal the decoded samples."""
+    rng = np.random.default_rng(4)
+    cases = []
+    px = rng.integers(0, 256, (5, 7, 3), dtype=np.uint8)
+    cases.append(("rgb8", _png(px, 2, 8, filt=[0, 1, 2, 3, 4]), px))
+    px = rng.integers(0, 256, (4, 6, 4), dtype=np.uint8)
+    cases.append(("rgba8", _png(px, 6, 8, filt=[4, 3, 2, 1]), px[:, :, :3]))
+    g = rng.integers(0, 256, (6, 5), dtype=np.uint8)
+    cases.append(("gray8", _png(g, 0, 8, filt=[1, 4, 0, 3, 2, 4]), np.repeat(g[:, :, None], 3, axis=2)))
+    ga = rng.integers(0, 256, (3, 9, 2), dtype=np.uint8)
+    cases.append(("graya8", _png(ga, 4, 8, filt=[3]), np.repeat(ga[:, :, :1], 3, axis=2)))
+    w16 = rng.integers(0, 65536, (4, 4, 3), dtype=np.uint16)
+    cases.append(("rgb16", _png(w16, 2, 16, filt=[2, 4, 1, 3]), (w16 >> 8).astype(np.uint8)))
+    pal = rng.integers(0, 256, (5, 3), dtype=np.uint8)
+    idx = rng.integers(0, 5, (3, 11), dtype=np.uint8)
+    cases.append(("pal8", _png(idx, 3, 8, filt=[1], plte=pal), pal[idx]))
+    cases.append(("pal8trns", _png(idx, 3, 8, filt=[0], plte=pal, trns=bytes([7, 9])), pal[idx]))
+    for depth in (1, 2, 4):
+        v = rng.integers(0, 1 << depth, (3, 13), dtype=np.uint8)
+        scale = 255 // ((1 << depth) - 1)
+        cases.append((f"gray{depth}", _png(v, 0, depth, filt=[4, 0, 2]), np.repeat((v * scale)[:, :, None], 3, axis=2)))
+        idx4 = rng.integers(0, min(5, 1 << depth), (2, 9), dtype=np.uint8)
+        cases.append((f"pal{depth}", _png(idx4, 3, depth, filt=[0, 1], plte=pal), pal[idx4]))
+    for name, data, want in cases:
+        d = tmp_path / name
+        d.mkdir()
+        (d / "t.png").write_bytes(data)
+        (d / "m.ptmat").write_text("DISNEY image(t.png) float(0.5) float(0.5) float(0.0) float(0.0) float(1.5) "
+                                   "rgb(0.5,0.5,1.0) float(1.0)\n")
+        (d / "q.ptobj").write_text("3 1 TRIANGLE_MESH\n0 0 0 0 0 0 0 1 1 0 0\n1 0 0 1 0 0 0 1 1 0 0\n"
+                                   "0 1 0 0 1 0 0 1 1 0 0\n0 1 2\n")
+        (d / "s.ptscene").write_text("camera 0 0 1 0 0 0 0 1 0 0.5\nobj q.ptobj m.ptmat\n")
+        (d / "s.settings").write_text(f"resolution 4 4\nscene {d / 's.ptscene'}\n")
+        r = subprocess.run([DUMP, str(d / "s.settings"), str(d / "x.bin")], capture_output=True, text=True)
+        assert r.returncode == 0, name + r.stdout + r.stderr
+        got, _ = _load_dump(str(d / "x.bin"))
+        h, w = want.shape[:2]
+        assert got["textures"].tolist() == [[w, h, 3, 0]], name
+        assert np.array_equal(got["texels"], cpt.scenes.image_texels(want, False, True).reshape(-1)), name
+
+
+def _png(px, ctype, depth, filt, plte=None, trns=None):
+    """A PNG with the given colour type and bit depth; row y uses filter filt[y % len]."""
+    import struct
+    import zlib
+    a = np.asarray(px)
+    if a.ndim == 2:
+        a = a[:, :, None]
+    h, w, ch = a.shape
+    rows = []
+    for y in range(h):
+        if depth == 16:
+            row = a[y].astype(">u2").tobytes()
+            bpp = 2 * ch
+        elif depth == 8:
+            row = a[y].astype(np.uint8).tobytes()
+            bpp = ch
+        else:
+            per = 8 // depth
+            b = bytearray((w + per - 1) // per)
+            for x in range(w):
+                b[x // per] |= int(a[y, x, 0]) << ((per - 1 - x % per) * depth)
+            row = bytes(b)
+            bpp = 1
+        rows.append(row)
+    raw = bytearray()
+    prev = bytes(len(rows[0]))
+    for y, row in enumerate(rows):
+        ft = filt[y % len(filt)]
+        out = bytearray(len(row))
+        for i in range(len(row)):
+            left = row[i - bpp] if i >= bpp else 0
+            up = prev[i]
+            ul = prev[i - bpp] if i >= bpp else 0
+            if ft == 0:
+                pred = 0
+            elif ft == 1:
+                pred = left
+            elif ft == 2:
+                pred = up
+            elif ft == 3:
+                pred = (left + up) >> 1
+            else:
+                p = left + up - ul
+                pa, pb, pc = abs(p - left), abs(p - up), abs(p - ul)
+                pred = left if pa <= pb and pa <= pc else (up if pb <= pc else ul)
+            out[i] = (row[i] - pred) & 255
+        raw += bytes([ft]) + out
+        prev = row
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    data = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 0))
+    if plte is not None:
+        data += chunk(b"PLTE", np.asarray(plte, np.uint8).tobytes())
+    if trns is not None:
+        data += chunk(b"tRNS", trns)
+    # split the image data over two IDAT chunks
+    z = zlib.compress(bytes(raw))
+    return data + chunk(b"IDAT", z[:len(z) // 2]) + chunk(b"IDAT", z[len(z) // 2:]) + chunk(b"IEND", b"")
+
+
 def test_loader_error_messages(tmp_path):
     bad = tmp_path / "bad.settings"
     bad.write_text("renderer pathtracer_mis\nfoo 3\n")
@@ -70,6 +226,13 @@ def test_loader_error_messages(tmp_path):
         f.write("light point 0 0 0 rgb(1,1,1) 1.0\n")
     r = subprocess.run([DUMP, settings, str(tmp_path / "x.bin")], capture_output=True, text=True)
     assert r.returncode == 1 and "Point lights" in r.stdout
+    # a missing image texture (loader.h:45-97 via rgb_image.h:161-197)
+    (tmp_path / "sc" / "mat_0.ptmat").write_text(
+        "DISNEY image(nope.png) float(0.5) float(0.5) float(0.0) float(0.0) float(1.5) rgb(0.5,0.5,1.0) float(1.0)\n")
+    with open(os.path.join(str(tmp_path / "sc"), "scene.ptscene"), "w") as f:
+        f.write("camera 0 0 3.9 0 0 0 0 1 0 0.35\nobj mesh_0.ptobj mat_0.ptmat\n")
+    r = subprocess.run([DUMP, settings, str(tmp_path / "x.bin")], capture_output=True, text=True)
+    assert r.returncode == 1 and "Cannot open image file" in r.stdout
     (tmp_path / "m.ptmat").write_text("PHONG rgb(1,1,1)\n")
     (tmp_path / "m.ptscene").write_text("camera 0 0 1 0 0 0 0 1 0 0.5\nobj missing.ptobj m.ptmat\n")
     (tmp_path / "m.settings").write_text(f"scene {tmp_path / 'm.ptscene'}\n")

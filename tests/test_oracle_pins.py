@@ -261,3 +261,89 @@ def test_one_bounce_sibling_integrators_match_lambert_polygon(integrator, spp_ax
     if integrator == 2:
         assert counts[2] == 0  # no shadow rays
     assert abs(got - want) / want < tol, (got, want)
+
+
+# ---------------------------------------------------------------- image textures
+def _textured_quad(seed=0):
+    """One quad z=0 over [-1,1]^2 facing +z, uv = ((x+1)*0.75, (y+1)*0.75) in [0, 1.5];
+    albedo 5x3 RGB (sRGB), roughness 4x7 gray, metal 6x2 gray, normal map 3x5 RGB."""
+    rng = np.random.default_rng(seed)
+    s = cpt.scenes.Scene()
+    imgs = dict(alb=rng.integers(0, 256, (3, 5, 3), dtype=np.uint8),
+                rough=rng.integers(0, 256, (7, 4), dtype=np.uint8),
+                metal=np.array([[0, 255, 20, 26, 255, 0], [255, 0, 25, 0, 30, 255]], np.uint8),
+                nmap=rng.integers(0, 256, (5, 3, 3), dtype=np.uint8))
+    ids = dict(alb=s.add_image(imgs["alb"], False, True), rough=s.add_image(imgs["rough"], True, False),
+               metal=s.add_image(imgs["metal"], True, False), nmap=s.add_image(imgs["nmap"], False, False))
+    m = s.add_material(cpt.scenes.material(cpt.scenes.MAT_DIFFUSE, albedo=(0.5, 0.5, 0.5), rough=0.2),
+                       tex=(ids["alb"], ids["rough"], ids["metal"], ids["nmap"]))
+    P = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    N = np.tile(np.float32([0, 0, 1]), (4, 1))
+    T = np.tile(np.float32([1, 0, 0]), (4, 1))
+    F = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    s.add_mesh(P, N, T, F, m, uvs=(P[:, :2] + 1.0) * 0.75)
+    s.set_camera((0, 0, 3), (0, 0, 0), (0, 1, 0), 0.5, 8, 8)
+    return s, imgs, ids
+
+
+def test_texture_lookup_closed_form():
+    """ImageTexture::sample, point filter with the REPEAT border (image_texture.h:24-28,
+    rgb_image.h:211-299), restated independently: for a ray hitting the quad at
+    (x, y), u = (x+1)*0.75, v = (y+1)*0.75, texel (floor(u*w) mod w, floor((1-v)*h)
+    mod h), row-major from the top.  Points within 1e-3 texel of a texel edge are
+    skipped (the barycentric uv rounds differently from the affine one there)."""
+    s, imgs, ids = _textured_quad()
+    o = oracle.OracleScene(s)
+    rng = np.random.default_rng(1)
+    xy = rng.uniform(-0.999, 0.999, (4000, 2))
+    rays = np.zeros((len(xy), 7), np.float32)
+    rays[:, :2] = xy
+    rays[:, 2] = 1.0
+    rays[:, 5] = -1.0
+    rays[:, 6] = 10.0
+    got = o.hit_material(rays)
+    assert (got[:, 0] >= 0).all()
+    u = (rays[:, 0].astype(np.float64) + 1.0) * 0.75
+    v = (rays[:, 1].astype(np.float64) + 1.0) * 0.75
+    assert np.abs(got[:, 8] - u).max() < 1e-6 and np.abs(got[:, 9] - v).max() < 1e-6
+
+    def texel(name, srgb, bw):
+        px = imgs[name]
+        h, w = px.shape[:2]
+        fx, fy = u * w, (1.0 - v) * h
+        ok = (np.abs(fx - np.round(fx)) > 1e-3) & (np.abs(fy - np.round(fy)) > 1e-3)
+        i = np.floor(fx).astype(int) % w
+        j = np.floor(fy).astype(int) % h
+        lin = cpt.scenes.image_texels(px, bw, srgb)
+        return lin[j, i], ok
+    alb, ok_a = texel("alb", True, False)
+    rough, ok_r = texel("rough", False, True)
+    metal, ok_m = texel("metal", False, True)
+    nmap, ok_n = texel("nmap", False, False)
+    assert ok_a.mean() > 0.9
+    assert np.array_equal(got[ok_a, 1:4], alb[ok_a])
+    assert np.array_equal(got[ok_r, 4], rough[ok_r, 0])
+    assert np.array_equal(got[ok_m, 0], np.where(metal[ok_m, 0] < 0.1, 0.0, 1.0))
+    assert np.array_equal(got[ok_n, 5:8], nmap[ok_n])
+    # both lobes occur (texels 0, 20, 25 -> diffuse; 26, 30, 255 -> GGX)
+    assert set(got[:, 0].tolist()) == {0.0, 1.0}
+
+
+def test_texture_repeat_border_quirk():
+    """Image<T>::handle_border, REPEAT (rgb_image.h:216-258): i >= w wraps to i % w,
+    i < 0 maps to i % w + w (C remainder), which is w itself for a negative
+    multiple of w: get_pixel then reads pixels[w + j*w], the first texel of the
+    next row."""
+    s, imgs, ids = _textured_quad()
+    o = oracle.OracleScene(s)
+    lin = cpt.scenes.image_texels(imgs["alb"], False, True)  # (3, 5, 3)
+    h, w = lin.shape[:2]
+    assert np.array_equal(o.image_texel(ids["alb"], 7, 1), lin[1, 2])
+    assert np.array_equal(o.image_texel(ids["alb"], -1, 0), lin[0, w - 1])
+    assert np.array_equal(o.image_texel(ids["alb"], -7, -2), lin[1, 3])
+    assert np.array_equal(o.image_texel(ids["alb"], -w, 0), lin[1, 0])   # the quirk
+    assert np.array_equal(o.image_texel(ids["alb"], 2, 3 * h + 1), lin[1, 2])
+    # past the last texel the reference reads beyond the image (undefined); both
+    # this restatement and the device take the last texel
+    assert np.array_equal(o.image_texel(ids["alb"], 2, -h), lin[h - 1, w - 1])
+    assert np.array_equal(o.image_texel(ids["alb"], -w, h - 1), lin[h - 1, w - 1])
