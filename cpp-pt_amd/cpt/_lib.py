@@ -60,6 +60,7 @@ class SceneDesc(C.Structure):
         ("n_lights", C.c_int32), ("light_triangles", C.c_void_p), ("camera", C.c_float * 16),
         ("env", C.c_float * 3), ("flags", C.c_int32), ("uvs", C.c_void_p), ("n_textures", C.c_int32),
         ("textures", C.c_void_p), ("texels", C.c_void_p), ("material_textures", C.c_void_p),
+        ("n_point_lights", C.c_int32), ("point_lights", C.c_void_p),
     ]
 
 
@@ -224,6 +225,9 @@ class Context:
         d.textures = _ptr(arrs["textures"]) if d.n_textures else None
         d.texels = _ptr(arrs["texels"]) if d.n_textures else None
         d.material_textures = _ptr(arrs["material_textures"])
+        pl = arrs.get("point_lights")
+        d.n_point_lights = 0 if pl is None else pl.shape[0]
+        d.point_lights = _ptr(pl) if d.n_point_lights else None
         self._check(self.lib.cpt_upload_scene(self.h, C.byref(d)), "cpt_upload_scene")
         self.n_triangles = d.n_triangles
 

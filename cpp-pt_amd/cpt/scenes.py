@@ -143,6 +143,7 @@ class Scene:
     env: tuple = (0.0, 0.0, 0.0)
     images: list = field(default_factory=list)        # (pixels uint8 (h,w,n), bw, srgb)
     material_tex: list = field(default_factory=list)  # per material: albedo, roughness, metal, normal image
+    point_lights: list = field(default_factory=list)  # (position, rgb colour, intensity)
     _flat: dict = None
 
     def add_material(self, rec, tex=(-1, -1, -1, -1)):
@@ -170,6 +171,12 @@ class Scene:
                                 np.ascontiguousarray(tangents, np.float32),
                                 np.ascontiguousarray(faces, np.int32), mat,
                                 None if uvs is None else np.ascontiguousarray(uvs, np.float32)))
+        self._flat = None
+
+    def add_point_light(self, position, colour, intensity):
+        """`light point x y z rgb(colour) intensity` (loader.h:129-141); intensity is
+        the total power, radiance colour*intensity/(4 pi) (point_light.h:20-28)."""
+        self.point_lights.append((tuple(float(x) for x in position), tuple(float(x) for x in colour), float(intensity)))
         self._flat = None
 
     def set_camera(self, origin, at, up, tan_fovy, res_x, res_y):
@@ -210,7 +217,10 @@ class Scene:
             indices=np.ascontiguousarray(np.concatenate(idx), np.int32),
             tri_material=np.ascontiguousarray(np.concatenate(tmat), np.int32),
             materials=np.ascontiguousarray(np.stack(self.material_list), np.float32),
-            light_tris=np.ascontiguousarray(np.concatenate(lights) if lights else np.zeros(0, np.int32), np.int32),
+            # LightGroup order: point lights (added while parsing), then emissive triangles
+            light_tris=np.ascontiguousarray(np.concatenate(
+                [-1 - np.arange(len(self.point_lights), dtype=np.int32)] + lights), np.int32),
+            point_lights=self._point_light_table(),
             camera=camera_frame(o, at, up, tf, rx, ry),
             env=np.asarray(self.env, np.float32),
             uvs=np.ascontiguousarray(np.concatenate(uvs), np.float32),
@@ -229,6 +239,13 @@ class Scene:
             material_textures=np.ascontiguousarray(np.stack(mt), np.int32),
         )
         return self._flat
+
+    def _point_light_table(self):
+        """Position and radiance per point light, radiance = (colour * intensity) *
+        float(1.0 / float(4 pi)) in float32, as Vec3 / double evaluates (math.h:104-109)."""
+        inv = f32(1.0 / float(f32(4.0 * math.pi)))
+        rows = [list(p) + [float(f32(f32(c) * f32(i)) * inv) for c in col] for p, col, i in self.point_lights]
+        return np.ascontiguousarray(np.array(rows, np.float32).reshape(-1, 6))
 
     @property
     def n_triangles(self):
@@ -259,6 +276,8 @@ class Scene:
                             f"{slot(1, f'float({_n(rec[4])})')} {slot(2, f'float({metal})')} float(0.0) float(1.5) "
                             f"{slot(3, f'rgb({_n(rec[9])},{_n(rec[10])},{_n(rec[11])})')} float(1.0)\n")
         lines = [f"camera {_fmt(o)} {_fmt(at)} {_fmt(up)} {_n(tf)}"]
+        for p, col, i in self.point_lights:
+            lines.append(f"light point {_fmt(p)} rgb({_n(col[0])},{_n(col[1])},{_n(col[2])}) {_n(i)}")
         if any(self.env):
             lines.append(f"environment {_fmt(self.env)}")
         for i, m in enumerate(self.meshes):

@@ -998,6 +998,7 @@ struct cpt_ctx {
     float2* d_uv = nullptr;     // per vertex texture coordinates
     int4* d_tex = nullptr;      // texture table
     float* d_texels = nullptr;
+    float4* d_plights = nullptr;
     int* d_lights = nullptr;
     float cam[16];
     float env[3];
@@ -1109,6 +1110,7 @@ void free_scene(cpt_ctx* c) {
     dfree(c->d_uv);
     dfree(c->d_tex);
     dfree(c->d_texels);
+    dfree(c->d_plights);
     dfree(c->d_lights);
     c->has_scene = false;
 }
@@ -1153,6 +1155,7 @@ DevScene dev_scene(const cpt_ctx* c) {
     s.tex = c->d_tex;
     s.texels = c->d_texels;
     s.lights = c->d_lights;
+    s.plights = c->d_plights;
     s.n_lights = c->nl;
     s.n_tris = c->nt;
     for (int k = 0; k < 3; k++) s.env[k] = c->env[k];
@@ -1507,9 +1510,13 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
         if ((int)type == CPT_MAT_STANDARD && (!d->material_textures || d->material_textures[4 * (size_t)m + 2] < 0))
             return fail(c, CPT_E_INVALID, "material %d: CPT_MAT_STANDARD needs a metal texture", m);
     }
+    if (d->n_point_lights < 0 || (d->n_point_lights > 0 && !d->point_lights))
+        return fail(c, CPT_E_INVALID, "bad point-light table");
+    if (d->n_lights > 0 && !d->light_triangles) return fail(c, CPT_E_INVALID, "missing light table");
     for (int i = 0; i < d->n_lights; i++) {
         int t = d->light_triangles[i];
-        if (t < 0 || t >= d->n_triangles) return fail(c, CPT_E_INVALID, "light %d: triangle %d out of range", i, t);
+        if (t >= d->n_triangles || (t < 0 && -1 - t >= d->n_point_lights))
+            return fail(c, CPT_E_INVALID, "light %d: entry %d out of range", i, t);
     }
     HIPCHK(c, hipSetDevice(c->device));
     drop_graph(c);
@@ -1607,6 +1614,7 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     r |= dalloc(c, c->d_tex, std::max<size_t>(1, (size_t)d->n_textures));
     r |= dalloc(c, c->d_texels, std::max<size_t>(1, n_texels));
     r |= dalloc(c, c->d_lights, std::max(1, c->nl));
+    r |= dalloc(c, c->d_plights, 2 * (size_t)std::max(1, d->n_point_lights));
     if (r) return CPT_E_HIP;
     HIPCHK(c, hipMemcpy(c->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4), hipMemcpyHostToDevice));
     if (!bvh.tris.empty())
@@ -1634,6 +1642,15 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     }
     if (c->nl > 0)
         HIPCHK(c, hipMemcpy(c->d_lights, d->light_triangles, sizeof(int) * (size_t)c->nl, hipMemcpyHostToDevice));
+    if (d->n_point_lights > 0) {
+        std::vector<float4> pl(2 * (size_t)d->n_point_lights);
+        for (int k = 0; k < d->n_point_lights; k++) {
+            const float* e = d->point_lights + 6 * (size_t)k;
+            pl[2 * k] = make_float4(e[0], e[1], e[2], 0.0f);
+            pl[2 * k + 1] = make_float4(e[3], e[4], e[5], 0.0f);
+        }
+        HIPCHK(c, hipMemcpy(c->d_plights, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
     c->has_scene = true;
     c->scene_gen++;
     return CPT_OK;

@@ -96,7 +96,8 @@ struct DevScene {
     const float2* uv;        // per vertex texture coordinates
     const int4* tex;         // per texture: width, height, channels, first texel
     const float* texels;     // linear, row 0 at the top
-    const int* lights;       // emitter triangle ids
+    const int* lights;       // LightGroup order: triangle id, or -1-k for point light k
+    const float4* plights;   // 2 per point light: position, radiance
     int n_lights;
     int n_tris;
     float env[3];

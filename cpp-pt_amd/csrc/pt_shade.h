@@ -353,6 +353,15 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, uint64_t& rn
     unsigned NL = (unsigned)sc.n_lights;
     int pick = (int)((unsigned)(int)(pcg_float(rng) * (float)NL) % NL);
     int id = sc.lights[pick];
+    LSample ls;
+    if (id < 0) {  // PointLight::connect_eye_path (point_light.h:20-28): no random numbers
+        const float4 p = sc.plights[2 * (-1 - id)], e = sc.plights[2 * (-1 - id) + 1];
+        ls.position = {p.x, p.y, p.z};
+        ls.intensity = {e.x, e.y, e.z};
+        ls.normal = {1.0f, 0.0f, 0.0f};
+        ls.pdf = (float)(1.0 * (1.0 / (double)(float)NL));
+        return ls;
+    }
     float r1 = pcg_float(rng);
     float r2 = pcg_float(rng);
     float alpha = r1, beta = r2;
@@ -364,7 +373,6 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, uint64_t& rn
     int4 ti = sc.tri_info[id];
     V3 a = ld3(sc.vpos[ti.x]), b = ld3(sc.vpos[ti.y]), c = ld3(sc.vpos[ti.z]);
     V3 n0 = ld3(sc.vnrm[ti.x]), n1 = ld3(sc.vnrm[ti.y]), n2 = ld3(sc.vnrm[ti.z]);
-    LSample ls;
     ls.position = add(add(mulf(a, alpha), mulf(b, beta)), mulf(c, gamma));
     ls.normal = normalized(add(add(mulf(n0, alpha), mulf(n1, beta)), mulf(n2, gamma)));
     const float4* mp = reinterpret_cast<const float4*>(sc.mats + 16 * ti.w);

@@ -126,6 +126,10 @@ struct Scene {
     std::vector<float> materials;  // 16 floats per material
     std::vector<float> uvs, texels;
     std::vector<int32_t> textures, material_textures;
+    // point lights in .ptscene order: position, radiance (6 floats each); they come
+    // first in the LightGroup (loader.h:129-141 runs while parsing, the shape lights
+    // are added after it, loader.h:353-367)
+    std::vector<float> point_lights;
     int n_textures = 0;
     int n_vertices = 0, n_triangles = 0, n_materials = 0;
     void flatten(const SceneData& sd);

@@ -155,6 +155,31 @@ void parse_camera(LoaderData* ld, std::istringstream& iss) {
     ld->s->camera->setup(ld->rs->resX, ld->rs->resY, tan_fovy, o, normalized(v(at.x - o.x, at.y - o.y, at.z - o.z)), up);
 }
 
+// loader.h:129-141: "light point x y z <colour texture> intensity"; the colour is
+// read as an sRGB Vec3 texture from the working directory and evaluated at uv 0
+// (PointLight::connect_eye_path, point_light.h:20-28), whose point lookup lands on
+// the first texel of an image.  Radiance = colour * intensity / (4 pi), with
+// Vec3 / a = Vec3 * float(1.0 / a) and a = float(4 pi) (math.h:104-109).
+void parse_light(LoaderData* ld, std::istringstream& iss) {
+    std::string type;
+    iss >> type;
+    if (type != "point") throw std::runtime_error("Unknown light type: " + type);
+    Vec3f pos;
+    std::string tex;
+    float intensity;
+    if (!(iss >> pos.x >> pos.y >> pos.z >> tex >> intensity)) throw std::runtime_error("Malformed light line");
+    Tex t = get_texture(ld->sd, "", tex, false, true);
+    Vec3f col = t.value;
+    if (t.image >= 0) {
+        const std::vector<float>& px = ld->sd->images[t.image].texels;
+        col = v(px[0], px[1], px[2]);
+    }
+    const float inv = (float)(1.0 / (double)(float)(4.0 * M_PI));
+    const float rec[6] = {pos.x, pos.y, pos.z, (col.x * intensity) * inv, (col.y * intensity) * inv,
+                          (col.z * intensity) * inv};
+    ld->s->point_lights.insert(ld->s->point_lights.end(), rec, rec + 6);
+}
+
 void parse_mat(LoaderData* ld, const std::string& dir, const std::string& filename) {
     auto it = ld->loaded_materials.find(filename);
     if (it != ld->loaded_materials.end()) {
@@ -340,6 +365,7 @@ void Scene::flatten(const SceneData& sd) {
     }
     n_textures = (int)sd.images.size();
     n_materials = (int)std::max<size_t>(1, table.size());
+    for (size_t k = 0; k < point_lights.size() / 6; k++) light_triangles.push_back(-1 - (int32_t)k);
     // loader.h:353-367: mesh order, then face order; emissive triangles become lights
     int voff = 0;
     for (size_t i = 0; i < sd.meshes.size(); i++) {
@@ -388,6 +414,8 @@ cpt_scene_desc Scene::desc() const {
     d.textures = textures.empty() ? nullptr : textures.data();
     d.texels = texels.empty() ? nullptr : texels.data();
     d.material_textures = material_textures.data();
+    d.n_point_lights = (int32_t)(point_lights.size() / 6);
+    d.point_lights = point_lights.empty() ? nullptr : point_lights.data();
     return d;
 }
 
@@ -458,10 +486,7 @@ void load_scene(Scene* s, SceneData* sd, RenderSettings* rs, const std::string& 
             parse_obj(&ld, dir_file.first, obj);
             parse_mat(&ld, dir_file.first, mat);
         } else if (type == "light") {
-            std::string lt;
-            iss >> lt;
-            if (lt == "point") throw std::runtime_error("Point lights are not supported by the GPU path");
-            throw std::runtime_error("Unknown light type: " + lt);
+            parse_light(&ld, iss);
         } else if (type == "environment") {
             // addition: constant environment radiance "environment r g b"
             iss >> s->env[0] >> s->env[1] >> s->env[2];

@@ -4,7 +4,7 @@
 // bitangents (nv*3 f32), indices (nt*3 i32), tri_material (nt i32), materials
 // (nm*16 f32), light triangles (nl i32), camera (16 f32), env (3 f32), uvs (nv*2 f32),
 // n_textures (i32), textures (n_textures*4 i32), n_texels (i32), texels (f32),
-// material textures (nm*4 i32).
+// material textures (nm*4 i32), n_point_lights (i32), point lights (n*6 f32).
 // No device is used.
 #include <cstdio>
 #include <iostream>
@@ -45,6 +45,9 @@ int main(int argc, char** argv) {
         fwrite(&nx, sizeof(nx), 1, f);
         fwrite(s.texels.data(), sizeof(float), s.texels.size(), f);
         fwrite(s.material_textures.data(), sizeof(int32_t), s.material_textures.size(), f);
+        const int32_t np = (int32_t)(s.point_lights.size() / 6);
+        fwrite(&np, sizeof(np), 1, f);
+        fwrite(s.point_lights.data(), sizeof(float), s.point_lights.size(), f);
         fclose(f);
     } catch (const std::runtime_error& e) {
         std::cout << e.what() << std::endl;

@@ -55,7 +55,10 @@ typedef struct cpt_ctx cpt_ctx;
  *  - materials: n_materials x 16 floats:
  *      [0] type  [1..3] albedo  [4] roughness  [5..7] emission (colour*intensity)
  *      [8] ior   [9..11] constant normal-map texel  [12] has normal map (1/0)
- *  - light_triangles: emissive triangle ids in LightGroup order (loader.h:362-364)
+ *  - light_triangles: the LightGroup in its order (light_group.h:57-60): point
+ *    lights as they appear in the .ptscene (loader.h:129-141), then every
+ *    emissive triangle (loader.h:362-364).  An entry >= 0 is a triangle id, an
+ *    entry e < 0 is point light -1-e.
  *  - camera: o[3], frame columns x[3] y[3] z[3], tan_fovy, aspect, resX, resY
  *    (CameraPerspective, src/camera/camera_perspective.h:19-27)
  *  - env: constant radiance returned for rays that leave the scene
@@ -69,6 +72,9 @@ typedef struct cpt_ctx cpt_ctx;
  *        row 0 at the top, channels interleaved
  *      material_textures: n_materials x 4 int32: albedo, roughness, metal and
  *        normal-map textures (-1: the material record's constant)
+ *  - point_lights: n_point_lights x 6 floats, position and the radiance that
+ *    PointLight::connect_eye_path returns, colour*intensity/(4 pi) computed as the
+ *    reference does (point_light.h:20-28); NEE treats its normal as (1,0,0)
  */
 typedef struct {
     int32_t n_vertices;
@@ -90,6 +96,8 @@ typedef struct {
     const int32_t* textures;
     const float* texels;
     const int32_t* material_textures;
+    int32_t n_point_lights;
+    const float* point_lights;
 } cpt_scene_desc;
 
 /* skip computing the reference BVH's test order (used only to break exact-t ties

@@ -36,6 +36,8 @@ def lib():
     L.orc_hit_material.argtypes = [vp, vp, i32, vp]
     L.orc_image_texel.restype = None
     L.orc_image_texel.argtypes = [vp, i32, i32, i32, vp]
+    L.orc_scene_set_point_lights.restype = None
+    L.orc_scene_set_point_lights.argtypes = [vp, i32, vp]
     L.orc_scene_set_textures.restype = None
     L.orc_scene_set_textures.argtypes = [vp, vp, i32, vp, i32, vp, vp]
     L.orc_scene_destroy.argtypes = [vp]
@@ -112,6 +114,9 @@ class OracleScene:
         if "textures" in a and len(a["textures"]):
             L.orc_scene_set_textures(self.h, _p(a["uvs"]), a["textures"].shape[0], _p(a["textures"]),
                                      a["texels"].size, _p(a["texels"]), _p(a["material_textures"]))
+        pl = a.get("point_lights") if hasattr(a, "get") else None
+        if pl is not None and len(pl):
+            L.orc_scene_set_point_lights(self.h, pl.shape[0], _p(pl))
         self.n_triangles = a["indices"].shape[0]
 
     def close(self):

@@ -146,7 +146,8 @@ struct Scene {
     std::vector<std::array<int, 3>> F;
     std::vector<int> tri_mat;
     std::vector<Material> mats;
-    std::vector<int> lights;       // triangle ids in LightGroup order (loader.h:362-364)
+    std::vector<int> lights;       // LightGroup order: triangle id, or -1-k for point light k
+    std::vector<std::array<float, 6>> point_lights;  // position, radiance
     std::vector<int> light_of_tri; // -1 if not emissive
     Camera cam;
     V3 env;
@@ -759,11 +760,23 @@ LSample tri_light_sample(const Scene& s, int id, float r1, float r2, const Isect
     return ls;
 }
 
-// light_group.h:25-30: uniform pick int(u*N)%N over all emitter triangles,
-// pdf *= 1.0/float(N) (a double factor applied to the float pdf)
+// light_group.h:25-30: uniform pick int(u*N)%N over all lights, pdf *= 1.0/float(N)
+// (a double factor applied to the float pdf).  A point light (point_light.h:20-28)
+// draws no further numbers: pdf 1, its radiance, and the normal (1,0,0) that NEE's
+// cosine term then uses.
 LSample light_sample(const Scene& s, Sampler& smp, const Isect& from) {
     size_t NL = s.lights.size();
     int pick = (int)(int(smp.sample() * (float)NL) % NL);
+    if (s.lights[pick] < 0) {
+        const auto& p = s.point_lights[-1 - s.lights[pick]];
+        LSample ls;
+        ls.position = V3(p[0], p[1], p[2]);
+        ls.intensity = V3(p[3], p[4], p[5]);
+        ls.normal = V3(1.0f, 0.0f, 0.0f);
+        ls.pdf = 1.0f;
+        ls.pdf = (float)((double)ls.pdf * (1.0 / (double)(float)NL));
+        return ls;
+    }
     float r1 = smp.sample();
     float r2 = smp.sample();
     LSample ls = tri_light_sample(s, s.lights[pick], r1, r2, from);
@@ -1056,7 +1069,8 @@ void* orc_scene_create(int nv, const float* pos, const float* nrm, const float* 
     }
     s->lights.assign(light_tris, light_tris + nl);
     s->light_of_tri.assign(nt, -1);
-    for (int i = 0; i < nl; i++) s->light_of_tri[light_tris[i]] = i;
+    for (int i = 0; i < nl; i++)
+        if (light_tris[i] >= 0) s->light_of_tri[light_tris[i]] = i;
     s->cam.o = ld(cam);
     s->cam.coords = M3(ld(cam + 3), ld(cam + 6), ld(cam + 9));
     s->cam.tan_fovy = cam[12];
@@ -1081,6 +1095,13 @@ void orc_scene_set_textures(void* sp, const float* uvs, int n_images, const int3
     for (size_t i = 0; i < s->mat_tex.size(); i++)
         s->mat_tex[i] = {material_textures[4 * i], material_textures[4 * i + 1], material_textures[4 * i + 2],
                          material_textures[4 * i + 3]};
+}
+
+void orc_scene_set_point_lights(void* sp, int n, const float* rec) {
+    Scene* s = (Scene*)sp;
+    s->point_lights.resize(n);
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < 6; j++) s->point_lights[k][j] = rec[6 * k + j];
 }
 
 void orc_scene_destroy(void* s) { delete (Scene*)s; }

@@ -50,6 +50,8 @@ void* orc_scene_create(int nv, const float* pos, const float* nrm, const float* 
  * constant).  A metal image makes the Lambertian/GGX choice per hit. */
 void orc_scene_set_textures(void* s, const float* uvs, int n_images, const int32_t* images, int n_texels,
                             const float* texels, const int32_t* material_textures);
+/* point lights (n x 6: position, radiance); light_tris entries -1-k refer to them */
+void orc_scene_set_point_lights(void* s, int n, const float* rec);
 /* per ray: the closest hit's material after its textures (10 floats: type or -1
  * for a miss, albedo rgb, roughness, normal-map texel rgb, u, v) */
 void orc_hit_material(void* s, const float* rays, int n, float* out);

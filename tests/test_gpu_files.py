@@ -1,6 +1,6 @@
 """The drop-in path through scene files: cpt_render (the reference's main,
 src/main.cpp:89-146) loads .settings/.ptscene/.ptobj/.ptmat files with image
-textures (loader.h:45-97), renders on the GPU and writes the 8-bit sRGB PNG
+textures (loader.h:45-97) and a point light (loader.h:129-141), renders on the GPU and writes the 8-bit sRGB PNG
 (rgb_image.h:300-318); the film equals an in-process render of the same
 scene arrays."""
 import os
@@ -50,6 +50,7 @@ def _to_srgb8(x):
 def test_cli_renders_textured_scene_files(ctx, tmp_path, renderer):
     W, H = 40, 32
     s = cpt.scenes.textured_room(res=(W, H))
+    s.add_point_light((0.3, 0.4, 0.3), (1.0, 0.9, 0.7), 2.0)
     settings = s.write_cpppt(str(tmp_path), spp=2, renderer=renderer, max_depth=16, seed=7)
     print(f"cpt_render {renderer} {W}x{H}", flush=True)
     r = subprocess.run([RENDER, settings], capture_output=True, text=True, timeout=120)

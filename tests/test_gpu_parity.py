@@ -219,6 +219,22 @@ def test_textured_scene_matches_oracle_per_path_stream(ctx, integrator):
     assert abs(g.sum() - r.sum()) / r.sum() < 1e-3
 
 
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_point_lights_match_oracle_per_path_stream(ctx, integrator):
+    """Point lights beside the area light in one LightGroup (point_light.h:20-28,
+    light_group.h:25-30): a pick that lands on a point light draws no further
+    random numbers, NEE's cosine term uses the (1,0,0) normal; GPU and oracle
+    agree path by path."""
+    s = cpt.scenes.box_room(res=(48, 48))
+    s.add_point_light((0.4, 0.5, 0.2), (1.0, 0.8, 0.6), 4.0)
+    s.add_point_light((-0.6, -0.3, 0.6), (0.3, 0.5, 1.0), 2.0)
+    print(f"box room + 2 point lights 48x48, integrator {integrator}", flush=True)
+    g, r = _path_parity(ctx, s, 48, 48, 4, 32, 29, integrator)
+    per_px = (np.abs(g - r) / np.maximum(np.abs(r), 1e-3)).max(axis=-1)
+    assert (per_px < 1e-4).mean() > 0.99
+    assert abs(g.sum() - r.sum()) / r.sum() < 1e-3
+
+
 def test_integrators_agree_statistically(ctx):
     """PurePt and Pathtracer estimate the same image (emitters do not reflect in
     either).  The light hangs 0.2 below the ceiling: a light 1e-3 from a wall

@@ -44,6 +44,8 @@ def _load_dump(path):
     n_texels = int(take(np.int32, 1)[0])
     out["texels"] = take(np.float32, n_texels)
     out["material_textures"] = take(np.int32, 4 * nm).reshape(-1, 4)
+    n_point = int(take(np.int32, 1)[0])
+    out["point_lights"] = take(np.float32, 6 * n_point).reshape(-1, 6)
     assert off == len(raw)
     return out, int(spp)
 
@@ -109,6 +111,104 @@ def test_loader_reads_image_textures(tmp_path):
     alb = image(want, 4, 0)[1]
     c = 230 / 255
     assert abs(alb[0] - ((c + 0.055) / 1.055) ** 2.4) < 1e-6
+
+
+def _write_blender_layout(s, d):
+    """The layout of the reference's Blender exporter (scripts/blender_export_cpppt.py):
+    models/, materials/ and images/ below the scene directory, one material token
+    per line, floats printed by Python's str(), point lights as `light point`."""
+    for sub in ("models", "materials", "images"):
+        os.makedirs(os.path.join(d, sub), exist_ok=True)
+    for k, (px, bw, srgb) in enumerate(s.images):
+        cpt.scenes.write_png(os.path.join(d, "images", f"img{k}.png"), px)
+
+    def f(x):
+        return str(float(x))
+    lines = []
+    for k, rec in enumerate(s.material_list):
+        tex = s.material_tex[k]
+        kind = int(rec[0])
+        with open(os.path.join(d, "materials", f"M{k}.ptmat"), "w") as fm:
+            if kind == cpt.scenes.MAT_EMISSIVE:
+                fm.write(f"EMISSION\nrgb({f(rec[5])},{f(rec[6])},{f(rec[7])})\nfloat(1.0)\n")
+                continue
+            slot = [f"image(images/img{t}.png)" if t >= 0 else None for t in tex]
+            metal = "1.0" if kind == cpt.scenes.MAT_GLOSSY else "0.0"
+            toks = [slot[0] or f"rgb({f(rec[1])},{f(rec[2])},{f(rec[3])})", "float(0.5)",
+                    slot[1] or f"float({f(rec[4])})", slot[2] or f"float({metal})", "float(0.0)", "float(1.3)",
+                    slot[3] or "rgb(0.5,0.5,1.0)", "float(1.0)"]
+            fm.write("DISNEY\n" + "\n".join(toks) + "\n")
+    for i, m in enumerate(s.meshes):
+        with open(os.path.join(d, "models", f"Obj{i}.ptobj"), "w") as fo:
+            fo.write(f"{len(m.positions)} {len(m.faces)} TRIANGLE_MESH\n")
+            uv = m.uvs if m.uvs is not None else np.zeros((len(m.positions), 2), np.float32)
+            for p, q, n, t in zip(m.positions, uv, m.normals, m.tangents):
+                fo.write(" ".join(f(x) for x in (*p, *q, *n, *t)) + "\n")
+            for fc in m.faces:
+                fo.write(f"{fc[0]} {fc[1]} {fc[2]}\n")
+        lines.append(f"obj models/Obj{i}.ptobj materials/M{m.material}.ptmat")
+    for p, col, power in s.point_lights:
+        lines.append(f"light point {f(p[0])} {f(p[1])} {f(p[2])} rgb({f(col[0])},{f(col[1])},{f(col[2])}) {f(power)}")
+    o, at, up, tf, rx, ry = s.camera_def
+    lines.append("camera " + " ".join(f(x) for x in (*o, *at, *up, tf)))
+    with open(os.path.join(d, "scene.ptscene"), "w") as fs:
+        fs.write("\n".join(lines) + "\n")
+    with open(os.path.join(d, "scene.settings"), "w") as fs:
+        fs.write(f"renderer pathtracer_mis\nspp 1\nresolution {rx} {ry}\nscene {os.path.join(d, 'scene.ptscene')}\n"
+                 f"output {os.path.join(d, 'out')}\n")
+    return os.path.join(d, "scene.settings")
+
+
+def test_loader_reads_blender_export_layout_and_point_lights(tmp_path):
+    """A textured room with two point lights written the way the reference's Blender
+    exporter writes scenes: geometry, uvs, textures and materials load as from
+    the generator, and the LightGroup holds the point lights first (they are added
+    while the .ptscene is parsed, loader.h:129-141), then the emissive triangles."""
+    s = cpt.scenes.textured_room(res=(24, 16))
+    s.add_point_light((0.4, 0.6, 0.1), (1.0, 0.8, 0.6), 3.0)
+    s.add_point_light((-0.5, 0.2, 0.5), (0.2, 0.4, 1.0), 1.5)
+    settings = _write_blender_layout(s, str(tmp_path))
+    r = subprocess.run([DUMP, settings, str(tmp_path / "d.bin")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got, _ = _load_dump(str(tmp_path / "d.bin"))
+    want = s.c_arrays()
+    for k in ("positions", "normals", "bitangents", "indices", "uvs", "light_tris", "point_lights"):
+        assert np.array_equal(got[k], want[k]), k
+    assert got["light_tris"][:2].tolist() == [-1, -2]
+    # radiance = colour * power / (4 pi) in float32
+    assert abs(got["point_lights"][0, 3] - 3.0 / (4 * np.pi)) < 1e-6
+    gm, wm = got["tri_material"], want["tri_material"]
+    assert np.array_equal(got["materials"][gm, 0], want["materials"][wm, 0])
+    assert len(got["textures"]) == 4
+    # camera: the generator's frame
+    assert np.allclose(got["camera"], want["camera"], atol=1e-6)
+
+
+def test_point_light_colour_textures(tmp_path):
+    """get_texture<false, true> for the point light's colour (loader.h:129-141):
+    rgb(...) is taken as written, float(x) is decoded from sRGB, and an image is
+    read at uv 0, which lands on its first texel."""
+    s = cpt.scenes.box_room(res=(8, 8))
+    settings = s.write_cpppt(str(tmp_path))
+    cpt.scenes.write_png(str(tmp_path / "c.png"), np.array([[[200, 100, 50], [0, 0, 0]]], np.uint8))
+    with open(tmp_path / "scene.ptscene", "a") as fs:
+        fs.write("light point 0 0.5 0 float(0.5) 2.0\n")
+        fs.write(f"light point 0 0.5 0 image({tmp_path / 'c.png'}) 2.0\n")
+    r = subprocess.run([DUMP, settings, str(tmp_path / "d.bin")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got, _ = _load_dump(str(tmp_path / "d.bin"))
+    inv = np.float32(1.0 / float(np.float32(4 * np.pi)))
+    srgb_half = cpt.scenes.srgb2linear(np.float32(0.5))
+    assert np.array_equal(got["point_lights"][0, 3:], np.full(3, np.float32(np.float32(srgb_half) * np.float32(2.0)) * inv))
+    first = cpt.scenes.image_texels(np.array([[[200, 100, 50]]], np.uint8), False, True)[0, 0]
+    assert np.array_equal(got["point_lights"][1, 3:], (first * np.float32(2.0)).astype(np.float32) * inv)
+    bad = tmp_path / "bad"
+    bad.mkdir()
+    settings = s.write_cpppt(str(bad))
+    with open(bad / "scene.ptscene", "a") as fs:
+        fs.write("light spot 0 0 0 rgb(1,1,1) 1.0\n")
+    r = subprocess.run([DUMP, settings, str(tmp_path / "x.bin")], capture_output=True, text=True)
+    assert r.returncode == 1 and "Unknown light type: spot" in r.stdout
 
 
 def test_png_reader_formats(tmp_path):
@@ -222,10 +322,6 @@ def test_loader_error_messages(tmp_path):
     assert r.returncode == 1 and "Unknown render setting: foo" in r.stdout
     s = cpt.scenes.box_room()
     settings = s.write_cpppt(str(tmp_path / "sc"))
-    with open(os.path.join(str(tmp_path / "sc"), "scene.ptscene"), "a") as f:
-        f.write("light point 0 0 0 rgb(1,1,1) 1.0\n")
-    r = subprocess.run([DUMP, settings, str(tmp_path / "x.bin")], capture_output=True, text=True)
-    assert r.returncode == 1 and "Point lights" in r.stdout
     # a missing image texture (loader.h:45-97 via rgb_image.h:161-197)
     (tmp_path / "sc" / "mat_0.ptmat").write_text(
         "DISNEY image(nope.png) float(0.5) float(0.5) float(0.0) float(0.0) float(1.5) rgb(0.5,0.5,1.0) float(1.0)\n")
