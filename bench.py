@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip cpt_autotune (default launch configuration)")
-    ap.add_argument("--workload", choices=["terrain", "views", "guided"], default="terrain",
+    ap.add_argument("--gpu-bvh", action="store_true", help="build the BVH on the device (CPT_SCENE_GPU_BVH)")
+    ap.add_argument("--workload", choices=["terrain", "views", "guided", "bvh"], default="terrain",
                     help="terrain: the headline 1080p line; views: 64-view batch of the box room; "
                          "guided: equal-time relMSE of guided vs plain PT on the hard-lighting interior")
     ap.add_argument("--budgets", type=str, default="10,30,60", help="guided: equal-time budgets in seconds")
@@ -240,6 +241,46 @@ def bench_views(args):
     print(json.dumps(out))
 
 
+def bench_bvh(args):
+    """Host binned-SAH build vs the device PLOC build on the terrain workload:
+    build time, BVH shape, and the traversal kernels' time and node visits over
+    the same rays (renders are identical by construction)."""
+    import cpt
+
+    W, H, S, depth = args.width, args.height, args.spp_axis, args.depth
+    gx, gz = (int(v) for v in args.grid.split("x"))
+    scene = cpt.scenes.terrain_scene(res=(W, H), n_spheres=args.spheres, grid=(gx, gz))
+    ctx = cpt.Context(0)
+    res = {}
+    for name, gpu in (("host_sah", False), ("device_ploc", True)):
+        t0 = time.perf_counter()
+        ctx.upload(scene, gpu_bvh=gpu)
+        up = time.perf_counter() - t0
+        info = ctx.bvh_info()
+        ctx.render(W, H, S, max_depth=depth, seed=1)
+        tm = ctx.render(W, H, S, max_depth=depth, seed=1, flags=cpt.CPT_RENDER_KERNEL_TIMING)
+        ct = ctx.render(W, H, S, max_depth=depth, seed=1, flags=cpt.CPT_RENDER_COUNT)
+        t0 = time.perf_counter()
+        st = ctx.render(W, H, S, max_depth=depth, seed=1)
+        frame = time.perf_counter() - t0
+        res[name] = {"build_ms": info["build_ms"], "upload_s": up, "nodes": info["nodes"], "depth": info["depth"],
+                     "leaves": info["leaves"], "ms_extend": tm["ms_extend"], "ms_shadow": tm["ms_shadow"],
+                     "ms_frame": frame * 1e3, "samples_per_s": st["paths"] / frame,
+                     "nodes_per_ray": ct["node_visits"] / max(1, ct["extension_rays"]),
+                     "tris_per_ray": ct["triangle_tests"] / max(1, ct["extension_rays"])}
+        print(f"bench: {name} {res[name]}", file=sys.stderr, flush=True)
+    ctx.close()
+    h, d = res["host_sah"], res["device_ploc"]
+    print(json.dumps({
+        "metric": "BVH build time (device PLOC vs host binned SAH) and traversal cost",
+        "value": d["build_ms"], "unit": "ms", "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"procedural terrain {gx}x{gz} + {args.spheres} icospheres, "
+                               f"{scene.n_triangles} triangles, {W}x{H}, {S * S} spp, max depth {depth}"},
+        "build_speedup": h["build_ms"] / d["build_ms"], "extend_time_ratio": d["ms_extend"] / h["ms_extend"],
+        "host_sah": h, "device_ploc": d}))
+
+
 def cpu_baseline(scene, W, H, depth, target_s):
     import numpy as np
 
@@ -285,6 +326,9 @@ def main():
     if args.workload == "guided":
         bench_guided(args)
         return
+    if args.workload == "bvh":
+        bench_bvh(args)
+        return
     import cpt
 
     W, H, S, depth = args.width, args.height, args.spp_axis, args.depth
@@ -292,10 +336,11 @@ def main():
     t0 = time.perf_counter()
     scene = cpt.scenes.terrain_scene(res=(W, H), n_spheres=args.spheres, grid=(gx, gz))
     ctx = cpt.Context(0)
-    ctx.upload(scene)
+    ctx.upload(scene, gpu_bvh=args.gpu_bvh)
     setup_s = time.perf_counter() - t0
     bvh = ctx.bvh_info()
-    print(f"bench: scene {bvh['triangles']} triangles, {bvh['nodes']} BVH4 nodes, setup {setup_s:.1f} s",
+    print(f"bench: scene {bvh['triangles']} triangles, {bvh['nodes']} BVH4 nodes "
+          f"({'device' if bvh['gpu_built'] else 'host'} build {bvh['build_ms']:.0f} ms), setup {setup_s:.1f} s",
           file=sys.stderr, flush=True)
     kw = dict(max_depth=depth, block_size=args.block, grid_blocks=args.grid_blocks, max_paths=args.max_paths)
     tune = None
@@ -334,6 +379,7 @@ def main():
                         f"{S * S} spp per step ({S * S * args.steps} spp total), max depth {depth}",
             "width": W, "height": H, "spp_per_step": S * S, "max_depth": depth,
             "triangles": bvh["triangles"],
+            "bvh_build": "device PLOC" if bvh["gpu_built"] else "host binned SAH",
             "launch": tune,
         },
         "mrays_per_s": (ext + shadow) / total / 1e6,

@@ -22,6 +22,7 @@ CPT_E_NODEVICE = -4
 CPT_E_LIMIT = -5
 
 CPT_SCENE_NO_TIE_RANK = 1
+CPT_SCENE_GPU_BVH = 2
 CPT_RENDER_NO_GRAPH = 1
 CPT_RENDER_KERNEL_TIMING = 2
 CPT_RENDER_COUNT = 4
@@ -199,8 +200,9 @@ class Context:
         if rc != CPT_OK:
             raise CptError(rc, f"{what}: {self.lib.cpt_last_error(self.h).decode()}")
 
-    def upload(self, scene, tie_rank: bool = True, camera=None):
-        """Upload a cpt.scenes.Scene; `camera` (16 floats) overrides its camera."""
+    def upload(self, scene, tie_rank: bool = True, camera=None, gpu_bvh: bool = False):
+        """Upload a cpt.scenes.Scene; `camera` (16 floats) overrides its camera;
+        `gpu_bvh` builds the BVH on the device."""
         arrs = scene.c_arrays()
         d = SceneDesc()
         d.n_vertices = arrs["positions"].shape[0]
@@ -219,7 +221,7 @@ class Context:
             d.camera[k] = float(cam[k])
         for k in range(3):
             d.env[k] = float(arrs["env"][k])
-        d.flags = 0 if tie_rank else CPT_SCENE_NO_TIE_RANK
+        d.flags = (0 if tie_rank else CPT_SCENE_NO_TIE_RANK) | (CPT_SCENE_GPU_BVH if gpu_bvh else 0)
         d.uvs = _ptr(arrs["uvs"])
         d.n_textures = arrs["textures"].shape[0]
         d.textures = _ptr(arrs["textures"]) if d.n_textures else None
@@ -232,9 +234,10 @@ class Context:
         self.n_triangles = d.n_triangles
 
     def bvh_info(self):
-        out = np.zeros(4, np.int64)
+        out = np.zeros(6, np.int64)
         self._check(self.lib.cpt_bvh_info(self.h, _ptr(out)), "cpt_bvh_info")
-        return dict(nodes=int(out[0]), leaves=int(out[1]), depth=int(out[2]), triangles=int(out[3]))
+        return dict(nodes=int(out[0]), leaves=int(out[1]), depth=int(out[2]), triangles=int(out[3]),
+                    build_ms=out[4] / 1000.0, gpu_built=bool(out[5]))
 
     def tie_rank(self):
         out = np.zeros(self.n_triangles, np.int32)

@@ -29,6 +29,7 @@
 
 #include "../../include/cpt.h"
 #include "bvh_build.h"
+#include "bvh_gpu.h"
 #include "sdtree.h"
 #include "sdtree_dev.h"
 #include "pt_device.h"
@@ -146,6 +147,14 @@ struct WfArgs {
 
 // idle lanes (of 64) that make a traversal wave fetch new rays of its chunk
 #define CPT_DEFAULT_REFILL 32u
+
+// device BVH build: largest leaf and the clustering's neighbour window
+#ifndef CPT_GPU_BVH_LEAF
+#define CPT_GPU_BVH_LEAF 4
+#endif
+#ifndef CPT_GPU_BVH_RADIUS
+#define CPT_GPU_BVH_RADIUS 16
+#endif
 
 __device__ __forceinline__ unsigned gthreads() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ unsigned gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -985,8 +994,9 @@ struct cpt_ctx {
         int B, G, SB, refill;
     } tuned;
     int nt = 0, nv = 0, nl = 0, nm = 0;
-    int64_t bvh_nodes = 0, bvh_leaves = 0;
+    int64_t bvh_nodes = 0, bvh_leaves = 0, bvh_build_us = 0;
     int bvh_depth = 0;
+    bool bvh_gpu = false;
     std::vector<int32_t> rank;
     Node4* d_nodes = nullptr;
     float4* d_tris = nullptr;
@@ -1531,12 +1541,18 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     c->rank.clear();
     if (!(d->flags & CPT_SCENE_NO_TIE_RANK)) reference_tie_rank(d->positions, d->indices, c->nt, c->rank);
     Bvh4 bvh;
-    build_bvh4(d->positions, d->indices, c->nt, c->rank, bvh);
-    if (3 * bvh.max_depth + 1 > CPT_LDS_SS + CPT_GSTACK)
-        return fail(c, CPT_E_LIMIT, "BVH depth %d exceeds the traversal stack", bvh.max_depth);
-    c->bvh_nodes = (int64_t)bvh.nodes.size();
-    c->bvh_leaves = bvh.leaves;
-    c->bvh_depth = bvh.max_depth;
+    const bool gpu_bvh = (d->flags & CPT_SCENE_GPU_BVH) && c->nt > 0;
+    if (!gpu_bvh) {
+        auto tb0 = std::chrono::steady_clock::now();
+        build_bvh4(d->positions, d->indices, c->nt, c->rank, bvh);
+        c->bvh_build_us = (int64_t)std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - tb0).count();
+        if (3 * bvh.max_depth + 1 > CPT_LDS_SS + CPT_GSTACK)
+            return fail(c, CPT_E_LIMIT, "BVH depth %d exceeds the traversal stack", bvh.max_depth);
+        c->bvh_nodes = (int64_t)bvh.nodes.size();
+        c->bvh_leaves = bvh.leaves;
+        c->bvh_depth = bvh.max_depth;
+    }
+    c->bvh_gpu = gpu_bvh;
 
     for (int k = 0; k < 3; k++) {
         c->bmin[k] = INFINITY;
@@ -1601,8 +1617,10 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
         std::memcpy(&sh[6 * (size_t)t], v, sizeof(v));
     }
     int r = 0;
-    r |= dalloc(c, c->d_nodes, bvh.nodes.size());
-    r |= dalloc(c, c->d_tris, 3 * std::max<size_t>(1, bvh.tris.size()));
+    if (!gpu_bvh) {
+        r |= dalloc(c, c->d_nodes, bvh.nodes.size());
+        r |= dalloc(c, c->d_tris, 3 * std::max<size_t>(1, bvh.tris.size()));
+    }
     r |= dalloc(c, c->d_tri_info, ti.size());
     r |= dalloc(c, c->d_shade, sh.size());
     r |= dalloc(c, c->d_vpos, vp.size());
@@ -1616,9 +1634,11 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     r |= dalloc(c, c->d_lights, std::max(1, c->nl));
     r |= dalloc(c, c->d_plights, 2 * (size_t)std::max(1, d->n_point_lights));
     if (r) return CPT_E_HIP;
-    HIPCHK(c, hipMemcpy(c->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4), hipMemcpyHostToDevice));
-    if (!bvh.tris.empty())
-        HIPCHK(c, hipMemcpy(c->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRec), hipMemcpyHostToDevice));
+    if (!gpu_bvh) {
+        HIPCHK(c, hipMemcpy(c->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node4), hipMemcpyHostToDevice));
+        if (!bvh.tris.empty())
+            HIPCHK(c, hipMemcpy(c->d_tris, bvh.tris.data(), bvh.tris.size() * sizeof(TriRec), hipMemcpyHostToDevice));
+    }
     HIPCHK(c, hipMemcpy(c->d_tri_info, ti.data(), ti.size() * sizeof(int4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_shade, sh.data(), sh.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_vpos, vp.data(), vp.size() * sizeof(float4), hipMemcpyHostToDevice));
@@ -1642,6 +1662,27 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     }
     if (c->nl > 0)
         HIPCHK(c, hipMemcpy(c->d_lights, d->light_triangles, sizeof(int) * (size_t)c->nl, hipMemcpyHostToDevice));
+    if (gpu_bvh) {
+        int32_t* d_rank = nullptr;
+        if (!c->rank.empty()) {
+            if (dalloc(c, d_rank, c->rank.size())) return CPT_E_HIP;
+            HIPCHK(c, hipMemcpy(d_rank, c->rank.data(), sizeof(int32_t) * c->rank.size(), hipMemcpyHostToDevice));
+        }
+        GpuBvh gb;
+        std::string berr;
+        hipError_t be = build_bvh4_gpu(c->d_vpos, c->d_tri_info, d_rank, c->nt, CPT_GPU_BVH_LEAF, CPT_GPU_BVH_RADIUS,
+                                       c->stream, &gb, &berr);
+        dfree(d_rank);
+        if (be != hipSuccess) return fail(c, CPT_E_HIP, "%s", berr.c_str());
+        c->d_nodes = gb.nodes;
+        c->d_tris = reinterpret_cast<float4*>(gb.tris);
+        c->bvh_nodes = gb.n_nodes;
+        c->bvh_leaves = gb.leaves;
+        c->bvh_depth = gb.max_depth;
+        c->bvh_build_us = (int64_t)(gb.ms * 1000.0);
+        if (3 * gb.max_depth + 1 > CPT_LDS_SS + CPT_GSTACK)
+            return fail(c, CPT_E_LIMIT, "BVH depth %d exceeds the traversal stack", gb.max_depth);
+    }
     if (d->n_point_lights > 0) {
         std::vector<float4> pl(2 * (size_t)d->n_point_lights);
         for (int k = 0; k < d->n_point_lights; k++) {
@@ -1663,6 +1704,8 @@ int cpt_bvh_info(cpt_ctx* c, int64_t* info) {
     info[1] = c->bvh_leaves;
     info[2] = c->bvh_depth;
     info[3] = c->nt;
+    info[4] = c->bvh_build_us;
+    info[5] = c->bvh_gpu ? 1 : 0;
     return CPT_OK;
 }
 

@@ -143,7 +143,9 @@ struct TravStack {
 };
 
 // Reference triangle test (triangle.h:36-77).  Returns true with t, beta, gamma
-// when t in [EPS, tmax] and all barycentrics in [0,1].
+// when t in [EPS, tmax] and all barycentrics in [0,1].  A ray in the triangle's
+// plane (determinant 0, t = 0 * inf = NaN) misses; the reference would accept it
+// with a NaN t that turns off every later comparison and all box culling.
 __device__ __forceinline__ bool tri_test(const float4& A, const float4& E1, const float4& E2, V3 o, V3 d, float tmax,
                                          float& t, float& beta, float& gamma) {
     V3 v1 = ld3(E1), v2 = ld3(E2);
@@ -154,7 +156,7 @@ __device__ __forceinline__ bool tri_test(const float4& A, const float4& E1, cons
     float inv = 1.0f / den;
     t = (v1.x * v2.y * vs.z + v2.x * vs.y * v1.z + vs.x * v1.y * v2.z - vs.x * v2.y * v1.z - vs.y * v2.z * v1.x -
          vs.z * v2.x * v1.y) * inv;
-    if (t < CPT_EPS_T || t > tmax) return false;
+    if (!(t >= CPT_EPS_T && t <= tmax)) return false;
     beta = (vs.x * v2.y * v3_.z + v2.x * v3_.y * vs.z + v3_.x * vs.y * v2.z - v3_.x * v2.y * vs.z -
             v3_.y * v2.z * vs.x - v3_.z * v2.x * vs.y) * inv;
     gamma = (v1.x * vs.y * v3_.z + vs.x * v3_.y * v1.z + v3_.x * v1.y * vs.z - v3_.x * vs.y * v1.z -

@@ -103,6 +103,10 @@ typedef struct {
 /* skip computing the reference BVH's test order (used only to break exact-t ties
  * the way the reference does; without it ties go to the lowest triangle id) */
 #define CPT_SCENE_NO_TIE_RANK 1
+/* build the BVH on the device (PLOC clustering of Morton-sorted triangles,
+ * collapsed to the same 4-wide layout) instead of the host's binned SAH build;
+ * intersections are identical, only build time and traversal speed differ */
+#define CPT_SCENE_GPU_BVH 2
 
 /* One render.  Replaces PathtracerMIS::render (pathtracerMIS.h:135-167):
  * width x height film, spp_axis^2 stratified samples per pixel (the settings
@@ -180,8 +184,9 @@ int cpt_version(void);
  * LightGroup population (loader.h:362-364). */
 int cpt_upload_scene(cpt_ctx* ctx, const cpt_scene_desc* desc);
 
-/* BVH statistics of the uploaded scene: [nodes, leaves, max depth, triangles] */
-int cpt_bvh_info(cpt_ctx* ctx, int64_t* info4);
+/* BVH statistics of the uploaded scene (6 entries): [nodes, leaves, max depth,
+ * triangles, build time in microseconds, 1 if built on the device else 0] */
+int cpt_bvh_info(cpt_ctx* ctx, int64_t* info6);
 /* The reference BVH's test order of every triangle (n_triangles int32). */
 int cpt_tie_rank(cpt_ctx* ctx, int32_t* rank_out);
 

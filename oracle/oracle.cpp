@@ -339,6 +339,10 @@ struct Hit {
 
 // triangle.h:18-77 / 117-177: Cramer's rule on [b-a, c-a, -d].  Returns
 // true and (t, beta, gamma) when t in [EPS, max_t] and barycentrics in [0,1].
+// Deviation shared with the library: a ray lying in the triangle's plane
+// (determinant 0, t = 0 * inf = NaN) misses.  The reference takes it as a hit
+// whose NaN max_t then lets every later triangle through, so its answer would
+// depend on the test order of the whole scene.
 inline bool tri_test(const Scene& s, int id, const Ray& r, float& t, float& beta, float& gamma) {
     const auto& f = s.F[id];
     V3 a = s.P[f[0]], b = s.P[f[1]], c = s.P[f[2]];
@@ -349,7 +353,7 @@ inline bool tri_test(const Scene& s, int id, const Ray& r, float& t, float& beta
                                        v3.x * v2.y * v1.z - v3.y * v2.z * v1.x - v3.z * v2.x * v1.y));
     t = (v1.x * v2.y * vs.z + v2.x * vs.y * v1.z + vs.x * v1.y * v2.z - vs.x * v2.y * v1.z -
          vs.y * v2.z * v1.x - vs.z * v2.x * v1.y) * inv;
-    if ((double)t < EPSD || t > r.max_t) return false;
+    if (!((double)t >= EPSD && t <= r.max_t)) return false;
     beta = (vs.x * v2.y * v3.z + v2.x * v3.y * vs.z + v3.x * vs.y * v2.z - v3.x * v2.y * vs.z -
             v3.y * v2.z * vs.x - v3.z * v2.x * vs.y) * inv;
     gamma = (v1.x * vs.y * v3.z + vs.x * v3.y * v1.z + v3.x * v1.y * vs.z - v3.x * vs.y * v1.z -

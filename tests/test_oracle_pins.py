@@ -403,3 +403,23 @@ def test_point_light_direct_lighting_closed_form(integrator):
         assert rel.max() < 1e-3 and rel.mean() < 2e-4, (c, rel.max(), rel.mean())
     if integrator == 0:  # the weight is far from 1 on this geometry
         assert (1.0 - want / want_unweighted).max() > 0.2
+
+
+def test_ray_in_a_triangle_plane_misses():
+    """A ray lying in a triangle's plane has determinant 0 and t = 0 * inf = NaN:
+    the oracle (like the library) reports a miss, and the reference BVH agrees
+    with brute force on coincident triangles hit by such rays."""
+    s = cpt.scenes.Scene()
+    m = s.add_material(cpt.scenes.material(cpt.scenes.MAT_DIFFUSE, albedo=(0.5,) * 3))
+    P = np.tile(np.array([[-0.5, -0.5, 0], [0.5, -0.5, 0], [0, 0.5, 0]], np.float32), (40, 1))
+    s.add_mesh(P, np.tile(np.float32([0, 0, 1]), (120, 1)), np.tile(np.float32([1, 0, 0]), (120, 1)),
+               np.arange(120, dtype=np.int32).reshape(-1, 3), m)
+    o = oracle.OracleScene(s)
+    rays = np.array([[-2.0, 0.0, 0.0, 1.0, 0.0, 0.0, np.inf],    # in the plane, through the triangles
+                     [0.0, -2.0, 0.0, 0.0, 1.0, 0.0, np.inf],
+                     [0.1, 0.0, 2.0, 0.0, 0.0, -1.0, np.inf]], np.float32)  # a proper hit
+    tri, t = o.trace_closest(rays)
+    btri, _ = o.trace_closest(rays, brute=True)
+    assert tri[0] == -1 and tri[1] == -1
+    assert tri[2] == 39 and abs(t[2] - 2.0) < 1e-6   # the last of 40 coincident triangles
+    assert np.array_equal(tri, btri)
