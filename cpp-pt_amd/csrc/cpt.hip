@@ -120,7 +120,7 @@ struct WfArgs {
     int guide_depth;         // vertices at depth >= this are not guided
     int guide_target;        // 0: train on L_i f cos, 1: on L_i
     unsigned refill;         // idle lanes that make a traversal wave fetch new rays
-    int integrator;          // 0 PathtracerMIS, 1 Pathtracer (NEE only), 2 PurePt, 3 PPG
+    int integrator;          // 0 PathtracerMIS, 1 Pathtracer (NEE only), 2 PurePt, 3 PPG, 4 Lighttracer
     // SD-tree guided arm (PPG): the tree, per path its vertices (origin, direction,
     // radiance factor: 9 floats each, max_depth per path) and vertex count
     SdDev sdt;
@@ -262,6 +262,49 @@ __global__ void k_raygen(WfArgs a) {
         a.meta[i] = 1u << 8;  // depth 0, sampled_delta = true (camera ray)
         if (i < a.n_rec)
             for (int k = 0; k < a.rec_depth; k++) a.rec[((size_t)i * a.rec_depth + k) * REC_FLOATS + 17] = 0.0f;
+    }
+}
+
+// Light paths (Lighttracer, lighttracer.h:27-35,138-148): one per camera sample
+// of the view, with the same per-path stream; no film coordinates are drawn.
+// Paths whose start fails get an empty ray (tmax -1) that ends at the first step.
+__global__ void k_raygen_lt(WfArgs a) {
+    const PassDesc* pd = a.pass;
+    const unsigned n_paths = pd->n_paths;
+    if (gtid() == 0) {
+        a.counters[0] = n_paths;
+        a.counters[1] = 0;
+        a.counters[2] = 0;
+        a.counters[3] = 0;
+        a.counters[4] = 0;
+        atomicAdd(&a.stats[ST_PATHS], (unsigned long long)n_paths);
+    }
+    for (unsigned i = gtid(); i < n_paths; i += gthreads()) {
+        const Segment sg = pd->seg[find_segment(pd, i, false)];
+        const ViewDesc& vd = a.views[sg.view];
+        unsigned local = i - sg.slot0;
+        unsigned sl = local / sg.npix, pl = local - sl * sg.npix;
+        unsigned pixel = sg.p0 + pl;
+        unsigned sample = sg.s0 + sl;
+        uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)vd.spp + sample;
+        uint64_t rng = pcg_init(a.seed, path_id);
+        V3 o, d, rad;
+        float pdf;
+        V3 mulv = {0.0f, 0.0f, 0.0f};
+        float tmax = INFINITY;
+        if (light_start(a.sc, rng, o, d, rad, pdf)) {
+            mulv = divs(rad, pdf);
+        } else {
+            o = v3(0.0f, 0.0f, 0.0f);
+            d = v3(1.0f, 0.0f, 0.0f);
+            tmax = -1.0f;
+        }
+        a.qo[0][i] = make_float4(o.x, o.y, o.z, tmax);
+        a.qd[0][i] = make_float4(d.x, d.y, d.z, __int_as_float((int)i));
+        a.mul_p[i] = make_float4(mulv.x, mulv.y, mulv.z, 1.0f);
+        a.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        a.rng[i] = rng;
+        a.meta[i] = 0u;
     }
 }
 
@@ -620,6 +663,102 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
     }
 }
 
+// ------------------------------------------------------------------ light tracer
+// Lighttracer::integrate (lighttracer.h:37-110), one loop iteration per queue
+// entry: emitters end the path; after a non-delta BSDF sample the vertex is
+// connected to the camera (shadow ray from the hit point itself, to the camera
+// point, contribution mul * f(normalized wo, normalized wi) * (1/spp) * the
+// camera factor, splatted by k_shadow); Russian roulette from depth 3 on; the
+// next ray leaves from the hit point itself (the reference offsets neither ray).
+template <int B>
+__global__ __launch_bounds__(B) void k_shade_lt(WfArgs a, int cur, int depth_cap) {
+    __shared__ unsigned qsh[2][2 * (B / 64) + 2];
+    const unsigned count = a.counters[2 * cur];
+    unsigned long long* const out_ctr = reinterpret_cast<unsigned long long*>(a.counters + 2 * (cur ^ 1));
+    const ViewDesc& vd = a.views[0];
+    const float factor = (float)(1.0 / (double)(vd.S * vd.S));
+    int par = 0;
+    for (unsigned base = blockIdx.x * B; base < count; base += gridDim.x * B, par ^= 1) {
+        const unsigned i = base + threadIdx.x;
+        const bool active = i < count;
+        bool push_ext = false, push_sh = false;
+        V3 nd = {0, 0, 0}, ho = {0, 0, 0}, shd = {0, 0, 0}, contrib = {0, 0, 0};
+        float shmax = 0.0f;
+        int slot = 0, pix = 0;
+        if (active) {
+            float4 O = a.qo[cur][i], D = a.qd[cur][i], HH = a.hits[i];
+            slot = __float_as_int(D.w);
+            V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
+            float4 MP = a.mul_p[slot];
+            uint64_t rng = a.rng[slot];
+            const int depth = (int)(a.meta[slot] & 0xffu);
+            V3 mulv = v3(MP.x, MP.y, MP.z);
+            Hit h;
+            h.t = HH.x;
+            h.beta = HH.y;
+            h.gamma = HH.z;
+            h.tri = __float_as_int(HH.w);
+            if (h.tri >= 0) {
+                Isect it;
+                int matid;
+                fill_isect(a.sc, o, d, h, it, matid);
+                Mat m = hit_mat(a.sc, matid, h);
+                if (depth == 0) first_hit_frame(m, it);
+                if (m.type != CPT_MAT_EMISSIVE) {
+                    const V3 wo = mulf(d, -1.0f);
+                    DSample ds = bsdf_sample(m, it, wo, rng);
+                    const V3 ev = bsdf_eval(m, it, wo, ds.wi);
+                    float p = ds.pdf;
+                    if (!ds.delta) {
+                        int ci, cj;
+                        float cf;
+                        camera_connect(vd.cam, vd.W, vd.H, it.hitpoint, ci, cj, cf);
+                        const V3 cp = v3(vd.cam[0], vd.cam[1], vd.cam[2]);
+                        const V3 sdir = normalized(sub(cp, it.hitpoint));
+                        if (bsdf_non_zero(m, it, wo, sdir) && ci > -1) {
+                            const V3 f = bsdf_eval(m, it, normalized(wo), normalized(sdir));
+                            contrib = mulf(mulf(mul(mulv, f), factor), cf);
+                            ho = it.hitpoint;
+                            shd = sdir;
+                            shmax = len3(sub(cp, it.hitpoint));
+                            pix = (int)(vd.film_off + (unsigned long long)cj * (unsigned)vd.W + (unsigned)ci);
+                            push_sh = true;
+                        }
+                    }
+                    bool survive = true;
+                    if (depth > 2) {
+                        const float rr = russian_roulette(ev);
+                        if (pcg_float(rng) > rr) survive = false;
+                        else p = p * rr;
+                    }
+                    if (survive && depth + 1 < depth_cap) {
+                        mulv = divs(mul(mulv, ev), p);
+                        ho = it.hitpoint;
+                        nd = ds.wi;
+                        push_ext = true;
+                    }
+                }
+            }
+            if (push_ext) {
+                a.mul_p[slot] = make_float4(mulv.x, mulv.y, mulv.z, 1.0f);
+                a.rng[slot] = rng;
+                a.meta[slot] = (uint32_t)(depth + 1);
+            }
+        }
+        unsigned qe, qs;
+        block_alloc2<B>(push_ext, push_sh, out_ctr, qsh[par], qe, qs);
+        if (push_ext) {
+            a.qo[cur ^ 1][qe] = make_float4(ho.x, ho.y, ho.z, INFINITY);
+            a.qd[cur ^ 1][qe] = make_float4(nd.x, nd.y, nd.z, __int_as_float(slot));
+        }
+        if (push_sh) {
+            a.so[qs] = make_float4(ho.x, ho.y, ho.z, shmax);
+            a.sd[qs] = make_float4(shd.x, shd.y, shd.z, __int_as_float(pix));
+            a.sc_rgb[qs] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ SD-tree guided arm
 // PPG::integrate (ppg.h:41-143), one loop iteration per queue entry: no NEE;
 // with probability 1/2 the BSDF, else the SD-tree; one-sample MIS weight
@@ -773,8 +912,20 @@ __global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_shadow(WfArgs a, int nxt)
         [&](unsigned i, const Hit& h) {
             if (h.tri < 0) {
                 int slot = __float_as_int(a.sd[i].w);
-                float4 c = a.col[slot], add_ = a.sc_rgb[i];
-                a.col[slot] = make_float4(c.x + add_.x, c.y + add_.y, c.z + add_.z, 0.0f);
+                float4 add_ = a.sc_rgb[i];
+                if (a.integrator == 4) {
+                    // light tracing: the camera sees the vertex; splat into the film
+                    // pixel (the reference adds to RgbImage::get_pixel, lighttracer.h:77-91).
+                    // The film holds spp times the image, as for camera paths.
+                    const double sc = (double)a.views[0].spp;
+                    double* f = a.film + 3 * (size_t)(unsigned)slot;
+                    unsafeAtomicAdd(f, (double)add_.x * sc);
+                    unsafeAtomicAdd(f + 1, (double)add_.y * sc);
+                    unsafeAtomicAdd(f + 2, (double)add_.z * sc);
+                } else {
+                    float4 c = a.col[slot];
+                    a.col[slot] = make_float4(c.x + add_.x, c.y + add_.y, c.z + add_.z, 0.0f);
+                }
             }
         },
         nv, nt);
@@ -1294,6 +1445,7 @@ struct LaunchCfg {
     // second round of waves would double the time of the CUs that run it.
     int GE = 0, GS = 0;
     bool ppg = false;  // SD-tree guided arm: k_shade_ppg, no shadow rays
+    bool lt = false;   // light tracer: k_raygen_lt, k_shade_lt, splatting shadow rays
 };
 
 template <int B>
@@ -1337,6 +1489,11 @@ void L_extend(const LaunchCfg& L, const WfArgs& a, int cur, hipStream_t s) {
     }
 }
 void L_shade(const LaunchCfg& L, const WfArgs& a, int cur, int cap, hipStream_t s) {
+    if (L.lt) {
+        const unsigned G = (unsigned)std::max(1, (int)(((long long)L.G * L.B) / 256));
+        hipLaunchKernelGGL((k_shade_lt<256>), dim3(G), dim3(256), 0, s, a, cur, cap);
+        return;
+    }
     if (L.ppg) {
         const unsigned G = (unsigned)std::max(1, (int)(((long long)L.G * L.B) / 256));
         hipLaunchKernelGGL((k_shade_ppg<256>), dim3(G), dim3(256), 0, s, a, cur, cap);
@@ -1374,7 +1531,8 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
         tm->kind.push_back(kind);
     };
     mark(-1);
-    hipLaunchKernelGGL(k_raygen, dim3(L.G), dim3(L.B), 0, s, a);
+    if (L.lt) hipLaunchKernelGGL(k_raygen_lt, dim3(L.G), dim3(L.B), 0, s, a);
+    else hipLaunchKernelGGL(k_raygen, dim3(L.G), dim3(L.B), 0, s, a);
     mark(3);
     for (int b = 0; b < a.max_depth; b++) {
         int cur = b & 1;
@@ -1929,12 +2087,16 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
                         int SB, int refill, int64_t max_paths, double* view_ms, cpt_render_stats* stats) {
     if (max_depth <= 0 || max_depth > 255) return fail(c, CPT_E_INVALID, "max_depth must be in [1, 255]");
     if (refill < 1 || refill > 64) return fail(c, CPT_E_INVALID, "refill must be in [1, 64]");
-    const int integrator = (flags & CPT_RENDER_PPG_STEP) ? 3
+    const int integrator = (flags & CPT_RENDER_LIGHT_TRACE) ? 4
+                           : (flags & CPT_RENDER_PPG_STEP) ? 3
                            : (flags & CPT_RENDER_PURE_PT) ? 2
                            : (flags & CPT_RENDER_NEE_ONLY) ? 1
                                                             : 0;
-    if (__builtin_popcount(flags & (CPT_RENDER_PURE_PT | CPT_RENDER_NEE_ONLY | CPT_RENDER_PPG_STEP)) > 1)
-        return fail(c, CPT_E_INVALID, "CPT_RENDER_NEE_ONLY, CPT_RENDER_PURE_PT and CPT_RENDER_PPG exclude each other");
+    if (__builtin_popcount(flags & (CPT_RENDER_PURE_PT | CPT_RENDER_NEE_ONLY | CPT_RENDER_PPG_STEP |
+                                    CPT_RENDER_LIGHT_TRACE)) > 1)
+        return fail(c, CPT_E_INVALID,
+                    "CPT_RENDER_NEE_ONLY, CPT_RENDER_PURE_PT, CPT_RENDER_PPG and CPT_RENDER_LIGHT_TRACE exclude each other");
+    if (integrator == 4 && vd.size() != 1) return fail(c, CPT_E_INVALID, "the light tracer renders one view at a time");
     if (integrator && (flags & CPT_RENDER_GUIDED))
         return fail(c, CPT_E_INVALID, "the learned guide uses the MIS integrator");
     if (B != 64 && B != 128 && B != 256) return fail(c, CPT_E_INVALID, "block_size must be 64, 128 or 256");
@@ -2064,6 +2226,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     L.GS = GS;
     L.guided = guided;
     L.ppg = integrator == 3;
+    L.lt = integrator == 4;
     L.infer_blocks = c->cus;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
     hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, a, (int)passes.size());
@@ -2172,6 +2335,7 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
     if (p->width <= 0 || p->height <= 0 || p->spp_axis <= 0)
         return fail(c, CPT_E_INVALID, "bad render size or spp");
     if (p->flags & CPT_RENDER_STREAM) {
+        if (p->flags & CPT_RENDER_LIGHT_TRACE) return fail(c, CPT_E_INVALID, "the light tracer does not stream");
         int32_t whs[3] = {p->width, p->height, p->spp_axis};
         return render_views_stream(c, 1, c->cam, whs, p->max_depth, p->seed, p->flags, nullptr, nullptr, stats);
     }

@@ -394,6 +394,96 @@ __device__ __forceinline__ float russian_roulette(V3 c) {  // pathtracerMIS.h:24
     return (float)(fabs((double)c.x * 0.2 + (double)c.y * 0.5 + (double)c.z * 0.3) * 0.5 + 0.4);
 }
 
+// Start of a light path (Lighttracer::integrate, lighttracer.h:27-35):
+// LightGroup::sample (light_group.h:33-39) picks a light; an emitter triangle
+// gives ShapeLight::sample (shape_light.h:50-68): Triangle::sample's point,
+// normal, bitangent and tangent (triangle.h:209-269), then
+// EmissiveBxDF::emit_sample (emissive_bxdf.h:36-55), a cosine direction about the
+// normal or, with probability 1/2, about the flipped normal and tangent, pdf
+// 0.5 cos / pi; a point light gives PointLight::sample (point_light.h:30-37) with
+// sample_sphere (math.h:655-660), whose acos argument leaves [-1, 1] for most
+// draws (those paths carry no light), pdf 1 / (4 pi^2).  The two draws of
+// sample_sphere's arguments are taken in source order.  Returns false when the
+// direction is not a number or the scene has no light.
+__device__ __forceinline__ bool light_start(const DevScene& sc, uint64_t& rng, V3& o, V3& d, V3& rad, float& pdf) {
+    const unsigned NL = (unsigned)sc.n_lights;
+    if (NL == 0) return false;
+    const int pick = (int)((unsigned)(int)(pcg_float(rng) * (float)NL) % NL);
+    const int id = sc.lights[pick];
+    if (id < 0) {
+        const float4 p = sc.plights[2 * (-1 - id)], e = sc.plights[2 * (-1 - id) + 1];
+        const float r1 = pcg_float(rng);
+        const float r2 = pcg_float(rng);
+        const float theta = (float)(2.0 * CPT_M_PI * (double)r1);
+        const float phi = (float)acos(1.0 - 2.0 * CPT_M_PI * (double)r2);
+        double st, ct, sp, cp;
+        sincos((double)theta, &st, &ct);
+        sincos((double)phi, &sp, &cp);
+        d = {(float)(sp * ct), (float)(sp * st), (float)cp};
+        o = {p.x, p.y, p.z};
+        rad = {e.x, e.y, e.z};
+        pdf = (float)(1.0 / (4.0 * CPT_M_PI * CPT_M_PI));
+    } else {
+        const float r1 = pcg_float(rng);
+        const float r2 = pcg_float(rng);
+        float alpha = r1, beta = r2;
+        if ((double)(alpha + beta) > 1.0) {
+            alpha = 1.0f - alpha;
+            beta = 1.0f - beta;
+        }
+        const float gamma = (float)(1.0 - (double)alpha - (double)beta);
+        const int4 ti = sc.tri_info[id];
+        const V3 a = ld3(sc.vpos[ti.x]), b = ld3(sc.vpos[ti.y]), c = ld3(sc.vpos[ti.z]);
+        const V3 n0 = ld3(sc.vnrm[ti.x]), n1 = ld3(sc.vnrm[ti.y]), n2 = ld3(sc.vnrm[ti.z]);
+        const V3 b0 = ld3(sc.vbit[ti.x]), b1 = ld3(sc.vbit[ti.y]), b2 = ld3(sc.vbit[ti.z]);
+        o = add(add(mulf(a, alpha), mulf(b, beta)), mulf(c, gamma));
+        const V3 n = normalized(add(add(mulf(n0, alpha), mulf(n1, beta)), mulf(n2, gamma)));
+        const V3 bt = normalized(add(add(mulf(b0, alpha), mulf(b1, beta)), mulf(b2, gamma)));
+        const V3 t = normalized(cross(bt, n));
+        const float e1 = pcg_float(rng);
+        const float e2 = pcg_float(rng);
+        const V3 hs = sample_hemisphere_cos(e1, e2);
+        const float p = (float)(0.5 * (double)hs.z / CPT_M_PI);
+        const float side = pcg_float(rng);
+        V3 nn = n, tt = t;
+        if (side < 0.5f) {
+            nn = negv(n);
+            tt = negv(t);
+        }
+        d = mat_cols_mul(tt, bt, nn, hs);
+        const float4 mb = reinterpret_cast<const float4*>(sc.mats + 16 * ti.w)[1];
+        rad = mulf(v3(mb.y, mb.z, mb.w), fabsf(dot(n, d)));
+        pdf = (float)((1.0 / (double)tri_area(sc, id)) * (double)p);
+    }
+    pdf = (float)((double)pdf * (1.0 / (double)(float)NL));
+    return !(isnan(d.x) || isnan(d.y) || isnan(d.z));
+}
+
+// CameraPerspective::connect_light_path (camera_perspective.h:46-82): the pixel a
+// point projects to and the importance factor 1 / (4 |cos^3| d^2 tan^2) (the
+// reference leaves the aspect ratio out).  ci = -1 when the point is behind the
+// camera, closer than EPS or outside the image.
+__device__ __forceinline__ void camera_connect(const float* cam, int W, int H, V3 p, int& ci, int& cj, float& factor) {
+    ci = cj = -1;
+    factor = 0.0f;
+    const V3 x = {cam[3], cam[4], cam[5]}, y = {cam[6], cam[7], cam[8]}, z = {cam[9], cam[10], cam[11]};
+    const V3 v = sub(p, v3(cam[0], cam[1], cam[2]));
+    V3 dir = {dot(x, v), dot(y, v), dot(z, v)};  // coords.transpose() * v
+    const float ndl = len3(dir);
+    if ((double)ndl < 1e-4 || dir.z >= 0.0f) return;
+    const V3 nd = normalized(divs(dir, ndl));
+    dir = mulf(divs(dir, dir.z), -1.0f);
+    dir.x = dir.x / (cam[12] * cam[13]);
+    dir.y = dir.y / cam[12];
+    const double fx = ((double)dir.x * 0.5 + 0.5) * (double)W;
+    const double fy = ((double)dir.y * 0.5 + 0.5) * (double)H;
+    if (!(fx >= 0.0 && fx < (double)W && fy >= 0.0 && fy < (double)H)) return;
+    ci = (int)fx;
+    cj = H - (int)fy - 1;
+    const float c3 = fabsf(nd.z * nd.z * nd.z);
+    factor = (float)(1.0 / (4.0 * (double)c3 * (double)ndl * (double)ndl * (double)cam[12] * (double)cam[12]));
+}
+
 // camera_perspective.h:30-39
 __device__ __forceinline__ void camera_ray(const float* cam, float cx, float cy, V3& o, V3& d) {
     V3 dir;

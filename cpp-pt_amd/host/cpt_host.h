@@ -175,6 +175,13 @@ public:
     explicit PurePtGPU(const RenderSettings& rs) : PathtracerMISGPU(rs, CPT_RENDER_PURE_PT) {}
 };
 
+// The reference's Lighttracer (src/renderer/lighttracer.h:15-154): paths from the
+// lights, every non-delta vertex connected to the camera.
+class LighttracerGPU : public PathtracerMISGPU {
+public:
+    explicit LighttracerGPU(const RenderSettings& rs) : PathtracerMISGPU(rs, CPT_RENDER_LIGHT_TRACE) {}
+};
+
 // The reference's PPG (src/renderer/ppg.h:17-214): SD-tree guided, iterations of
 // 1, 2, 4, ... spp, the image of the last one.
 class PpgGPU : public PathtracerMISGPU {

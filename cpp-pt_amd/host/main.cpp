@@ -2,7 +2,7 @@
 // (src/main.cpp:89-146): argv[1] = settings file, renderer chosen by the
 // settings token, wall time of render() printed as "<seconds>s", runtime
 // errors printed.  Renderer tokens `pathtracer_mis` / `pathtracer_mis_gpu`,
-// `pathtracer` / `pathtracer_gpu`, `pure_pt` / `pure_pt_gpu` and `ppg` select the
+// `pathtracer` / `pathtracer_gpu`, `pure_pt` / `pure_pt_gpu`, `ppg` and `lighttracer` select the
 // GPU integrators; the reference's other algorithms are not part of this build
 // and are reported as such.
 #include <chrono>
@@ -33,8 +33,11 @@ int main(int argc, char** argv) {
             renderer = std::make_unique<PurePtGPU>(rs);
         } else if (rs.renderer == RenderSettings::PPG) {
             renderer = std::make_unique<PpgGPU>(rs);
+        } else if (rs.renderer == RenderSettings::LIGHTTRACER) {
+            renderer = std::make_unique<LighttracerGPU>(rs);
         } else {
-            throw std::runtime_error("renderer not provided by this build (pathtracer_mis, pathtracer, pure_pt, ppg)");
+            throw std::runtime_error(
+                "renderer not provided by this build (pathtracer_mis, pathtracer, pure_pt, ppg, lighttracer)");
         }
         auto t1 = std::chrono::high_resolution_clock::now();
         renderer->render(s, rs.output_name);

@@ -149,6 +149,12 @@ typedef struct {
 #define CPT_RENDER_PPG_STEP 1024    /* one PPG iteration with the current SD-tree (ppg.h:41-143):
                                        spp_axis = samples per pixel (whole-pixel jitter); splats
                                        accumulate until cpt_ppg_update */
+#define CPT_RENDER_LIGHT_TRACE 2048 /* Lighttracer (src/renderer/lighttracer.h:17-154): width x height x
+                                       spp_axis^2 paths start on the lights, every non-delta vertex
+                                       is connected to the camera and splatted into the film (the
+                                       reference's image, aspect factor left out as it does);
+                                       max_depth bounces (the reference fixes 32); no sums of
+                                       squares; one view, no streaming */
 
 typedef struct {
     double seconds;           /* wall time of the render call (device synchronised) */

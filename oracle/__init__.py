@@ -36,6 +36,8 @@ def lib():
     L.orc_hit_material.argtypes = [vp, vp, i32, vp]
     L.orc_image_texel.restype = None
     L.orc_image_texel.argtypes = [vp, i32, i32, i32, vp]
+    L.orc_render_lt.restype = i32
+    L.orc_render_lt.argtypes = [vp, i32, i32, i32, i32, i32, C.c_uint64, vp, i32, vp]
     L.orc_scene_set_point_lights.restype = None
     L.orc_scene_set_point_lights.argtypes = [vp, i32, vp]
     L.orc_scene_set_textures.restype = None
@@ -189,6 +191,14 @@ class OracleScene:
         lib().orc_render(self.h, W, H, spp_axis, max_depth, 0 if sampler == "pcg" else 1, seed,
                          c0, c1, r0, r1, _p(sums), _p(sq), _p(film), threads, _p(counts))
         return (sums, film, counts, sq) if sumsq else (sums, film, counts)
+
+    def render_lt(self, W, H, spp_axis, max_depth=32, sampler="pcg", seed=1, threads=0):
+        """Lighttracer: returns (film sums (H,W,3) float64 = spp x the image, counts)."""
+        sums = np.zeros((H, W, 3), np.float64)
+        counts = np.zeros(3, np.int64)
+        lib().orc_render_lt(self.h, W, H, spp_axis, max_depth, 0 if sampler == "pcg" else 1, seed, _p(sums),
+                            threads, _p(counts))
+        return sums, counts
 
     def ppg_render(self, W, H, spp, max_depth, seed, view, tree, threads=0):
         """One PPG iteration with the fixed SD-tree `tree` (a dict as returned by

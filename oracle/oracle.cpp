@@ -1026,6 +1026,141 @@ V3 ppg_integrate(const Scene& s, const SdTreeView& t, float cx, float cy, Sample
     return col;
 }
 
+// ---------------------------------------------------------------- light tracer
+struct LightStart {
+    bool ok = false;
+    V3 o, d, rad;
+    float pdf = 0;
+};
+
+// LightGroup::sample (light_group.h:33-39) over ShapeLight::sample
+// (shape_light.h:50-68: Triangle::sample, triangle.h:209-269, then
+// EmissiveBxDF::emit_sample, emissive_bxdf.h:36-55) or PointLight::sample
+// (point_light.h:30-37 with sample_sphere, math.h:655-660, arguments drawn in
+// source order).  ok = false for a direction that is not a number (most point
+// light draws: acos of an argument below -1) or a scene without lights.
+LightStart light_start(const Scene& s, Sampler& smp) {
+    LightStart L;
+    const size_t NL = s.lights.size();
+    if (NL == 0) return L;
+    const int pick = (int)(int(smp.sample() * (float)NL) % NL);
+    const int id = s.lights[pick];
+    if (id < 0) {
+        const auto& pl = s.point_lights[-1 - id];
+        const float r1 = smp.sample();
+        const float r2 = smp.sample();
+        const float theta = (float)(2.0 * M_PI * (double)r1);
+        const float phi = (float)std::acos(1.0 - 2.0 * M_PI * (double)r2);
+        L.d = V3((float)(std::sin((double)phi) * std::cos((double)theta)),
+                 (float)(std::sin((double)phi) * std::sin((double)theta)), (float)std::cos((double)phi));
+        L.o = V3(pl[0], pl[1], pl[2]);
+        L.rad = V3(pl[3], pl[4], pl[5]);
+        L.pdf = (float)(1.0 / (4.0 * M_PI * M_PI));
+    } else {
+        const auto& f = s.F[id];
+        const float r1 = smp.sample();
+        const float r2 = smp.sample();
+        float alpha = r1, beta = r2;
+        if ((double)(alpha + beta) > 1.0) {
+            alpha = (float)(1.0 - (double)alpha);
+            beta = (float)(1.0 - (double)beta);
+        }
+        const float gamma = (float)(1.0 - (double)alpha - (double)beta);
+        const V3 a = s.P[f[0]], b = s.P[f[1]], c = s.P[f[2]];
+        L.o = a * alpha + b * beta + c * gamma;
+        const V3 n = normalized(s.N[f[0]] * alpha + s.N[f[1]] * beta + s.N[f[2]] * gamma);
+        const V3 bt = normalized(s.B[f[0]] * alpha + s.B[f[1]] * beta + s.B[f[2]] * gamma);
+        const V3 t = normalized(cross(bt, n));
+        const float e1 = smp.sample();
+        const float e2 = smp.sample();
+        const V3 hs = sample_hemisphere_cos(e1, e2);
+        const float p = (float)(0.5 * (double)hs.z / M_PI);
+        const float side = smp.sample();
+        V3 nn = n, tt = t;
+        if ((double)side < 0.5) {
+            nn = neg(n);
+            tt = neg(t);
+        }
+        L.d = M3(tt, bt, nn).mul(hs);
+        L.rad = s.mats[s.tri_mat[id]].emission * (float)std::fabs((double)dot(n, L.d));
+        L.pdf = (float)((1.0 / (double)tri_area(s, id)) * (double)p);
+    }
+    L.pdf = (float)((double)L.pdf * (1.0 / (double)(float)NL));
+    L.ok = !(std::isnan(L.d.x) || std::isnan(L.d.y) || std::isnan(L.d.z));
+    return L;
+}
+
+// CameraPerspective::connect_light_path (camera_perspective.h:46-82)
+struct CamConn {
+    int i = -1, j = -1;
+    float factor = 0;
+};
+CamConn camera_connect(const Camera& c, int W, int H, V3 p) {
+    CamConn cc;
+    V3 dir = c.coords.transpose().mul(p - c.o);
+    const float ndl = length(dir);
+    if ((double)ndl < EPSD || dir.z >= 0) return cc;
+    const V3 nd = normalized(divs(dir, ndl));
+    dir = divs(dir, dir.z) * (float)(-1.0);
+    dir.x = dir.x / (c.tan_fovy * c.aspect);
+    dir.y = dir.y / c.tan_fovy;
+    const double fx = ((double)dir.x * 0.5 + 0.5) * (double)W;
+    const double fy = ((double)dir.y * 0.5 + 0.5) * (double)H;
+    if (!(fx >= 0.0 && fx < (double)W && fy >= 0.0 && fy < (double)H)) return cc;  // int() then bounds
+    cc.i = (int)fx;
+    cc.j = H - (int)fy - 1;
+    cc.factor = (float)(1.0 / (4.0 * (double)std::fabs(nd.z * nd.z * nd.z) * (double)ndl * (double)ndl *
+                               (double)c.tan_fovy * (double)c.tan_fovy));
+    return cc;
+}
+
+// Lighttracer::integrate (lighttracer.h:27-110).  Splats `color * scale` into the
+// FP64 film (scale = spp, so that film / spp is the reference's image).
+void integrate_lt(const Scene& s, Sampler& smp, float factor, int W, int H, int max_depth, double scale, double* film,
+                  Counts* cnt) {
+    LightStart L = light_start(s, smp);
+    if (!L.ok) return;
+    Ray ray{L.o, L.d, std::numeric_limits<float>::infinity()};
+    V3 mul = divs(L.rad, L.pdf);
+    Isect it;
+    for (int i = 0; i < max_depth; i++) {
+        Hit h;
+        if (cnt) cnt->ext++;
+        if (!intersect(s, ray, h)) break;
+        fill_isect(s, ray, h, it);
+        const Material m = hit_material(s, it);
+        first_hit_frame(m, it);
+        if (m.type == 2) break;
+        const V3 wo = neg(ray.d);
+        DSample ds = bsdf_sample(m, it, wo, smp);
+        const V3 eval = bsdf_eval(m, it, wo, ds.wi);
+        float p = ds.pdf;
+        if (!ds.delta) {
+            CamConn cc = camera_connect(s.cam, W, H, it.hitpoint);
+            const V3 to = s.cam.o - it.hitpoint;
+            Ray sh{it.hitpoint, normalized(to), length(to)};
+            if (bsdf_non_zero(m, it, wo, sh.d) && cc.i > -1) {
+                if (cnt) cnt->shadow++;
+                if (!ref_any(s, s.root, sh)) {
+                    V3 color = mul * bsdf_eval(m, it, normalized(wo), normalized(sh.d)) * factor;
+                    color = color * cc.factor;
+                    double* f = film + 3 * ((size_t)cc.i + (size_t)cc.j * W);
+                    f[0] += (double)color.x * scale;
+                    f[1] += (double)color.y * scale;
+                    f[2] += (double)color.z * scale;
+                }
+            }
+        }
+        if (i > 2) {
+            float rr = russian_roulette(eval);
+            if (smp.sample() > rr) break;
+            p = p * rr;
+        }
+        mul = divs(mul * eval, p);
+        ray = Ray{it.hitpoint, ds.wi, std::numeric_limits<float>::infinity()};
+    }
+}
+
 // pathtracerMIS.h:150-153 film coordinates of stratum (k,l) of pixel (i,j)
 inline void film_coords(int i, int j, int k, int l, int S, int W, int H, float r1, float r2, float* cx,
                         float* cy) {
@@ -1307,6 +1442,49 @@ int orc_ppg_render(void* sp, int W, int H, int spp, int max_depth, uint64_t seed
 }
 
 void orc_set_integrator(void* sp, int integrator) { ((Scene*)sp)->integrator = integrator; }
+
+// Lighttracer::render (lighttracer.h:118-154): W*H*S*S light paths, one per camera
+// sample, each with factor 1/(S*S).  sampler 0: per-path PCG stream (path id as
+// for camera paths); 1: the reference's per-column mt19937 (RandomSampler(i)).
+int orc_render_lt(void* sp, int W, int H, int S, int max_depth, int sampler, uint64_t seed, double* sum_out,
+                  int nthreads, int64_t* counts) {
+    const Scene* s = (const Scene*)sp;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const float factor = (float)(1.0 / (double)(S * S));
+    const double scale = (double)(S * S);
+    std::fill(sum_out, sum_out + 3 * (size_t)W * H, 0.0);
+    int64_t paths = 0, ext = 0, shadow = 0;
+#pragma omp parallel reduction(+ : paths, ext, shadow)
+    {
+        std::vector<double> film(3 * (size_t)W * H, 0.0);
+        Counts cnt;
+#pragma omp for schedule(dynamic, 1)
+        for (int i = 0; i < W; i++) {
+            MtSampler mt(i + (int)(seed & 0x7fff) * W);
+            for (int j = 0; j < H; j++)
+                for (int k = 0; k < S; k++)
+                    for (int l = 0; l < S; l++) {
+                        if (sampler == 1) {
+                            integrate_lt(*s, mt, factor, W, H, max_depth, scale, film.data(), &cnt);
+                        } else {
+                            PcgSampler pcg(seed, (uint64_t)(i + (int64_t)j * W) * (uint64_t)(S * S) + (uint64_t)(k * S + l));
+                            integrate_lt(*s, pcg, factor, W, H, max_depth, scale, film.data(), &cnt);
+                        }
+                        paths++;
+                    }
+        }
+#pragma omp critical
+        for (size_t q = 0; q < film.size(); q++) sum_out[q] += film[q];
+        ext += cnt.ext;
+        shadow += cnt.shadow;
+    }
+    if (counts) {
+        counts[0] = paths;
+        counts[1] = ext;
+        counts[2] = shadow;
+    }
+    return 0;
+}
 
 void orc_path(void* sp, int W, int H, int S, int max_depth, uint64_t seed, int pixel, int sample, float* rgb) {
     const Scene* s = (const Scene*)sp;

@@ -50,6 +50,10 @@ void* orc_scene_create(int nv, const float* pos, const float* nrm, const float* 
  * constant).  A metal image makes the Lambertian/GGX choice per hit. */
 void orc_scene_set_textures(void* s, const float* uvs, int n_images, const int32_t* images, int n_texels,
                             const float* texels, const int32_t* material_textures);
+/* Lighttracer (lighttracer.h:17-154): W*H*S*S light paths splatted into sum_out
+ * (H x W x 3, spp times the image); sampler 0 per-path PCG, 1 per-column mt19937 */
+int orc_render_lt(void* s, int W, int H, int S, int max_depth, int sampler, uint64_t seed, double* sum_out,
+                  int nthreads, int64_t* counts);
 /* point lights (n x 6: position, radiance); light_tris entries -1-k refer to them */
 void orc_scene_set_point_lights(void* s, int n, const float* rec);
 /* per ray: the closest hit's material after its textures (10 floats: type or -1
