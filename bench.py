@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip cpt_autotune (default launch configuration)")
     ap.add_argument("--gpu-bvh", action="store_true", help="build the BVH on the device (CPT_SCENE_GPU_BVH)")
-    ap.add_argument("--workload", choices=["terrain", "views", "guided", "bvh"], default="terrain",
+    ap.add_argument("--workload", choices=["terrain", "views", "guided", "bvh", "box"], default="terrain",
                     help="terrain: the headline 1080p line; views: 64-view batch of the box room; "
                          "guided: equal-time relMSE of guided vs plain PT on the hard-lighting interior")
     ap.add_argument("--budgets", type=str, default="10,30,60", help="guided: equal-time budgets in seconds")
@@ -281,6 +281,48 @@ def bench_bvh(args):
         "host_sah": h, "device_ploc": d}))
 
 
+def bench_box(args):
+    """The reference's own CPU-runnable case: the 36-triangle box room, 512x512, 64 spp
+    (settings spp 8), max depth 8; also at the reference's fixed 32 bounces.  One
+    step = one 64-spp frame."""
+    import cpt
+
+    W = H = 512
+    S = 8
+    scene = cpt.scenes.box_room(res=(W, H))
+    ctx = cpt.Context(0)
+    ctx.upload(scene)
+    res = {}
+    for depth in (8, 32):
+        ctx.autotune(W, H, max_depth=depth)
+        for w in range(args.warmup):
+            ctx.render(W, H, S, max_depth=depth, seed=1000 + w)
+        t0 = time.perf_counter()
+        paths = rays = 0
+        for k in range(args.steps):
+            st = ctx.render(W, H, S, max_depth=depth, seed=1 + k, flags=cpt.CPT_RENDER_ACCUMULATE if k else 0)
+            paths += st["paths"]
+            rays += st["extension_rays"] + st["shadow_rays"]
+        total = time.perf_counter() - t0
+        res[depth] = {"samples_per_s": paths / total, "ms_per_step": total / args.steps * 1e3,
+                      "mrays_per_s": rays / total / 1e6}
+        print(f"bench: box room depth {depth}: {res[depth]}", file=sys.stderr, flush=True)
+    ctx.close()
+    out = {
+        "metric": "samples/s (box room, the reference's CPU-runnable case)",
+        "value": res[8]["samples_per_s"], "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res[8]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "box room, 36 triangles, 512x512, 64 spp per step, max depth 8"},
+        "mrays_per_s": res[8]["mrays_per_s"], "depth32": res[32],
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(scene, W, H, 8, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = out["value"] / cb["value"]
+    print(json.dumps(out))
+
+
 def cpu_baseline(scene, W, H, depth, target_s):
     import numpy as np
 
@@ -328,6 +370,9 @@ def main():
         return
     if args.workload == "bvh":
         bench_bvh(args)
+        return
+    if args.workload == "box":
+        bench_box(args)
         return
     import cpt
 

@@ -35,6 +35,7 @@ CPT_RENDER_PURE_PT = 256
 CPT_RENDER_PPG = 512
 CPT_RENDER_PPG_STEP = 1024
 CPT_RENDER_LIGHT_TRACE = 2048
+CPT_RENDER_CORRECTED = 4096
 
 # every function include/cpt.h declares
 ABI_FUNCTIONS = (

@@ -121,6 +121,7 @@ struct WfArgs {
     int guide_target;        // 0: train on L_i f cos, 1: on L_i
     unsigned refill;         // idle lanes that make a traversal wave fetch new rays
     int integrator;          // 0 PathtracerMIS, 1 Pathtracer (NEE only), 2 PurePt, 3 PPG, 4 Lighttracer
+    int corrected;           // CPT_RENDER_CORRECTED: the reference's biased quirks switched off
     // SD-tree guided arm (PPG): the tree, per path its vertices (origin, direction,
     // radiance factor: 9 floats each, max_depth per path) and vertex count
     SdDev sdt;
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
                     int matid;
                     fill_isect(a.sc, o, d, h, it, matid);
                     int slot = __float_as_int(D.w);
-                    if ((a.meta[slot] & 0xffu) == 0) first_hit_frame(hit_mat(a.sc, matid, h), it);
+                    if ((a.meta[slot] & 0xffu) == 0 || a.corrected) first_hit_frame(hit_mat(a.sc, matid, h), it);
                     guide::features(a.sc.bmin, a.sc.bext, it.hitpoint.x, it.hitpoint.y, it.hitpoint.z, it.normal.x,
                                     it.normal.y, it.normal.z, -d.x, -d.y, -d.z, in);
                 }
@@ -535,7 +536,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                 int matid;
                 fill_isect(a.sc, o, d, h, it, matid);
                 Mat m = hit_mat(a.sc, matid, h);
-                if (depth == 0) first_hit_frame(m, it);
+                if (depth == 0 || a.corrected) first_hit_frame(m, it);
                 V3 wo = mulf(d, -1.0f);
                 if (m.type == CPT_MAT_EMISSIVE) {
                     if (a.integrator == 2 || sampled_delta) {
@@ -588,6 +589,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                                 float ct = (float)(fabs((double)dot(ls.normal, sdir)) + 1e-4);
                                 float pnn = ls.pdf * len * len / ct;
                                 float w = pnn / (pnb + pnn);
+                                if (a.corrected && ls.delta) w = 1.0f;  // no BSDF sample reaches a point
                                 V3 ev = bsdf_eval(m, it, wo, sdir);
                                 if (a.integrator == 1) w = 1.0f;  // pathtracer.h:70-76: no MIS weight
                                 contrib = mulf(divs(mul(mul(mulv, ev), ls.intensity), ls.pdf * len * len), w);
@@ -604,13 +606,16 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                         V3 ev = bsdf_eval(m, it, wo, ds.wi);
                         bool survive = true;
                         if (depth > 2) {
-                            float rr = russian_roulette(ev);
+                            float rr = rr_prob(ev, a.corrected);
                             if (pcg_float(rng) > rr) survive = false;
                             else p = p * rr;
                         }
                         if (survive) {
                             const V3 mul_prev = mulv;
                             mulv = divs(mul(mulv, ev), p);
+                            // the next emitter hit's MIS weight: the reference keeps the
+                            // pdf after Russian roulette, the corrected mode the BSDF's own
+                            if (a.corrected) p = p_sampled;
                             if (depth + 1 < depth_cap) {
                                 no = add(it.hitpoint, mulf(ds.wi, CPT_EPSF));
                                 nd = ds.wi;
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(B) void k_shade_lt(WfArgs a, int cur, int depth_cap
                 int matid;
                 fill_isect(a.sc, o, d, h, it, matid);
                 Mat m = hit_mat(a.sc, matid, h);
-                if (depth == 0) first_hit_frame(m, it);
+                if (depth == 0 || a.corrected) first_hit_frame(m, it);
                 if (m.type != CPT_MAT_EMISSIVE) {
                     const V3 wo = mulf(d, -1.0f);
                     DSample ds = bsdf_sample(m, it, wo, rng);
@@ -712,7 +717,7 @@ __global__ __launch_bounds__(B) void k_shade_lt(WfArgs a, int cur, int depth_cap
                     if (!ds.delta) {
                         int ci, cj;
                         float cf;
-                        camera_connect(vd.cam, vd.W, vd.H, it.hitpoint, ci, cj, cf);
+                        camera_connect(vd.cam, vd.W, vd.H, it.hitpoint, ci, cj, cf, a.corrected != 0);
                         const V3 cp = v3(vd.cam[0], vd.cam[1], vd.cam[2]);
                         const V3 sdir = normalized(sub(cp, it.hitpoint));
                         if (bsdf_non_zero(m, it, wo, sdir) && ci > -1) {
@@ -727,7 +732,7 @@ __global__ __launch_bounds__(B) void k_shade_lt(WfArgs a, int cur, int depth_cap
                     }
                     bool survive = true;
                     if (depth > 2) {
-                        const float rr = russian_roulette(ev);
+                        const float rr = rr_prob(ev, a.corrected);
                         if (pcg_float(rng) > rr) survive = false;
                         else p = p * rr;
                     }
@@ -801,7 +806,7 @@ __global__ __launch_bounds__(B) void k_shade_ppg(WfArgs a, int cur, int depth_ca
                 int matid;
                 fill_isect(a.sc, o, d, h, it, matid);
                 Mat m = hit_mat(a.sc, matid, h);
-                if (depth == 0) first_hit_frame(m, it);
+                if (depth == 0 || a.corrected) first_hit_frame(m, it);
                 V3 wo = mulf(d, -1.0f);
                 if (m.type == CPT_MAT_EMISSIVE) {
                     colv = add(colv, mul(mulv, m.emission));
@@ -824,7 +829,7 @@ __global__ __launch_bounds__(B) void k_shade_ppg(WfArgs a, int cur, int depth_ca
                         V3 ev = bsdf_eval(m, it, wo, ds.wi);
                         bool survive = true;
                         if (depth > 2) {
-                            float rr = russian_roulette(ev);
+                            float rr = rr_prob(ev, a.corrected);
                             if (pcg_float(rng) > rr) survive = false;
                             else p = p * rr;
                         }
@@ -1417,6 +1422,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.ppg_nv = c->ppg_nv;
     a.refill = CPT_DEFAULT_REFILL;
     a.integrator = 0;
+    a.corrected = 0;
     a.gsamples = c->g_samples;
     a.gcount = c->g_count;
     a.gwt = c->g_wt;
@@ -2220,6 +2226,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     a.n_rec = (uint32_t)nrec;
     a.refill = (unsigned)refill;
     a.integrator = integrator;
+    a.corrected = (flags & CPT_RENDER_CORRECTED) ? 1 : 0;
     LaunchCfg L{B, G, (flags & CPT_RENDER_COUNT) != 0};
     L.SB = SB;
     L.GE = GE;
@@ -2238,10 +2245,11 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     if (use_graph) {
         char key[512];
         snprintf(key, sizeof(key),
-                 "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d,%d,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
+                 "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d,%d,%d,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
                  max_depth, (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
                  (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator,
+                 a.corrected,
                  (void*)c->sd_snode, (void*)c->sd_dinfo, (void*)c->sd_quad, (void*)c->sd_rect, (void*)c->sd_sum,
                  (void*)c->ppg_vtx, (void*)c->ppg_nv, a.sdt.bmin[0], a.sdt.bmin[1], a.sdt.bmin[2], a.sdt.bmax[0],
                  a.sdt.bmax[1], a.sdt.bmax[2]);
@@ -2631,6 +2639,7 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
         sa.view_tend = d_sc + 2 + n_views;
         WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
         a.integrator = (flags & CPT_RENDER_PURE_PT) ? 2 : (flags & CPT_RENDER_NEE_ONLY) ? 1 : 0;
+        a.corrected = (flags & CPT_RENDER_CORRECTED) ? 1 : 0;
         LaunchCfg L{B, G, false, true};
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->stats, (int)ST_N);
         hipLaunchKernelGGL(k_stream_init, dim3(G), dim3(B), 0, c->stream, a, sa, (unsigned)cap);

@@ -347,6 +347,7 @@ __device__ __forceinline__ float tri_area(const DevScene& sc, int id) {  // tria
 struct LSample {
     float pdf;
     V3 position, intensity, normal;
+    bool delta;  // a point light, which BSDF sampling never reaches
 };
 
 __device__ __forceinline__ LSample light_sample(const DevScene& sc, uint64_t& rng, V3 from) {
@@ -360,6 +361,7 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, uint64_t& rn
         ls.intensity = {e.x, e.y, e.z};
         ls.normal = {1.0f, 0.0f, 0.0f};
         ls.pdf = (float)(1.0 * (1.0 / (double)(float)NL));
+        ls.delta = true;
         return ls;
     }
     float r1 = pcg_float(rng);
@@ -381,6 +383,7 @@ __device__ __forceinline__ LSample light_sample(const DevScene& sc, uint64_t& rn
     ls.intensity = mulf(em, fabsf(dot(ls.normal, normalized(sub(from, ls.position)))));
     ls.pdf = 1.0f / tri_area(sc, id);
     ls.pdf = (float)((double)ls.pdf * (1.0 / (double)(float)NL));
+    ls.delta = false;
     return ls;
 }
 
@@ -392,6 +395,12 @@ __device__ __forceinline__ float light_pdf(const DevScene& sc, int id) {
 
 __device__ __forceinline__ float russian_roulette(V3 c) {  // pathtracerMIS.h:24-27
     return (float)(fabs((double)c.x * 0.2 + (double)c.y * 0.5 + (double)c.z * 0.3) * 0.5 + 0.4);
+}
+
+// survival probability; the corrected mode clamps it at 1 (the reference's can exceed 1)
+__device__ __forceinline__ float rr_prob(V3 c, int corrected) {
+    const float q = russian_roulette(c);
+    return corrected ? fminf(q, 1.0f) : q;
 }
 
 // Start of a light path (Lighttracer::integrate, lighttracer.h:27-35):
@@ -462,8 +471,10 @@ __device__ __forceinline__ bool light_start(const DevScene& sc, uint64_t& rng, V
 // CameraPerspective::connect_light_path (camera_perspective.h:46-82): the pixel a
 // point projects to and the importance factor 1 / (4 |cos^3| d^2 tan^2) (the
 // reference leaves the aspect ratio out).  ci = -1 when the point is behind the
-// camera, closer than EPS or outside the image.
-__device__ __forceinline__ void camera_connect(const float* cam, int W, int H, V3 p, int& ci, int& cj, float& factor) {
+// camera, closer than EPS or outside the image.  with_aspect adds the aspect ratio
+// to the image-plane area (the corrected mode).
+__device__ __forceinline__ void camera_connect(const float* cam, int W, int H, V3 p, int& ci, int& cj, float& factor,
+                                               bool with_aspect) {
     ci = cj = -1;
     factor = 0.0f;
     const V3 x = {cam[3], cam[4], cam[5]}, y = {cam[6], cam[7], cam[8]}, z = {cam[9], cam[10], cam[11]};
@@ -481,7 +492,9 @@ __device__ __forceinline__ void camera_connect(const float* cam, int W, int H, V
     ci = (int)fx;
     cj = H - (int)fy - 1;
     const float c3 = fabsf(nd.z * nd.z * nd.z);
-    factor = (float)(1.0 / (4.0 * (double)c3 * (double)ndl * (double)ndl * (double)cam[12] * (double)cam[12]));
+    double den = 4.0 * (double)c3 * (double)ndl * (double)ndl * (double)cam[12] * (double)cam[12];
+    if (with_aspect) den *= (double)cam[13];
+    factor = (float)(1.0 / den);
 }
 
 // camera_perspective.h:30-39

@@ -149,6 +149,13 @@ typedef struct {
 #define CPT_RENDER_PPG_STEP 1024    /* one PPG iteration with the current SD-tree (ppg.h:41-143):
                                        spp_axis = samples per pixel (whole-pixel jitter); splats
                                        accumulate until cpt_ppg_update */
+#define CPT_RENDER_CORRECTED 4096   /* switch off the reference's biased quirks (every integrator):
+                                       MIS weights from the BSDF pdf before Russian roulette
+                                       (pathtracerMIS.h:54,123), point lights weighted 1 in NEE
+                                       (their (1,0,0) normal, point_light.h:25), survival
+                                       probability clamped at 1 (pathtracerMIS.h:24-27), normal
+                                       maps at every hit (intersection.h:26-31), and the light
+                                       tracer's aspect ratio (camera_perspective.h:75-79) */
 #define CPT_RENDER_LIGHT_TRACE 2048 /* Lighttracer (src/renderer/lighttracer.h:17-154): width x height x
                                        spp_axis^2 paths start on the lights, every non-delta vertex
                                        is connected to the camera and splatted into the film (the

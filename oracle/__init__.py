@@ -36,6 +36,8 @@ def lib():
     L.orc_hit_material.argtypes = [vp, vp, i32, vp]
     L.orc_image_texel.restype = None
     L.orc_image_texel.argtypes = [vp, i32, i32, i32, vp]
+    L.orc_set_corrected.restype = None
+    L.orc_set_corrected.argtypes = [vp, i32]
     L.orc_render_lt.restype = i32
     L.orc_render_lt.argtypes = [vp, i32, i32, i32, i32, i32, C.c_uint64, vp, i32, vp]
     L.orc_scene_set_point_lights.restype = None
@@ -175,13 +177,14 @@ class OracleScene:
         return o, d
 
     def render(self, W, H, spp_axis, max_depth=32, sampler="pcg", seed=1, cols=None, rows=None,
-               threads=0, sumsq=False, integrator=0):
+               threads=0, sumsq=False, integrator=0, corrected=False):
         """Returns (sums (H,W,3) float64, film (H,W,3) float32, counts [paths, ext, shadow])
         and, with sumsq=True, the per-pixel sums of squares as a fourth element.
         sampler "pcg" = the GPU's per-path stream; "mt" = the reference's per-column
         mt19937 (seed 0 reproduces the reference's seeding).  integrator 0 =
         PathtracerMIS, 1 = Pathtracer (NEE without MIS), 2 = PurePt."""
         lib().orc_set_integrator(self.h, integrator)
+        lib().orc_set_corrected(self.h, int(corrected))
         sums = np.zeros((H, W, 3), np.float64)
         sq = np.zeros((H, W, 3), np.float64)
         film = np.zeros((H, W, 3), np.float32)
@@ -192,8 +195,9 @@ class OracleScene:
                          c0, c1, r0, r1, _p(sums), _p(sq), _p(film), threads, _p(counts))
         return (sums, film, counts, sq) if sumsq else (sums, film, counts)
 
-    def render_lt(self, W, H, spp_axis, max_depth=32, sampler="pcg", seed=1, threads=0):
+    def render_lt(self, W, H, spp_axis, max_depth=32, sampler="pcg", seed=1, threads=0, corrected=False):
         """Lighttracer: returns (film sums (H,W,3) float64 = spp x the image, counts)."""
+        lib().orc_set_corrected(self.h, int(corrected))
         sums = np.zeros((H, W, 3), np.float64)
         counts = np.zeros(3, np.int64)
         lib().orc_render_lt(self.h, W, H, spp_axis, max_depth, 0 if sampler == "pcg" else 1, seed, _p(sums),
@@ -208,6 +212,7 @@ class OracleScene:
         ssum = np.zeros(ns, np.float64)
         scnt = np.zeros(ns, np.int64)
         t = {k: np.ascontiguousarray(v) for k, v in tree.items()}
+        lib().orc_set_corrected(self.h, 0)
         lib().orc_ppg_render(self.h, W, H, spp, max_depth, seed, view, _p(t["snode"]), _p(t["dinfo"]), _p(t["quad"]),
                              _p(t["rect"]), _p(t["bounds"]), ns, _p(sums), _p(ssum), _p(scnt), threads)
         return sums, ssum, scnt
@@ -215,6 +220,7 @@ class OracleScene:
     def path(self, W, H, spp_axis, max_depth, seed, pixel, sample, integrator=0):
         out = np.zeros(3, np.float32)
         lib().orc_set_integrator(self.h, integrator)
+        lib().orc_set_corrected(self.h, 0)
         lib().orc_path(self.h, W, H, spp_axis, max_depth, seed, pixel, sample, _p(out))
         return out
 

@@ -158,6 +158,10 @@ struct Scene {
     // 0 PathtracerMIS (pathtracerMIS.h:29-130), 1 Pathtracer: NEE without MIS
     // (pathtracer.h:27-110), 2 PurePt: BSDF sampling only (pure_pt.h:27-67)
     int integrator = 0;
+    // the reference's biased quirks switched off (the library's CPT_RENDER_CORRECTED):
+    // MIS from the pre-roulette pdf, point lights weighted 1 in NEE, survival
+    // probability clamped at 1, normal maps at every hit, light tracer aspect ratio
+    bool corrected = false;
     // image textures: per-vertex uv, images (w, h, channels, texel offset),
     // per material the image of its albedo, roughness, metal and normal slot (-1: constant)
     std::vector<std::array<float, 2>> UV;
@@ -742,6 +746,7 @@ inline float tri_area(const Scene& s, int id) {  // triangle.h:197-207
 struct LSample {
     float pdf;
     V3 position, intensity, normal;
+    bool delta = false;
 };
 
 // shape_light.h:24-44 + triangle.h:209-269: uniform area sample of emitter
@@ -777,6 +782,7 @@ LSample light_sample(const Scene& s, Sampler& smp, const Isect& from) {
         ls.position = V3(p[0], p[1], p[2]);
         ls.intensity = V3(p[3], p[4], p[5]);
         ls.normal = V3(1.0f, 0.0f, 0.0f);
+        ls.delta = true;
         ls.pdf = 1.0f;
         ls.pdf = (float)((double)ls.pdf * (1.0 / (double)(float)NL));
         return ls;
@@ -798,6 +804,10 @@ inline float light_pdf(const Scene& s, int id) {
 // pathtracerMIS.h:24-27
 inline float russian_roulette(V3 c) {
     return (float)(std::fabs((double)c.x * 0.2 + (double)c.y * 0.5 + (double)c.z * 0.3) * 0.5 + 0.4);
+}
+inline float rr_prob(const Scene& s, V3 c) {
+    const float q = russian_roulette(c);
+    return s.corrected ? std::min(q, 1.0f) : q;
 }
 
 struct Counts {
@@ -828,6 +838,7 @@ V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Co
         }
         fill_isect(s, ray, h, it);
         const Material m = hit_material(s, it);
+        if (s.corrected) it.computed_normal = false;
         first_hit_frame(m, it);
         V3 wo = ray.d * (float)(-1.0);
         if (m.type == 2) {
@@ -870,6 +881,7 @@ V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Co
                         float ct = (float)(std::fabs((double)dot(ls.normal, sd)) + EPSD);
                         float pnn = ls.pdf * r * r / ct;
                         float w = pnn / (pnb + pnn);
+                        if (s.corrected && ls.delta) w = 1.0f;
                         col = col + divs(mul * ev * ls.intensity, ls.pdf * r * r) * w;
                     }
                 }
@@ -881,12 +893,14 @@ V3 integrate(const Scene& s, float cx, float cy, Sampler& smp, int max_depth, Co
     no_nee:
         if (!bsdf_non_zero(m, it, wo, sdir)) break;
         V3 ev = bsdf_eval(m, it, wo, sdir);
+        const float p_sampled = p;
         if (i > 2) {
-            float rr = russian_roulette(ev);
+            float rr = rr_prob(s, ev);
             if (smp.sample() > rr) break;
             p = p * rr;
         }
         mul = divs(mul * ev, p);
+        if (s.corrected) p = p_sampled;
         ray = Ray{it.hitpoint + sdir * EPSF, sdir, std::numeric_limits<float>::infinity()};
     }
     return col;
@@ -991,6 +1005,7 @@ V3 ppg_integrate(const Scene& s, const SdTreeView& t, float cx, float cy, Sample
         }
         fill_isect(s, ray, h, it);
         const Material m = hit_material(s, it);
+        if (s.corrected) it.computed_normal = false;
         first_hit_frame(m, it);
         V3 wo = ray.d * (float)(-1.0);
         if (m.type == 2) {
@@ -1014,7 +1029,7 @@ V3 ppg_integrate(const Scene& s, const SdTreeView& t, float cx, float cy, Sample
         if (!bsdf_non_zero(m, it, wo, sdir)) break;
         V3 ev = bsdf_eval(m, it, wo, sdir);
         if (i > 2) {
-            float rr = russian_roulette(ev);
+            float rr = rr_prob(s, ev);
             if (smp.sample() > rr) break;
             p = p * rr;
         }
@@ -1095,7 +1110,7 @@ struct CamConn {
     int i = -1, j = -1;
     float factor = 0;
 };
-CamConn camera_connect(const Camera& c, int W, int H, V3 p) {
+CamConn camera_connect(const Camera& c, int W, int H, V3 p, bool with_aspect) {
     CamConn cc;
     V3 dir = c.coords.transpose().mul(p - c.o);
     const float ndl = length(dir);
@@ -1109,8 +1124,10 @@ CamConn camera_connect(const Camera& c, int W, int H, V3 p) {
     if (!(fx >= 0.0 && fx < (double)W && fy >= 0.0 && fy < (double)H)) return cc;  // int() then bounds
     cc.i = (int)fx;
     cc.j = H - (int)fy - 1;
-    cc.factor = (float)(1.0 / (4.0 * (double)std::fabs(nd.z * nd.z * nd.z) * (double)ndl * (double)ndl *
-                               (double)c.tan_fovy * (double)c.tan_fovy));
+    double den = 4.0 * (double)std::fabs(nd.z * nd.z * nd.z) * (double)ndl * (double)ndl * (double)c.tan_fovy *
+                 (double)c.tan_fovy;
+    if (with_aspect) den *= (double)c.aspect;  // the corrected mode: image-plane area 4 tan^2 aspect
+    cc.factor = (float)(1.0 / den);
     return cc;
 }
 
@@ -1129,6 +1146,7 @@ void integrate_lt(const Scene& s, Sampler& smp, float factor, int W, int H, int 
         if (!intersect(s, ray, h)) break;
         fill_isect(s, ray, h, it);
         const Material m = hit_material(s, it);
+        if (s.corrected) it.computed_normal = false;
         first_hit_frame(m, it);
         if (m.type == 2) break;
         const V3 wo = neg(ray.d);
@@ -1136,7 +1154,7 @@ void integrate_lt(const Scene& s, Sampler& smp, float factor, int W, int H, int 
         const V3 eval = bsdf_eval(m, it, wo, ds.wi);
         float p = ds.pdf;
         if (!ds.delta) {
-            CamConn cc = camera_connect(s.cam, W, H, it.hitpoint);
+            CamConn cc = camera_connect(s.cam, W, H, it.hitpoint, s.corrected);
             const V3 to = s.cam.o - it.hitpoint;
             Ray sh{it.hitpoint, normalized(to), length(to)};
             if (bsdf_non_zero(m, it, wo, sh.d) && cc.i > -1) {
@@ -1152,7 +1170,7 @@ void integrate_lt(const Scene& s, Sampler& smp, float factor, int W, int H, int 
             }
         }
         if (i > 2) {
-            float rr = russian_roulette(eval);
+            float rr = rr_prob(s, eval);
             if (smp.sample() > rr) break;
             p = p * rr;
         }
@@ -1442,6 +1460,7 @@ int orc_ppg_render(void* sp, int W, int H, int spp, int max_depth, uint64_t seed
 }
 
 void orc_set_integrator(void* sp, int integrator) { ((Scene*)sp)->integrator = integrator; }
+void orc_set_corrected(void* sp, int corrected) { ((Scene*)sp)->corrected = corrected != 0; }
 
 // Lighttracer::render (lighttracer.h:118-154): W*H*S*S light paths, one per camera
 // sample, each with factor 1/(S*S).  sampler 0: per-path PCG stream (path id as

@@ -50,6 +50,8 @@ void* orc_scene_create(int nv, const float* pos, const float* nrm, const float* 
  * constant).  A metal image makes the Lambertian/GGX choice per hit. */
 void orc_scene_set_textures(void* s, const float* uvs, int n_images, const int32_t* images, int n_texels,
                             const float* texels, const int32_t* material_textures);
+/* 1: switch off the reference's biased quirks (the library's CPT_RENDER_CORRECTED) */
+void orc_set_corrected(void* s, int corrected);
 /* Lighttracer (lighttracer.h:17-154): W*H*S*S light paths splatted into sum_out
  * (H x W x 3, spp times the image); sampler 0 per-path PCG, 1 per-column mt19937 */
 int orc_render_lt(void* s, int W, int H, int S, int max_depth, int sampler, uint64_t seed, double* sum_out,

@@ -416,6 +416,13 @@ def test_point_light_direct_lighting_closed_form(integrator):
     if integrator == 0:  # the weight is far from 1 on this geometry
         unweighted = _floor_pixel_integral(s, W, H, 32, _point_light_floor(pos, albedo, False))
         assert (1.0 - want / unweighted).max() > 0.2
+        # the corrected mode weighs a point light 1 (no BSDF sample can reach it)
+        _, fc, _ = oracle.OracleScene(s).render(W, H, 32, max_depth=4, sampler="pcg", seed=5, integrator=0,
+                                                corrected=True)
+        for c in range(3):
+            rad = col[c] * power / (4 * math.pi)
+            rel = np.abs(fc[..., c] - unweighted * rad) / (unweighted * rad)
+            assert rel.max() < 1e-3 and rel.mean() < 2e-4
 
 
 def _blocks(x, b=4):
@@ -442,6 +449,21 @@ def test_light_tracer_point_light_closed_form():
     assert abs(img.mean() / want.mean() - 1) < 3e-3
     assert np.abs(_blocks(img) / _blocks(want) - 1).max() < 0.04
     assert counts[0] == W * H * 256 ** 2
+
+
+@pytest.mark.parametrize("corrected", [False, True])
+def test_light_tracer_aspect_ratio(corrected):
+    """A 2:1 image: the reference's importance factor leaves out the aspect ratio
+    (camera_perspective.h:75-79, "todo: account for x"), so its light-traced image
+    is `aspect` times the closed form; the corrected mode divides it out."""
+    W, H = 32, 16
+    pos, col, power, albedo = (0.35, 0.5, -0.2), (1.0, 0.5, 0.25), 2.0, 0.5
+    s = _floor_under_point_light((W, H), pos, col, power, albedo)
+    sums, _ = oracle.OracleScene(s).render_lt(W, H, 256, max_depth=4, seed=3, corrected=corrected)
+    img = sums[..., 0] / 256 ** 2
+    want = _floor_pixel_integral(s, W, H, 8, _point_light_floor(pos, albedo, False)) * col[0] * power / (4 * math.pi)
+    ratio = img.mean() / want.mean()
+    assert abs(ratio - (1.0 if corrected else W / H)) < 0.01 * (W / H)
 
 
 def test_light_tracer_area_light_matches_lambert_polygon():
