@@ -722,3 +722,43 @@ def textured_room(res=(64, 64), tex_res=16, seed=9):
         m.uvs = uv_of(m.positions, axes)
     s._flat = None
     return s
+
+
+# --------------------------------------------------------------------- grid of interiors
+def interior_grid_scene(res=(3840, 2160), grid=4, n_boxes=10000, seed=40, glossy_frac=0.4, lights_per_room=2,
+                        light_radiance=3000.0):
+    """Large-scale guided workload: grid x grid copies of the hard-lighting interior
+    (interior_scene) with their own random furniture, a share `glossy_frac` of each
+    room's furniture materials GGX with roughness U[0.02, 0.8], and `lights_per_room`
+    small 0.2 x 0.2 emitters in each room's corridor (32 for the 4 x 4 grid).  The
+    camera stands in the first room; rooms are 14 apart along x and 11 along z."""
+    rng = np.random.default_rng(seed)
+    out = Scene()
+    for gi in range(grid):
+        for gj in range(grid):
+            room = interior_scene(res=res, n_boxes=n_boxes, seed=seed + 1 + gi * grid + gj)
+            off = np.array([14.0 * gi, 0.0, 11.0 * gj], np.float32)
+            base = len(out.material_list)
+            furn = [k for k, m in enumerate(room.material_list) if m[0] == MAT_DIFFUSE][1:]  # not the walls
+            for k, m in enumerate(room.material_list):
+                rec = m.copy()
+                if k in furn and rng.random() < glossy_frac:
+                    rec[0] = MAT_GLOSSY
+                    rec[4] = rng.uniform(0.02, 0.8)
+                    rec[12] = 1.0
+                out.add_material(rec)
+            for m in room.meshes:
+                if room.material_list[m.material][0] == MAT_EMISSIVE:
+                    continue
+                out.add_mesh(m.positions + off, m.normals, m.tangents, m.faces, base + m.material)
+            light = out.add_material(material(MAT_EMISSIVE, emission=(light_radiance,) * 3))
+            h = 0.1
+            for q in range(lights_per_room):
+                z = -3.0 + 6.0 * (q + 0.5) / lights_per_room
+                P, N, T, F = _quad((7.9, 1.5 - h, z - h), (7.9, 1.5 - h, z + h), (7.9, 1.5 + h, z + h),
+                                   (7.9, 1.5 + h, z - h), (-1, 0, 0), (0, 0, 1))
+                if np.cross(P[1] - P[0], P[2] - P[0])[0] > 0:
+                    P = P[[0, 3, 2, 1]]
+                out.add_mesh(P + off, N, T, F, light)
+    out.set_camera((-4.0, 1.5, -3.0), (5.0, 1.0, 0.0), (0.0, 1.0, 0.0), math.tan(math.radians(30.0)), *res)
+    return out

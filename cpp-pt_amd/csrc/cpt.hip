@@ -396,18 +396,21 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     guide::SmemFwd& sm = *reinterpret_cast<guide::SmemFwd*>(smem_raw);
     __shared__ float o_tile[guide::TILE * guide::NOUT];
-    __shared__ float in_s[guide::TILE * guide::NIN];
+    // features of a group of 4 tiles, one sample per thread: the hit record,
+    // material and feature maths are latency-bound, so all four waves take part
+    __shared__ float in_s[4 * guide::TILE * guide::NIN];
+    constexpr unsigned GROUP = 4 * guide::TILE;
     const unsigned count = a.counters[2 * cur];
-    if (blockIdx.x * guide::TILE >= count) return;
+    if (blockIdx.x * GROUP >= count) return;
     guide::load_weights(sm, a.gwt, a.gparams);
-    for (unsigned t = blockIdx.x; t * guide::TILE < count; t += gridDim.x) {
-        const unsigned e0 = t * guide::TILE;
-        const int nt = (int)min((unsigned)guide::TILE, count - e0);
-        if ((int)threadIdx.x < guide::TILE) {
+    for (unsigned g = blockIdx.x; g * GROUP < count; g += gridDim.x) {
+        const unsigned eg = g * GROUP;
+        const int ng = (int)min(GROUP, count - eg);
+        {
             float* in = in_s + threadIdx.x * guide::NIN;
             for (int k = 0; k < guide::NIN; k++) in[k] = 0.0f;
-            if ((int)threadIdx.x < nt) {
-                unsigned e = e0 + threadIdx.x;
+            if ((int)threadIdx.x < ng) {
+                unsigned e = eg + threadIdx.x;
                 float4 O = a.qo[cur][e], D = a.qd[cur][e], HH = a.hits[e];
                 Hit h;
                 h.t = HH.x;
@@ -427,23 +430,27 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
             }
         }
         __syncthreads();
-        guide::encode_tile(sm, in_s, nt);
-        __syncthreads();
-        guide::forward_tile(sm, o_tile);
-        if ((int)threadIdx.x < nt) {
-            guide::Lobes L;
-            guide::decode(o_tile + threadIdx.x * guide::NOUT, L);
-            float* dst = a.lobes + (size_t)(e0 + threadIdx.x) * 40;
+        for (int q = 0; q * guide::TILE < ng; q++) {
+            const unsigned e0 = eg + q * guide::TILE;
+            const int nt = min(guide::TILE, ng - q * guide::TILE);
+            guide::encode_tile(sm, in_s + q * guide::TILE * guide::NIN, nt);
+            __syncthreads();
+            guide::forward_tile(sm, o_tile);
+            if ((int)threadIdx.x < nt) {
+                guide::Lobes L;
+                guide::decode(o_tile + threadIdx.x * guide::NOUT, L);
+                float* dst = a.lobes + (size_t)(e0 + threadIdx.x) * 40;
 #pragma unroll
-            for (int k = 0; k < guide::NLOBES; k++) {
-                dst[k] = L.pi[k];
-                dst[8 + k] = L.kappa[k];
-                dst[16 + k] = L.mx[k];
-                dst[24 + k] = L.my[k];
-                dst[32 + k] = L.mz[k];
+                for (int k = 0; k < guide::NLOBES; k++) {
+                    dst[k] = L.pi[k];
+                    dst[8 + k] = L.kappa[k];
+                    dst[16 + k] = L.mx[k];
+                    dst[24 + k] = L.my[k];
+                    dst[32 + k] = L.mz[k];
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
