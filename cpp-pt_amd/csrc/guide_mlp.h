@@ -31,10 +31,10 @@ constexpr int HID = 64;
 constexpr int NLOBES = 8;
 constexpr int NOUT = 48;        // 8 x 5 used
 constexpr int TILE = 64;        // samples per block tile
-constexpr int PAD = 8;          // bf16 padding of LDS rows
-constexpr int LD0 = K0 + PAD;   // 232
-constexpr int LDH = HID + PAD;  // 72
-constexpr int LDO = NOUT + PAD; // 56
+constexpr int PAD = 12;         // bf16 padding of LDS rows (12: fewest bank conflicts measured)
+constexpr int LD0 = K0 + PAD;   // 236
+constexpr int LDH = HID + PAD;  // 76
+constexpr int LDO = NOUT + PAD; // 60
 
 // parameter offsets in the flat FP32 vector: W_l (K_l x N_l, row-major), then b_l
 constexpr int W0 = 0;
