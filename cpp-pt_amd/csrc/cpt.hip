@@ -130,6 +130,9 @@ struct WfArgs {
     float* gsamples;         // training samples out (11 floats)
     uint32_t* gcount;
     const uint16_t* gwt;
+    const uint8_t* gwt8;     // FP8 weights (k_quant8) and their inverse scales
+    const float* ginv8;
+    int guide_fp8;           // guide inference in FP8
     const float* gparams;
     int W, H, S, max_depth;
     uint64_t seed;
@@ -392,9 +395,12 @@ __device__ __forceinline__ bool guidable(const Mat& m) {
 
 // Per bounce: network inputs of every queued hit, one MLP forward per tile of 64,
 // decoded mixture written per queue entry.
+// FP8: the e4m3 network (k_quant8's weights) on FP8 MFMA; else BF16
+template <bool FP8>
 __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     guide::SmemFwd& sm = *reinterpret_cast<guide::SmemFwd*>(smem_raw);
+    guide::SmemFwd8& sm8 = *reinterpret_cast<guide::SmemFwd8*>(smem_raw);
     __shared__ float o_tile[guide::TILE * guide::NOUT];
     // features of a group of 4 tiles, one sample per thread: the hit record,
     // material and feature maths are latency-bound, so all four waves take part
@@ -402,7 +408,8 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     constexpr unsigned GROUP = 4 * guide::TILE;
     const unsigned count = a.counters[2 * cur];
     if (blockIdx.x * GROUP >= count) return;
-    guide::load_weights(sm, a.gwt, a.gparams);
+    if constexpr (FP8) guide::load_weights8(sm8, a.gwt8, a.gparams, a.ginv8);
+    else guide::load_weights(sm, a.gwt, a.gparams);
     for (unsigned g = blockIdx.x; g * GROUP < count; g += gridDim.x) {
         const unsigned eg = g * GROUP;
         const int ng = (int)min(GROUP, count - eg);
@@ -433,9 +440,15 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
         for (int q = 0; q * guide::TILE < ng; q++) {
             const unsigned e0 = eg + q * guide::TILE;
             const int nt = min(guide::TILE, ng - q * guide::TILE);
-            guide::encode_tile(sm, in_s + q * guide::TILE * guide::NIN, nt);
-            __syncthreads();
-            guide::forward_tile(sm, o_tile);
+            if constexpr (FP8) {
+                guide::encode_tile8(sm8, in_s + q * guide::TILE * guide::NIN, nt);
+                __syncthreads();
+                guide::forward_tile8(sm8, o_tile);
+            } else {
+                guide::encode_tile(sm, in_s + q * guide::TILE * guide::NIN, nt);
+                __syncthreads();
+                guide::forward_tile(sm, o_tile);
+            }
             if ((int)threadIdx.x < nt) {
                 guide::Lobes L;
                 guide::decode(o_tile + threadIdx.x * guide::NOUT, L);
@@ -1207,6 +1220,8 @@ struct cpt_ctx {
     float* g_v = nullptr;
     float* g_grad = nullptr;
     uint16_t* g_wt = nullptr;    // transposed BF16 weights
+    uint8_t* g_wt8 = nullptr;    // transposed FP8 weights (inference) and inverse layer scales
+    float* g_inv8 = nullptr;
     float* g_samples = nullptr;  // training samples (11 floats each)
     size_t g_samples_cap = 0;
     uint16_t* g_rows = nullptr;  // per-sample activation / gradient rows
@@ -1433,6 +1448,9 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.gsamples = c->g_samples;
     a.gcount = c->g_count;
     a.gwt = c->g_wt;
+    a.gwt8 = c->g_wt8;
+    a.ginv8 = c->g_inv8;
+    a.guide_fp8 = c->g_cfg.infer_fp8 != 0;
     a.gparams = c->g_params;
     a.W = W;
     a.H = H;
@@ -1551,8 +1569,16 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
         int cur = b & 1;
         L_extend(L, a, cur, s);
         mark(0);
-        if (L.guided && b < a.guide_depth)
-            hipLaunchKernelGGL(k_guide_infer, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a, cur);
+        if (L.guided && b < a.guide_depth) {
+            // the FP8 tile state (77 KB of LDS) lets two blocks share a CU
+            if (a.guide_fp8) {
+                hipLaunchKernelGGL(k_guide_infer<true>, dim3(2 * L.infer_blocks), dim3(256), sizeof(guide::SmemFwd8), s,
+                                   a, cur);
+            } else {
+                hipLaunchKernelGGL(k_guide_infer<false>, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a,
+                                   cur);
+            }
+        }
         L_shade(L, a, cur, a.max_depth, s);
         mark(1);
         if (!L.ppg) {
@@ -1630,6 +1656,8 @@ void cpt_destroy(cpt_ctx* c) {
     dfree(c->g_v);
     dfree(c->g_grad);
     dfree(c->g_wt);
+    dfree(c->g_wt8);
+    dfree(c->g_inv8);
     dfree(c->g_samples);
     dfree(c->g_rows);
     dfree(c->g_partial);
@@ -2256,7 +2284,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
                  max_depth, (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
                  (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator,
-                 a.corrected,
+                 a.corrected * 2 + a.guide_fp8,
                  (void*)c->sd_snode, (void*)c->sd_dinfo, (void*)c->sd_quad, (void*)c->sd_rect, (void*)c->sd_sum,
                  (void*)c->ppg_vtx, (void*)c->ppg_nv, a.sdt.bmin[0], a.sdt.bmin[1], a.sdt.bmin[2], a.sdt.bmax[0],
                  a.sdt.bmax[1], a.sdt.bmax[2]);
@@ -2814,13 +2842,17 @@ int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
         r |= dalloc(c, c->g_v, NPARAM);
         r |= dalloc(c, c->g_grad, NPARAM);
         r |= dalloc(c, c->g_wt, NWT);
+        r |= dalloc(c, c->g_wt8, NWQ);
+        r |= dalloc(c, c->g_inv8, 8);
         r |= dalloc(c, c->g_partial, (size_t)GUIDE_DW_BLOCKS * NPARAM);
         r |= dalloc(c, c->g_loss_part, GUIDE_TRAIN_BLOCKS);
         r |= dalloc(c, c->g_wscale, 1);
         if (r) return CPT_E_HIP;
         HIPCHK(c, hipFuncSetAttribute((const void*)k_mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
         HIPCHK(c, hipFuncSetAttribute((const void*)k_train_tile, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
-        HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<false>, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<true>, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd8)));
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_mlp_forward8, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd8)));
         r |= dalloc(c, c->g_count, 4);
         if (r) return CPT_E_HIP;
         HIPCHK(c, hipHostMalloc((void**)&c->h_count, 4 * sizeof(uint32_t), hipHostMallocDefault));
@@ -2861,6 +2893,7 @@ int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
     HIPCHK(c, hipMemset(c->g_v, 0, NPARAM * sizeof(float)));
     hipLaunchKernelGGL(k_pack_weights, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
     hipLaunchKernelGGL(k_pack_weights2, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
+    hipLaunchKernelGGL(k_quant8, dim3(5), dim3(256), 0, c->stream, c->g_params, c->g_wt8, c->g_inv8);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->g_step = 0;
     c->has_guide = true;
@@ -2884,6 +2917,7 @@ int cpt_guide_set_params(cpt_ctx* c, const float* in) {
     HIPCHK(c, hipMemset(c->g_v, 0, guide::NPARAM * sizeof(float)));
     hipLaunchKernelGGL(guide::k_pack_weights, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
     hipLaunchKernelGGL(guide::k_pack_weights2, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
+    hipLaunchKernelGGL(guide::k_quant8, dim3(5), dim3(256), 0, c->stream, c->g_params, c->g_wt8, c->g_inv8);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->g_step = 0;
     return CPT_OK;
@@ -2905,7 +2939,12 @@ int cpt_guide_forward(cpt_ctx* c, const float* inputs, int32_t n, float* outputs
         rc = fail(c, CPT_E_HIP, "upload failed");
     if (!rc) {
         int G = std::min(c->cus, (n + TILE - 1) / TILE);
-        hipLaunchKernelGGL(k_mlp_forward, dim3(G), dim3(256), sizeof(SmemFwd), c->stream, c->g_wt, c->g_params, d_in, n, d_out);
+        if (c->g_cfg.infer_fp8)
+            hipLaunchKernelGGL(k_mlp_forward8, dim3(G), dim3(256), sizeof(SmemFwd8), c->stream, c->g_wt8, c->g_params,
+                               c->g_inv8, d_in, n, d_out);
+        else
+            hipLaunchKernelGGL(k_mlp_forward, dim3(G), dim3(256), sizeof(SmemFwd), c->stream, c->g_wt, c->g_params, d_in,
+                               n, d_out);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(outputs, d_out, sizeof(float) * NOUT * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
@@ -2935,6 +2974,8 @@ static int guide_train_device(cpt_ctx* c, int n, bool apply) {
         c->g_step++;
         hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_m, c->g_v, c->g_grad, c->g_step,
                            c->g_cfg.lr, c->g_cfg.beta1, c->g_cfg.beta2, c->g_cfg.eps, c->g_wt);
+        if (c->g_cfg.infer_fp8)
+            hipLaunchKernelGGL(k_quant8, dim3(5), dim3(256), 0, c->stream, c->g_params, c->g_wt8, c->g_inv8);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, CPT_E_HIP, "guide training launch failed: %s", hipGetErrorString(e));

@@ -345,6 +345,187 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* gwt, const 
     }
 }
 
+// --------------------------------------------------------------------------
+// FP8 inference (e4m3, OCP format of gfx950) on v_mfma_f32_16x16x32_fp8_fp8.
+// Weights: per layer scaled by 448 / max|W_l| and rounded to e4m3 (k_quant8);
+// one-blob features scaled by 448; each hidden layer's ReLU outputs scaled per
+// tile by 448 / (their maximum), found with one block reduction.  FP32
+// accumulation; the accumulator is rescaled by the two inverse scales before the
+// bias.  The conversion does not saturate (it returns NaN above 448), so every
+// value is clamped first.  Lane layout of the 16x16x32 MFMA: lane l supplies
+// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j = 0..7, as 8 bytes.
+// Training keeps the BF16 path; only inference uses these weights.
+constexpr int PAD8 = 8;             // byte padding of FP8 LDS rows
+constexpr int LQ0 = K0 + PAD8;      // 232
+constexpr int LQH = HID + PAD8;     // 72
+constexpr int Q0 = 0;
+constexpr int Q1 = Q0 + HID * LQ0;
+constexpr int Q2 = Q1 + HID * LQH;
+constexpr int Q3 = Q2 + HID * LQH;
+constexpr int Q4 = Q3 + HID * LQH;
+constexpr int NWQ = Q4 + NOUT * LQH;  // 32128 bytes
+constexpr float FP8_MAX = 448.0f;
+
+__device__ __forceinline__ uint32_t fp8x2(float a, float b) {
+    a = fminf(fmaxf(a, -FP8_MAX), FP8_MAX);
+    b = fminf(fmaxf(b, -FP8_MAX), FP8_MAX);
+    return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xffffu;
+}
+__device__ __forceinline__ uint8_t fp8x1(float a) { return (uint8_t)(fp8x2(a, 0.0f) & 0xffu); }
+
+struct SmemFwd8 {
+    uint8_t wt[NWQ];
+    float bias[4 * HID + NOUT];
+    float inv_w[8];       // 1 / weight scale per layer (5 used)
+    float redmax[2][4];   // per-wave maxima, double-buffered over layers
+    uint8_t x0[TILE * LQ0];
+    uint8_t h[2][TILE * LQH];
+};
+
+__device__ __forceinline__ void load_weights8(SmemFwd8& sm, const uint8_t* gwt8, const float* params,
+                                              const float* inv_scale) {
+    const uint4* src = reinterpret_cast<const uint4*>(gwt8);
+    uint4* dst = reinterpret_cast<uint4*>(sm.wt);
+    for (int i = threadIdx.x; i < NWQ / 16; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < 4 * HID + NOUT; i += blockDim.x) {
+        int off = i < HID ? B0 + i : i < 2 * HID ? B1 + i - HID : i < 3 * HID ? B2 + i - 2 * HID
+                : i < 4 * HID ? B3 + i - 3 * HID : B4 + i - 4 * HID;
+        sm.bias[i] = params[off];
+    }
+    if (threadIdx.x < 5) sm.inv_w[threadIdx.x] = inv_scale[threadIdx.x];
+}
+
+// one-blob features x 448 as FP8, two per store (bins 2b, 2b+1 of one input)
+__device__ __forceinline__ void encode_tile8(SmemFwd8& sm, const float* in, int n) {
+    for (int i = threadIdx.x; i < TILE * K0 / 2; i += blockDim.x) {
+        const int f2 = 2 * i, s = f2 / K0, f = f2 - s * K0;
+        const int d = f / NBINS, b = f - d * NBINS;
+        float v0 = 0.0f, v1 = 0.0f;
+        if (s < n) {
+            const float x = in[s * NIN + d];
+            v0 = blob(x, b) * FP8_MAX;
+            v1 = blob(x, b + 1) * FP8_MAX;
+        }
+        *reinterpret_cast<uint16_t*>(sm.x0 + s * LQ0 + f) = (uint16_t)fp8x2(v0, v1);
+    }
+}
+
+template <int K, int NT>
+__device__ __forceinline__ void gemm8_rows_nk(const uint8_t* A, int lda, const uint8_t* Wt, int ldb, f4* acc,
+                                              int lane) {
+#pragma unroll
+    for (int kt = 0; kt < K / 32; kt++) {
+        const int kk = kt * 32 + 8 * (lane >> 4);
+        const long a = *reinterpret_cast<const long*>(A + (lane & 15) * lda + kk);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const long b = *reinterpret_cast<const long*>(Wt + (nt * 16 + (lane & 15)) * ldb + kk);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc[nt], 0, 0, 0);
+        }
+    }
+}
+
+// forward through all layers with FP8 operands; out as forward_tile
+__device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = 16 * wave;
+    float inv_a = 1.0f / FP8_MAX;  // inverse scale of the current layer's input
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+        f4 acc[4] = {};
+        if (l == 0) gemm8_rows_nk<K0, 4>(sm.x0 + r0 * LQ0, LQ0, sm.wt + Q0, LQ0, acc, lane);
+        else gemm8_rows_nk<HID, 4>(sm.h[(l - 1) & 1] + r0 * LQH, LQH, sm.wt + (l == 1 ? Q1 : l == 2 ? Q2 : Q3), LQH,
+                                   acc, lane);
+        const float sc = inv_a * sm.inv_w[l];
+        float mx = 0.0f;
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int n = nt * 16 + (lane & 15);
+                const float z = fmaxf(acc[nt][j] * sc + sm.bias[l * HID + n], 0.0f);
+                acc[nt][j] = z;
+                mx = fmaxf(mx, z);
+            }
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+        if (lane == 0) sm.redmax[l & 1][wave] = mx;
+        __syncthreads();
+        const float M = fmaxf(fmaxf(sm.redmax[l & 1][0], sm.redmax[l & 1][1]),
+                              fmaxf(sm.redmax[l & 1][2], sm.redmax[l & 1][3]));
+        const float s = M > 0.0f ? FP8_MAX / M : 1.0f;
+        inv_a = M > 0.0f ? M / FP8_MAX : 1.0f;
+        uint8_t* hd = sm.h[l & 1];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
+                hd[m * LQH + n] = fp8x1(acc[nt][j] * s);
+            }
+        __syncthreads();
+    }
+    {
+        f4 acc[3] = {};
+        gemm8_rows_nk<HID, 3>(sm.h[1] + r0 * LQH, LQH, sm.wt + Q4, LQH, acc, lane);
+        const float sc = inv_a * sm.inv_w[4];
+#pragma unroll
+        for (int nt = 0; nt < 3; nt++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
+                out[m * NOUT + n] = acc[nt][j] * sc + sm.bias[4 * HID + n];
+            }
+    }
+    __syncthreads();
+}
+
+// FP8 copy of the weights: one block per layer; inv_scale[l] = max|W_l| / 448
+__global__ __launch_bounds__(256) void k_quant8(const float* params, uint8_t* gwt8, float* inv_scale) {
+    __shared__ float red[256];
+    const int l = blockIdx.x;
+    const int off = l == 0 ? W0 : l == 1 ? W1 : l == 2 ? W2 : l == 3 ? W3 : W4;
+    const int K = l == 0 ? K0 : HID, N = l == 4 ? NOUT : HID;
+    const int q = l == 0 ? Q0 : l == 1 ? Q1 : l == 2 ? Q2 : l == 3 ? Q3 : Q4;
+    const int ld = l == 0 ? LQ0 : LQH;
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < K * N; i += blockDim.x) m = fmaxf(m, fabsf(params[off + i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    const float M = red[0];
+    const float sc = M > 0.0f ? FP8_MAX / M : 1.0f;
+    if (threadIdx.x == 0) inv_scale[l] = M > 0.0f ? M / FP8_MAX : 1.0f;
+    for (int i = threadIdx.x; i < K * N; i += blockDim.x) {
+        const int k = i / N, n = i - k * N;
+        gwt8[q + n * ld + k] = fp8x1(params[off + i] * sc);
+    }
+    for (int i = threadIdx.x; i < N * (ld - K); i += blockDim.x) {
+        const int n = i / (ld - K), k = K + (i - n * (ld - K));
+        gwt8[q + n * ld + k] = 0;
+    }
+}
+
+// FP8 inference over n input vectors: raw outputs (n x NOUT)
+__global__ __launch_bounds__(256) void k_mlp_forward8(const uint8_t* gwt8, const float* params, const float* inv_scale,
+                                                      const float* in, int n, float* out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    SmemFwd8& sm = *reinterpret_cast<SmemFwd8*>(smem_raw);
+    __shared__ float o_tile[TILE * NOUT];
+    load_weights8(sm, gwt8, params, inv_scale);
+    __syncthreads();
+    for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
+        int nt = min(TILE, n - t * TILE);
+        encode_tile8(sm, in + (size_t)t * TILE * NIN, nt);
+        __syncthreads();
+        forward_tile8(sm, o_tile);
+        for (int i = threadIdx.x; i < nt * NOUT; i += blockDim.x) out[(size_t)t * TILE * NOUT + i] = o_tile[i];
+        __syncthreads();
+    }
+}
+
 // Forward + loss gradient + backward-data for tiles of training samples; writes
 // the per-sample rows used by the weight-gradient GEMMs and each block's loss sum.
 __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const float* params, const float* samples,

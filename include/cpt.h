@@ -292,7 +292,9 @@ typedef struct {
     int32_t guide_depth;   /* guided renders: guide the vertices of the first this many bounces (0 = all) */
     int32_t guide_target;  /* training target: 0 = the product L_i f cos (default), 1 = the incident
                               radiance L_i alone (what PPG splats, ppg.h:130-141) */
-    int32_t reserved;
+    int32_t infer_fp8;     /* 1: guided renders and cpt_guide_forward run the network in FP8 (e4m3
+                              weights with per-layer scales, per-tile activation scales, FP8 MFMA,
+                              FP32 accumulation); training stays BF16 */
 } cpt_guide_params;
 
 int cpt_guide_num_params(void);

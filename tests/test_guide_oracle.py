@@ -78,3 +78,79 @@ def test_bf16_emulation_close_to_fp64():
     a = oracle.mlp_forward(P, X, bf16=True)
     b = oracle.mlp_forward(P, X, bf16=False)
     assert np.abs(a - b).max() < 0.05 * np.abs(b).max()
+
+
+def e4m3(x):
+    """Round to the nearest OCP e4m3 value (ties to even), after clamping to
+    +-448: 3 mantissa bits, exponent bias 7, subnormal step 2^-9."""
+    x = np.clip(np.asarray(x, np.float64), -448.0, 448.0)
+    a = np.abs(x)
+    e = np.maximum(np.floor(np.log2(np.maximum(a, 2.0 ** -10))), -6.0)
+    step = 2.0 ** (e - 3)
+    return np.sign(x) * np.round(a / step) * step
+
+
+def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64):
+    """FP8 inference pipeline of the guide network restated in numpy: weights
+    scaled per layer by 448 / max|W|, one-blob features by 448, each hidden
+    layer's ReLU outputs per tile of 64 samples by 448 / (tile maximum), the
+    padding rows of a partial tile taking part in that maximum as zero-feature
+    samples.  quant=identity gives the exact network."""
+    K = [224, 64, 64, 64, 64]
+    N = [64, 64, 64, 64, 48]
+    P = np.asarray(params, np.float64)
+    Ws, bs, o = [], [], 0
+    for l in range(5):
+        Ws.append(P[o:o + K[l] * N[l]].reshape(K[l], N[l]))
+        o += K[l] * N[l]
+        bs.append(P[o:o + N[l]])
+        o += N[l]
+    Wq, inv_w = [], []
+    for W in Ws:
+        m = np.abs(W).max()
+        Wq.append(quant(W * (448.0 / m)))
+        inv_w.append(m / 448.0)
+    X = np.asarray(inputs, np.float32).reshape(-1, 7)
+    n = X.shape[0]
+    out = np.zeros((n, 48))
+    centres = np.arange(32, dtype=np.float32) + np.float32(0.5)
+    for t0 in range(0, n, tile):
+        xs = X[t0:t0 + tile]
+        d = xs[:, :, None] * np.float32(32) - centres[None, None, :]
+        f = np.zeros((tile, 224))
+        f[:len(xs)] = np.exp(np.float32(-0.5) * d * d).reshape(len(xs), 224)
+        h = quant(f * 448.0)
+        inv_a = 1.0 / 448.0
+        for l in range(4):
+            z = np.maximum(h @ Wq[l] * (inv_a * inv_w[l]) + bs[l], 0.0)
+            M = z.max()
+            s, inv_a = (448.0 / M, M / 448.0) if M > 0 else (1.0, 1.0)
+            h = quant(z * s)
+        out[t0:t0 + len(xs)] = (h @ Wq[4] * (inv_a * inv_w[4]) + bs[4])[:len(xs)]
+    return out
+
+
+def test_e4m3_rounding_known_values():
+    # largest normal, smallest subnormal, ties to even in both directions,
+    # the binade edge, clamping
+    assert e4m3(448.0) == 448.0 and e4m3(1e6) == 448.0 and e4m3(-500.0) == -448.0
+    assert e4m3(2.0 ** -9) == 2.0 ** -9 and e4m3(2.0 ** -11) == 0.0 and e4m3(0.75 * 2.0 ** -9) == 2.0 ** -9
+    assert e4m3(1.0625) == 1.0 and e4m3(1.1875) == 1.25 and e4m3(15.5) == 16.0
+    assert e4m3(2.0 ** -6 * 1.0625) == 2.0 ** -6   # smallest normal binade, step 2^-9
+    assert e4m3(240.0) == 240.0 and e4m3(232.0) == 224.0 and e4m3(432.0) == 448.0
+    # every representable positive value maps to itself: 7 subnormals + 15 binades x 8
+    vals = [k * 2.0 ** -9 for k in range(1, 8)] + [(1 + m / 8) * 2.0 ** e for e in range(-6, 9) for m in range(8)]
+    vals = [v for v in vals if v <= 448.0]
+    assert len(vals) == 7 + 15 * 8 - 1
+    assert np.array_equal(e4m3(np.array(vals)), np.array(vals))
+
+
+def test_fp8_pipeline_without_rounding_is_the_network():
+    P = random_params(11)
+    X = np.random.default_rng(12).random((150, 7)).astype(np.float32)
+    exact = mlp_forward_fp8(P, X, quant=lambda v: v)
+    r = oracle.mlp_forward(P, X, bf16=False)
+    assert np.abs(exact - r).max() < 1e-5 * np.abs(r).max()
+    q = mlp_forward_fp8(P, X)
+    rel = np.linalg.norm(q - r) / np.linalg.norm(r)
+    assert 1e-3 < rel < 0.1, rel
