@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--no-ppg", action="store_true", help="guided: skip the SD-tree (PPG) arm")
     ap.add_argument("--fp8", action="store_true", help="guided: run the guide network in FP8 (training stays BF16)")
+    ap.add_argument("--guide-width", type=int, choices=[64, 128], default=64,
+                    help="guided: hidden width of the guide network (64: 8 lobes, 128: 16 lobes)")
     ap.add_argument("--guide-depth", type=int, default=0,
                     help="guided: guide the vertices below this depth only (0: every vertex)")
     ap.add_argument("--scene", choices=["interior", "grid"], default="interior",
@@ -132,7 +134,7 @@ def bench_guided(args):
     def run(guided):
         if guided:
             ctx.guide_init(seed=11, lr=3e-3, train_paths=min(W * H, 1 << 18), train_steps=1,
-                           guide_depth=args.guide_depth, infer_fp8=args.fp8)
+                           guide_depth=args.guide_depth, infer_fp8=args.fp8, width=args.guide_width)
         flags = (cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN) if guided else 0
         curve = []
         elapsed, spp, k = 0.0, 0, 0
@@ -203,6 +205,7 @@ def bench_guided(args):
         "value": rows[-1]["plain_relmse"] / max(rows[-1]["guided_relmse"], 1e-30),
         "unit": "relMSE ratio plain/guided at the largest budget",
         "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "f32 (guide: " + ("fp8 e4m3 MFMA inference, bf16 training" if args.fp8 else "bf16 MFMA") + ", fp32 acc)",
+        "guide": {"width": args.guide_width, "lobes": 8 if args.guide_width == 64 else 16, "infer_fp8": bool(args.fp8)},
         "config": {"workload": f"{'4x4 grid of hard-lighting interiors, 32 emitters, mixed GGX' if args.scene == 'grid' else 'hard-lighting interior'}, "
                                f"{scene.n_triangles} triangles, {W}x{H}, max depth {depth}, "
                                f"reference {ref_spp} spp FP64", "budgets_s": budgets},

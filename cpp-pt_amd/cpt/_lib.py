@@ -42,7 +42,7 @@ ABI_FUNCTIONS = (
     "cpt_version", "cpt_create", "cpt_destroy", "cpt_last_error", "cpt_upload_scene",
     "cpt_bvh_info", "cpt_tie_rank", "cpt_trace_closest", "cpt_trace_any",
     "cpt_trace_closest_device", "cpt_render", "cpt_read_film", "cpt_render_views",
-    "cpt_reference_tie_rank", "cpt_guide_num_params", "cpt_guide_init", "cpt_guide_get_params",
+    "cpt_reference_tie_rank", "cpt_guide_num_params", "cpt_guide_shape", "cpt_guide_init", "cpt_guide_get_params",
     "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
     "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export",
 )
@@ -52,7 +52,7 @@ class GuideParams(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("seed", C.c_uint64), ("guide_fraction", C.c_float), ("train_paths", C.c_int32),
                 ("train_steps", C.c_int32), ("guide_depth", C.c_int32), ("guide_target", C.c_int32),
-                ("infer_fp8", C.c_int32)]
+                ("infer_fp8", C.c_int32), ("width", C.c_int32)]
 
 
 class SceneDesc(C.Structure):
@@ -138,6 +138,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_reference_tie_rank.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     lib.cpt_guide_num_params.restype = C.c_int
     lib.cpt_guide_init.argtypes = [C.c_void_p, C.POINTER(GuideParams)]
+    lib.cpt_guide_shape.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_guide_get_params.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_guide_set_params.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_guide_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
@@ -348,30 +349,40 @@ Context.render_views = _render_views
 
 
 def _guide_init(self, lr=1e-3, seed=1, guide_fraction=0.5, train_paths=0, train_steps=1, beta1=0.9, beta2=0.99,
-                eps=1e-8, guide_depth=0, guide_target=0, infer_fp8=False):
+                eps=1e-8, guide_depth=0, guide_target=0, infer_fp8=False, width=64):
     gp = GuideParams()
     gp.lr, gp.beta1, gp.beta2, gp.eps = lr, beta1, beta2, eps
     gp.seed, gp.guide_fraction, gp.train_paths, gp.train_steps = seed, guide_fraction, train_paths, train_steps
     gp.guide_depth = guide_depth
     gp.guide_target = guide_target
     gp.infer_fp8 = 1 if infer_fp8 else 0
+    gp.width = width
     self._check(self.lib.cpt_guide_init(self.h, C.byref(gp)), "cpt_guide_init")
 
 
+def _guide_shape(self):
+    """(hidden width, lobes, outputs per vector, parameters) of the current guide."""
+    sh = np.zeros(4, np.int32)
+    self._check(self.lib.cpt_guide_shape(self.h, _ptr(sh)), "cpt_guide_shape")
+    return tuple(int(v) for v in sh)
+
+
 def _guide_params(self):
-    out = np.zeros(self.lib.cpt_guide_num_params(), np.float32)
+    out = np.zeros(self.guide_shape()[3], np.float32)
     self._check(self.lib.cpt_guide_get_params(self.h, _ptr(out)), "cpt_guide_get_params")
     return out
 
 
 def _guide_set_params(self, params):
     p = np.ascontiguousarray(params, np.float32)
+    if p.size != self.guide_shape()[3]:
+        raise ValueError(f"guide has {self.guide_shape()[3]} parameters, got {p.size}")
     self._check(self.lib.cpt_guide_set_params(self.h, _ptr(p)), "cpt_guide_set_params")
 
 
 def _guide_forward(self, inputs):
     x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 7)
-    out = np.zeros((x.shape[0], 48), np.float32)
+    out = np.zeros((x.shape[0], self.guide_shape()[2]), np.float32)
     self._check(self.lib.cpt_guide_forward(self.h, _ptr(x), x.shape[0], _ptr(out)), "cpt_guide_forward")
     return out
 
@@ -379,13 +390,14 @@ def _guide_forward(self, inputs):
 def _guide_train_step(self, samples, apply=True, want_grad=False):
     s = np.ascontiguousarray(samples, np.float32).reshape(-1, 11)
     loss = np.zeros(1, np.float32)
-    g = np.zeros(self.lib.cpt_guide_num_params(), np.float32) if want_grad else None
+    g = np.zeros(self.guide_shape()[3], np.float32) if want_grad else None
     self._check(self.lib.cpt_guide_train_step(self.h, _ptr(s), s.shape[0], 1 if apply else 0, _ptr(loss), _ptr(g)),
                 "cpt_guide_train_step")
     return (float(loss[0]), g) if want_grad else float(loss[0])
 
 
 Context.guide_init = _guide_init
+Context.guide_shape = _guide_shape
 Context.guide_params = _guide_params
 Context.guide_set_params = _guide_set_params
 Context.guide_forward = _guide_forward

@@ -133,6 +133,7 @@ struct WfArgs {
     const uint8_t* gwt8;     // FP8 weights (k_quant8) and their inverse scales
     const float* ginv8;
     int guide_fp8;           // guide inference in FP8
+    int guide_lobes;         // 8 (width-64 network) or 16 (width 128); lobes holds 5 x this per entry
     const float* gparams;
     int W, H, S, max_depth;
     uint64_t seed;
@@ -350,35 +351,39 @@ __global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_extend(WfArgs a, int cur)
 
 // ------------------------------------------------------------------ learned guide in the path
 
+// Decoded mixture of one queue entry, NL lobes: weights lob[k], kappa lob[NL + k],
+// mean lob[2 NL + k], lob[3 NL + k], lob[4 NL + k].
+template <int NL>
 __device__ __forceinline__ float gmix_pdf(const float* lob, V3 w) {
     float p = 0.0f;
 #pragma unroll
-    for (int k = 0; k < guide::NLOBES; k++) {
-        float c = lob[16 + k] * w.x + lob[24 + k] * w.y + lob[32 + k] * w.z;
-        p += lob[k] * guide::vmf_pdf(lob[8 + k], c);
+    for (int k = 0; k < NL; k++) {
+        float c = lob[2 * NL + k] * w.x + lob[3 * NL + k] * w.y + lob[4 * NL + k] * w.z;
+        p += lob[k] * guide::vmf_pdf(lob[NL + k], c);
     }
     return p;
 }
 
+template <int NL>
 __device__ __forceinline__ V3 gmix_sample(const float* lob, float u0, float u1, float u2) {
     // lobe by its weight, then a vMF sample around its mean (read from memory:
     // a local array indexed by the chosen lobe would live in scratch)
-    int k = guide::NLOBES - 1;
+    int k = NL - 1;
     float c = 0.0f;
-    for (int j = 0; j < guide::NLOBES; j++) {
+    for (int j = 0; j < NL; j++) {
         c += lob[j];
         if (u0 < c) {
             k = j;
             break;
         }
     }
-    float kap = lob[8 + k];
+    float kap = lob[NL + k];
     float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kap)) / kap;
     w = fminf(fmaxf(w, -1.0f), 1.0f);
     float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     float sn, co;
     __sincosf(guide::TWO_PI * u2, &sn, &co);
-    float mx = lob[16 + k], my = lob[24 + k], mz = lob[32 + k];
+    float mx = lob[2 * NL + k], my = lob[3 * NL + k], mz = lob[4 * NL + k];
     float sign = copysignf(1.0f, mz);
     float a = -1.0f / (sign + mz);
     float b = mx * my * a;
@@ -389,27 +394,36 @@ __device__ __forceinline__ V3 gmix_sample(const float* lob, float u0, float u1, 
     return {d.x / n, d.y / n, d.z / n};
 }
 
+// the guide's lobe count is 8 (width-64 network) or 16 (width 128)
+__device__ __forceinline__ float gmix_pdf_n(const float* lob, int nl, V3 w) {
+    return nl == 16 ? gmix_pdf<16>(lob, w) : gmix_pdf<8>(lob, w);
+}
+__device__ __forceinline__ V3 gmix_sample_n(const float* lob, int nl, float u0, float u1, float u2) {
+    return nl == 16 ? gmix_sample<16>(lob, u0, u1, u2) : gmix_sample<8>(lob, u0, u1, u2);
+}
+
 __device__ __forceinline__ bool guidable(const Mat& m) {
     return m.type == CPT_MAT_DIFFUSE || (m.type == CPT_MAT_GLOSSY && m.rough >= 0.01f);
 }
 
 // Per bounce: network inputs of every queued hit, one MLP forward per tile of 64,
-// decoded mixture written per queue entry.
-// FP8: the e4m3 network (k_quant8's weights) on FP8 MFMA; else BF16
-template <bool FP8>
+// decoded mixture written per queue entry (5 x lobes floats).
+// WIDTH: 64 (namespace guide) or 128 (guide128); FP8: the e4m3 network (k_quant8's
+// weights) on FP8 MFMA, else BF16
+template <int WIDTH, bool FP8>
 __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    guide::SmemFwd& sm = *reinterpret_cast<guide::SmemFwd*>(smem_raw);
-    guide::SmemFwd8& sm8 = *reinterpret_cast<guide::SmemFwd8*>(smem_raw);
-    __shared__ float o_tile[guide::TILE * guide::NOUT];
+    constexpr int NOUT = WIDTH == 128 ? guide128::NOUT : guide::NOUT;
+    constexpr int LSTRIDE = 5 * (WIDTH == 128 ? guide128::NLOBES : guide::NLOBES);
+    __shared__ float o_tile[guide::TILE * NOUT];
     // features of a group of 4 tiles, one sample per thread: the hit record,
     // material and feature maths are latency-bound, so all four waves take part
     __shared__ float in_s[4 * guide::TILE * guide::NIN];
     constexpr unsigned GROUP = 4 * guide::TILE;
     const unsigned count = a.counters[2 * cur];
     if (blockIdx.x * GROUP >= count) return;
-    if constexpr (FP8) guide::load_weights8(sm8, a.gwt8, a.gparams, a.ginv8);
-    else guide::load_weights(sm, a.gwt, a.gparams);
+    if constexpr (WIDTH == 128) guide128::infer_load<FP8>(smem_raw, a.gwt, a.gwt8, a.gparams, a.ginv8);
+    else guide::infer_load<FP8>(smem_raw, a.gwt, a.gwt8, a.gparams, a.ginv8);
     for (unsigned g = blockIdx.x; g * GROUP < count; g += gridDim.x) {
         const unsigned eg = g * GROUP;
         const int ng = (int)min(GROUP, count - eg);
@@ -440,29 +454,10 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
         for (int q = 0; q * guide::TILE < ng; q++) {
             const unsigned e0 = eg + q * guide::TILE;
             const int nt = min(guide::TILE, ng - q * guide::TILE);
-            if constexpr (FP8) {
-                guide::encode_tile8(sm8, in_s + q * guide::TILE * guide::NIN, nt);
-                __syncthreads();
-                guide::forward_tile8(sm8, o_tile);
-            } else {
-                guide::encode_tile(sm, in_s + q * guide::TILE * guide::NIN, nt);
-                __syncthreads();
-                guide::forward_tile(sm, o_tile);
-            }
-            if ((int)threadIdx.x < nt) {
-                guide::Lobes L;
-                guide::decode(o_tile + threadIdx.x * guide::NOUT, L);
-                float* dst = a.lobes + (size_t)(e0 + threadIdx.x) * 40;
-#pragma unroll
-                for (int k = 0; k < guide::NLOBES; k++) {
-                    dst[k] = L.pi[k];
-                    dst[8 + k] = L.kappa[k];
-                    dst[16 + k] = L.mx[k];
-                    dst[24 + k] = L.my[k];
-                    dst[32 + k] = L.mz[k];
-                }
-            }
-            __syncthreads();
+            const float* in = in_s + q * guide::TILE * guide::NIN;
+            float* dst = a.lobes + (size_t)e0 * LSTRIDE;
+            if constexpr (WIDTH == 128) guide128::infer_tile<FP8>(smem_raw, a.gwt, o_tile, in, nt, dst);
+            else guide::infer_tile<FP8>(smem_raw, a.gwt, o_tile, in, nt, dst);
         }
     }
 }
@@ -575,7 +570,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                     }
                 } else {
                     sampled_delta = false;
-                    const float* lob = GUIDED ? a.lobes + (size_t)i * 40 : nullptr;
+                    const float* lob = GUIDED ? a.lobes + (size_t)i * 5 * a.guide_lobes : nullptr;
                     const bool gv = GUIDED && depth < a.guide_depth && guidable(m);
                     const float alpha = a.guide_frac;
                     DSample ds;
@@ -585,13 +580,13 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                         if (u < alpha) {
                             float u1 = pcg_float(rng);
                             float u2 = pcg_float(rng);
-                            ds.wi = gmix_sample(lob, u / alpha, u1, u2);
+                            ds.wi = gmix_sample_n(lob, a.guide_lobes, u / alpha, u1, u2);
                             ds.delta = false;
                             ds.pdf = bsdf_pdf(m, it, wo, ds.wi);
                         } else {
                             ds = bsdf_sample(m, it, wo, rng);
                         }
-                        ds.pdf = alpha * gmix_pdf(lob, ds.wi) + (1.0f - alpha) * ds.pdf;
+                        ds.pdf = alpha * gmix_pdf_n(lob, a.guide_lobes, ds.wi) + (1.0f - alpha) * ds.pdf;
                     } else {
                         ds = bsdf_sample(m, it, wo, rng);
                     }
@@ -605,7 +600,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                             sdir = divs(sdir, len);
                             if (bsdf_non_zero(m, it, wo, sdir)) {
                                 float pnb = bsdf_pdf(m, it, wo, sdir);
-                                if (gv) pnb = alpha * gmix_pdf(lob, sdir) + (1.0f - alpha) * pnb;
+                                if (gv) pnb = alpha * gmix_pdf_n(lob, a.guide_lobes, sdir) + (1.0f - alpha) * pnb;
                                 float ct = (float)(fabs((double)dot(ls.normal, sdir)) + 1e-4);
                                 float pnn = ls.pdf * len * len / ct;
                                 float w = pnn / (pnb + pnn);
@@ -1231,8 +1226,9 @@ struct cpt_ctx {
     float* g_wscale = nullptr;
     int g_step = 0;
     cpt_guide_params g_cfg{};
-    float* g_lobes = nullptr;    // per queue entry mixture (40 floats)
-    size_t g_lobes_cap = 0;
+    int g_width = 0;             // network of the allocated guide buffers: 64 or 128 (0: none)
+    float* g_lobes = nullptr;    // per queue entry mixture (5 floats per lobe)
+    size_t g_lobes_cap = 0;      // floats
     float* g_rec = nullptr;      // training records
     size_t g_rec_cap = 0;
     uint32_t* g_count = nullptr;
@@ -1451,6 +1447,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.gwt8 = c->g_wt8;
     a.ginv8 = c->g_inv8;
     a.guide_fp8 = c->g_cfg.infer_fp8 != 0;
+    a.guide_lobes = c->g_width == 128 ? 16 : 8;
     a.gparams = c->g_params;
     a.W = W;
     a.H = H;
@@ -1570,13 +1567,20 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
         L_extend(L, a, cur, s);
         mark(0);
         if (L.guided && b < a.guide_depth) {
-            // the FP8 tile state (77 KB of LDS) lets two blocks share a CU
-            if (a.guide_fp8) {
-                hipLaunchKernelGGL(k_guide_infer<true>, dim3(2 * L.infer_blocks), dim3(256), sizeof(guide::SmemFwd8), s,
-                                   a, cur);
+            // the width-64 FP8 tile state (77 KB of LDS) lets two blocks share a CU
+            if (a.guide_lobes == 16) {
+                if (a.guide_fp8)
+                    hipLaunchKernelGGL((k_guide_infer<128, true>), dim3(L.infer_blocks), dim3(256),
+                                       sizeof(guide128::SmemFwd8), s, a, cur);
+                else
+                    hipLaunchKernelGGL((k_guide_infer<128, false>), dim3(L.infer_blocks), dim3(256),
+                                       sizeof(guide128::SmemFwd), s, a, cur);
+            } else if (a.guide_fp8) {
+                hipLaunchKernelGGL((k_guide_infer<64, true>), dim3(2 * L.infer_blocks), dim3(256),
+                                   sizeof(guide::SmemFwd8), s, a, cur);
             } else {
-                hipLaunchKernelGGL(k_guide_infer<false>, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a,
-                                   cur);
+                hipLaunchKernelGGL((k_guide_infer<64, false>), dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd),
+                                   s, a, cur);
             }
         }
         L_shade(L, a, cur, a.max_depth, s);
@@ -2193,11 +2197,12 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     size_t nrec = 0;
     if (guided) {
         if (!c->has_guide) return fail(c, CPT_E_INVALID, "guided render needs cpt_guide_init");
-        if (c->g_lobes_cap < slots || !c->g_lobes) {
+        const size_t lfloats = slots * 5 * (c->g_width == 128 ? 16 : 8);
+        if (c->g_lobes_cap < lfloats || !c->g_lobes) {
             drop_graph(c);
-            r = dalloc(c, c->g_lobes, slots * 40);
+            r = dalloc(c, c->g_lobes, lfloats);
             if (r) return r;
-            c->g_lobes_cap = slots;
+            c->g_lobes_cap = lfloats;
         }
         if (train) {
             nrec = std::min<size_t>(slots, (size_t)c->g_cfg.train_paths);
@@ -2284,7 +2289,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
                  max_depth, (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
                  (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator,
-                 a.corrected * 2 + a.guide_fp8,
+                 a.guide_lobes * 4 + a.corrected * 2 + a.guide_fp8,
                  (void*)c->sd_snode, (void*)c->sd_dinfo, (void*)c->sd_quad, (void*)c->sd_rect, (void*)c->sd_sum,
                  (void*)c->ppg_vtx, (void*)c->ppg_nv, a.sdt.bmin[0], a.sdt.bmin[1], a.sdt.bmin[2], a.sdt.bmax[0],
                  a.sdt.bmax[1], a.sdt.bmax[2]);
@@ -2831,44 +2836,60 @@ static const int GUIDE_DW_BLOCKS = 256;
 
 int cpt_guide_num_params(void) { return guide::NPARAM; }
 
-int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
-    using namespace guide;
-    if (!c || !gp) return CPT_E_INVALID;
-    HIPCHK(c, hipSetDevice(c->device));
+}  // extern "C"
+
+namespace {
+
+// guide buffers of one network width (freed first when the width changes)
+template <class N>
+int guide_alloc(cpt_ctx* c) {
+    if (c->g_width == N::kHid && c->g_params) return CPT_OK;
+    drop_graph(c);
+    dfree(c->g_params);
+    dfree(c->g_m);
+    dfree(c->g_v);
+    dfree(c->g_grad);
+    dfree(c->g_wt);
+    dfree(c->g_wt8);
+    dfree(c->g_partial);
+    dfree(c->g_rows);
+    c->g_rows_cap = 0;
+    c->g_width = 0;
     int r = 0;
-    if (!c->g_params) {
-        r |= dalloc(c, c->g_params, NPARAM);
-        r |= dalloc(c, c->g_m, NPARAM);
-        r |= dalloc(c, c->g_v, NPARAM);
-        r |= dalloc(c, c->g_grad, NPARAM);
-        r |= dalloc(c, c->g_wt, NWT);
-        r |= dalloc(c, c->g_wt8, NWQ);
-        r |= dalloc(c, c->g_inv8, 8);
-        r |= dalloc(c, c->g_partial, (size_t)GUIDE_DW_BLOCKS * NPARAM);
-        r |= dalloc(c, c->g_loss_part, GUIDE_TRAIN_BLOCKS);
-        r |= dalloc(c, c->g_wscale, 1);
-        if (r) return CPT_E_HIP;
-        HIPCHK(c, hipFuncSetAttribute((const void*)k_mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
-        HIPCHK(c, hipFuncSetAttribute((const void*)k_train_tile, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
-        HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<false>, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd)));
-        HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<true>, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd8)));
-        HIPCHK(c, hipFuncSetAttribute((const void*)k_mlp_forward8, hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(SmemFwd8)));
-        r |= dalloc(c, c->g_count, 4);
-        if (r) return CPT_E_HIP;
-        HIPCHK(c, hipHostMalloc((void**)&c->h_count, 4 * sizeof(uint32_t), hipHostMallocDefault));
-    }
-    c->g_cfg = *gp;
-    if (c->g_cfg.lr <= 0) c->g_cfg.lr = 1e-3f;
-    if (c->g_cfg.beta1 <= 0) c->g_cfg.beta1 = 0.9f;
-    if (c->g_cfg.beta2 <= 0) c->g_cfg.beta2 = 0.99f;
-    if (c->g_cfg.eps <= 0) c->g_cfg.eps = 1e-8f;
-    if (c->g_cfg.guide_fraction <= 0 || c->g_cfg.guide_fraction >= 1) c->g_cfg.guide_fraction = 0.5f;
-    if (c->g_cfg.train_paths <= 0) c->g_cfg.train_paths = 65536;
-    if (c->g_cfg.train_steps <= 0) c->g_cfg.train_steps = 1;
-    // Glorot-uniform weights from a seeded splitmix64 stream; zero biases except the
-    // lobe means, which start at the 8 cube diagonals (a near-uniform mixture)
-    std::vector<float> h(NPARAM, 0.0f);
-    uint64_t st = gp->seed ^ 0x243F6A8885A308D3ull;
+    r |= dalloc(c, c->g_params, N::kParams);
+    r |= dalloc(c, c->g_m, N::kParams);
+    r |= dalloc(c, c->g_v, N::kParams);
+    r |= dalloc(c, c->g_grad, N::kParams);
+    r |= dalloc(c, c->g_wt, N::kWt);
+    r |= dalloc(c, c->g_wt8, N::kWq);
+    r |= dalloc(c, c->g_partial, (size_t)GUIDE_DW_BLOCKS * N::kParams);
+    if (r) return CPT_E_HIP;
+    HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd));
+    HIPCHK(c, hipFuncSetAttribute((const void*)N::train_tile, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd));
+    HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_forward8, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd8));
+    HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<N::kHid, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  N::kSmemFwd));
+    HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<N::kHid, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  N::kSmemFwd8));
+    c->g_width = N::kHid;
+    return CPT_OK;
+}
+
+// BF16 transposed copy and FP8 copy of the parameters at c->g_params
+template <class N>
+void guide_pack(cpt_ctx* c) {
+    hipLaunchKernelGGL(N::pack_weights, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
+    hipLaunchKernelGGL(N::pack_weights2, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
+    hipLaunchKernelGGL(N::quant8, dim3(5), dim3(256), 0, c->stream, c->g_params, c->g_wt8, c->g_inv8);
+}
+
+// Glorot-uniform weights from a seeded splitmix64 stream; zero biases except the
+// lobe means, which start at the 8 cube diagonals (a near-uniform mixture; with
+// 16 lobes each diagonal twice)
+template <class N>
+int guide_init_params(cpt_ctx* c, uint64_t seed) {
+    std::vector<float> h(N::kParams, 0.0f);
+    uint64_t st = seed ^ 0x243F6A8885A308D3ull;
     auto rnd = [&]() {
         st += 0x9E3779B97F4A7C15ull;
         uint64_t z = st;
@@ -2877,26 +2898,115 @@ int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
         z ^= z >> 31;
         return (float)((z >> 40) * (1.0 / 16777216.0)) * 2.0f - 1.0f;
     };
-    const int offs[5] = {W0, W1, W2, W3, W4}, ks[5] = {K0, HID, HID, HID, HID}, ns[5] = {HID, HID, HID, HID, NOUT};
+    const int ks[5] = {guide::K0, N::kHid, N::kHid, N::kHid, N::kHid};
+    const int ns[5] = {N::kHid, N::kHid, N::kHid, N::kHid, N::kOut};
     for (int l = 0; l < 5; l++) {
         float a = std::sqrt(6.0f / (float)(ks[l] + ns[l]));
         if (l == 4) a *= 0.1f;
-        for (int i = 0; i < ks[l] * ns[l]; i++) h[offs[l] + i] = a * rnd();
+        for (int i = 0; i < ks[l] * ns[l]; i++) h[N::kW[l] + i] = a * rnd();
     }
-    for (int k = 0; k < NLOBES; k++) {
-        h[B4 + 5 * k + 2] = (k & 1) ? 1.0f : -1.0f;
-        h[B4 + 5 * k + 3] = (k & 2) ? 1.0f : -1.0f;
-        h[B4 + 5 * k + 4] = (k & 4) ? 1.0f : -1.0f;
+    for (int k = 0; k < N::kLobes; k++) {
+        h[N::kB4 + 5 * k + 2] = (k & 1) ? 1.0f : -1.0f;
+        h[N::kB4 + 5 * k + 3] = (k & 2) ? 1.0f : -1.0f;
+        h[N::kB4 + 5 * k + 4] = (k & 4) ? 1.0f : -1.0f;
     }
-    HIPCHK(c, hipMemcpy(c->g_params, h.data(), NPARAM * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->g_m, 0, NPARAM * sizeof(float)));
-    HIPCHK(c, hipMemset(c->g_v, 0, NPARAM * sizeof(float)));
-    hipLaunchKernelGGL(k_pack_weights, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
-    hipLaunchKernelGGL(k_pack_weights2, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
-    hipLaunchKernelGGL(k_quant8, dim3(5), dim3(256), 0, c->stream, c->g_params, c->g_wt8, c->g_inv8);
+    HIPCHK(c, hipMemcpy(c->g_params, h.data(), N::kParams * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->g_m, 0, N::kParams * sizeof(float)));
+    HIPCHK(c, hipMemset(c->g_v, 0, N::kParams * sizeof(float)));
+    guide_pack<N>(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CPT_OK;
+}
+
+template <class N>
+int guide_forward_impl(cpt_ctx* c, const float* d_in, int n, float* d_out) {
+    int G = std::min(c->cus, (n + N::kTile - 1) / N::kTile);
+    if (c->g_cfg.infer_fp8)
+        hipLaunchKernelGGL(N::mlp_forward8, dim3(G), dim3(256), N::kSmemFwd8, c->stream, c->g_wt8, c->g_params,
+                           c->g_inv8, d_in, n, d_out);
+    else
+        hipLaunchKernelGGL(N::mlp_forward, dim3(G), dim3(256), N::kSmemFwd, c->stream, c->g_wt, c->g_params, d_in, n,
+                           d_out);
+    return hipGetLastError() == hipSuccess ? CPT_OK : CPT_E_HIP;
+}
+
+// One Adam step on device-resident samples (n x 11 floats at c->g_samples).
+template <class N>
+int guide_train_impl(cpt_ctx* c, int n, bool apply) {
+    if (c->g_rows_cap < (size_t)n) {
+        int r = dalloc(c, c->g_rows, (size_t)n * N::kRow);
+        if (r) return r;
+        c->g_rows_cap = n;
+    }
+    int per_block = ((n + GUIDE_DW_BLOCKS - 1) / GUIDE_DW_BLOCKS + N::kChunk - 1) / N::kChunk * N::kChunk;
+    hipLaunchKernelGGL(N::weight_scale, dim3(1), dim3(256), 0, c->stream, c->g_samples, n, c->g_wscale);
+    hipLaunchKernelGGL(N::train_tile, dim3(GUIDE_TRAIN_BLOCKS), dim3(256), N::kSmemFwd, c->stream, c->g_wt,
+                       c->g_params, c->g_samples, n, c->g_wscale, c->g_rows, c->g_loss_part);
+    hipLaunchKernelGGL(N::train_dw, dim3(GUIDE_DW_BLOCKS, N::kDwGroups), dim3(256), 0, c->stream, c->g_rows, n,
+                       per_block, c->g_partial);
+    hipLaunchKernelGGL(N::reduce_partials, dim3(64), dim3(256), 0, c->stream, c->g_partial, GUIDE_DW_BLOCKS,
+                       N::kParams, N::kParams, c->g_grad);
+    if (apply) {
+        c->g_step++;
+        hipLaunchKernelGGL(N::adam, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_m, c->g_v, c->g_grad,
+                           c->g_step, c->g_cfg.lr, c->g_cfg.beta1, c->g_cfg.beta2, c->g_cfg.eps, c->g_wt);
+        if (c->g_cfg.infer_fp8)
+            hipLaunchKernelGGL(N::quant8, dim3(5), dim3(256), 0, c->stream, c->g_params, c->g_wt8, c->g_inv8);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, CPT_E_HIP, "guide training launch failed: %s", hipGetErrorString(e));
+    return CPT_OK;
+}
+
+int guide_nparams(const cpt_ctx* c) { return c->g_width == 128 ? guide128::NPARAM : guide::NPARAM; }
+
+}  // namespace
+
+static int guide_train_device(cpt_ctx* c, int n, bool apply) {
+    return c->g_width == 128 ? guide_train_impl<guide128::Net>(c, n, apply) : guide_train_impl<guide::Net>(c, n, apply);
+}
+
+extern "C" {
+
+int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
+    if (!c || !gp) return CPT_E_INVALID;
+    if (gp->width != 0 && gp->width != 64 && gp->width != 128)
+        return fail(c, CPT_E_INVALID, "guide width %d: 64 or 128", (int)gp->width);
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool wide = gp->width == 128;
+    int r = wide ? guide_alloc<guide128::Net>(c) : guide_alloc<guide::Net>(c);
+    if (r) return r;
+    if (!c->g_count) {
+        r |= dalloc(c, c->g_inv8, 8);
+        r |= dalloc(c, c->g_loss_part, GUIDE_TRAIN_BLOCKS);
+        r |= dalloc(c, c->g_wscale, 1);
+        r |= dalloc(c, c->g_count, 4);
+        if (r) return CPT_E_HIP;
+        HIPCHK(c, hipHostMalloc((void**)&c->h_count, 4 * sizeof(uint32_t), hipHostMallocDefault));
+    }
+    c->g_cfg = *gp;
+    c->g_cfg.width = wide ? 128 : 64;
+    if (c->g_cfg.lr <= 0) c->g_cfg.lr = 1e-3f;
+    if (c->g_cfg.beta1 <= 0) c->g_cfg.beta1 = 0.9f;
+    if (c->g_cfg.beta2 <= 0) c->g_cfg.beta2 = 0.99f;
+    if (c->g_cfg.eps <= 0) c->g_cfg.eps = 1e-8f;
+    if (c->g_cfg.guide_fraction <= 0 || c->g_cfg.guide_fraction >= 1) c->g_cfg.guide_fraction = 0.5f;
+    if (c->g_cfg.train_paths <= 0) c->g_cfg.train_paths = 65536;
+    if (c->g_cfg.train_steps <= 0) c->g_cfg.train_steps = 1;
+    r = wide ? guide_init_params<guide128::Net>(c, gp->seed) : guide_init_params<guide::Net>(c, gp->seed);
+    if (r) return r;
     c->g_step = 0;
     c->has_guide = true;
+    return CPT_OK;
+}
+
+int cpt_guide_shape(cpt_ctx* c, int32_t* shape4) {
+    if (!c || !shape4) return CPT_E_INVALID;
+    const bool wide = c->has_guide && c->g_width == 128;
+    shape4[0] = wide ? guide128::HID : guide::HID;
+    shape4[1] = wide ? guide128::NLOBES : guide::NLOBES;
+    shape4[2] = wide ? guide128::NOUT : guide::NOUT;
+    shape4[3] = wide ? guide128::NPARAM : guide::NPARAM;
     return CPT_OK;
 }
 
@@ -2904,82 +3014,50 @@ int cpt_guide_get_params(cpt_ctx* c, float* out) {
     if (!c || !out) return CPT_E_INVALID;
     if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(out, c->g_params, guide::NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(out, c->g_params, guide_nparams(c) * sizeof(float), hipMemcpyDeviceToHost));
     return CPT_OK;
 }
 
 int cpt_guide_set_params(cpt_ctx* c, const float* in) {
     if (!c || !in) return CPT_E_INVALID;
     if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
+    const int np = guide_nparams(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(c->g_params, in, guide::NPARAM * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(c->g_m, 0, guide::NPARAM * sizeof(float)));
-    HIPCHK(c, hipMemset(c->g_v, 0, guide::NPARAM * sizeof(float)));
-    hipLaunchKernelGGL(guide::k_pack_weights, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
-    hipLaunchKernelGGL(guide::k_pack_weights2, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_wt);
-    hipLaunchKernelGGL(guide::k_quant8, dim3(5), dim3(256), 0, c->stream, c->g_params, c->g_wt8, c->g_inv8);
+    HIPCHK(c, hipMemcpy(c->g_params, in, np * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(c->g_m, 0, np * sizeof(float)));
+    HIPCHK(c, hipMemset(c->g_v, 0, np * sizeof(float)));
+    if (c->g_width == 128) guide_pack<guide128::Net>(c);
+    else guide_pack<guide::Net>(c);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->g_step = 0;
     return CPT_OK;
 }
 
 int cpt_guide_forward(cpt_ctx* c, const float* inputs, int32_t n, float* outputs) {
-    using namespace guide;
     if (!c || n < 0 || (n > 0 && (!inputs || !outputs))) return CPT_E_INVALID;
     if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
     if (n == 0) return CPT_OK;
+    const int nin = guide::NIN, nout = c->g_width == 128 ? guide128::NOUT : guide::NOUT;
     float *d_in = nullptr, *d_out = nullptr;
-    HIPCHK(c, hipMalloc((void**)&d_in, sizeof(float) * NIN * (size_t)n));
-    if (hipMalloc((void**)&d_out, sizeof(float) * NOUT * (size_t)n) != hipSuccess) {
+    HIPCHK(c, hipMalloc((void**)&d_in, sizeof(float) * nin * (size_t)n));
+    if (hipMalloc((void**)&d_out, sizeof(float) * nout * (size_t)n) != hipSuccess) {
         (void)hipFree(d_in);
         return fail(c, CPT_E_HIP, "hipMalloc failed");
     }
     int rc = CPT_OK;
-    if (hipMemcpyAsync(d_in, inputs, sizeof(float) * NIN * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    if (hipMemcpyAsync(d_in, inputs, sizeof(float) * nin * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess)
         rc = fail(c, CPT_E_HIP, "upload failed");
     if (!rc) {
-        int G = std::min(c->cus, (n + TILE - 1) / TILE);
-        if (c->g_cfg.infer_fp8)
-            hipLaunchKernelGGL(k_mlp_forward8, dim3(G), dim3(256), sizeof(SmemFwd8), c->stream, c->g_wt8, c->g_params,
-                               c->g_inv8, d_in, n, d_out);
-        else
-            hipLaunchKernelGGL(k_mlp_forward, dim3(G), dim3(256), sizeof(SmemFwd), c->stream, c->g_wt, c->g_params, d_in,
-                               n, d_out);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(outputs, d_out, sizeof(float) * NOUT * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        int lr = c->g_width == 128 ? guide_forward_impl<guide128::Net>(c, d_in, n, d_out)
+                                   : guide_forward_impl<guide::Net>(c, d_in, n, d_out);
+        if (lr != CPT_OK ||
+            hipMemcpyAsync(outputs, d_out, sizeof(float) * nout * (size_t)n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             rc = fail(c, CPT_E_HIP, "guide forward failed");
     }
     (void)hipFree(d_in);
     (void)hipFree(d_out);
     return rc;
-}
-
-// One Adam step on device-resident samples (n x 11 floats at c->g_samples).
-static int guide_train_device(cpt_ctx* c, int n, bool apply) {
-    using namespace guide;
-    if (c->g_rows_cap < (size_t)n) {
-        int r = dalloc(c, c->g_rows, (size_t)n * ROW);
-        if (r) return r;
-        c->g_rows_cap = n;
-    }
-    int per_block = ((n + GUIDE_DW_BLOCKS - 1) / GUIDE_DW_BLOCKS + CH - 1) / CH * CH;
-    hipLaunchKernelGGL(k_weight_scale, dim3(1), dim3(256), 0, c->stream, c->g_samples, n, c->g_wscale);
-    hipLaunchKernelGGL(k_train_tile, dim3(GUIDE_TRAIN_BLOCKS), dim3(256), sizeof(SmemFwd), c->stream, c->g_wt,
-                       c->g_params, c->g_samples, n, c->g_wscale, c->g_rows, c->g_loss_part);
-    hipLaunchKernelGGL(k_train_dw, dim3(GUIDE_DW_BLOCKS), dim3(256), 0, c->stream, c->g_rows, n, per_block, c->g_partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(64), dim3(256), 0, c->stream, c->g_partial, GUIDE_DW_BLOCKS, NPARAM,
-                       NPARAM, c->g_grad);
-    if (apply) {
-        c->g_step++;
-        hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, c->stream, c->g_params, c->g_m, c->g_v, c->g_grad, c->g_step,
-                           c->g_cfg.lr, c->g_cfg.beta1, c->g_cfg.beta2, c->g_cfg.eps, c->g_wt);
-        if (c->g_cfg.infer_fp8)
-            hipLaunchKernelGGL(k_quant8, dim3(5), dim3(256), 0, c->stream, c->g_params, c->g_wt8, c->g_inv8);
-    }
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return fail(c, CPT_E_HIP, "guide training launch failed: %s", hipGetErrorString(e));
-    return CPT_OK;
 }
 
 int cpt_guide_train_step(cpt_ctx* c, const float* samples, int32_t n, int32_t apply, float* loss, float* grad) {
@@ -2997,7 +3075,7 @@ int cpt_guide_train_step(cpt_ctx* c, const float* samples, int32_t n, int32_t ap
     if (r) return r;
     std::vector<float> lp(GUIDE_TRAIN_BLOCKS);
     HIPCHK(c, hipMemcpyAsync(lp.data(), c->g_loss_part, sizeof(float) * GUIDE_TRAIN_BLOCKS, hipMemcpyDeviceToHost, c->stream));
-    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->g_grad, sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c->stream));
+    if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->g_grad, sizeof(float) * guide_nparams(c), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (loss) {
         double s = 0;

@@ -12,7 +12,9 @@
 //
 // Network: one-blob encoding of 7 inputs x 32 bins = 224 features ->
 // 4 hidden layers of width 64 (ReLU) -> 48 outputs (8 lobes x [logit,
-// log kappa, mu.xyz], 8 spare).  BF16 weights and activations on MFMA
+// log kappa, mu.xyz], 8 spare), or width 128 -> 80 outputs (16 lobes) for
+// large scenes; the two are instances of guide_net.h in the namespaces guide
+// and guide128.  BF16 weights and activations on MFMA
 // (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; FP32 master weights and
 // Adam moments.  A block of 4 waves processes tiles of 64 samples; wave w owns
 // rows 16w..16w+15 of every activation matrix, so the layers of one tile need
@@ -22,773 +24,21 @@
 #include <stdint.h>
 
 namespace cpt {
-namespace guide {
 
-constexpr int NIN = 7;          // pos.xyz, normal (theta, phi), wo (theta, phi), all in [0,1]
-constexpr int NBINS = 32;
-constexpr int K0 = NIN * NBINS; // 224 input features
-constexpr int HID = 64;
-constexpr int NLOBES = 8;
-constexpr int NOUT = 48;        // 8 x 5 used
-constexpr int TILE = 64;        // samples per block tile
-constexpr int PAD = 12;         // bf16 padding of LDS rows (12: fewest bank conflicts measured)
-constexpr int LD0 = K0 + PAD;   // 236
-constexpr int LDH = HID + PAD;  // 76
-constexpr int LDO = NOUT + PAD; // 60
-
-// parameter offsets in the flat FP32 vector: W_l (K_l x N_l, row-major), then b_l
-constexpr int W0 = 0;
-constexpr int B0 = W0 + K0 * HID;
-constexpr int W1 = B0 + HID;
-constexpr int B1 = W1 + HID * HID;
-constexpr int W2 = B1 + HID;
-constexpr int B2 = W2 + HID * HID;
-constexpr int W3 = B2 + HID;
-constexpr int B3 = W3 + HID * HID;
-constexpr int W4 = B3 + HID;
-constexpr int B4 = W4 + HID * NOUT;
-constexpr int NPARAM = B4 + NOUT;  // 30000
-
-// transposed BF16 weights as stored in global memory and LDS: Wt_l[n][k]
-constexpr int T0 = 0;
-constexpr int T1 = T0 + HID * LD0;
-constexpr int T2 = T1 + HID * LDH;
-constexpr int T3 = T2 + HID * LDH;
-constexpr int T4 = T3 + HID * LDH;
-constexpr int NWT = T4 + NOUT * LDH;  // bf16 elements
-
-constexpr float KAPPA_LOG_MIN = -3.0f, KAPPA_LOG_MAX = 6.0f;
-constexpr float TWO_PI = 6.28318530717958647692f;
-
-typedef short s4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-
-// one-blob encoding: bin b of input x gets exp(-0.5 ((x*NBINS - (b+0.5)))^2)
-__device__ __forceinline__ float blob(float x, int b) {
-    float d = x * (float)NBINS - ((float)b + 0.5f);
-    return __expf(-0.5f * d * d);
-}
-
-// acc[nt] += A(rows of this wave) x Wt^T, A row-major (lda), Wt n-major (ldb):
-// lane l supplies A[l%16][k0 + 4(l/16) + j] and B[k0 + 4(l/16) + j][n0 + l%16]
-template <int K, int NT>
-__device__ __forceinline__ void gemm_rows_nk(const uint16_t* A, int lda, const uint16_t* Wt, int ldb, f4* acc,
-                                             int lane) {
-#pragma unroll
-    for (int kt = 0; kt < K / 16; kt++) {
-        const int kk = kt * 16 + 4 * (lane >> 4);
-        s4 a = *reinterpret_cast<const s4*>(A + (lane & 15) * lda + kk);
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) {
-            s4 b = *reinterpret_cast<const s4*>(Wt + (nt * 16 + (lane & 15)) * ldb + kk);
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
-        }
-    }
-}
-
-// acc[nt] += A x B with B stored k-major (Bk[k][n], ldb): 4 strided reads per lane
-template <int K, int NT>
-__device__ __forceinline__ void gemm_rows_kn(const uint16_t* A, int lda, const uint16_t* Bk, int ldb, f4* acc,
-                                             int lane) {
-#pragma unroll
-    for (int kt = 0; kt < K / 16; kt++) {
-        const int kk = kt * 16 + 4 * (lane >> 4);
-        s4 a = *reinterpret_cast<const s4*>(A + (lane & 15) * lda + kk);
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) {
-            s4 b;
-            const uint16_t* bp = Bk + kk * ldb + nt * 16 + (lane & 15);
-            b[0] = (short)bp[0];
-            b[1] = (short)bp[ldb];
-            b[2] = (short)bp[2 * ldb];
-            b[3] = (short)bp[3 * ldb];
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
-        }
-    }
-}
-
-struct Lobes {
-    float pi[NLOBES], kappa[NLOBES], mx[NLOBES], my[NLOBES], mz[NLOBES];
-};
-
-// raw outputs -> mixture (softmax weights, kappa = exp(clamped), unit mean)
-__device__ __forceinline__ void decode(const float* o, Lobes& L) {
-    float mxv = -INFINITY;
-#pragma unroll
-    for (int k = 0; k < NLOBES; k++) mxv = fmaxf(mxv, o[5 * k]);
-    float s = 0.0f;
-#pragma unroll
-    for (int k = 0; k < NLOBES; k++) {
-        L.pi[k] = __expf(o[5 * k] - mxv);
-        s += L.pi[k];
-    }
-    float inv = 1.0f / s;
-#pragma unroll
-    for (int k = 0; k < NLOBES; k++) {
-        L.pi[k] *= inv;
-        L.kappa[k] = __expf(fminf(fmaxf(o[5 * k + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
-        float x = o[5 * k + 2], y = o[5 * k + 3], z = o[5 * k + 4];
-        float n = sqrtf(x * x + y * y + z * z);
-        if (n < 1e-12f) {
-            x = 0.0f;
-            y = 0.0f;
-            z = 1.0f;
-            n = 1.0f;
-        }
-        L.mx[k] = x / n;
-        L.my[k] = y / n;
-        L.mz[k] = z / n;
-    }
-}
-
-// vMF density kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (mu.w - 1))
-__device__ __forceinline__ float vmf_pdf(float kappa, float cosang) {
-    return kappa / (TWO_PI * (1.0f - __expf(-2.0f * kappa))) * __expf(kappa * (cosang - 1.0f));
-}
-
-__device__ __forceinline__ float mixture_pdf(const Lobes& L, float wx, float wy, float wz) {
-    float p = 0.0f;
-#pragma unroll
-    for (int k = 0; k < NLOBES; k++) p += L.pi[k] * vmf_pdf(L.kappa[k], L.mx[k] * wx + L.my[k] * wy + L.mz[k] * wz);
-    return p;
-}
-
-// lobe chosen by u0; vMF sample (Jakob 2012, "Numerically stable sampling of the von Mises Fisher distribution")
-__device__ __forceinline__ void mixture_sample(const Lobes& L, float u0, float u1, float u2, float& wx, float& wy,
-                                               float& wz) {
-    int k = NLOBES - 1;
-    float c = 0.0f;
-#pragma unroll
-    for (int j = 0; j < NLOBES; j++) {
-        c += L.pi[j];
-        if (u0 < c) {
-            k = j;
-            break;
-        }
-    }
-    float kap = L.kappa[k];
-    float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kap)) / kap;
-    w = fminf(fmaxf(w, -1.0f), 1.0f);
-    float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
-    float s, co;
-    __sincosf(TWO_PI * u2, &s, &co);
-    float mx = L.mx[k], my = L.my[k], mz = L.mz[k];
-    // orthonormal basis around mu (Duff et al. 2017)
-    float sign = copysignf(1.0f, mz);
-    float a = -1.0f / (sign + mz);
-    float b = mx * my * a;
-    float t1x = 1.0f + sign * mx * mx * a, t1y = sign * b, t1z = -sign * mx;
-    float t2x = b, t2y = sign + my * my * a, t2z = -my;
-    wx = w * mx + r * (co * t1x + s * t2x);
-    wy = w * my + r * (co * t1y + s * t2y);
-    wz = w * mz + r * (co * t1z + s * t2z);
-}
-
-// network inputs in [0,1]: position in the scene box, normal and wo as (theta/pi, phi/2pi)
-__device__ __forceinline__ void features(const float* bmin, const float* bext, float px, float py, float pz,
-                                         float nx, float ny, float nz, float ox, float oy, float oz, float* in) {
-    in[0] = fminf(fmaxf((px - bmin[0]) / bext[0], 0.0f), 1.0f);
-    in[1] = fminf(fmaxf((py - bmin[1]) / bext[1], 0.0f), 1.0f);
-    in[2] = fminf(fmaxf((pz - bmin[2]) / bext[2], 0.0f), 1.0f);
-    in[3] = acosf(fminf(fmaxf(nz, -1.0f), 1.0f)) * (1.0f / 3.14159265f);
-    in[4] = (atan2f(ny, nx) + 3.14159265f) * (1.0f / TWO_PI);
-    in[5] = acosf(fminf(fmaxf(oz, -1.0f), 1.0f)) * (1.0f / 3.14159265f);
-    in[6] = (atan2f(oy, ox) + 3.14159265f) * (1.0f / TWO_PI);
-}
-
-// --------------------------------------------------------------------------
-// Shared-memory tile: weights (transposed bf16), biases, activations.
-struct SmemFwd {
-    uint16_t wt[NWT];
-    float bias[4 * HID + NOUT];
-    uint16_t x0[TILE * LD0];
-    uint16_t h[4][TILE * LDH];
-};
-
-__device__ __forceinline__ void load_weights(SmemFwd& sm, const uint16_t* gwt, const float* params) {
-    const uint4* src = reinterpret_cast<const uint4*>(gwt);
-    uint4* dst = reinterpret_cast<uint4*>(sm.wt);
-    for (int i = threadIdx.x; i < NWT / 8; i += blockDim.x) dst[i] = src[i];
-    for (int i = threadIdx.x; i < 4 * HID + NOUT; i += blockDim.x) {
-        int off = i < HID ? B0 + i : i < 2 * HID ? B1 + i - HID : i < 3 * HID ? B2 + i - 2 * HID
-                : i < 4 * HID ? B3 + i - 3 * HID : B4 + i - 4 * HID;
-        sm.bias[i] = params[off];
-    }
-}
-
-// one-blob features of the tile's samples into sm.x0 (samples beyond n are zero)
-__device__ __forceinline__ void encode_tile(SmemFwd& sm, const float* in, int n) {
-    for (int i = threadIdx.x; i < TILE * K0; i += blockDim.x) {
-        int s = i / K0, f = i - s * K0;
-        int d = f / NBINS, b = f - d * NBINS;
-        float v = s < n ? blob(in[s * NIN + d], b) : 0.0f;
-        sm.x0[s * LD0 + f] = f2bf(v);
-    }
-}
-
-// forward through all layers; out (TILE x NOUT fp32, row stride NOUT) receives raw outputs
-__device__ __forceinline__ void forward_tile(SmemFwd& sm, float* out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r0 = 16 * wave;
-    {
-        f4 acc[4] = {};
-        gemm_rows_nk<K0, 4>(sm.x0 + r0 * LD0, LD0, sm.wt + T0, LD0, acc, lane);
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
-                float z = acc[nt][j] + sm.bias[n];
-                sm.h[0][m * LDH + n] = f2bf(fmaxf(z, 0.0f));
-            }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int l = 1; l < 4; l++) {
-        f4 acc[4] = {};
-        const int toff = l == 1 ? T1 : l == 2 ? T2 : T3;
-        gemm_rows_nk<HID, 4>(sm.h[l - 1] + r0 * LDH, LDH, sm.wt + toff, LDH, acc, lane);
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
-                float z = acc[nt][j] + sm.bias[l * HID + n];
-                sm.h[l][m * LDH + n] = f2bf(fmaxf(z, 0.0f));
-            }
-        __syncthreads();
-    }
-    {
-        f4 acc[3] = {};
-        gemm_rows_nk<HID, 3>(sm.h[3] + r0 * LDH, LDH, sm.wt + T4, LDH, acc, lane);
-#pragma unroll
-        for (int nt = 0; nt < 3; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
-                out[m * NOUT + n] = acc[nt][j] + sm.bias[4 * HID + n];
-            }
-    }
-    __syncthreads();
-}
-
-// d(-log q(w))/d outputs for one sample (scaled by wgt); returns -log q
-__device__ __forceinline__ float loss_grad(const float* o, float wx, float wy, float wz, float wgt, float* g) {
-    Lobes L;
-    decode(o, L);
-    float v[NLOBES], q = 0.0f;
-#pragma unroll
-    for (int k = 0; k < NLOBES; k++) {
-        v[k] = vmf_pdf(L.kappa[k], L.mx[k] * wx + L.my[k] * wy + L.mz[k] * wz);
-        q += L.pi[k] * v[k];
-    }
-    q = fmaxf(q, 1e-30f);
-#pragma unroll
-    for (int k = 0; k < NLOBES; k++) {
-        float r = L.pi[k] * v[k] / q;  // responsibility
-        g[5 * k] = -wgt * (r - L.pi[k]);
-        float kap = L.kappa[k];
-        float ls = o[5 * k + 1];
-        float e2 = __expf(-2.0f * kap);
-        float c = L.mx[k] * wx + L.my[k] * wy + L.mz[k] * wz;
-        float dlogv_dk = 1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c - 1.0f);
-        bool inside = ls > KAPPA_LOG_MIN && ls < KAPPA_LOG_MAX;
-        g[5 * k + 1] = inside ? -wgt * r * dlogv_dk * kap : 0.0f;
-        float ux = o[5 * k + 2], uy = o[5 * k + 3], uz = o[5 * k + 4];
-        float un = sqrtf(ux * ux + uy * uy + uz * uz);
-        float f = un < 1e-12f ? 0.0f : -wgt * r * kap / un;
-        g[5 * k + 2] = f * (wx - L.mx[k] * c);
-        g[5 * k + 3] = f * (wy - L.my[k] * c);
-        g[5 * k + 4] = f * (wz - L.mz[k] * c);
-    }
-#pragma unroll
-    for (int k = 5 * NLOBES; k < NOUT; k++) g[k] = 0.0f;
-    return -__logf(q);
-}
-
+namespace guide {  // hidden width 64, 8 lobes
+#define CPT_GUIDE_HID 64
+#define CPT_GUIDE_LOBES 8
+#include "guide_net.h"
+#undef CPT_GUIDE_HID
+#undef CPT_GUIDE_LOBES
 }  // namespace guide
-}  // namespace cpt
 
-namespace cpt {
-namespace guide {
+namespace guide128 {  // hidden width 128, 16 lobes
+#define CPT_GUIDE_HID 128
+#define CPT_GUIDE_LOBES 16
+#include "guide_net.h"
+#undef CPT_GUIDE_HID
+#undef CPT_GUIDE_LOBES
+}  // namespace guide128
 
-// Training rows in global memory (bf16, one row per sample):
-// [x0 224 | h0 64 | h1 64 | h2 64 | h3 64 | dz0 64 | dz1 64 | dz2 64 | dz3 64 | dz4 48]
-constexpr int RX0 = 0, RH = K0, RDZ = K0 + 4 * HID, RDZ4 = RDZ + 4 * HID;
-constexpr int ROW = RDZ4 + NOUT;  // 784
-
-// Training samples: 11 floats (7 inputs, direction xyz, weight = target / sampling pdf)
-constexpr int SAMPLE_FLOATS = 11;
-
-// Inference over n input vectors: raw outputs (n x NOUT).
-__global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* gwt, const float* params, const float* in,
-                                                     int n, float* out) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    SmemFwd& sm = *reinterpret_cast<SmemFwd*>(smem_raw);
-    __shared__ float o_tile[TILE * NOUT];
-    load_weights(sm, gwt, params);
-    __syncthreads();
-    for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
-        int nt = min(TILE, n - t * TILE);
-        encode_tile(sm, in + (size_t)t * TILE * NIN, nt);
-        __syncthreads();
-        forward_tile(sm, o_tile);
-        for (int i = threadIdx.x; i < nt * NOUT; i += blockDim.x) out[(size_t)t * TILE * NOUT + i] = o_tile[i];
-        __syncthreads();
-    }
-}
-
-// --------------------------------------------------------------------------
-// FP8 inference (e4m3, OCP format of gfx950) on v_mfma_f32_16x16x32_fp8_fp8.
-// Weights: per layer scaled by 448 / max|W_l| and rounded to e4m3 (k_quant8);
-// one-blob features scaled by 448; each hidden layer's ReLU outputs scaled per
-// tile by 448 / (their maximum), found with one block reduction.  FP32
-// accumulation; the accumulator is rescaled by the two inverse scales before the
-// bias.  The conversion does not saturate (it returns NaN above 448), so every
-// value is clamped first.  Lane layout of the 16x16x32 MFMA: lane l supplies
-// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j = 0..7, as 8 bytes.
-// Training keeps the BF16 path; only inference uses these weights.
-constexpr int PAD8 = 8;             // byte padding of FP8 LDS rows
-constexpr int LQ0 = K0 + PAD8;      // 232
-constexpr int LQH = HID + PAD8;     // 72
-constexpr int Q0 = 0;
-constexpr int Q1 = Q0 + HID * LQ0;
-constexpr int Q2 = Q1 + HID * LQH;
-constexpr int Q3 = Q2 + HID * LQH;
-constexpr int Q4 = Q3 + HID * LQH;
-constexpr int NWQ = Q4 + NOUT * LQH;  // 32128 bytes
-constexpr float FP8_MAX = 448.0f;
-
-__device__ __forceinline__ uint32_t fp8x2(float a, float b) {
-    a = fminf(fmaxf(a, -FP8_MAX), FP8_MAX);
-    b = fminf(fmaxf(b, -FP8_MAX), FP8_MAX);
-    return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xffffu;
-}
-__device__ __forceinline__ uint8_t fp8x1(float a) { return (uint8_t)(fp8x2(a, 0.0f) & 0xffu); }
-
-struct SmemFwd8 {
-    uint8_t wt[NWQ];
-    float bias[4 * HID + NOUT];
-    float inv_w[8];       // 1 / weight scale per layer (5 used)
-    float redmax[2][4];   // per-wave maxima, double-buffered over layers
-    uint8_t x0[TILE * LQ0];
-    uint8_t h[2][TILE * LQH];
-};
-
-__device__ __forceinline__ void load_weights8(SmemFwd8& sm, const uint8_t* gwt8, const float* params,
-                                              const float* inv_scale) {
-    const uint4* src = reinterpret_cast<const uint4*>(gwt8);
-    uint4* dst = reinterpret_cast<uint4*>(sm.wt);
-    for (int i = threadIdx.x; i < NWQ / 16; i += blockDim.x) dst[i] = src[i];
-    for (int i = threadIdx.x; i < 4 * HID + NOUT; i += blockDim.x) {
-        int off = i < HID ? B0 + i : i < 2 * HID ? B1 + i - HID : i < 3 * HID ? B2 + i - 2 * HID
-                : i < 4 * HID ? B3 + i - 3 * HID : B4 + i - 4 * HID;
-        sm.bias[i] = params[off];
-    }
-    if (threadIdx.x < 5) sm.inv_w[threadIdx.x] = inv_scale[threadIdx.x];
-}
-
-// one-blob features x 448 as FP8, two per store (bins 2b, 2b+1 of one input)
-__device__ __forceinline__ void encode_tile8(SmemFwd8& sm, const float* in, int n) {
-    for (int i = threadIdx.x; i < TILE * K0 / 2; i += blockDim.x) {
-        const int f2 = 2 * i, s = f2 / K0, f = f2 - s * K0;
-        const int d = f / NBINS, b = f - d * NBINS;
-        float v0 = 0.0f, v1 = 0.0f;
-        if (s < n) {
-            const float x = in[s * NIN + d];
-            v0 = blob(x, b) * FP8_MAX;
-            v1 = blob(x, b + 1) * FP8_MAX;
-        }
-        *reinterpret_cast<uint16_t*>(sm.x0 + s * LQ0 + f) = (uint16_t)fp8x2(v0, v1);
-    }
-}
-
-template <int K, int NT>
-__device__ __forceinline__ void gemm8_rows_nk(const uint8_t* A, int lda, const uint8_t* Wt, int ldb, f4* acc,
-                                              int lane) {
-#pragma unroll
-    for (int kt = 0; kt < K / 32; kt++) {
-        const int kk = kt * 32 + 8 * (lane >> 4);
-        const long a = *reinterpret_cast<const long*>(A + (lane & 15) * lda + kk);
-#pragma unroll
-        for (int nt = 0; nt < NT; nt++) {
-            const long b = *reinterpret_cast<const long*>(Wt + (nt * 16 + (lane & 15)) * ldb + kk);
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc[nt], 0, 0, 0);
-        }
-    }
-}
-
-// forward through all layers with FP8 operands; out as forward_tile
-__device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r0 = 16 * wave;
-    float inv_a = 1.0f / FP8_MAX;  // inverse scale of the current layer's input
-#pragma unroll
-    for (int l = 0; l < 4; l++) {
-        f4 acc[4] = {};
-        if (l == 0) gemm8_rows_nk<K0, 4>(sm.x0 + r0 * LQ0, LQ0, sm.wt + Q0, LQ0, acc, lane);
-        else gemm8_rows_nk<HID, 4>(sm.h[(l - 1) & 1] + r0 * LQH, LQH, sm.wt + (l == 1 ? Q1 : l == 2 ? Q2 : Q3), LQH,
-                                   acc, lane);
-        const float sc = inv_a * sm.inv_w[l];
-        float mx = 0.0f;
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int n = nt * 16 + (lane & 15);
-                const float z = fmaxf(acc[nt][j] * sc + sm.bias[l * HID + n], 0.0f);
-                acc[nt][j] = z;
-                mx = fmaxf(mx, z);
-            }
-        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-        if (lane == 0) sm.redmax[l & 1][wave] = mx;
-        __syncthreads();
-        const float M = fmaxf(fmaxf(sm.redmax[l & 1][0], sm.redmax[l & 1][1]),
-                              fmaxf(sm.redmax[l & 1][2], sm.redmax[l & 1][3]));
-        const float s = M > 0.0f ? FP8_MAX / M : 1.0f;
-        inv_a = M > 0.0f ? M / FP8_MAX : 1.0f;
-        uint8_t* hd = sm.h[l & 1];
-#pragma unroll
-        for (int nt = 0; nt < 4; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
-                hd[m * LQH + n] = fp8x1(acc[nt][j] * s);
-            }
-        __syncthreads();
-    }
-    {
-        f4 acc[3] = {};
-        gemm8_rows_nk<HID, 3>(sm.h[1] + r0 * LQH, LQH, sm.wt + Q4, LQH, acc, lane);
-        const float sc = inv_a * sm.inv_w[4];
-#pragma unroll
-        for (int nt = 0; nt < 3; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
-                out[m * NOUT + n] = acc[nt][j] * sc + sm.bias[4 * HID + n];
-            }
-    }
-    __syncthreads();
-}
-
-// FP8 copy of the weights: one block per layer; inv_scale[l] = max|W_l| / 448
-__global__ __launch_bounds__(256) void k_quant8(const float* params, uint8_t* gwt8, float* inv_scale) {
-    __shared__ float red[256];
-    const int l = blockIdx.x;
-    const int off = l == 0 ? W0 : l == 1 ? W1 : l == 2 ? W2 : l == 3 ? W3 : W4;
-    const int K = l == 0 ? K0 : HID, N = l == 4 ? NOUT : HID;
-    const int q = l == 0 ? Q0 : l == 1 ? Q1 : l == 2 ? Q2 : l == 3 ? Q3 : Q4;
-    const int ld = l == 0 ? LQ0 : LQH;
-    float m = 0.0f;
-    for (int i = threadIdx.x; i < K * N; i += blockDim.x) m = fmaxf(m, fabsf(params[off + i]));
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
-        __syncthreads();
-    }
-    const float M = red[0];
-    const float sc = M > 0.0f ? FP8_MAX / M : 1.0f;
-    if (threadIdx.x == 0) inv_scale[l] = M > 0.0f ? M / FP8_MAX : 1.0f;
-    for (int i = threadIdx.x; i < K * N; i += blockDim.x) {
-        const int k = i / N, n = i - k * N;
-        gwt8[q + n * ld + k] = fp8x1(params[off + i] * sc);
-    }
-    for (int i = threadIdx.x; i < N * (ld - K); i += blockDim.x) {
-        const int n = i / (ld - K), k = K + (i - n * (ld - K));
-        gwt8[q + n * ld + k] = 0;
-    }
-}
-
-// FP8 inference over n input vectors: raw outputs (n x NOUT)
-__global__ __launch_bounds__(256) void k_mlp_forward8(const uint8_t* gwt8, const float* params, const float* inv_scale,
-                                                      const float* in, int n, float* out) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    SmemFwd8& sm = *reinterpret_cast<SmemFwd8*>(smem_raw);
-    __shared__ float o_tile[TILE * NOUT];
-    load_weights8(sm, gwt8, params, inv_scale);
-    __syncthreads();
-    for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
-        int nt = min(TILE, n - t * TILE);
-        encode_tile8(sm, in + (size_t)t * TILE * NIN, nt);
-        __syncthreads();
-        forward_tile8(sm, o_tile);
-        for (int i = threadIdx.x; i < nt * NOUT; i += blockDim.x) out[(size_t)t * TILE * NOUT + i] = o_tile[i];
-        __syncthreads();
-    }
-}
-
-// Forward + loss gradient + backward-data for tiles of training samples; writes
-// the per-sample rows used by the weight-gradient GEMMs and each block's loss sum.
-__global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const float* params, const float* samples,
-                                                    int n, const float* wscale, uint16_t* rows, float* loss_part) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    SmemFwd& sm = *reinterpret_cast<SmemFwd*>(smem_raw);
-    __shared__ float in_s[TILE * NIN];
-    __shared__ float dir_s[TILE * 4];
-    __shared__ float red[256];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r0 = 16 * wave;
-    load_weights(sm, gwt, params);
-    const float ws = *wscale;
-    float loss_acc = 0.0f;
-    __syncthreads();
-    for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
-        const int nt = min(TILE, n - t * TILE);
-        const size_t s0 = (size_t)t * TILE;
-        for (int i = threadIdx.x; i < TILE; i += blockDim.x) {
-            const float* sp = samples + (s0 + i) * SAMPLE_FLOATS;
-            for (int d = 0; d < NIN; d++) in_s[i * NIN + d] = i < nt ? sp[d] : 0.0f;
-            for (int d = 0; d < 4; d++) dir_s[i * 4 + d] = i < nt ? sp[NIN + d] : 0.0f;
-        }
-        __syncthreads();
-        encode_tile(sm, in_s, nt);
-        __syncthreads();
-        for (int i = threadIdx.x; i < nt * K0; i += blockDim.x) {
-            int s = i / K0, f = i - s * K0;
-            rows[(s0 + s) * ROW + RX0 + f] = sm.x0[s * LD0 + f];
-        }
-        __syncthreads();
-        float* o_tile = reinterpret_cast<float*>(sm.x0);                 // x0 is free now
-        uint16_t* dz4 = reinterpret_cast<uint16_t*>(o_tile + TILE * NOUT);
-        forward_tile(sm, o_tile);
-        for (int i = threadIdx.x; i < nt * 4 * HID; i += blockDim.x) {
-            int s = i / (4 * HID), f = i - s * 4 * HID;
-            rows[(s0 + s) * ROW + RH + f] = sm.h[f / HID][s * LDH + (f % HID)];
-        }
-        if (threadIdx.x < TILE) {
-            int s = threadIdx.x;
-            float g[NOUT];
-            if (s < nt) {
-                float l = loss_grad(o_tile + s * NOUT, dir_s[4 * s], dir_s[4 * s + 1], dir_s[4 * s + 2],
-                                    dir_s[4 * s + 3] * ws, g);
-                loss_acc += dir_s[4 * s + 3] * ws * l;
-            } else {
-                for (int k = 0; k < NOUT; k++) g[k] = 0.0f;
-            }
-            for (int k = 0; k < NOUT; k++) dz4[s * LDO + k] = f2bf(g[k]);
-            for (int k = NOUT; k < LDO; k++) dz4[s * LDO + k] = 0;
-        }
-        __syncthreads();
-        for (int i = threadIdx.x; i < nt * NOUT; i += blockDim.x) {
-            int s = i / NOUT, f = i - s * NOUT;
-            rows[(s0 + s) * ROW + RDZ4 + f] = dz4[s * LDO + f];
-        }
-        // backward-data: dz_{l-1} = (dz_l W_l^T) .* (h_{l-1} > 0), written over h_{l-1}
-        {
-            f4 acc[4] = {};
-            gemm_rows_kn<NOUT, 4>(dz4 + r0 * LDO, LDO, sm.wt + T4, LDH, acc, lane);
-#pragma unroll
-            for (int ntl = 0; ntl < 4; ntl++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    int m = r0 + 4 * (lane >> 4) + j, u = ntl * 16 + (lane & 15);
-                    float hv = bf2f(sm.h[3][m * LDH + u]);
-                    acc[ntl][j] = hv > 0.0f ? acc[ntl][j] : 0.0f;
-                }
-            __syncthreads();  // all reads of h[3] as activations are done (rows already stored)
-#pragma unroll
-            for (int ntl = 0; ntl < 4; ntl++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    int m = r0 + 4 * (lane >> 4) + j, u = ntl * 16 + (lane & 15);
-                    sm.h[3][m * LDH + u] = f2bf(acc[ntl][j]);
-                }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int l = 3; l >= 1; l--) {
-            const int toff = l == 1 ? T1 : l == 2 ? T2 : T3;
-            f4 acc[4] = {};
-            gemm_rows_kn<HID, 4>(sm.h[l] + r0 * LDH, LDH, sm.wt + toff, LDH, acc, lane);
-#pragma unroll
-            for (int ntl = 0; ntl < 4; ntl++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    int m = r0 + 4 * (lane >> 4) + j, u = ntl * 16 + (lane & 15);
-                    float hv = bf2f(sm.h[l - 1][m * LDH + u]);
-                    acc[ntl][j] = hv > 0.0f ? acc[ntl][j] : 0.0f;
-                }
-            __syncthreads();
-#pragma unroll
-            for (int ntl = 0; ntl < 4; ntl++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    int m = r0 + 4 * (lane >> 4) + j, u = ntl * 16 + (lane & 15);
-                    sm.h[l - 1][m * LDH + u] = f2bf(acc[ntl][j]);
-                }
-            __syncthreads();
-        }
-        // sm.h[l] now holds dz_l for l = 0..3
-        for (int i = threadIdx.x; i < nt * 4 * HID; i += blockDim.x) {
-            int s = i / (4 * HID), f = i - s * 4 * HID;
-            rows[(s0 + s) * ROW + RDZ + f] = sm.h[f / HID][s * LDH + (f % HID)];
-        }
-        __syncthreads();
-    }
-    red[threadIdx.x] = loss_acc;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
-}
-
-// Weight gradients dW_l = sum_s a_l[s]^T dz_l[s] over a slice of the batch per
-// block (MFMA, 116 output tiles of 16x16 spread over 4 waves), bias gradients
-// by plain sums; each block writes its partial NPARAM vector.
-constexpr int NTILES = (K0 / 16) * (HID / 16) + 3 * (HID / 16) * (HID / 16) + (HID / 16) * (NOUT / 16);  // 116
-constexpr int TPW = (NTILES + 3) / 4;  // 29 tiles per wave
-constexpr int CH = 32;                 // samples staged per step
-
-__device__ __forceinline__ void tile_of(int t, int& layer, int& mt, int& nt) {
-    if (t < 56) { layer = 0; mt = t / 4; nt = t % 4; return; }
-    t -= 56;
-    if (t < 48) { layer = 1 + t / 16; mt = (t % 16) / 4; nt = t % 4; return; }
-    t -= 48;
-    layer = 4; mt = t / 3; nt = t % 3;
-}
-
-__global__ __launch_bounds__(256) void k_train_dw(const uint16_t* rows, int n, int per_block, float* partial) {
-    __shared__ __align__(16) uint16_t chunk[CH * ROW];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    f4 acc[TPW];
-#pragma unroll
-    for (int i = 0; i < TPW; i++) acc[i] = f4{0, 0, 0, 0};
-    float bacc = 0.0f;  // bias gradient for entry threadIdx.x (304 biases: threads 0..255 and a second pass)
-    float bacc2 = 0.0f;
-    const int begin = blockIdx.x * per_block;
-    const int end = min(n, begin + per_block);
-    for (int c0 = begin; c0 < end; c0 += CH) {
-        const int nc = min(CH, end - c0);
-        for (int i = threadIdx.x; i < CH * ROW / 8; i += blockDim.x) {
-            int s = (i * 8) / ROW;
-            uint4 v = s < nc ? reinterpret_cast<const uint4*>(rows + (size_t)c0 * ROW)[i] : uint4{0, 0, 0, 0};
-            reinterpret_cast<uint4*>(chunk)[i] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < CH / 16; ks++) {
-            const int sk = ks * 16 + 4 * (lane >> 4);
-#pragma unroll
-            for (int i = 0; i < TPW; i++) {
-                const int t = wave + 4 * i;
-                if (t < NTILES) {
-                    int layer, mt, nt;
-                    tile_of(t, layer, mt, nt);
-                    const int aoff = layer == 0 ? RX0 : RH + (layer - 1) * HID;
-                    const int doff = layer == 4 ? RDZ4 : RDZ + layer * HID;
-                    s4 a, b;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        a[j] = (short)chunk[(sk + j) * ROW + aoff + mt * 16 + (lane & 15)];
-                        b[j] = (short)chunk[(sk + j) * ROW + doff + nt * 16 + (lane & 15)];
-                    }
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
-                }
-            }
-        }
-        for (int s = 0; s < nc; s++) {
-            const int e = threadIdx.x;  // bias entry e: layers 0..3 -> dz_l, then dz4
-            bacc += bf2f(chunk[s * ROW + RDZ + e]);
-            if (e < NOUT) bacc2 += bf2f(chunk[s * ROW + RDZ4 + e]);
-        }
-        __syncthreads();
-    }
-    float* out = partial + (size_t)blockIdx.x * NPARAM;
-#pragma unroll
-    for (int i = 0; i < TPW; i++) {
-        const int t = wave + 4 * i;
-        if (t < NTILES) {
-            int layer, mt, nt;
-            tile_of(t, layer, mt, nt);
-            const int N = layer == 4 ? NOUT : HID;
-            const int woff = layer == 0 ? W0 : layer == 1 ? W1 : layer == 2 ? W2 : layer == 3 ? W3 : W4;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                int k = mt * 16 + 4 * (lane >> 4) + j, nn = nt * 16 + (lane & 15);
-                out[woff + k * N + nn] = acc[i][j];
-            }
-        }
-    }
-    {
-        const int e = threadIdx.x;
-        const int l = e / HID, u = e % HID;
-        const int boff = l == 0 ? B0 : l == 1 ? B1 : l == 2 ? B2 : B3;
-        out[boff + u] = bacc;
-        if (e < NOUT) out[B4 + e] = bacc2;
-    }
-}
-
-// sum of the per-block partials in block order (deterministic)
-__global__ void k_reduce_partials(const float* partial, int nblocks, int stride, int count, float* out) {
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
-        float s = 0.0f;
-        for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * stride + p];
-        out[p] = s;
-    }
-}
-
-// 1 / sum of sample weights (one block)
-__global__ void k_weight_scale(const float* samples, int n, float* wscale) {
-    __shared__ float red[256];
-    float s = 0.0f;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += samples[(size_t)i * SAMPLE_FLOATS + 10];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *wscale = red[0] > 0.0f ? 1.0f / red[0] : 0.0f;
-}
-
-// Adam on the FP32 master weights, then refresh the transposed BF16 copy
-__global__ void k_adam(float* params, float* m, float* v, const float* grad, int step, float lr, float b1, float b2,
-                       float eps, uint16_t* gwt) {
-    const float c1 = 1.0f / (1.0f - powf(b1, (float)step));
-    const float c2 = 1.0f / (1.0f - powf(b2, (float)step));
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < NPARAM; p += gridDim.x * blockDim.x) {
-        float g = grad[p];
-        if (!isfinite(g)) g = 0.0f;
-        float mm = b1 * m[p] + (1.0f - b1) * g;
-        float vv = b2 * v[p] + (1.0f - b2) * g * g;
-        m[p] = mm;
-        v[p] = vv;
-        float w = params[p] - lr * (mm * c1) / (sqrtf(vv * c2) + eps);
-        params[p] = w;
-        int off, K, N, ld;
-        if (p < B0) { off = W0; K = K0; N = HID; ld = LD0; (void)K; gwt[T0 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
-        else if (p >= W1 && p < B1) { off = W1; N = HID; ld = LDH; gwt[T1 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
-        else if (p >= W2 && p < B2) { off = W2; N = HID; ld = LDH; gwt[T2 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
-        else if (p >= W3 && p < B3) { off = W3; N = HID; ld = LDH; gwt[T3 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
-        else if (p >= W4 && p < B4) { off = W4; N = NOUT; ld = LDH; gwt[T4 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
-    }
-}
-
-// BF16 transposed copy of all weights (after set_weights)
-__global__ void k_pack_weights(const float* params, uint16_t* gwt) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NWT; i += gridDim.x * blockDim.x) gwt[i] = 0;
-    __syncthreads();
-}
-__global__ void k_pack_weights2(const float* params, uint16_t* gwt) {
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < NPARAM; p += gridDim.x * blockDim.x) {
-        float w = params[p];
-        if (p < B0) gwt[T0 + ((p - W0) % HID) * LD0 + (p - W0) / HID] = f2bf(w);
-        else if (p >= W1 && p < B1) gwt[T1 + ((p - W1) % HID) * LDH + (p - W1) / HID] = f2bf(w);
-        else if (p >= W2 && p < B2) gwt[T2 + ((p - W2) % HID) * LDH + (p - W2) / HID] = f2bf(w);
-        else if (p >= W3 && p < B3) gwt[T3 + ((p - W3) % HID) * LDH + (p - W3) / HID] = f2bf(w);
-        else if (p >= W4 && p < B4) gwt[T4 + ((p - W4) % NOUT) * LDH + (p - W4) / NOUT] = f2bf(w);
-    }
-}
-
-}  // namespace guide
 }  // namespace cpt

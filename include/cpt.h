@@ -276,11 +276,14 @@ int cpt_render_views(cpt_ctx* ctx, int32_t n_views, const float* cameras, const 
  * (src/renderer/ppg.h:41-214, src/algorithms/sdtree.h:445-538): a directional
  * distribution per path vertex, here an MLP (one-blob encoding of position,
  * normal and outgoing direction, 7 x 32 bins -> 4 x 64 ReLU -> 8 von Mises-Fisher
- * lobes) with BF16 weights on MFMA and FP32 accumulation.  SDTree::sample/pdf ->
- * the mixture's sample/pdf inside the guided shade kernel; SDTree::splat/update ->
- * training samples and cpt_guide_train_step (forward, backward, Adam on the GPU).
- * Parameters: 30000 floats; layer l weights W_l (K_l x N_l row-major, K = 224, 64,
- * 64, 64, 64; N = 64, 64, 64, 64, 48) each followed by its bias (N_l). */
+ * lobes, or 4 x 128 -> 16 lobes) with BF16 weights on MFMA and FP32 accumulation.
+ * SDTree::sample/pdf -> the mixture's sample/pdf inside the guided shade kernel;
+ * SDTree::splat/update -> training samples and cpt_guide_train_step (forward,
+ * backward, Adam on the GPU).
+ * Parameters (width H, O outputs = 48 for 8 lobes, 80 for 16): layer l weights W_l
+ * (K_l x N_l row-major, K = 224, H, H, H, H; N = H, H, H, H, O) each followed by
+ * its bias (N_l): 30000 floats at width 64, 88656 at width 128.  Outputs per
+ * vector: 5 per lobe (logit, log kappa, unnormalised mean xyz), padded to O. */
 typedef struct {
     float lr;              /* Adam step size (0 = 1e-3) */
     float beta1, beta2;    /* 0 = 0.9, 0.99 */
@@ -295,19 +298,26 @@ typedef struct {
     int32_t infer_fp8;     /* 1: guided renders and cpt_guide_forward run the network in FP8 (e4m3
                               weights with per-layer scales, per-tile activation scales, FP8 MFMA,
                               FP32 accumulation); training stays BF16 */
+    int32_t width;         /* hidden width: 64 (8 lobes; 0 = 64) or 128 (16 lobes, the large-scene
+                              network: BF16 inference reads its weights through the L2, FP8
+                              inference keeps them in LDS) */
 } cpt_guide_params;
 
-int cpt_guide_num_params(void);
+int cpt_guide_num_params(void);  /* of the width-64 network */
+/* shape4 = [hidden width, lobes, outputs per vector, parameters] of the context's
+ * guide (of the width-64 network before cpt_guide_init). */
+int cpt_guide_shape(cpt_ctx* ctx, int32_t* shape4);
 /* Allocate (first call) and initialise the guide: Glorot-uniform weights from `seed`. */
 int cpt_guide_init(cpt_ctx* ctx, const cpt_guide_params* params);
 int cpt_guide_get_params(cpt_ctx* ctx, float* params_out);
 int cpt_guide_set_params(cpt_ctx* ctx, const float* params_in);
-/* Raw network outputs (n x 48) for n input vectors (n x 7 floats in [0,1]). */
+/* Raw network outputs (n x 48, or n x 80 at width 128) for n input vectors (n x 7
+ * floats in [0,1]). */
 int cpt_guide_forward(cpt_ctx* ctx, const float* inputs, int32_t n, float* outputs);
 /* One training step on n samples (n x 11 floats: 7 inputs, direction xyz, weight =
  * target radiance / sampling pdf).  Loss = sum_i w_i (-log q(dir_i)) / sum_i w_i.
- * loss_out (may be NULL) = that loss; grad_out (may be NULL, 30000 floats) = its
- * gradient; apply != 0 also takes the Adam step. */
+ * loss_out (may be NULL) = that loss; grad_out (may be NULL, one float per
+ * parameter) = its gradient; apply != 0 also takes the Adam step. */
 int cpt_guide_train_step(cpt_ctx* ctx, const float* samples, int32_t n, int32_t apply, float* loss_out,
                          float* grad_out);
 

@@ -81,6 +81,8 @@ def lib():
     L.orc_mt_floats.argtypes = [C.c_uint32, i32, vp]
     L.orc_mt_floats.restype = None
     L.orc_mlp_num_params.restype = i32
+    L.orc_mlp_set_width.argtypes = [i32]
+    L.orc_mlp_set_width.restype = None
     L.orc_mlp_forward.argtypes = [vp, vp, i32, i32, vp]
     L.orc_mlp_forward.restype = None
     L.orc_mlp_mixture_pdf_batch.argtypes = [vp, vp, i32, vp]
@@ -282,30 +284,43 @@ def mt_floats(seed, n):
 
 
 # ---------------------------------------------------------------- guide MLP oracle
-def mlp_num_params():
+def _mlp_width(width):
+    if width not in (64, 128):
+        raise ValueError("guide network width is 64 or 128")
+    lib().orc_mlp_set_width(width)
+    return 48 if width == 64 else 80
+
+
+def mlp_num_params(width=64):
+    _mlp_width(width)
     return lib().orc_mlp_num_params()
 
 
-def mlp_forward(params, inputs, bf16=True):
+def mlp_forward(params, inputs, bf16=True, width=64):
+    nout = _mlp_width(width)
     P = np.ascontiguousarray(params, np.float32)
+    assert P.size == lib().orc_mlp_num_params()
     X = np.ascontiguousarray(inputs, np.float32).reshape(-1, 7)
-    out = np.zeros((X.shape[0], 48), np.float64)
+    out = np.zeros((X.shape[0], nout), np.float64)
     lib().orc_mlp_forward(_p(P), _p(X), X.shape[0], 1 if bf16 else 0, _p(out))
     return out
 
 
-def mlp_mixture_pdf(outputs48, dirs):
-    o = np.ascontiguousarray(outputs48, np.float64).reshape(48)
+def mlp_mixture_pdf(outputs, dirs, width=64):
+    nout = _mlp_width(width)
+    o = np.ascontiguousarray(outputs, np.float64).reshape(nout)
     d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
     out = np.zeros(d.shape[0], np.float64)
     lib().orc_mlp_mixture_pdf_batch(_p(o), _p(d), d.shape[0], _p(out))
     return out
 
 
-def mlp_loss_grad(params, samples, bf16=True):
+def mlp_loss_grad(params, samples, bf16=True, width=64):
+    _mlp_width(width)
     P = np.ascontiguousarray(params, np.float32)
+    assert P.size == lib().orc_mlp_num_params()
     S = np.ascontiguousarray(samples, np.float32).reshape(-1, 11)
-    g = np.zeros(mlp_num_params(), np.float64)
+    g = np.zeros(P.size, np.float64)
     loss = lib().orc_mlp_loss_grad(_p(P), _p(S), S.shape[0], 1 if bf16 else 0, _p(g))
     return loss, g
 
@@ -315,5 +330,7 @@ def mlp_adam(params, m, v, grad, step, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8):
     m = np.array(m, np.float64)
     v = np.array(v, np.float64)
     g = np.ascontiguousarray(grad, np.float64)
+    _mlp_width(64 if P.size == mlp_num_params(64) else 128)
+    assert P.size == m.size == v.size == g.size == lib().orc_mlp_num_params()
     lib().orc_mlp_adam(_p(P), _p(m), _p(v), _p(g), step, lr, b1, b2, eps)
     return P, m, v

@@ -1605,11 +1605,31 @@ void orc_mt_floats(uint32_t seed, int n, float* out) {
 // (cpt_guide_train_step): parameters W_l (K_l x N_l row-major) then b_l.
 namespace {
 namespace mlp {
-const int NIN = 7, NB = 32, K0 = NIN * NB, H = 64, NOUT = 48, NL = 8;
-const int W0 = 0, B0 = W0 + K0 * H, W1 = B0 + H, B1 = W1 + H * H, W2 = B1 + H, B2 = W2 + H * H, W3 = B2 + H,
-          B3 = W3 + H * H, W4 = B3 + H, B4 = W4 + H * NOUT, NP = B4 + NOUT;
-const int KS[5] = {K0, H, H, H, H}, NS[5] = {H, H, H, H, NOUT}, WO[5] = {W0, W1, W2, W3, W4},
-          BO[5] = {B0, B1, B2, B3, B4};
+// two network shapes: hidden width 64 with 8 lobes (48 outputs) and width 128
+// with 16 lobes (96 outputs); the arrays below are sized for the larger one
+const int NIN = 7, NB = 32, K0 = NIN * NB, HMAX = 128, NOUTMAX = 96, NLMAX = 16;
+struct Shape {
+    int H, NL, NOUT, NP;
+    int KS[5], NS[5], WO[5], BO[5];
+};
+Shape make_shape(int width) {
+    Shape S;
+    S.H = width == 128 ? 128 : 64;
+    S.NL = S.H == 128 ? 16 : 8;
+    S.NOUT = (5 * S.NL + 15) / 16 * 16;
+    int off = 0;
+    for (int l = 0; l < 5; l++) {
+        S.KS[l] = l == 0 ? K0 : S.H;
+        S.NS[l] = l == 4 ? S.NOUT : S.H;
+        S.WO[l] = off;
+        off += S.KS[l] * S.NS[l];
+        S.BO[l] = off;
+        off += S.NS[l];
+    }
+    S.NP = off;
+    return S;
+}
+Shape g_shape = make_shape(64);
 
 inline double bf(double x, int on) {
     if (!on) return x;
@@ -1624,10 +1644,12 @@ inline double bf(double x, int on) {
 
 struct Acts {
     double a[5][K0];   // inputs of layer l (a[0] = features)
-    double o[NOUT];
+    double o[NOUTMAX];
 };
 
 void forward(const float* P, const float* in, int bfon, Acts& A) {
+    const Shape& S = g_shape;
+    const int *KS = S.KS, *NS = S.NS, *WO = S.WO, *BO = S.BO;
     for (int d = 0; d < NIN; d++)
         for (int b = 0; b < NB; b++) {
             double x = (double)in[d] * NB - (b + 0.5);
@@ -1645,10 +1667,11 @@ void forward(const float* P, const float* in, int bfon, Acts& A) {
 }
 
 struct Mix {
-    double pi[NL], kap[NL], m[NL][3], un[NL], ls[NL];
+    double pi[NLMAX], kap[NLMAX], m[NLMAX][3], un[NLMAX], ls[NLMAX];
 };
 
 void decode(const double* o, Mix& M) {
+    const int NL = g_shape.NL;
     double mx = -1e300;
     for (int k = 0; k < NL; k++) mx = std::max(mx, o[5 * k]);
     double s = 0;
@@ -1672,6 +1695,7 @@ inline double vmf(double kap, double c) {
 }
 
 double mix_pdf(const Mix& M, const double* w) {
+    const int NL = g_shape.NL;
     double p = 0;
     for (int k = 0; k < NL; k++) p += M.pi[k] * vmf(M.kap[k], M.m[k][0] * w[0] + M.m[k][1] * w[1] + M.m[k][2] * w[2]);
     return p;
@@ -1679,9 +1703,10 @@ double mix_pdf(const Mix& M, const double* w) {
 
 // -wgt * log q(w) and its gradient w.r.t. the raw outputs (analytic)
 double loss_grad(const double* o, const double* w, double wgt, double* g) {
+    const int NL = g_shape.NL, NOUT = g_shape.NOUT;
     Mix M;
     decode(o, M);
-    double v[NL], q = 0;
+    double v[NLMAX], q = 0;
     for (int k = 0; k < NL; k++) {
         v[k] = vmf(M.kap[k], M.m[k][0] * w[0] + M.m[k][1] * w[1] + M.m[k][2] * w[2]);
         q += M.pi[k] * v[k];
@@ -1706,14 +1731,18 @@ double loss_grad(const double* o, const double* w, double wgt, double* g) {
 
 extern "C" {
 
-int orc_mlp_num_params(void) { return mlp::NP; }
+int orc_mlp_num_params(void) { return mlp::g_shape.NP; }
+
+// network shape of the orc_mlp_* functions: width 64 (8 lobes) or 128 (16 lobes)
+void orc_mlp_set_width(int width) { mlp::g_shape = mlp::make_shape(width); }
 
 void orc_mlp_forward(const float* P, const float* in, int n, int bfon, double* out) {
+    const int NOUT = mlp::g_shape.NOUT;
 #pragma omp parallel for
     for (int s = 0; s < n; s++) {
         mlp::Acts A;
         mlp::forward(P, in + (size_t)s * mlp::NIN, bfon, A);
-        for (int k = 0; k < mlp::NOUT; k++) out[(size_t)s * mlp::NOUT + k] = A.o[k];
+        for (int k = 0; k < NOUT; k++) out[(size_t)s * NOUT + k] = A.o[k];
     }
 }
 
@@ -1732,6 +1761,9 @@ void orc_mlp_mixture_pdf_batch(const double* o, const double* w, int n, double* 
 // loss = sum_s w_s (-log q_s) / sum_s w_s and its gradient (NP doubles)
 double orc_mlp_loss_grad(const float* P, const float* samples, int n, int bfon, double* grad) {
     using namespace mlp;
+    const Shape& S = g_shape;
+    const int *KS = S.KS, *NS = S.NS, *WO = S.WO, *BO = S.BO;
+    const int NP = S.NP, NOUT = S.NOUT;
     double wsum = 0;
     for (int s = 0; s < n; s++) wsum += samples[(size_t)s * 11 + 10];
     // the GPU forms 1/sum in FP32 and multiplies per sample in FP32
@@ -1745,7 +1777,7 @@ double orc_mlp_loss_grad(const float* P, const float* samples, int n, int bfon, 
         double w[3] = {sp[7], sp[8], sp[9]};
         double wgt = bfon ? (double)(sp[10] * (float)ws) : sp[10] * ws;
         double dz[5][K0];
-        double g[NOUT];
+        double g[NOUTMAX];
         loss += loss_grad(A.o, w, wgt, g);
         for (int k = 0; k < NOUT; k++) dz[4][k] = bf(g[k], bfon);
         for (int l = 4; l >= 1; l--) {
@@ -1772,7 +1804,7 @@ double orc_mlp_loss_grad(const float* P, const float* samples, int n, int bfon, 
 void orc_mlp_adam(float* P, double* m, double* v, const double* g, int step, double lr, double b1, double b2,
                   double eps) {
     double c1 = 1 / (1 - std::pow(b1, step)), c2 = 1 / (1 - std::pow(b2, step));
-    for (int p = 0; p < mlp::NP; p++) {
+    for (int p = 0; p < mlp::g_shape.NP; p++) {
         m[p] = b1 * m[p] + (1 - b1) * g[p];
         v[p] = b2 * v[p] + (1 - b2) * g[p] * g[p];
         P[p] = (float)(P[p] - lr * (m[p] * c1) / (std::sqrt(v[p] * c2) + eps));

@@ -138,6 +138,7 @@ extern "C" {
 #endif
 /* Guide MLP (FP64; bf16 != 0 rounds to BF16 where the GPU does). */
 int orc_mlp_num_params(void);
+void orc_mlp_set_width(int width);  /* 64 (8 lobes, default) or 128 (16 lobes) */
 void orc_mlp_forward(const float* params, const float* inputs, int n, int bf16, double* outputs);
 double orc_mlp_mixture_pdf(const double* outputs48, const double* dir3);
 void orc_mlp_mixture_pdf_batch(const double* outputs48, const double* dirs, int n, double* out);

@@ -33,14 +33,14 @@ def test_guided_training_produces_samples_and_steps(ctx):
     assert np.isfinite(img).all() and img.mean() > 0
 
 
-@pytest.mark.parametrize("fp8", [False, True])
-def test_guided_is_unbiased(ctx, fp8):
+@pytest.mark.parametrize("fp8,width", [(False, 64), (True, 64), (True, 128), (False, 128)])
+def test_guided_is_unbiased(ctx, fp8, width):
     W = H = 48
     s = cpt.scenes.box_room(res=(W, H))
     ctx.upload(s)
     ctx.render(W, H, 16, max_depth=16, seed=5)
     _, ps, pq = ctx.read_film(sums=True, sumsq=True)
-    ctx.guide_init(seed=2, lr=3e-3, train_paths=8192, infer_fp8=fp8)
+    ctx.guide_init(seed=2, lr=3e-3, train_paths=8192, infer_fp8=fp8, width=width)
     ctx.render(W, H, 16, max_depth=16, seed=6, flags=cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN)
     _, gs, gq = ctx.read_film(sums=True, sumsq=True)
     z = _z(gs, gq, 256, ps, pq, 256)

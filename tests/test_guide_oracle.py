@@ -8,17 +8,29 @@ import pytest
 import oracle
 
 
-def random_params(seed=0, out_scale=0.3):
+def net_shape(width=64):
+    """Layer fan-ins, fan-outs and (W, b) offsets of the guide network: width 64
+    with 8 lobes (48 outputs) or width 128 with 16 lobes (80 outputs)."""
+    nout = 48 if width == 64 else 80
+    K = [224, width, width, width, width]
+    N = [width, width, width, width, nout]
+    offs, o = [], 0
+    for l in range(5):
+        offs.append((o, o + K[l] * N[l]))
+        o += K[l] * N[l] + N[l]
+    return K, N, offs, o
+
+
+def random_params(seed=0, out_scale=0.3, width=64):
     rng = np.random.default_rng(seed)
-    K = [224, 64, 64, 64, 64]
-    N = [64, 64, 64, 64, 48]
+    K, N, _, _ = net_shape(width)
     parts = []
     for l in range(5):
         a = math.sqrt(6.0 / (K[l] + N[l])) * (out_scale if l == 4 else 1.0)
         parts.append(rng.uniform(-a, a, K[l] * N[l]))
         parts.append(rng.uniform(-0.1, 0.1, N[l]))
     p = np.concatenate(parts).astype(np.float32)
-    assert p.size == oracle.mlp_num_params()
+    assert p.size == oracle.mlp_num_params(width)
     return p
 
 
@@ -32,14 +44,16 @@ def random_samples(n, seed=1):
     return s
 
 
-def test_analytic_gradient_matches_finite_differences():
-    P = random_params(3)
+@pytest.mark.parametrize("width", [64, 128])
+def test_analytic_gradient_matches_finite_differences(width):
+    P = random_params(3, width=width)
     S = random_samples(6, 4)
-    _, g = oracle.mlp_loss_grad(P, S, bf16=False)
+    _, g = oracle.mlp_loss_grad(P, S, bf16=False, width=width)
     rng = np.random.default_rng(5)
-    offs = [0, 14336, 14400, 18496, 18560, 22656, 22720, 26816, 26880, 29952]
-    idx = np.concatenate([rng.integers(offs[i], offs[i + 1], 8) for i in range(len(offs) - 1)] +
-                         [np.arange(29952, 29992)])
+    _, N, offs, total = net_shape(width)
+    bounds = [b for wo, bo in offs for b in (wo, bo)]
+    idx = np.concatenate([rng.integers(bounds[i], bounds[i + 1], 8) for i in range(len(bounds) - 1)] +
+                         [np.arange(offs[4][1], offs[4][1] + 5 * (8 if width == 64 else 16))])
     checked = 0
     for p in idx:
         h = np.float32(1e-4)  # small enough not to cross ReLU kinks
@@ -47,8 +61,8 @@ def test_analytic_gradient_matches_finite_differences():
         Pp[p] += h
         Pm[p] -= h
         d = float(Pp[p]) - float(Pm[p])
-        lp, _ = oracle.mlp_loss_grad(Pp, S, bf16=False)
-        lm, _ = oracle.mlp_loss_grad(Pm, S, bf16=False)
+        lp, _ = oracle.mlp_loss_grad(Pp, S, bf16=False, width=width)
+        lm, _ = oracle.mlp_loss_grad(Pm, S, bf16=False, width=width)
         fd = (lp - lm) / d
         if abs(g[p]) < 1e-7 and abs(fd) < 1e-7:
             continue
@@ -57,10 +71,10 @@ def test_analytic_gradient_matches_finite_differences():
     assert checked > 60
 
 
-@pytest.mark.parametrize("seed", [0, 1, 2])
-def test_mixture_pdf_integrates_to_one(seed):
+@pytest.mark.parametrize("seed,width", [(0, 64), (1, 64), (2, 64), (3, 128)])
+def test_mixture_pdf_integrates_to_one(seed, width):
     rng = np.random.default_rng(seed)
-    o = rng.standard_normal(48)
+    o = rng.standard_normal(48 if width == 64 else 80)
     o[1::5] = np.clip(o[1::5], -3, 3)      # kappa up to e^3 for a quadrature-friendly check
     n = 400_000  # Fibonacci sphere
     i = np.arange(n) + 0.5
@@ -68,7 +82,7 @@ def test_mixture_pdf_integrates_to_one(seed):
     r = np.sqrt(1 - z * z)
     phi = math.pi * (1 + 5 ** 0.5) * i
     d = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
-    q = oracle.mlp_mixture_pdf(o, d)
+    q = oracle.mlp_mixture_pdf(o, d, width=width)
     assert abs(q.mean() * 4 * math.pi - 1.0) < 2e-3
 
 
@@ -90,14 +104,13 @@ def e4m3(x):
     return np.sign(x) * np.round(a / step) * step
 
 
-def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64):
+def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64, width=64):
     """FP8 inference pipeline of the guide network restated in numpy: weights
     scaled per layer by 448 / max|W|, one-blob features by 448, each hidden
     layer's ReLU outputs per tile of 64 samples by 448 / (tile maximum), the
     padding rows of a partial tile taking part in that maximum as zero-feature
     samples.  quant=identity gives the exact network."""
-    K = [224, 64, 64, 64, 64]
-    N = [64, 64, 64, 64, 48]
+    K, N, _, _ = net_shape(width)
     P = np.asarray(params, np.float64)
     Ws, bs, o = [], [], 0
     for l in range(5):
@@ -112,7 +125,7 @@ def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64):
         inv_w.append(m / 448.0)
     X = np.asarray(inputs, np.float32).reshape(-1, 7)
     n = X.shape[0]
-    out = np.zeros((n, 48))
+    out = np.zeros((n, N[4]))
     centres = np.arange(32, dtype=np.float32) + np.float32(0.5)
     for t0 in range(0, n, tile):
         xs = X[t0:t0 + tile]
@@ -145,12 +158,15 @@ def test_e4m3_rounding_known_values():
     assert np.array_equal(e4m3(np.array(vals)), np.array(vals))
 
 
-def test_fp8_pipeline_without_rounding_is_the_network():
-    P = random_params(11)
+@pytest.mark.parametrize("width", [64, 128])
+def test_fp8_pipeline_without_rounding_is_the_network(width):
+    """Also pins the oracle's forward pass and parameter layout: numpy matrix
+    products over the documented layout give the same outputs."""
+    P = random_params(11, width=width)
     X = np.random.default_rng(12).random((150, 7)).astype(np.float32)
-    exact = mlp_forward_fp8(P, X, quant=lambda v: v)
-    r = oracle.mlp_forward(P, X, bf16=False)
+    exact = mlp_forward_fp8(P, X, quant=lambda v: v, width=width)
+    r = oracle.mlp_forward(P, X, bf16=False, width=width)
     assert np.abs(exact - r).max() < 1e-5 * np.abs(r).max()
-    q = mlp_forward_fp8(P, X)
+    q = mlp_forward_fp8(P, X, width=width)
     rel = np.linalg.norm(q - r) / np.linalg.norm(r)
     assert 1e-3 < rel < 0.1, rel
