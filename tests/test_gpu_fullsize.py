@@ -1,0 +1,90 @@
+"""Parity at the measured workload's full size: the procedural terrain of the
+1080p benchmark (1024x512 heightfield + 200 icospheres, 2,069,522 triangles).
+The oracle's own BVH keeps exact closest and any hits affordable at this size;
+whole-frame checks use properties that hold at any size (the same film from
+the captured graph, eager launches, other launch shapes and multi-pass plans)."""
+import numpy as np
+import pytest
+
+import cpt
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+W, H = 1920, 1080
+
+
+@pytest.fixture(scope="module")
+def terrain():
+    s = cpt.scenes.terrain_scene(res=(W, H))
+    assert s.n_triangles == 2069522
+    return s
+
+
+@pytest.fixture(scope="module")
+def oracle_terrain(terrain):
+    return oracle.OracleScene(terrain)
+
+
+def test_full_terrain_closest_hit_bit_exact(ctx, terrain, oracle_terrain):
+    print(f"full-size closest hit: {terrain.n_triangles} triangles", flush=True)
+    ctx.upload(terrain)
+    rays = cpt.raybatch.make_rays(terrain, 1 << 17, seed=41)
+    g_tri, g_t = ctx.trace_closest(rays)
+    r_tri, r_t = oracle_terrain.trace_closest(rays)
+    # a brute-force sweep over all 2M triangles pins the oracle's traversal on a sample
+    b_tri, b_t = oracle_terrain.trace_closest(rays[:128], brute=True)
+    assert np.array_equal(b_tri, r_tri[:128])
+    # the reference's box test can drop the true hit of a rare grazing ray; the
+    # GPU returns the true closest hit, so every disagreement is settled by brute force
+    diff = (g_tri != r_tri) | ((r_tri >= 0) & (g_t.view(np.int32) != r_t.view(np.int32)))
+    print("rays where GPU and the reference traversal differ:", int(diff.sum()))
+    assert diff.mean() < 1e-3
+    if diff.any():
+        d_tri, d_t = oracle_terrain.trace_closest(rays[diff], brute=True)
+        assert np.array_equal(g_tri[diff], d_tri)
+        assert np.array_equal(g_t[diff][d_tri >= 0].view(np.int32), d_t[d_tri >= 0].view(np.int32))
+    hit = g_tri >= 0
+    assert 0.3 < hit.mean() < 0.95
+
+
+def test_full_terrain_any_hit_exact(ctx, terrain, oracle_terrain):
+    ctx.upload(terrain)
+    rays = cpt.raybatch.make_rays(terrain, 1 << 17, seed=42, shadow=True)
+    g = ctx.trace_any(rays)
+    r = oracle_terrain.trace_any(rays)
+    diff = g != r
+    print("shadow rays where GPU and the reference traversal differ:", int(diff.sum()))
+    assert diff.mean() < 1e-3
+    if diff.any():
+        assert np.array_equal(g[diff], oracle_terrain.trace_any(rays[diff], brute=True))
+    assert 0.05 < r.mean() < 0.95
+
+
+def test_full_terrain_paths_match_oracle(ctx, terrain, oracle_terrain):
+    """The same PCG stream per path through the full scene (a 96x54 film, 4 spp,
+    depth 12): GPU and oracle agree pixel by pixel."""
+    w, h = 96, 54
+    ctx.upload(terrain)
+    ctx.render(w, h, 2, max_depth=12, seed=43)
+    _, g = ctx.read_film(sums=True)
+    r, _, _ = oracle_terrain.render(w, h, 2, max_depth=12, sampler="pcg", seed=43)
+    per_px = (np.abs(g - r) / np.maximum(np.abs(r), 1e-3)).max(axis=-1)
+    print("full-size path parity: pixels within 1e-3:", (per_px < 1e-3).mean())
+    assert (per_px < 1e-3).mean() > 0.97
+    assert abs(g.sum() - r.sum()) / r.sum() < 5e-3
+
+
+def test_full_frame_independent_of_launch_and_plan(ctx, terrain):
+    """A 1080p frame (1 spp, depth 12): the captured graph, eager launches, other
+    launch shapes and a plan of several passes all give the same FP64 sums."""
+    ctx.upload(terrain)
+    ctx.render(W, H, 1, max_depth=12, seed=44)
+    _, a, qa = ctx.read_film(sums=True, sumsq=True)
+    assert np.isfinite(a).all() and a.mean() > 0
+    variants = [dict(flags=cpt.CPT_RENDER_NO_GRAPH), dict(block_size=64, shade_block=512, refill=17),
+                dict(max_paths=W * H // 3 + 7)]
+    for kw in variants:
+        ctx.render(W, H, 1, max_depth=12, seed=44, **kw)
+        _, b, qb = ctx.read_film(sums=True, sumsq=True)
+        assert np.array_equal(a, b) and np.array_equal(qa, qb), kw
