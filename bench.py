@@ -177,6 +177,19 @@ def bench_guided(args):
             print(f"bench: ppg budget {T:.0f}s: {el:.1f}s, last iteration {last} spp, relMSE={ppg[T][2]:.4f}",
                   file=sys.stderr, flush=True)
 
+    # per-kernel time of a 1-spp frame from HIP events (eager launches), with the
+    # guide as trained by the guided run
+    def frame_ms(flags):
+        acc = {}
+        for k in range(4):
+            st = ctx.render(W, H, 1, max_depth=depth, seed=900 + k, flags=flags | cpt.CPT_RENDER_KERNEL_TIMING)
+            for key in ("ms_extend", "ms_shade", "ms_shadow", "ms_guide", "ms_other"):
+                acc[key] = acc.get(key, 0.0) + st[key] / 4
+        return {k: round(v, 3) for k, v in acc.items()}
+
+    kernel_ms = {"plain": frame_ms(0), "guided": frame_ms(cpt.CPT_RENDER_GUIDED)}
+    print(f"bench: per-frame kernel ms {kernel_ms}", file=sys.stderr, flush=True)
+
     def at(curve, T):
         best = curve[0]
         for c in curve:
@@ -210,6 +223,7 @@ def bench_guided(args):
                                f"{scene.n_triangles} triangles, {W}x{H}, max depth {depth}, "
                                f"reference {ref_spp} spp FP64", "budgets_s": budgets},
         "budgets": rows,
+        "frame_kernel_ms": kernel_ms,
         "reference_seconds": ref_s,
     }
     ctx.close()

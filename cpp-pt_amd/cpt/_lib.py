@@ -85,7 +85,7 @@ class RenderStats(C.Structure):
         ("ms_other", C.c_double), ("n_extend", C.c_int64), ("n_shade", C.c_int64),
         ("n_shadow", C.c_int64), ("passes", C.c_int32), ("paths_per_pass", C.c_int32),
         ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("guide_samples", C.c_int64),
-        ("guide_steps", C.c_int32), ("reserved_", C.c_int32),
+        ("guide_steps", C.c_int32), ("reserved_", C.c_int32), ("ms_guide", C.c_double), ("n_guide", C.c_int64),
     ]
 
     def as_dict(self):

@@ -1582,6 +1582,7 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
                 hipLaunchKernelGGL((k_guide_infer<64, false>), dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd),
                                    s, a, cur);
             }
+            mark(4);
         }
         L_shade(L, a, cur, a.max_depth, s);
         mark(1);
@@ -2365,6 +2366,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
                     case 0: stats->ms_extend += ms; stats->n_extend++; break;
                     case 1: stats->ms_shade += ms; stats->n_shade++; break;
                     case 2: stats->ms_shadow += ms; stats->n_shadow++; break;
+                    case 4: stats->ms_guide += ms; stats->n_guide++; break;
                     default: stats->ms_other += ms; break;
                 }
             }

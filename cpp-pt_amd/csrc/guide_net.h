@@ -214,12 +214,20 @@ __device__ __forceinline__ void load_weights(SmemFwd& sm, const uint16_t* gwt, c
 }
 
 // one-blob features of the tile's samples into sm.x0 (samples beyond n are zero)
+// (4 consecutive bins of one input per thread step, one 8-byte store)
 __device__ __forceinline__ void encode_tile(SmemFwd& sm, const float* in, int n) {
-    for (int i = threadIdx.x; i < TILE * K0; i += blockDim.x) {
-        int s = i / K0, f = i - s * K0;
-        int d = f / NBINS, b = f - d * NBINS;
-        float v = s < n ? blob(in[s * NIN + d], b) : 0.0f;
-        sm.x0[s * LD0 + f] = f2bf(v);
+    constexpr int CPS = K0 / 4;  // chunks per sample
+    static_assert(NBINS % 4 == 0 && LD0 % 4 == 0, "8-byte chunks");
+    for (int c = threadIdx.x; c < TILE * CPS; c += blockDim.x) {
+        const int s = c / CPS, f = (c - s * CPS) * 4;
+        const int d = f / NBINS, b = f - d * NBINS;
+        uint2 w = {0u, 0u};
+        if (s < n) {
+            const float x = in[s * NIN + d];
+            w.x = (uint32_t)f2bf(blob(x, b)) | ((uint32_t)f2bf(blob(x, b + 1)) << 16);
+            w.y = (uint32_t)f2bf(blob(x, b + 2)) | ((uint32_t)f2bf(blob(x, b + 3)) << 16);
+        }
+        *reinterpret_cast<uint2*>(sm.x0 + s * LD0 + f) = w;
     }
 }
 
@@ -382,18 +390,23 @@ __device__ __forceinline__ void load_weights8(SmemFwd8& sm, const uint8_t* gwt8,
     if (threadIdx.x < 5) sm.inv_w[threadIdx.x] = inv_scale[threadIdx.x];
 }
 
-// one-blob features x 448 as FP8, two per store (bins 2b, 2b+1 of one input)
+// one-blob features x 448 as FP8 (8 consecutive bins of one input per thread
+// step, one 8-byte store)
 __device__ __forceinline__ void encode_tile8(SmemFwd8& sm, const float* in, int n) {
-    for (int i = threadIdx.x; i < TILE * K0 / 2; i += blockDim.x) {
-        const int f2 = 2 * i, s = f2 / K0, f = f2 - s * K0;
+    constexpr int CPS = K0 / 8;  // chunks per sample
+    static_assert(NBINS % 8 == 0 && LQ0 % 8 == 0, "8-byte chunks");
+    for (int c = threadIdx.x; c < TILE * CPS; c += blockDim.x) {
+        const int s = c / CPS, f = (c - s * CPS) * 8;
         const int d = f / NBINS, b = f - d * NBINS;
-        float v0 = 0.0f, v1 = 0.0f;
+        uint2 w = {0u, 0u};
         if (s < n) {
             const float x = in[s * NIN + d];
-            v0 = blob(x, b) * FP8_MAX;
-            v1 = blob(x, b + 1) * FP8_MAX;
+            w.x = fp8x2(blob(x, b) * FP8_MAX, blob(x, b + 1) * FP8_MAX) |
+                  (fp8x2(blob(x, b + 2) * FP8_MAX, blob(x, b + 3) * FP8_MAX) << 16);
+            w.y = fp8x2(blob(x, b + 4) * FP8_MAX, blob(x, b + 5) * FP8_MAX) |
+                  (fp8x2(blob(x, b + 6) * FP8_MAX, blob(x, b + 7) * FP8_MAX) << 16);
         }
-        *reinterpret_cast<uint16_t*>(sm.x0 + s * LQ0 + f) = (uint16_t)fp8x2(v0, v1);
+        *reinterpret_cast<uint2*>(sm.x0 + s * LQ0 + f) = w;
     }
 }
 
@@ -796,6 +809,49 @@ __global__ void k_pack_weights2(const float* params, uint16_t* gwt) {
     }
 }
 
+// Mixtures of a tile's samples from their raw outputs, as decode() but spread over
+// 4 consecutive threads per sample (NLOBES / 4 lobes each; the softmax maximum and
+// sum are combined across the 4 lanes); needs blockDim.x >= 4 TILE.
+__device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* dst) {
+    constexpr int LPT = NLOBES / 4;
+    static_assert(NLOBES % 4 == 0, "lobes per thread");
+    const int s = threadIdx.x >> 2, k0 = (threadIdx.x & 3) * LPT;
+    if (s >= nt) return;  // the 4 lanes of a sample share s
+    const float* o = o_tile + s * NOUT;
+    float mxv = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < LPT; j++) mxv = fmaxf(mxv, o[5 * (k0 + j)]);
+    mxv = fmaxf(mxv, __shfl_xor(mxv, 1));
+    mxv = fmaxf(mxv, __shfl_xor(mxv, 2));
+    float e[LPT], sum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < LPT; j++) {
+        e[j] = __expf(o[5 * (k0 + j)] - mxv);
+        sum += e[j];
+    }
+    sum += __shfl_xor(sum, 1);
+    sum += __shfl_xor(sum, 2);
+    const float inv = 1.0f / sum;
+    float* d = dst + (size_t)s * 5 * NLOBES;
+#pragma unroll
+    for (int j = 0; j < LPT; j++) {
+        const int k = k0 + j;
+        float x = o[5 * k + 2], y = o[5 * k + 3], z = o[5 * k + 4];
+        float nrm = sqrtf(x * x + y * y + z * z);
+        if (nrm < 1e-12f) {
+            x = 0.0f;
+            y = 0.0f;
+            z = 1.0f;
+            nrm = 1.0f;
+        }
+        d[k] = e[j] * inv;
+        d[NLOBES + k] = __expf(fminf(fmaxf(o[5 * k + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
+        d[2 * NLOBES + k] = x / nrm;
+        d[3 * NLOBES + k] = y / nrm;
+        d[4 * NLOBES + k] = z / nrm;
+    }
+}
+
 // The path tracer's per-bounce inference: weights into LDS once per block, then per
 // tile of up to TILE samples (NIN features each at `in`) the forward pass and the
 // decoded mixtures, sample s's lobe k at dst[s 5 NLOBES + {k, NLOBES + k, ...}]
@@ -821,19 +877,7 @@ __device__ __forceinline__ void infer_tile(unsigned char* smem, const uint16_t* 
         __syncthreads();
         forward_tile(sm, WT_LDS ? sm.wt : gwt, o_tile);
     }
-    if ((int)threadIdx.x < nt) {
-        Lobes L;
-        decode(o_tile + threadIdx.x * NOUT, L);
-        float* d = dst + (size_t)threadIdx.x * 5 * NLOBES;
-#pragma unroll
-        for (int k = 0; k < NLOBES; k++) {
-            d[k] = L.pi[k];
-            d[NLOBES + k] = L.kappa[k];
-            d[2 * NLOBES + k] = L.mx[k];
-            d[3 * NLOBES + k] = L.my[k];
-            d[4 * NLOBES + k] = L.mz[k];
-        }
-    }
+    decode_tile(o_tile, nt, dst);
     __syncthreads();
 }
 

@@ -181,6 +181,8 @@ typedef struct {
     int64_t guide_samples;    /* CPT_RENDER_GUIDE_TRAIN: training samples used */
     int32_t guide_steps;      /* CPT_RENDER_GUIDE_TRAIN: Adam steps taken */
     int32_t reserved;
+    double ms_guide;          /* CPT_RENDER_KERNEL_TIMING, guided renders: guide inference */
+    int64_t n_guide;
 } cpt_render_stats;
 
 /* Create a context on HIP device `device`.  Fails with CPT_E_NODEVICE when the
