@@ -112,7 +112,10 @@ struct WfArgs {
     double* film;      // per-pixel radiance sums
     double* film2;     // per-pixel sums of squared radiance
     // learned guide (guided renders only)
-    float* lobes;            // per queue entry: pi[8], kappa[8], mu.x[8], mu.y[8], mu.z[8]
+    float* lobes;            // decoded mixtures, structure of arrays: value v (0..5 L - 1: pi[L], kappa[L],
+                             // mu.x[L], mu.y[L], mu.z[L], L = guide_lobes) of queue entry e at
+                             // lobes[v * lobes_stride + e]
+    size_t lobes_stride;
     float* rec;              // training records: [slot][depth][REC_FLOATS]
     uint32_t n_rec;          // slots that record
     int rec_depth;
@@ -351,39 +354,41 @@ __global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_extend(WfArgs a, int cur)
 
 // ------------------------------------------------------------------ learned guide in the path
 
-// Decoded mixture of one queue entry, NL lobes: weights lob[k], kappa lob[NL + k],
-// mean lob[2 NL + k], lob[3 NL + k], lob[4 NL + k].
+// Decoded mixture of one queue entry (lob = lobes + entry), NL lobes: weight
+// lob[k st], kappa lob[(NL + k) st], mean lob[(2 NL + k) st], lob[(3 NL + k) st],
+// lob[(4 NL + k) st]; st = the structure-of-arrays stride, so a wave's loads of
+// one value for consecutive entries are contiguous.
 template <int NL>
-__device__ __forceinline__ float gmix_pdf(const float* lob, V3 w) {
+__device__ __forceinline__ float gmix_pdf(const float* lob, size_t st, V3 w) {
     float p = 0.0f;
 #pragma unroll
     for (int k = 0; k < NL; k++) {
-        float c = lob[2 * NL + k] * w.x + lob[3 * NL + k] * w.y + lob[4 * NL + k] * w.z;
-        p += lob[k] * guide::vmf_pdf(lob[NL + k], c);
+        float c = lob[(2 * NL + k) * st] * w.x + lob[(3 * NL + k) * st] * w.y + lob[(4 * NL + k) * st] * w.z;
+        p += lob[k * st] * guide::vmf_pdf(lob[(NL + k) * st], c);
     }
     return p;
 }
 
 template <int NL>
-__device__ __forceinline__ V3 gmix_sample(const float* lob, float u0, float u1, float u2) {
+__device__ __forceinline__ V3 gmix_sample(const float* lob, size_t st, float u0, float u1, float u2) {
     // lobe by its weight, then a vMF sample around its mean (read from memory:
     // a local array indexed by the chosen lobe would live in scratch)
     int k = NL - 1;
     float c = 0.0f;
     for (int j = 0; j < NL; j++) {
-        c += lob[j];
+        c += lob[j * st];
         if (u0 < c) {
             k = j;
             break;
         }
     }
-    float kap = lob[NL + k];
+    float kap = lob[(NL + k) * st];
     float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kap)) / kap;
     w = fminf(fmaxf(w, -1.0f), 1.0f);
     float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     float sn, co;
     __sincosf(guide::TWO_PI * u2, &sn, &co);
-    float mx = lob[2 * NL + k], my = lob[3 * NL + k], mz = lob[4 * NL + k];
+    float mx = lob[(2 * NL + k) * st], my = lob[(3 * NL + k) * st], mz = lob[(4 * NL + k) * st];
     float sign = copysignf(1.0f, mz);
     float a = -1.0f / (sign + mz);
     float b = mx * my * a;
@@ -395,11 +400,11 @@ __device__ __forceinline__ V3 gmix_sample(const float* lob, float u0, float u1, 
 }
 
 // the guide's lobe count is 8 (width-64 network) or 16 (width 128)
-__device__ __forceinline__ float gmix_pdf_n(const float* lob, int nl, V3 w) {
-    return nl == 16 ? gmix_pdf<16>(lob, w) : gmix_pdf<8>(lob, w);
+__device__ __forceinline__ float gmix_pdf_n(const float* lob, size_t st, int nl, V3 w) {
+    return nl == 16 ? gmix_pdf<16>(lob, st, w) : gmix_pdf<8>(lob, st, w);
 }
-__device__ __forceinline__ V3 gmix_sample_n(const float* lob, int nl, float u0, float u1, float u2) {
-    return nl == 16 ? gmix_sample<16>(lob, u0, u1, u2) : gmix_sample<8>(lob, u0, u1, u2);
+__device__ __forceinline__ V3 gmix_sample_n(const float* lob, size_t st, int nl, float u0, float u1, float u2) {
+    return nl == 16 ? gmix_sample<16>(lob, st, u0, u1, u2) : gmix_sample<8>(lob, st, u0, u1, u2);
 }
 
 __device__ __forceinline__ bool guidable(const Mat& m) {
@@ -407,14 +412,13 @@ __device__ __forceinline__ bool guidable(const Mat& m) {
 }
 
 // Per bounce: network inputs of every queued hit, one MLP forward per tile of 64,
-// decoded mixture written per queue entry (5 x lobes floats).
+// decoded mixture written per queue entry (5 x lobes values, structure of arrays).
 // WIDTH: 64 (namespace guide) or 128 (guide128); FP8: the e4m3 network (k_quant8's
 // weights) on FP8 MFMA, else BF16
 template <int WIDTH, bool FP8>
 __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int NOUT = WIDTH == 128 ? guide128::NOUT : guide::NOUT;
-    constexpr int LSTRIDE = 5 * (WIDTH == 128 ? guide128::NLOBES : guide::NLOBES);
     __shared__ float o_tile[guide::TILE * NOUT];
     // features of a group of 4 tiles, one sample per thread: the hit record,
     // material and feature maths are latency-bound, so all four waves take part
@@ -455,9 +459,10 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
             const unsigned e0 = eg + q * guide::TILE;
             const int nt = min(guide::TILE, ng - q * guide::TILE);
             const float* in = in_s + q * guide::TILE * guide::NIN;
-            float* dst = a.lobes + (size_t)e0 * LSTRIDE;
-            if constexpr (WIDTH == 128) guide128::infer_tile<FP8>(smem_raw, a.gwt, o_tile, in, nt, dst);
-            else guide::infer_tile<FP8>(smem_raw, a.gwt, o_tile, in, nt, dst);
+            float* dst = a.lobes + e0;
+            if constexpr (WIDTH == 128)
+                guide128::infer_tile<FP8>(smem_raw, a.gwt, o_tile, in, nt, dst, a.lobes_stride);
+            else guide::infer_tile<FP8>(smem_raw, a.gwt, o_tile, in, nt, dst, a.lobes_stride);
         }
     }
 }
@@ -570,7 +575,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                     }
                 } else {
                     sampled_delta = false;
-                    const float* lob = GUIDED ? a.lobes + (size_t)i * 5 * a.guide_lobes : nullptr;
+                    const float* lob = GUIDED ? a.lobes + i : nullptr;
                     const bool gv = GUIDED && depth < a.guide_depth && guidable(m);
                     const float alpha = a.guide_frac;
                     DSample ds;
@@ -580,13 +585,13 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                         if (u < alpha) {
                             float u1 = pcg_float(rng);
                             float u2 = pcg_float(rng);
-                            ds.wi = gmix_sample_n(lob, a.guide_lobes, u / alpha, u1, u2);
+                            ds.wi = gmix_sample_n(lob, a.lobes_stride, a.guide_lobes, u / alpha, u1, u2);
                             ds.delta = false;
                             ds.pdf = bsdf_pdf(m, it, wo, ds.wi);
                         } else {
                             ds = bsdf_sample(m, it, wo, rng);
                         }
-                        ds.pdf = alpha * gmix_pdf_n(lob, a.guide_lobes, ds.wi) + (1.0f - alpha) * ds.pdf;
+                        ds.pdf = alpha * gmix_pdf_n(lob, a.lobes_stride, a.guide_lobes, ds.wi) + (1.0f - alpha) * ds.pdf;
                     } else {
                         ds = bsdf_sample(m, it, wo, rng);
                     }
@@ -600,7 +605,8 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                             sdir = divs(sdir, len);
                             if (bsdf_non_zero(m, it, wo, sdir)) {
                                 float pnb = bsdf_pdf(m, it, wo, sdir);
-                                if (gv) pnb = alpha * gmix_pdf_n(lob, a.guide_lobes, sdir) + (1.0f - alpha) * pnb;
+                                if (gv)
+                                    pnb = alpha * gmix_pdf_n(lob, a.lobes_stride, a.guide_lobes, sdir) + (1.0f - alpha) * pnb;
                                 float ct = (float)(fabs((double)dot(ls.normal, sdir)) + 1e-4);
                                 float pnn = ls.pdf * len * len / ct;
                                 float w = pnn / (pnb + pnn);
@@ -1420,6 +1426,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.film = c->film;
     a.film2 = c->film2;
     a.lobes = c->g_lobes;
+    a.lobes_stride = c->g_lobes_cap / (5 * (c->g_width == 128 ? 16 : 8));
     a.rec = c->g_rec;
     a.n_rec = 0;
     a.rec_depth = depth;
@@ -2286,11 +2293,11 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     if (use_graph) {
         char key[512];
         snprintf(key, sizeof(key),
-                 "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d,%d,%d,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
+                 "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
                  max_depth, (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
                  (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator,
-                 a.guide_lobes * 4 + a.corrected * 2 + a.guide_fp8,
+                 a.guide_lobes * 4 + a.corrected * 2 + a.guide_fp8, a.lobes_stride,
                  (void*)c->sd_snode, (void*)c->sd_dinfo, (void*)c->sd_quad, (void*)c->sd_rect, (void*)c->sd_sum,
                  (void*)c->ppg_vtx, (void*)c->ppg_nv, a.sdt.bmin[0], a.sdt.bmin[1], a.sdt.bmin[2], a.sdt.bmax[0],
                  a.sdt.bmax[1], a.sdt.bmax[2]);

@@ -811,8 +811,9 @@ __global__ void k_pack_weights2(const float* params, uint16_t* gwt) {
 
 // Mixtures of a tile's samples from their raw outputs, as decode() but spread over
 // 4 consecutive threads per sample (NLOBES / 4 lobes each; the softmax maximum and
-// sum are combined across the 4 lanes); needs blockDim.x >= 4 TILE.
-__device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* dst) {
+// sum are combined across the 4 lanes); needs blockDim.x >= 4 TILE.  Value v of
+// sample s goes to dst[v * stride + s] (structure of arrays).
+__device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* dst, size_t stride) {
     constexpr int LPT = NLOBES / 4;
     static_assert(NLOBES % 4 == 0, "lobes per thread");
     const int s = threadIdx.x >> 2, k0 = (threadIdx.x & 3) * LPT;
@@ -832,7 +833,7 @@ __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* 
     sum += __shfl_xor(sum, 1);
     sum += __shfl_xor(sum, 2);
     const float inv = 1.0f / sum;
-    float* d = dst + (size_t)s * 5 * NLOBES;
+    float* d = dst + s;
 #pragma unroll
     for (int j = 0; j < LPT; j++) {
         const int k = k0 + j;
@@ -844,17 +845,17 @@ __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* 
             z = 1.0f;
             nrm = 1.0f;
         }
-        d[k] = e[j] * inv;
-        d[NLOBES + k] = __expf(fminf(fmaxf(o[5 * k + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
-        d[2 * NLOBES + k] = x / nrm;
-        d[3 * NLOBES + k] = y / nrm;
-        d[4 * NLOBES + k] = z / nrm;
+        d[k * stride] = e[j] * inv;
+        d[(NLOBES + k) * stride] = __expf(fminf(fmaxf(o[5 * k + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
+        d[(2 * NLOBES + k) * stride] = x / nrm;
+        d[(3 * NLOBES + k) * stride] = y / nrm;
+        d[(4 * NLOBES + k) * stride] = z / nrm;
     }
 }
 
 // The path tracer's per-bounce inference: weights into LDS once per block, then per
 // tile of up to TILE samples (NIN features each at `in`) the forward pass and the
-// decoded mixtures, sample s's lobe k at dst[s 5 NLOBES + {k, NLOBES + k, ...}]
+// decoded mixtures, sample s's lobe k at dst[{k, NLOBES + k, ...} stride + s]
 // (weight, kappa, mean x, y, z).  FP8: the e4m3 network of k_quant8.
 template <bool FP8>
 __device__ __forceinline__ void infer_load(unsigned char* smem, const uint16_t* gwt, const uint8_t* gwt8,
@@ -865,7 +866,7 @@ __device__ __forceinline__ void infer_load(unsigned char* smem, const uint16_t* 
 
 template <bool FP8>
 __device__ __forceinline__ void infer_tile(unsigned char* smem, const uint16_t* gwt, float* o_tile, const float* in,
-                                           int nt, float* dst) {
+                                           int nt, float* dst, size_t stride) {
     if constexpr (FP8) {
         SmemFwd8& sm = *reinterpret_cast<SmemFwd8*>(smem);
         encode_tile8(sm, in, nt);
@@ -877,7 +878,7 @@ __device__ __forceinline__ void infer_tile(unsigned char* smem, const uint16_t* 
         __syncthreads();
         forward_tile(sm, WT_LDS ? sm.wt : gwt, o_tile);
     }
-    decode_tile(o_tile, nt, dst);
+    decode_tile(o_tile, nt, dst, stride);
     __syncthreads();
 }
 
