@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--no-ppg", action="store_true", help="guided: skip the SD-tree (PPG) arm")
     ap.add_argument("--fp8", action="store_true", help="guided: run the guide network in FP8 (training stays BF16)")
+    ap.add_argument("--train-paths", type=int, default=0,
+                    help="guided: paths per pass that record training samples (0: min(pixels, 2^18))")
     ap.add_argument("--guide-width", type=int, choices=[64, 128], default=64,
                     help="guided: hidden width of the guide network (64: 8 lobes, 128: 16 lobes)")
     ap.add_argument("--guide-depth", type=int, default=0,
@@ -133,16 +135,19 @@ def bench_guided(args):
 
     def run(guided):
         if guided:
-            ctx.guide_init(seed=11, lr=3e-3, train_paths=min(W * H, 1 << 18), train_steps=1,
+            ctx.guide_init(seed=11, lr=3e-3, train_paths=args.train_paths or min(W * H, 1 << 18), train_steps=1,
                            guide_depth=args.guide_depth, infer_fp8=args.fp8, width=args.guide_width)
         flags = (cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN) if guided else 0
         curve = []
         elapsed, spp, k = 0.0, 0, 0
+        gsamples = gsteps = 0
         next_check = 0.25
         while elapsed < budgets[-1]:
             t = time.perf_counter()
-            ctx.render(W, H, 1, max_depth=depth, seed=1 + k, flags=flags | (cpt.CPT_RENDER_ACCUMULATE if k else 0))
+            st = ctx.render(W, H, 1, max_depth=depth, seed=1 + k, flags=flags | (cpt.CPT_RENDER_ACCUMULATE if k else 0))
             elapsed += time.perf_counter() - t
+            gsamples += st["guide_samples"]
+            gsteps += st["guide_steps"]
             spp += 1
             k += 1
             if elapsed >= next_check or elapsed >= budgets[-1]:
@@ -150,7 +155,9 @@ def bench_guided(args):
                 curve.append((elapsed, spp, relmse(sums / spp), relmse_trim(sums / spp)))
                 next_check = elapsed * 1.25
                 print(f"bench: {'guided' if guided else 'plain'} t={elapsed:.2f}s spp={spp} relMSE={curve[-1][2]:.4f} "
-                      f"trimmed={curve[-1][3]:.5f}", file=sys.stderr, flush=True)
+                      f"trimmed={curve[-1][3]:.5f}" +
+                      (f" training samples/step {gsamples / max(gsteps, 1):.0f}" if guided else ""),
+                      file=sys.stderr, flush=True)
         return curve
 
     plain = run(False)
@@ -218,7 +225,8 @@ def bench_guided(args):
         "value": rows[-1]["plain_relmse"] / max(rows[-1]["guided_relmse"], 1e-30),
         "unit": "relMSE ratio plain/guided at the largest budget",
         "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "f32 (guide: " + ("fp8 e4m3 MFMA inference, bf16 training" if args.fp8 else "bf16 MFMA") + ", fp32 acc)",
-        "guide": {"width": args.guide_width, "lobes": 8 if args.guide_width == 64 else 16, "infer_fp8": bool(args.fp8)},
+        "guide": {"width": args.guide_width, "lobes": 8 if args.guide_width == 64 else 16, "infer_fp8": bool(args.fp8),
+                  "train_paths": args.train_paths or min(W * H, 1 << 18)},
         "config": {"workload": f"{'4x4 grid of hard-lighting interiors, 32 emitters, mixed GGX' if args.scene == 'grid' else 'hard-lighting interior'}, "
                                f"{scene.n_triangles} triangles, {W}x{H}, max depth {depth}, "
                                f"reference {ref_spp} spp FP64", "budgets_s": budgets},
