@@ -117,6 +117,7 @@ struct WfArgs {
                              // lobes[v * lobes_stride + e]
     size_t lobes_stride;
     float* rec;              // training records: [slot][depth][REC_FLOATS]
+    uint8_t* rec_flag;       // per record: 0 empty, 1 vertex recorded, 2 radiance after it known
     uint32_t n_rec;          // slots that record
     int rec_depth;
     float guide_frac;
@@ -146,7 +147,10 @@ struct WfArgs {
 // training record of one guided path vertex: inputs[7], dir[3], pdf, path throughput
 // before the vertex [3], radiance snapshot[3], state (0 empty, 1 direction traced,
 // 2 complete), path throughput after the vertex [3]
-#define REC_FLOATS 21
+// record: features [0, 7), sampled direction [7, 10), its pdf [10], throughput before
+// the vertex [11, 14), path radiance when the vertex was recorded [14, 17), throughput
+// after it [17, 20)
+#define REC_FLOATS 20
 
 // waves per SIMD the traversal kernels are compiled for (caps their VGPRs)
 #ifndef CPT_TRAV_WAVES
@@ -269,7 +273,7 @@ __global__ void k_raygen(WfArgs a) {
         a.rng[i] = rng;
         a.meta[i] = 1u << 8;  // depth 0, sampled_delta = true (camera ray)
         if (i < a.n_rec)
-            for (int k = 0; k < a.rec_depth; k++) a.rec[((size_t)i * a.rec_depth + k) * REC_FLOATS + 17] = 0.0f;
+            for (int k = 0; k < a.rec_depth; k++) a.rec_flag[(size_t)i * a.rec_depth + k] = 0;
     }
 }
 
@@ -477,25 +481,23 @@ __global__ void k_guide_collect(WfArgs a) {
         unsigned idx = base + __lane_id();
         bool emit = false;
         float smp[guide::SAMPLE_FLOATS];
-        if (idx < total) {
+        if (idx < total && a.rec_flag[idx] == 2) {
             const float* r = a.rec + (size_t)idx * REC_FLOATS;
-            if (r[17] == 2.0f) {
-                unsigned slot = idx / (unsigned)a.rec_depth;
-                float4 c = a.col[slot];
-                // radiance gathered through the sampled direction divided by the
-                // throughput after the vertex is L_i; divided by the throughput
-                // before it, L_i f cos / p.  Weight = target / sampling density.
-                const int m = a.guide_target ? 18 : 11;
-                float lr = r[m] > 0.0f ? (c.x - r[14]) / r[m] : 0.0f;
-                float lg = r[m + 1] > 0.0f ? (c.y - r[15]) / r[m + 1] : 0.0f;
-                float lb = r[m + 2] > 0.0f ? (c.z - r[16]) / r[m + 2] : 0.0f;
-                float w = fmaxf(0.2126f * lr + 0.7152f * lg + 0.0722f * lb, 0.0f);
-                if (a.guide_target) w = r[10] > 0.0f ? w / r[10] : 0.0f;
-                if (w > 0.0f && isfinite(w)) {
-                    emit = true;
-                    for (int k = 0; k < 10; k++) smp[k] = r[k];
-                    smp[10] = w;
-                }
+            unsigned slot = idx / (unsigned)a.rec_depth;
+            float4 c = a.col[slot];
+            // radiance gathered through the sampled direction divided by the
+            // throughput after the vertex is L_i; divided by the throughput
+            // before it, L_i f cos / p.  Weight = target / sampling density.
+            const int m = a.guide_target ? 17 : 11;
+            float lr = r[m] > 0.0f ? (c.x - r[14]) / r[m] : 0.0f;
+            float lg = r[m + 1] > 0.0f ? (c.y - r[15]) / r[m + 1] : 0.0f;
+            float lb = r[m + 2] > 0.0f ? (c.z - r[16]) / r[m + 2] : 0.0f;
+            float w = fmaxf(0.2126f * lr + 0.7152f * lg + 0.0722f * lb, 0.0f);
+            if (a.guide_target) w = r[10] > 0.0f ? w / r[10] : 0.0f;
+            if (w > 0.0f && isfinite(w)) {
+                emit = true;
+                for (int k = 0; k < 10; k++) smp[k] = r[k];
+                smp[10] = w;
             }
         }
         unsigned q = wave_alloc(emit, a.gcount);
@@ -540,12 +542,13 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
             float* rec = nullptr;
             if (GUIDED && (unsigned)slot < a.n_rec) {
                 rec = a.rec + (size_t)slot * a.rec_depth * REC_FLOATS;
-                if (depth > 0 && rec[(depth - 1) * REC_FLOATS + 17] == 1.0f) {
+                uint8_t* fl = a.rec_flag + (size_t)slot * a.rec_depth;
+                if (depth > 0 && fl[depth - 1] == 1) {
                     float* r = rec + (depth - 1) * REC_FLOATS;
                     r[14] = colv.x;
                     r[15] = colv.y;
                     r[16] = colv.z;
-                    r[17] = 2.0f;
+                    fl[depth - 1] = 2;
                 }
             }
             if (h.tri < 0) {
@@ -652,10 +655,10 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                                     r[11] = mul_prev.x;
                                     r[12] = mul_prev.y;
                                     r[13] = mul_prev.z;
-                                    r[18] = mulv.x;
-                                    r[19] = mulv.y;
-                                    r[20] = mulv.z;
-                                    r[17] = 1.0f;
+                                    r[17] = mulv.x;
+                                    r[18] = mulv.y;
+                                    r[19] = mulv.z;
+                                    a.rec_flag[(size_t)slot * a.rec_depth + depth] = 1;
                                 }
                             }
                         }
@@ -1230,12 +1233,14 @@ struct cpt_ctx {
     float* g_partial = nullptr;  // per-block dW partials
     float* g_loss_part = nullptr;
     float* g_wscale = nullptr;
+    float* g_wpart = nullptr;    // per-block partial sums of the sample weights
     int g_step = 0;
     cpt_guide_params g_cfg{};
     int g_width = 0;             // network of the allocated guide buffers: 64 or 128 (0: none)
     float* g_lobes = nullptr;    // per queue entry mixture (5 floats per lobe)
     size_t g_lobes_cap = 0;      // floats
     float* g_rec = nullptr;      // training records
+    uint8_t* g_rec_flag = nullptr;
     size_t g_rec_cap = 0;
     uint32_t* g_count = nullptr;
     uint32_t* h_count = nullptr; // pinned
@@ -1428,6 +1433,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.lobes = c->g_lobes;
     a.lobes_stride = c->g_lobes_cap / (5 * (c->g_width == 128 ? 16 : 8));
     a.rec = c->g_rec;
+    a.rec_flag = c->g_rec_flag;
     a.n_rec = 0;
     a.rec_depth = depth;
     a.guide_frac = c->g_cfg.guide_fraction;
@@ -1675,6 +1681,7 @@ void cpt_destroy(cpt_ctx* c) {
     dfree(c->g_partial);
     dfree(c->g_loss_part);
     dfree(c->g_wscale);
+    dfree(c->g_wpart);
     dfree(c->g_lobes);
     dfree(c->sd_snode);
     dfree(c->sd_quad);
@@ -1685,6 +1692,7 @@ void cpt_destroy(cpt_ctx* c) {
     dfree(c->ppg_vtx);
     dfree(c->ppg_nv);
     dfree(c->g_rec);
+    dfree(c->g_rec_flag);
     dfree(c->g_count);
     if (c->h_count) (void)hipHostFree(c->h_count);
     if (c->h_pass) (void)hipHostFree(c->h_pass);
@@ -2218,6 +2226,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
             if (c->g_rec_cap < nr || !c->g_rec) {
                 drop_graph(c);
                 r = dalloc(c, c->g_rec, nr * REC_FLOATS);
+                if (!r) r = dalloc(c, c->g_rec_flag, nr);
                 if (r) return r;
                 c->g_rec_cap = nr;
             }
@@ -2842,6 +2851,7 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
 // ------------------------------------------------------------------ learned guide
 static const int GUIDE_TRAIN_BLOCKS = 256;
 static const int GUIDE_DW_BLOCKS = 256;
+static const int GUIDE_WSUM_BLOCKS = 128;
 
 int cpt_guide_num_params(void) { return guide::NPARAM; }
 
@@ -2948,7 +2958,9 @@ int guide_train_impl(cpt_ctx* c, int n, bool apply) {
         c->g_rows_cap = n;
     }
     int per_block = ((n + GUIDE_DW_BLOCKS - 1) / GUIDE_DW_BLOCKS + N::kChunk - 1) / N::kChunk * N::kChunk;
-    hipLaunchKernelGGL(N::weight_scale, dim3(1), dim3(256), 0, c->stream, c->g_samples, n, c->g_wscale);
+    hipLaunchKernelGGL(N::weight_partial, dim3(GUIDE_WSUM_BLOCKS), dim3(256), 0, c->stream, c->g_samples, n,
+                       c->g_wpart);
+    hipLaunchKernelGGL(N::weight_scale, dim3(1), dim3(256), 0, c->stream, c->g_wpart, GUIDE_WSUM_BLOCKS, c->g_wscale);
     hipLaunchKernelGGL(N::train_tile, dim3(GUIDE_TRAIN_BLOCKS), dim3(256), N::kSmemFwd, c->stream, c->g_wt,
                        c->g_params, c->g_samples, n, c->g_wscale, c->g_rows, c->g_loss_part);
     hipLaunchKernelGGL(N::train_dw, dim3(GUIDE_DW_BLOCKS, N::kDwGroups), dim3(256), 0, c->stream, c->g_rows, n,
@@ -2989,6 +3001,7 @@ int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
         r |= dalloc(c, c->g_inv8, 8);
         r |= dalloc(c, c->g_loss_part, GUIDE_TRAIN_BLOCKS);
         r |= dalloc(c, c->g_wscale, 1);
+        r |= dalloc(c, c->g_wpart, GUIDE_WSUM_BLOCKS);
         r |= dalloc(c, c->g_count, 4);
         if (r) return CPT_E_HIP;
         HIPCHK(c, hipHostMalloc((void**)&c->h_count, 4 * sizeof(uint32_t), hipHostMallocDefault));

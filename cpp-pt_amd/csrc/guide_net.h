@@ -756,11 +756,26 @@ __global__ void k_reduce_partials(const float* partial, int nblocks, int stride,
     }
 }
 
-// 1 / sum of sample weights (one block)
-__global__ void k_weight_scale(const float* samples, int n, float* wscale) {
+// 1 / sum of sample weights, deterministic in two steps: per-block partial sums
+// (grid-stride, fixed grid) and their sum in block order by one block.
+__global__ __launch_bounds__(256) void k_weight_partial(const float* samples, int n, float* part) {
     __shared__ float red[256];
     float s = 0.0f;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += samples[(size_t)i * SAMPLE_FLOATS + 10];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        s += samples[(size_t)i * SAMPLE_FLOATS + 10];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_weight_scale(const float* part, int nparts, float* wscale) {
+    __shared__ float red[256];
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
     red[threadIdx.x] = s;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
@@ -893,6 +908,7 @@ struct Net {
     static constexpr auto train_tile = &k_train_tile;
     static constexpr auto train_dw = &k_train_dw;
     static constexpr auto reduce_partials = &k_reduce_partials;
+    static constexpr auto weight_partial = &k_weight_partial;
     static constexpr auto weight_scale = &k_weight_scale;
     static constexpr auto adam = &k_adam;
     static constexpr auto pack_weights = &k_pack_weights;
