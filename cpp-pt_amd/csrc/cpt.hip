@@ -112,9 +112,9 @@ struct WfArgs {
     double* film;      // per-pixel radiance sums
     double* film2;     // per-pixel sums of squared radiance
     // learned guide (guided renders only)
-    float* lobes;            // decoded mixtures, structure of arrays: value v (0..5 L - 1: pi[L], kappa[L],
-                             // mu.x[L], mu.y[L], mu.z[L], L = guide_lobes) of queue entry e at
-                             // lobes[v * lobes_stride + e]
+    float* lobes;            // decoded mixtures, structure of arrays: value v (0..6 L - 1: pi[L], kappa[L],
+                             // mu.x[L], mu.y[L], mu.z[L], pi kappa / (2 pi (1 - e^-2kappa))[L],
+                             // L = guide_lobes) of queue entry e at lobes[v * lobes_stride + e]
     size_t lobes_stride;
     float* rec;              // training records: [slot][depth][REC_FLOATS]
     uint8_t* rec_flag;       // per record: 0 empty, 1 vertex recorded, 2 radiance after it known
@@ -360,15 +360,16 @@ __global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_extend(WfArgs a, int cur)
 
 // Decoded mixture of one queue entry (lob = lobes + entry), NL lobes: weight
 // lob[k st], kappa lob[(NL + k) st], mean lob[(2 NL + k) st], lob[(3 NL + k) st],
-// lob[(4 NL + k) st]; st = the structure-of-arrays stride, so a wave's loads of
-// one value for consecutive entries are contiguous.
+// lob[(4 NL + k) st], weight x vMF normalisation lob[(5 NL + k) st]; st = the
+// structure-of-arrays stride, so a wave's loads of one value for consecutive
+// entries are contiguous.
 template <int NL>
 __device__ __forceinline__ float gmix_pdf(const float* lob, size_t st, V3 w) {
     float p = 0.0f;
 #pragma unroll
     for (int k = 0; k < NL; k++) {
         float c = lob[(2 * NL + k) * st] * w.x + lob[(3 * NL + k) * st] * w.y + lob[(4 * NL + k) * st] * w.z;
-        p += lob[k * st] * guide::vmf_pdf(lob[(NL + k) * st], c);
+        p += lob[(5 * NL + k) * st] * __expf(lob[(NL + k) * st] * (c - 1.0f));
     }
     return p;
 }
@@ -416,7 +417,7 @@ __device__ __forceinline__ bool guidable(const Mat& m) {
 }
 
 // Per bounce: network inputs of every queued hit, one MLP forward per tile of 64,
-// decoded mixture written per queue entry (5 x lobes values, structure of arrays).
+// decoded mixture written per queue entry (6 x lobes values, structure of arrays).
 // WIDTH: 64 (namespace guide) or 128 (guide128); FP8: the e4m3 network (k_quant8's
 // weights) on FP8 MFMA, else BF16
 template <int WIDTH, bool FP8>
@@ -1431,7 +1432,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.film = c->film;
     a.film2 = c->film2;
     a.lobes = c->g_lobes;
-    a.lobes_stride = c->g_lobes_cap / (5 * (c->g_width == 128 ? 16 : 8));
+    a.lobes_stride = c->g_lobes_cap / (6 * (c->g_width == 128 ? 16 : 8));
     a.rec = c->g_rec;
     a.rec_flag = c->g_rec_flag;
     a.n_rec = 0;
@@ -2213,7 +2214,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     size_t nrec = 0;
     if (guided) {
         if (!c->has_guide) return fail(c, CPT_E_INVALID, "guided render needs cpt_guide_init");
-        const size_t lfloats = slots * 5 * (c->g_width == 128 ? 16 : 8);
+        const size_t lfloats = slots * 6 * (c->g_width == 128 ? 16 : 8);
         if (c->g_lobes_cap < lfloats || !c->g_lobes) {
             drop_graph(c);
             r = dalloc(c, c->g_lobes, lfloats);

@@ -850,7 +850,8 @@ __global__ void k_pack_weights2(const float* params, uint16_t* gwt) {
 // Mixtures of a tile's samples from their raw outputs, as decode() but spread over
 // 4 consecutive threads per sample (NLOBES / 4 lobes each; the softmax maximum and
 // sum are combined across the 4 lanes); needs blockDim.x >= 4 TILE.  Value v of
-// sample s goes to dst[v * stride + s] (structure of arrays).
+// sample s goes to dst[v * stride + s] (structure of arrays): weights, kappas, mean
+// x, y, z, then weight x vMF normalisation, NLOBES values each.
 __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* dst, size_t stride) {
     constexpr int LPT = NLOBES / 4;
     static_assert(NLOBES % 4 == 0, "lobes per thread");
@@ -883,18 +884,23 @@ __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* 
             z = 1.0f;
             nrm = 1.0f;
         }
-        d[k * stride] = e[j] * inv;
-        d[(NLOBES + k) * stride] = __expf(fminf(fmaxf(o[5 * k + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
+        const float pi = e[j] * inv;
+        const float kappa = __expf(fminf(fmaxf(o[5 * k + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
+        d[k * stride] = pi;
+        d[(NLOBES + k) * stride] = kappa;
         d[(2 * NLOBES + k) * stride] = x / nrm;
         d[(3 * NLOBES + k) * stride] = y / nrm;
         d[(4 * NLOBES + k) * stride] = z / nrm;
+        // the lobe's weight times its vMF normalisation (vmf_pdf without the exponential)
+        d[(5 * NLOBES + k) * stride] = pi * kappa / (TWO_PI * (1.0f - __expf(-2.0f * kappa)));
     }
 }
 
 // The path tracer's per-bounce inference: weights into LDS once per block, then per
 // tile of up to TILE samples (NIN features each at `in`) the forward pass and the
 // decoded mixtures, sample s's lobe k at dst[{k, NLOBES + k, ...} stride + s]
-// (weight, kappa, mean x, y, z).  FP8: the e4m3 network of k_quant8.
+// (weight, kappa, mean x, y, z, weight x normalisation).  FP8: the e4m3 network of
+// k_quant8.
 template <bool FP8>
 __device__ __forceinline__ void infer_load(unsigned char* smem, const uint16_t* gwt, const uint8_t* gwt8,
                                            const float* params, const float* inv8) {
