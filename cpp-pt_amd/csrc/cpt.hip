@@ -424,7 +424,7 @@ template <int WIDTH, bool FP8>
 __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int NOUT = WIDTH == 128 ? guide128::NOUT : guide::NOUT;
-    __shared__ float o_tile[guide::TILE * NOUT];
+    __shared__ __align__(16) float o_tile[guide::TILE * NOUT];
     // features of a group of 4 tiles, one sample per thread: the hit record,
     // material and feature maths are latency-bound, so all four waves take part
     __shared__ float in_s[4 * guide::TILE * guide::NIN];

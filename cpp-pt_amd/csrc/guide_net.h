@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* gwt, const 
                                                      int n, float* out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     SmemFwd& sm = *reinterpret_cast<SmemFwd*>(smem_raw);
-    __shared__ float o_tile[TILE * NOUT];
+    __shared__ __align__(16) float o_tile[TILE * NOUT];
     load_weights(sm, gwt, params);
     __syncthreads();
     for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
@@ -433,9 +433,12 @@ __device__ __forceinline__ void encode_tile8(SmemFwd8& sm, const float* in, int 
     }
 }
 
+// acc[nt] += (A(rows of this wave) x Wt^T)^T: the weights are the MFMA's first
+// operand, so lane l holds features nt 16 + 4 (l / 16) + j (j = 0..3) of sample row
+// l % 16, four consecutive outputs of one sample for packed stores
 template <int K, int NT>
-__device__ __forceinline__ void gemm8_rows_nk(const uint8_t* A, int lda, const uint8_t* Wt, int ldb, f4* acc,
-                                              int lane) {
+__device__ __forceinline__ void gemm8_rows_nk_t(const uint8_t* A, int lda, const uint8_t* Wt, int ldb, f4* acc,
+                                                int lane) {
 #pragma unroll
     for (int kt = 0; kt < K / 32; kt++) {
         const int kk = kt * 32 + 8 * (lane >> 4);
@@ -443,33 +446,44 @@ __device__ __forceinline__ void gemm8_rows_nk(const uint8_t* A, int lda, const u
 #pragma unroll
         for (int nt = 0; nt < NT; nt++) {
             const long b = *reinterpret_cast<const long*>(Wt + (nt * 16 + (lane & 15)) * ldb + kk);
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc[nt], 0, 0, 0);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(b, a, acc[nt], 0, 0, 0);
         }
     }
+}
+
+// four non-negative values (<= 448 up to rounding) as packed e4m3
+__device__ __forceinline__ uint32_t fp8x4_pos(float v0, float v1, float v2, float v3) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(fminf(v0, FP8_MAX), fminf(v1, FP8_MAX), 0, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(fminf(v2, FP8_MAX), fminf(v3, FP8_MAX), 0, false);
+    return (lo & 0xffffu) | (hi << 16);
 }
 
 // forward through all layers with FP8 operands; out as forward_tile
 __device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = 16 * wave;
+    const int m = r0 + (lane & 15);      // this lane's sample row
+    const int n4 = 4 * (lane >> 4);      // its first feature within a 16-wide tile
     float inv_a = 1.0f / FP8_MAX;  // inverse scale of the current layer's input
 #pragma unroll
     for (int l = 0; l < 4; l++) {
         f4 acc[NTH] = {};
-        if (l == 0) gemm8_rows_nk<K0, NTH>(sm.x0 + r0 * LQ0, LQ0, sm.wt + Q0, LQ0, acc, lane);
-        else gemm8_rows_nk<HID, NTH>(sm.h[(l - 1) & 1] + r0 * LQH, LQH, sm.wt + (l == 1 ? Q1 : l == 2 ? Q2 : Q3), LQH,
-                                     acc, lane);
+        if (l == 0) gemm8_rows_nk_t<K0, NTH>(sm.x0 + r0 * LQ0, LQ0, sm.wt + Q0, LQ0, acc, lane);
+        else gemm8_rows_nk_t<HID, NTH>(sm.h[(l - 1) & 1] + r0 * LQH, LQH, sm.wt + (l == 1 ? Q1 : l == 2 ? Q2 : Q3),
+                                       LQH, acc, lane);
         const float sc = inv_a * sm.inv_w[l];
         float mx = 0.0f;
 #pragma unroll
-        for (int nt = 0; nt < NTH; nt++)
+        for (int nt = 0; nt < NTH; nt++) {
+            const float4 bv = *reinterpret_cast<const float4*>(sm.bias + l * HID + nt * 16 + n4);
+            const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int n = nt * 16 + (lane & 15);
-                const float z = fmaxf(acc[nt][j] * sc + sm.bias[l * HID + n], 0.0f);
+                const float z = fmaxf(acc[nt][j] * sc + bb[j], 0.0f);
                 acc[nt][j] = z;
                 mx = fmaxf(mx, z);
             }
+        }
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
         if (lane == 0) sm.redmax[l & 1][wave] = mx;
         __syncthreads();
@@ -477,27 +491,24 @@ __device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
                               fmaxf(sm.redmax[l & 1][2], sm.redmax[l & 1][3]));
         const float s = M > 0.0f ? FP8_MAX / M : 1.0f;
         inv_a = M > 0.0f ? M / FP8_MAX : 1.0f;
-        uint8_t* hd = sm.h[l & 1];
+        uint8_t* hd = sm.h[l & 1] + m * LQH + n4;
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
-                hd[m * LQH + n] = fp8x1(acc[nt][j] * s);
-            }
+            *reinterpret_cast<uint32_t*>(hd + nt * 16) =
+                fp8x4_pos(acc[nt][0] * s, acc[nt][1] * s, acc[nt][2] * s, acc[nt][3] * s);
         __syncthreads();
     }
     {
         f4 acc[NTO] = {};
-        gemm8_rows_nk<HID, NTO>(sm.h[1] + r0 * LQH, LQH, sm.wt + Q4, LQH, acc, lane);
+        gemm8_rows_nk_t<HID, NTO>(sm.h[1] + r0 * LQH, LQH, sm.wt + Q4, LQH, acc, lane);
         const float sc = inv_a * sm.inv_w[4];
 #pragma unroll
-        for (int nt = 0; nt < NTO; nt++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int m = r0 + 4 * (lane >> 4) + j, n = nt * 16 + (lane & 15);
-                out[m * NOUT + n] = acc[nt][j] * sc + sm.bias[4 * HID + n];
-            }
+        for (int nt = 0; nt < NTO; nt++) {
+            const float4 bv = *reinterpret_cast<const float4*>(sm.bias + 4 * HID + nt * 16 + n4);
+            *reinterpret_cast<float4*>(out + m * NOUT + nt * 16 + n4) =
+                make_float4(acc[nt][0] * sc + bv.x, acc[nt][1] * sc + bv.y, acc[nt][2] * sc + bv.z,
+                            acc[nt][3] * sc + bv.w);
+        }
     }
     __syncthreads();
 }
@@ -536,7 +547,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward8(const uint8_t* gwt8, const
                                                       const float* in, int n, float* out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     SmemFwd8& sm = *reinterpret_cast<SmemFwd8*>(smem_raw);
-    __shared__ float o_tile[TILE * NOUT];
+    __shared__ __align__(16) float o_tile[TILE * NOUT];
     load_weights8(sm, gwt8, params, inv_scale);
     __syncthreads();
     for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
