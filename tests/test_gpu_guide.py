@@ -123,3 +123,17 @@ def test_training_learns_a_target_distribution(ctx, width):
     print("loss", losses[0], "->", losses[-1], "q(mu)", q_mu.mean(), "uniform", 1 / (4 * math.pi))
     assert np.mean(losses[-10:]) < np.mean(losses[:10]) - 0.5
     assert q_mu.min() > 10 / (4 * math.pi)
+
+
+def test_guide_shape_errors(ctx):
+    """Width other than 64 or 128 is rejected; a parameter vector of the other
+    width's size is refused before it reaches the device."""
+    with pytest.raises(Exception):
+        ctx.guide_init(seed=1, width=96)
+    ctx.guide_init(seed=1, width=128)
+    assert ctx.guide_shape() == (128, 16, 80, 88656)
+    with pytest.raises(ValueError):
+        ctx.guide_set_params(random_params(1))          # width-64 vector
+    ctx.guide_init(seed=1)
+    assert ctx.guide_shape() == (64, 8, 48, 30000)
+    assert ctx.guide_forward(np.zeros((3, 7), np.float32)).shape == (3, 48)
