@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* gwt, const 
 // FP8 inference (e4m3, OCP format of gfx950) on v_mfma_f32_16x16x32_fp8_fp8.
 // Weights: per layer scaled by 448 / max|W_l| and rounded to e4m3 (k_quant8);
 // one-blob features scaled by 448; each hidden layer's ReLU outputs scaled per
-// tile by 448 / (their maximum), found with one block reduction.  FP32
+// 16-row group (one wave's rows of the tile) by 448 / (their maximum).  FP32
 // accumulation; the accumulator is rescaled by the two inverse scales before the
 // bias.  The conversion does not saturate (it returns NaN above 448), so every
 // value is clamped first.  Lane layout of the 16x16x32 MFMA: lane l supplies
@@ -396,7 +396,6 @@ struct SmemFwd8 {
     uint8_t wt[NWQ];
     float bias[4 * HID + NOUT];
     float inv_w[8];       // 1 / weight scale per layer (5 used)
-    float redmax[2][4];   // per-wave maxima, double-buffered over layers
     uint8_t x0[TILE * LQ0];
     uint8_t h[2][TILE * LQH];
 };
@@ -484,19 +483,20 @@ __device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
                 mx = fmaxf(mx, z);
             }
         }
+        // one scale per wave: its 16 rows are the whole input of its next layer, so
+        // the layers of a tile need no block barrier
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-        if (lane == 0) sm.redmax[l & 1][wave] = mx;
-        __syncthreads();
-        const float M = fmaxf(fmaxf(sm.redmax[l & 1][0], sm.redmax[l & 1][1]),
-                              fmaxf(sm.redmax[l & 1][2], sm.redmax[l & 1][3]));
-        const float s = M > 0.0f ? FP8_MAX / M : 1.0f;
-        inv_a = M > 0.0f ? M / FP8_MAX : 1.0f;
+        const float s = mx > 0.0f ? FP8_MAX / mx : 1.0f;
+        inv_a = mx > 0.0f ? mx / FP8_MAX : 1.0f;
         uint8_t* hd = sm.h[l & 1] + m * LQH + n4;
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
             *reinterpret_cast<uint32_t*>(hd + nt * 16) =
                 fp8x4_pos(acc[nt][0] * s, acc[nt][1] * s, acc[nt][2] * s, acc[nt][3] * s);
-        __syncthreads();
+        // the next layer reads what the other lanes of this wave just wrote
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     {
         f4 acc[NTO] = {};

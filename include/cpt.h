@@ -298,7 +298,7 @@ typedef struct {
     int32_t guide_target;  /* training target: 0 = the product L_i f cos (default), 1 = the incident
                               radiance L_i alone (what PPG splats, ppg.h:130-141) */
     int32_t infer_fp8;     /* 1: guided renders and cpt_guide_forward run the network in FP8 (e4m3
-                              weights with per-layer scales, per-tile activation scales, FP8 MFMA,
+                              weights with per-layer scales, activation scales per 16 samples, FP8 MFMA,
                               FP32 accumulation); training stays BF16 */
     int32_t width;         /* hidden width: 64 (8 lobes; 0 = 64) or 128 (16 lobes, the large-scene
                               network: BF16 inference reads its weights through the L2, FP8

@@ -104,12 +104,12 @@ def e4m3(x):
     return np.sign(x) * np.round(a / step) * step
 
 
-def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64, width=64):
+def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64, width=64, group=16):
     """FP8 inference pipeline of the guide network restated in numpy: weights
     scaled per layer by 448 / max|W|, one-blob features by 448, each hidden
-    layer's ReLU outputs per tile of 64 samples by 448 / (tile maximum), the
-    padding rows of a partial tile taking part in that maximum as zero-feature
-    samples.  quant=identity gives the exact network."""
+    layer's ReLU outputs per group of 16 rows of a 64-sample tile by 448 /
+    (group maximum), the padding rows of a partial tile taking part in that
+    maximum as zero-feature samples.  quant=identity gives the exact network."""
     K, N, _, _ = net_shape(width)
     P = np.asarray(params, np.float64)
     Ws, bs, o = [], [], 0
@@ -133,11 +133,12 @@ def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64, width=64):
         f = np.zeros((tile, 224))
         f[:len(xs)] = np.exp(np.float32(-0.5) * d * d).reshape(len(xs), 224)
         h = quant(f * 448.0)
-        inv_a = 1.0 / 448.0
+        inv_a = np.full((tile, 1), 1.0 / 448.0)
         for l in range(4):
             z = np.maximum(h @ Wq[l] * (inv_a * inv_w[l]) + bs[l], 0.0)
-            M = z.max()
-            s, inv_a = (448.0 / M, M / 448.0) if M > 0 else (1.0, 1.0)
+            M = np.repeat(z.reshape(tile // group, -1).max(axis=1), group)[:, None]
+            s = np.where(M > 0, 448.0 / np.where(M > 0, M, 1.0), 1.0)
+            inv_a = np.where(M > 0, M / 448.0, 1.0)
             h = quant(z * s)
         out[t0:t0 + len(xs)] = (h @ Wq[4] * (inv_a * inv_w[4]) + bs[4])[:len(xs)]
     return out
