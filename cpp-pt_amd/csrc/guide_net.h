@@ -510,7 +510,11 @@ __device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
                             acc[nt][3] * sc + bv.w);
         }
     }
-    __syncthreads();
+    // out holds this wave's 16 rows; a reader of other waves' rows needs a block
+    // barrier of its own
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // FP8 copy of the weights: one block per layer; inv_scale[l] = max|W_l| / 448
@@ -555,6 +559,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward8(const uint8_t* gwt8, const
         encode_tile8(sm, in + (size_t)t * TILE * NIN, nt);
         __syncthreads();
         forward_tile8(sm, o_tile);
+        __syncthreads();
         for (int i = threadIdx.x; i < nt * NOUT; i += blockDim.x) out[(size_t)t * TILE * NOUT + i] = o_tile[i];
         __syncthreads();
     }
@@ -911,7 +916,8 @@ __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* 
 // tile of up to TILE samples (NIN features each at `in`) the forward pass and the
 // decoded mixtures, sample s's lobe k at dst[{k, NLOBES + k, ...} stride + s]
 // (weight, kappa, mean x, y, z, weight x normalisation).  FP8: the e4m3 network of
-// k_quant8.
+// k_quant8.  The decode reads only rows its own wave computed (4 lanes per sample,
+// 16 samples per wave), so after the FP8 forward it needs no block barrier.
 template <bool FP8>
 __device__ __forceinline__ void infer_load(unsigned char* smem, const uint16_t* gwt, const uint8_t* gwt8,
                                            const float* params, const float* inv8) {
