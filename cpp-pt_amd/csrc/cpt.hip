@@ -376,8 +376,10 @@ __device__ __forceinline__ float gmix_pdf(const float* lob, size_t st, V3 w) {
 
 template <int NL>
 __device__ __forceinline__ V3 gmix_sample(const float* lob, size_t st, float u0, float u1, float u2) {
-    // lobe by its weight, then a vMF sample around its mean (read from memory:
-    // a local array indexed by the chosen lobe would live in scratch)
+    // lobe by its weight, then a vMF sample around its mean (Jakob 2012, "Numerically
+    // stable sampling of the von Mises Fisher distribution"; orthonormal basis after
+    // Duff et al. 2017); values read from memory: a local array indexed by the
+    // chosen lobe would live in scratch
     int k = NL - 1;
     float c = 0.0f;
     for (int j = 0; j < NL; j++) {

@@ -163,44 +163,6 @@ __device__ __forceinline__ float vmf_pdf(float kappa, float cosang) {
     return kappa / (TWO_PI * (1.0f - __expf(-2.0f * kappa))) * __expf(kappa * (cosang - 1.0f));
 }
 
-__device__ __forceinline__ float mixture_pdf(const Lobes& L, float wx, float wy, float wz) {
-    float p = 0.0f;
-#pragma unroll
-    for (int k = 0; k < NLOBES; k++) p += L.pi[k] * vmf_pdf(L.kappa[k], L.mx[k] * wx + L.my[k] * wy + L.mz[k] * wz);
-    return p;
-}
-
-// lobe chosen by u0; vMF sample (Jakob 2012, "Numerically stable sampling of the von Mises Fisher distribution")
-__device__ __forceinline__ void mixture_sample(const Lobes& L, float u0, float u1, float u2, float& wx, float& wy,
-                                               float& wz) {
-    int k = NLOBES - 1;
-    float c = 0.0f;
-#pragma unroll
-    for (int j = 0; j < NLOBES; j++) {
-        c += L.pi[j];
-        if (u0 < c) {
-            k = j;
-            break;
-        }
-    }
-    float kap = L.kappa[k];
-    float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kap)) / kap;
-    w = fminf(fmaxf(w, -1.0f), 1.0f);
-    float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
-    float s, co;
-    __sincosf(TWO_PI * u2, &s, &co);
-    float mx = L.mx[k], my = L.my[k], mz = L.mz[k];
-    // orthonormal basis around mu (Duff et al. 2017)
-    float sign = copysignf(1.0f, mz);
-    float a = -1.0f / (sign + mz);
-    float b = mx * my * a;
-    float t1x = 1.0f + sign * mx * mx * a, t1y = sign * b, t1z = -sign * mx;
-    float t2x = b, t2y = sign + my * my * a, t2z = -my;
-    wx = w * mx + r * (co * t1x + s * t2x);
-    wy = w * my + r * (co * t1y + s * t2y);
-    wz = w * mz + r * (co * t1z + s * t2z);
-}
-
 // network inputs in [0,1]: position in the scene box, normal and wo as (theta/pi, phi/2pi)
 __device__ __forceinline__ void features(const float* bmin, const float* bext, float px, float py, float pz,
                                          float nx, float ny, float nz, float ox, float oy, float oz, float* in) {
