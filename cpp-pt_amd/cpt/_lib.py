@@ -44,7 +44,7 @@ ABI_FUNCTIONS = (
     "cpt_trace_closest_device", "cpt_render", "cpt_read_film", "cpt_render_views",
     "cpt_reference_tie_rank", "cpt_guide_num_params", "cpt_guide_shape", "cpt_guide_init", "cpt_guide_get_params",
     "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
-    "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export",
+    "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export", "cpt_debug_fail_alloc",
 )
 
 
@@ -120,6 +120,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_destroy.restype = None
     lib.cpt_last_error.argtypes = [C.c_void_p]
     lib.cpt_last_error.restype = C.c_char_p
+    lib.cpt_debug_fail_alloc.argtypes = [C.c_void_p, C.c_int32]
     lib.cpt_upload_scene.argtypes = [C.c_void_p, C.POINTER(SceneDesc)]
     lib.cpt_bvh_info.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_tie_rank.argtypes = [C.c_void_p, C.c_void_p]
@@ -202,6 +203,10 @@ class Context:
     def _check(self, rc, what):
         if rc != CPT_OK:
             raise CptError(rc, f"{what}: {self.lib.cpt_last_error(self.h).decode()}")
+
+    def debug_fail_alloc(self, after):
+        """Make the device allocation after the next `after` fail (tests; -1: off)."""
+        self._check(self.lib.cpt_debug_fail_alloc(self.h, after), "cpt_debug_fail_alloc")
 
     def upload(self, scene, tie_rank: bool = True, camera=None, gpu_bvh: bool = False):
         """Upload a cpt.scenes.Scene; `camera` (16 floats) overrides its camera;

@@ -1165,6 +1165,9 @@ struct cpt_ctx {
     int cus = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    // fault injection for tests (cpt_debug_fail_alloc): device allocations
+    // left before one fails; -1 = off
+    int fail_alloc_in = -1;
 
     // scene
     bool has_scene = false;
@@ -1219,6 +1222,10 @@ struct cpt_ctx {
     size_t film_px = 0;
     int film_w = 0, film_h = 0;
     unsigned film_spp = 1;  // samples per pixel of the last single-view render
+    // (width, height) of every view in the film of the last completed render;
+    // empty while the film holds nothing a render can add to (just allocated,
+    // or a render failed part way)
+    std::vector<std::pair<int, int>> film_layout;
 
     // learned guide
     bool has_guide = false;
@@ -1365,17 +1372,31 @@ DevScene dev_scene(const cpt_ctx* c) {
     return s;
 }
 
+// Free p and allocate count elements.  On failure p is null: every caller
+// keeps the capacity of a buffer group at 0 until all of the group's buffers
+// are allocated, so a failed allocation is retried by the next call instead of
+// leaving a stale capacity in front of a null or short buffer.
 template <typename T>
 int dalloc(cpt_ctx* c, T*& p, size_t count) {
     dfree(p);
     if (count == 0) count = 1;
-    HIPCHK(c, hipMalloc((void**)&p, count * sizeof(T)));
+    if (c->fail_alloc_in == 0) {
+        c->fail_alloc_in = -1;
+        return fail(c, CPT_E_HIP, "hipMalloc of %zu bytes failed (injected)", count * sizeof(T));
+    }
+    if (c->fail_alloc_in > 0) c->fail_alloc_in--;
+    hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(c, CPT_E_HIP, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+    }
     return CPT_OK;
 }
 
 int ensure_gstack(cpt_ctx* c, size_t threads) {
     if (c->gcap >= threads && c->gstack) return CPT_OK;
     drop_graph(c);
+    c->gcap = 0;
     int r = dalloc(c, c->gstack, threads * (size_t)CPT_GSTACK);
     if (r) return r;
     c->gcap = threads;
@@ -1402,8 +1423,25 @@ int ensure_wavefront(cpt_ctx* c, size_t n) {
     r |= dalloc(c, c->pix, n);
     r |= dalloc(c, c->vid, n);
     r |= dalloc(c, c->done, n);
-    if (r) return CPT_E_HIP;
+    if (r) {
+        free_wavefront(c);
+        return CPT_E_HIP;
+    }
     c->cap = n;
+    return CPT_OK;
+}
+
+// FP64 film sums and sums of squares for px pixels.  A new film holds nothing
+// to add to, so its layout record is cleared.
+int ensure_film(cpt_ctx* c, size_t px) {
+    if (c->film_px >= px && c->film && c->film2) return CPT_OK;
+    drop_graph(c);
+    c->film_px = 0;
+    c->film_layout.clear();
+    int r = dalloc(c, c->film, 3 * px);
+    if (!r) r = dalloc(c, c->film2, 3 * px);
+    if (r) return r;
+    c->film_px = px;
     return CPT_OK;
 }
 
@@ -1704,6 +1742,12 @@ void cpt_destroy(cpt_ctx* c) {
 }
 
 const char* cpt_last_error(cpt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int cpt_debug_fail_alloc(cpt_ctx* c, int32_t after) {
+    if (!c) return CPT_E_INVALID;
+    c->fail_alloc_in = after < 0 ? -1 : after;
+    return CPT_OK;
+}
 
 int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     if (!c || !d) return CPT_E_INVALID;
@@ -2203,13 +2247,16 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     if (r) return r;
     r = ensure_gstack(c, (size_t)G * B);
     if (r) return r;
-    if (c->film_px < film_px || !c->film) {
-        drop_graph(c);
-        r = dalloc(c, c->film, 3 * (size_t)film_px);
-        if (r) return r;
-        r = dalloc(c, c->film2, 3 * (size_t)film_px);
-        if (r) return r;
-        c->film_px = film_px;
+    r = ensure_film(c, film_px);
+    if (r) return r;
+    // adding to the film needs the previous render's film, views and sizes
+    if (flags & CPT_RENDER_ACCUMULATE) {
+        bool same = c->film_layout.size() == vd.size();
+        for (size_t v = 0; same && v < vd.size(); v++)
+            same = c->film_layout[v] == std::make_pair(vd[v].W, vd[v].H);
+        if (!same)
+            return fail(c, CPT_E_INVALID,
+                        "CPT_RENDER_ACCUMULATE needs a film of the same views and sizes from a completed render");
     }
     const bool guided = (flags & CPT_RENDER_GUIDED) != 0;
     const bool train = guided && (flags & CPT_RENDER_GUIDE_TRAIN) != 0;
@@ -2219,6 +2266,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         const size_t lfloats = slots * 6 * (c->g_width == 128 ? 16 : 8);
         if (c->g_lobes_cap < lfloats || !c->g_lobes) {
             drop_graph(c);
+            c->g_lobes_cap = 0;
             r = dalloc(c, c->g_lobes, lfloats);
             if (r) return r;
             c->g_lobes_cap = lfloats;
@@ -2226,8 +2274,9 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         if (train) {
             nrec = std::min<size_t>(slots, (size_t)c->g_cfg.train_paths);
             size_t nr = nrec * (size_t)max_depth;
-            if (c->g_rec_cap < nr || !c->g_rec) {
+            if (c->g_rec_cap < nr || !c->g_rec || !c->g_rec_flag) {
                 drop_graph(c);
+                c->g_rec_cap = 0;
                 r = dalloc(c, c->g_rec, nr * REC_FLOATS);
                 if (!r) r = dalloc(c, c->g_rec_flag, nr);
                 if (r) return r;
@@ -2235,6 +2284,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
             }
             if (c->g_samples_cap < nr || !c->g_samples) {
                 drop_graph(c);
+                c->g_samples_cap = 0;
                 r = dalloc(c, c->g_samples, nr * guide::SAMPLE_FLOATS);
                 if (r) return r;
                 c->g_samples_cap = nr;
@@ -2243,12 +2293,14 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     }
     if (c->views_cap < vd.size() || !c->d_views) {
         drop_graph(c);
+        c->views_cap = 0;
         r = dalloc(c, c->d_views, vd.size());
         if (r) return r;
         c->views_cap = vd.size();
     }
     if (c->tend_cap < passes.size() + 1 || !c->pass_tend) {
         drop_graph(c);
+        c->tend_cap = 0;
         r = dalloc(c, c->pass_tend, passes.size() + 1);
         if (r) return r;
         c->tend_cap = passes.size() + 1;
@@ -2264,6 +2316,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     }
     std::memcpy(c->h_pass, passes.data(), passes.size() * sizeof(PassDesc));
     HIPCHK(c, hipMemcpyAsync(c->d_views, vd.data(), vd.size() * sizeof(ViewDesc), hipMemcpyHostToDevice, c->stream));
+    c->film_layout.clear();  // until this render completes
     if (!(flags & CPT_RENDER_ACCUMULATE)) {
         HIPCHK(c, hipMemsetAsync(c->film, 0, 3 * sizeof(double) * film_px, c->stream));
         HIPCHK(c, hipMemsetAsync(c->film2, 0, 3 * sizeof(double) * film_px, c->stream));
@@ -2273,8 +2326,9 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         r = sd_upload(c);
         if (r) return r;
         const size_t need = slots * (size_t)max_depth * 9;
-        if (c->ppg_cap < need || !c->ppg_vtx) {
+        if (c->ppg_cap < need || !c->ppg_vtx || !c->ppg_nv) {
             drop_graph(c);
+            c->ppg_cap = 0;
             dfree(c->ppg_nv);
             r = dalloc(c, c->ppg_vtx, need);
             if (!r) r = dalloc(c, c->ppg_nv, slots);
@@ -2395,6 +2449,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     c->film_w = vd[0].W;
     c->film_h = vd[0].H;
     c->film_spp = vd[0].spp;
+    for (auto& v : vd) c->film_layout.emplace_back(v.W, v.H);
     return CPT_OK;
 }
 
@@ -2405,6 +2460,8 @@ int cpt_render(cpt_ctx* c, const cpt_render_params* p, cpt_render_stats* stats) 
         return fail(c, CPT_E_INVALID, "bad render size or spp");
     if (p->flags & CPT_RENDER_STREAM) {
         if (p->flags & CPT_RENDER_LIGHT_TRACE) return fail(c, CPT_E_INVALID, "the light tracer does not stream");
+        if (p->flags & CPT_RENDER_ACCUMULATE)
+            return fail(c, CPT_E_INVALID, "the streaming renderer starts from a cleared film (no CPT_RENDER_ACCUMULATE)");
         int32_t whs[3] = {p->width, p->height, p->spp_axis};
         return render_views_stream(c, 1, c->cam, whs, p->max_depth, p->seed, p->flags, nullptr, nullptr, stats);
     }
@@ -2660,14 +2717,9 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
     if (r) return r;
     r = ensure_gstack(c, (size_t)G * B);
     if (r) return r;
-    if (c->film_px < film_px || !c->film) {
-        drop_graph(c);
-        r = dalloc(c, c->film, 3 * (size_t)film_px);
-        if (r) return r;
-        r = dalloc(c, c->film2, 3 * (size_t)film_px);
-        if (r) return r;
-        c->film_px = film_px;
-    }
+    r = ensure_film(c, film_px);
+    if (r) return r;
+    c->film_layout.clear();  // the streaming renderer always starts from a cleared film
     ViewDesc* d_views = nullptr;
     unsigned long long* d_sc = nullptr;  // next_path, t_start, view_done[n], view_tend[n]
     HIPCHK(c, hipMalloc((void**)&d_views, sizeof(ViewDesc) * n_views));
@@ -2815,6 +2867,8 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
     c->film_w = vd[0].W;
     c->film_h = vd[0].H;
     c->film_spp = vd[0].spp;
+    if (rc == CPT_OK)
+        for (auto& v : vd) c->film_layout.emplace_back(v.W, v.H);
     return rc;
 }
 
@@ -2955,7 +3009,8 @@ int guide_forward_impl(cpt_ctx* c, const float* d_in, int n, float* d_out) {
 // One Adam step on device-resident samples (n x 11 floats at c->g_samples).
 template <class N>
 int guide_train_impl(cpt_ctx* c, int n, bool apply) {
-    if (c->g_rows_cap < (size_t)n) {
+    if (c->g_rows_cap < (size_t)n || !c->g_rows) {
+        c->g_rows_cap = 0;
         int r = dalloc(c, c->g_rows, (size_t)n * N::kRow);
         if (r) return r;
         c->g_rows_cap = n;
@@ -2997,17 +3052,28 @@ int cpt_guide_init(cpt_ctx* c, const cpt_guide_params* gp) {
     if (gp->width != 0 && gp->width != 64 && gp->width != 128)
         return fail(c, CPT_E_INVALID, "guide width %d: 64 or 128", (int)gp->width);
     HIPCHK(c, hipSetDevice(c->device));
+    // work of an earlier call (a render that failed part way) may still read the parameters
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     const bool wide = gp->width == 128;
     int r = wide ? guide_alloc<guide128::Net>(c) : guide_alloc<guide::Net>(c);
     if (r) return r;
-    if (!c->g_count) {
+    if (!c->g_count || !c->h_count) {
+        if (c->h_count) (void)hipHostFree(c->h_count);
+        c->h_count = nullptr;
         r |= dalloc(c, c->g_inv8, 8);
         r |= dalloc(c, c->g_loss_part, GUIDE_TRAIN_BLOCKS);
         r |= dalloc(c, c->g_wscale, 1);
         r |= dalloc(c, c->g_wpart, GUIDE_WSUM_BLOCKS);
         r |= dalloc(c, c->g_count, 4);
-        if (r) return CPT_E_HIP;
-        HIPCHK(c, hipHostMalloc((void**)&c->h_count, 4 * sizeof(uint32_t), hipHostMallocDefault));
+        if (r) {
+            dfree(c->g_count);  // retried as a whole by the next call
+            return CPT_E_HIP;
+        }
+        hipError_t e = hipHostMalloc((void**)&c->h_count, 4 * sizeof(uint32_t), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            c->h_count = nullptr;
+            return fail(c, CPT_E_HIP, "hipHostMalloc failed: %s", hipGetErrorString(e));
+        }
     }
     c->g_cfg = *gp;
     c->g_cfg.width = wide ? 128 : 64;
@@ -3090,7 +3156,9 @@ int cpt_guide_train_step(cpt_ctx* c, const float* samples, int32_t n, int32_t ap
     if (!c || n <= 0 || !samples) return CPT_E_INVALID;
     if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
     HIPCHK(c, hipSetDevice(c->device));
-    if (c->g_samples_cap < (size_t)n) {
+    if (c->g_samples_cap < (size_t)n || !c->g_samples) {
+        drop_graph(c);  // guided renders' graphs hold the sample buffer's address
+        c->g_samples_cap = 0;
         int r = dalloc(c, c->g_samples, (size_t)n * SAMPLE_FLOATS);
         if (r) return r;
         c->g_samples_cap = n;

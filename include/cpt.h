@@ -192,6 +192,11 @@ void cpt_destroy(cpt_ctx* ctx);
 const char* cpt_last_error(cpt_ctx* ctx);
 /* ABI version (major*1000 + minor) */
 int cpt_version(void);
+/* Fault injection for tests; no counterpart in the reference.  The context's
+ * next `after` device allocations succeed and the one after them fails as if
+ * hipMalloc had failed (after < 0: off).  Lets a test check that a call after a
+ * failed allocation reallocates instead of using a stale buffer. */
+int cpt_debug_fail_alloc(cpt_ctx* ctx, int32_t after);
 
 /* Upload a scene, build the 4-wide BVH and (unless CPT_SCENE_NO_TIE_RANK) the
  * reference test-order table.  Replaces Loader::load_scene's BVH construction
