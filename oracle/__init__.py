@@ -78,6 +78,21 @@ def lib():
     L.orc_triangle_sample.restype = None
     L.orc_pcg_floats.argtypes = [C.c_uint64, C.c_uint64, i32, vp]
     L.orc_pcg_floats.restype = None
+    L.orc_ggx_lambda.argtypes = [C.c_float, vp]
+    L.orc_ggx_lambda.restype = C.c_float
+    L.orc_ggx_g1.argtypes = [C.c_float, vp]
+    L.orc_ggx_g1.restype = C.c_float
+    L.orc_dielectric_sample.argtypes = [vp, vp, C.c_float, vp]
+    L.orc_dielectric_sample.restype = None
+    L.orc_russian_roulette.argtypes = [vp, i32]
+    L.orc_russian_roulette.restype = C.c_float
+    for f in ("orc_sd_lookup", "orc_sd_pdf", "orc_sd_sample"):
+        getattr(L, f).restype = None
+    L.orc_sd_lookup.argtypes = [vp] * 5 + [i32, vp, vp]
+    L.orc_sd_pdf.argtypes = [vp] * 5 + [i32, vp, vp, vp]
+    L.orc_sd_sample.argtypes = [vp] * 5 + [i32, vp, vp, vp, vp]
+    L.orc_ggx_sample_batch.argtypes = [C.c_float, vp, i32, vp, vp]
+    L.orc_ggx_sample_batch.restype = None
     L.orc_mt_floats.argtypes = [C.c_uint32, i32, vp]
     L.orc_mt_floats.restype = None
     L.orc_mlp_num_params.restype = i32
@@ -255,6 +270,66 @@ def glossy_eval(t, b, n, rough, wo, wi):
 
 def glossy_pdf(t, b, n, rough, wo, wi):
     return lib().orc_glossy_pdf(_p(_v3(t)), _p(_v3(b)), _p(_v3(n)), rough, _p(_v3(wo)), _p(_v3(wi)))
+
+
+def ggx_lambda(alpha, w):
+    return lib().orc_ggx_lambda(alpha, _p(_v3(w)))
+
+
+def ggx_g1(alpha, w):
+    return lib().orc_ggx_g1(alpha, _p(_v3(w)))
+
+
+def dielectric_sample(n, wo, ior):
+    wi = np.zeros(3, np.float32)
+    lib().orc_dielectric_sample(_p(_v3(n)), _p(_v3(wo)), ior, _p(wi))
+    return wi
+
+
+def russian_roulette(c, corrected=False):
+    return lib().orc_russian_roulette(_p(_v3(c)), int(corrected))
+
+
+def ggx_sample_batch(alpha, wo, u):
+    """Visible-normal samples for an (n, 2) array of uniforms."""
+    u = np.ascontiguousarray(u, np.float32)
+    out = np.zeros((u.shape[0], 3), np.float32)
+    lib().orc_ggx_sample_batch(alpha, _p(_v3(wo)), u.shape[0], _p(u), _p(out))
+    return out
+
+
+def _sd_arrays(tree):
+    t = {k: np.ascontiguousarray(tree[k], np.int32 if k in ("snode", "dinfo", "quad") else np.float32)
+         for k in ("snode", "dinfo", "quad", "rect", "bounds")}
+    return t, [_p(t[k]) for k in ("snode", "dinfo", "quad", "rect", "bounds")]
+
+
+def sd_lookup(tree, pts):
+    """Directional tree index of each point (flattened SD-tree as cpt_ppg_export gives it)."""
+    keep, a = _sd_arrays(tree)
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    out = np.zeros(p.shape[0], np.int32)
+    lib().orc_sd_lookup(*a, p.shape[0], _p(p), _p(out))
+    return out
+
+
+def sd_pdf(tree, pts, dirs):
+    keep, a = _sd_arrays(tree)
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros(p.shape[0], np.float32)
+    lib().orc_sd_pdf(*a, p.shape[0], _p(p), _p(d), _p(out))
+    return out
+
+
+def sd_sample(tree, pts, u3):
+    keep, a = _sd_arrays(tree)
+    p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    u = np.ascontiguousarray(u3, np.float32).reshape(-1, 3)
+    dirs = np.zeros((p.shape[0], 3), np.float32)
+    pdf = np.zeros(p.shape[0], np.float32)
+    lib().orc_sd_sample(*a, p.shape[0], _p(p), _p(u), _p(dirs), _p(pdf))
+    return dirs, pdf
 
 
 def ggx_d(alpha, wh):

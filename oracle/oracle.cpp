@@ -1594,6 +1594,72 @@ void orc_mt_floats(uint32_t seed, int n, float* out) {
     for (int i = 0; i < n; i++) out[i] = m.sample();
 }
 
+// microfacet_distribution.h:137-150 (the Lambda with |cos/sin|) in the local frame
+float orc_ggx_lambda(float alpha, const float* w) { return Ggx{alpha, alpha}.lambda(ld(w)); }
+float orc_ggx_g1(float alpha, const float* w) { return Ggx{alpha, alpha}.g1(ld(w)); }
+
+// refraction_bxdf.h:26-52 through the oracle's BSDF dispatch: shading normal n, ior
+void orc_dielectric_sample(const float* n, const float* wo, float ior, float* wi) {
+    Isect it;
+    it.normal = ld(n);
+    it.g_normal = ld(n);
+    Material m{};
+    m.type = 3;
+    m.ior = ior;
+    FixedSampler fs(nullptr);
+    DSample d = bsdf_sample(m, it, ld(wo), fs);
+    wi[0] = d.wi.x; wi[1] = d.wi.y; wi[2] = d.wi.z;
+}
+
+// SD-tree lookups on the flattened layout (sdtree.h:268-396, 400-423, 525-538):
+// per point the directional tree it falls in, per (point, direction) the pdf,
+// and per (point, 3 uniforms) a sampled direction with its pdf
+static SdTreeView sd_view(const int32_t* snode, const int32_t* dinfo, const int32_t* quad, const float* rect,
+                          const float* bounds) {
+    SdTreeView t{snode, dinfo, quad, rect, {bounds[0], bounds[1], bounds[2]}, {bounds[3], bounds[4], bounds[5]}};
+    return t;
+}
+void orc_sd_lookup(const int32_t* snode, const int32_t* dinfo, const int32_t* quad, const float* rect,
+                   const float* bounds, int n, const float* pts, int32_t* dtree_out) {
+    SdTreeView t = sd_view(snode, dinfo, quad, rect, bounds);
+    for (int i = 0; i < n; i++) dtree_out[i] = sd_dtree_of(t, ld(pts + 3 * (size_t)i));
+}
+void orc_sd_pdf(const int32_t* snode, const int32_t* dinfo, const int32_t* quad, const float* rect,
+                const float* bounds, int n, const float* pts, const float* dirs, float* pdf_out) {
+    SdTreeView t = sd_view(snode, dinfo, quad, rect, bounds);
+    for (int i = 0; i < n; i++) pdf_out[i] = sd_pdf_of(t, ld(pts + 3 * (size_t)i), ld(dirs + 3 * (size_t)i));
+}
+void orc_sd_sample(const int32_t* snode, const int32_t* dinfo, const int32_t* quad, const float* rect,
+                   const float* bounds, int n, const float* pts, const float* u3, float* dirs_out, float* pdf_out) {
+    SdTreeView t = sd_view(snode, dinfo, quad, rect, bounds);
+    for (int i = 0; i < n; i++) {
+        FixedSampler fs(u3 + 3 * (size_t)i);
+        V3 d = sd_sample_of(t, ld(pts + 3 * (size_t)i), fs, pdf_out + i);
+        dirs_out[3 * (size_t)i] = d.x;
+        dirs_out[3 * (size_t)i + 1] = d.y;
+        dirs_out[3 * (size_t)i + 2] = d.z;
+    }
+}
+
+// microfacet_distribution.h:96-118 for n (u1, u2) pairs
+void orc_ggx_sample_batch(float alpha, const float* wo, int n, const float* u2, float* wh) {
+    Ggx g{alpha, alpha};
+    V3 w = ld(wo);
+    for (int i = 0; i < n; i++) {
+        V3 h = g.sample(w, u2[2 * (size_t)i], u2[2 * (size_t)i + 1]);
+        wh[3 * (size_t)i] = h.x;
+        wh[3 * (size_t)i + 1] = h.y;
+        wh[3 * (size_t)i + 2] = h.z;
+    }
+}
+
+// pathtracerMIS.h:24-27 (corrected: clamped at 1)
+float orc_russian_roulette(const float* c, int corrected) {
+    Scene s;
+    s.corrected = corrected != 0;
+    return rr_prob(s, ld(c));
+}
+
 }  // extern "C"
 
 // ============================================================================

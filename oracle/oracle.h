@@ -127,6 +127,23 @@ void orc_triangle_sample(void* s, int tri, float u1, float u2, float* pos, float
 void orc_pcg_floats(uint64_t seed, uint64_t path_id, int n, float* out);
 /* std::mt19937(seed) + uniform_real_distribution<float>(0,1): first n floats */
 void orc_mt_floats(uint32_t seed, int n, float* out);
+/* GGX Lambda and G1 in the local frame (microfacet_distribution.h:137-150) */
+float orc_ggx_lambda(float alpha, const float* w);
+float orc_ggx_g1(float alpha, const float* w);
+/* RefractionBxDF::sample (refraction_bxdf.h:26-52): shading normal n, outgoing wo -> wi */
+void orc_dielectric_sample(const float* n, const float* wo, float ior, float* wi);
+/* SD-tree lookups on the flattened layout of cpt_ppg_export (sdtree.h:268-396 get_leaf,
+ * 400-423 sample_direction, 525-538 pdf): pts, dirs n x 3; u3 n x 3 uniforms per sample */
+void orc_sd_lookup(const int32_t* snode, const int32_t* dinfo, const int32_t* quad, const float* rect,
+                   const float* bounds, int n, const float* pts, int32_t* dtree_out);
+void orc_sd_pdf(const int32_t* snode, const int32_t* dinfo, const int32_t* quad, const float* rect,
+                const float* bounds, int n, const float* pts, const float* dirs, float* pdf_out);
+void orc_sd_sample(const int32_t* snode, const int32_t* dinfo, const int32_t* quad, const float* rect,
+                   const float* bounds, int n, const float* pts, const float* u3, float* dirs_out, float* pdf_out);
+/* GGX visible-normal samples for n (u1, u2) pairs (microfacet_distribution.h:96-118) */
+void orc_ggx_sample_batch(float alpha, const float* wo, int n, const float* u2, float* wh);
+/* survival probability of Russian roulette (pathtracerMIS.h:24-27); corrected = clamped at 1 */
+float orc_russian_roulette(const float* c, int corrected);
 
 #ifdef __cplusplus
 }

@@ -170,3 +170,45 @@ class SDTree:
                 walk(n[2])
         walk(self.root)
         return out
+
+
+def flatten(sd):
+    """The flattened layout of cpt.Context.ppg_export (without splat sums) for an
+    SDTree of this module: snode (>= 0: children v, v+1; < 0: dtree -1-v), dinfo
+    (quad base, leaf base, leaf count, 0 per dtree), quad (>= 0: first of four
+    children, child i = x half i % 2, y half i // 2; < 0: accumulator -1-v), rect (the
+    sampling leaves), bounds (lo, hi)."""
+    snode, dinfo, quad, rect = [0], [], [], []
+    slots = [0]
+
+    def qfill(idx, n):
+        if len(n) == 1:
+            quad[idx] = -1 - slots[0]
+            slots[0] += 1
+            return
+        base = len(quad)
+        quad.extend([0] * 4)
+        quad[idx] = base
+        for i in range(4):
+            qfill(base + i, n[1 + i])
+
+    def sfill(idx, n):
+        if n[0] == "L":
+            d = n[1]
+            snode[idx] = -1 - len(dinfo)
+            qb = len(quad)
+            quad.append(0)
+            qfill(qb, d.root)
+            dinfo.append((qb, len(rect), len(d.leaves), 0))
+            rect.extend(d.leaves)
+            return
+        base = len(snode)
+        snode.extend([0, 0])
+        snode[idx] = base
+        sfill(base, n[1])
+        sfill(base + 1, n[2])
+
+    sfill(0, sd.root)
+    return {"snode": np.array(snode, np.int32), "dinfo": np.array(dinfo, np.int32).reshape(-1, 4),
+            "quad": np.array(quad, np.int32), "rect": np.array(rect, np.float32).reshape(-1, 4),
+            "bounds": np.array(list(sd.lo) + list(sd.hi), np.float32)}
