@@ -7,12 +7,18 @@ resident in HBM before timing starts; cpt_render returns after the device has
 finished (it synchronises its stream).
 
 Prints one JSON line.  `roofline` describes the dominant kernel (closest-hit
-traversal, `extend`): algorithmic bytes per extension ray = 32 (ray) + 16 (hit)
-+ 128 x BVH4 nodes visited + 48 x triangles tested, counted by a counting run of
-the same kernel on the same rays, divided by that kernel's summed launch time
-measured with HIP events on the stream it runs on.  `cpu_baseline` renders a
-row sample of the same workload with the CPU oracle (a restatement of the
-reference's PathtracerMIS with its mt19937 column sampler and binary BVH).
+traversal, `k_extend`): achieved = HBM bytes per launch (two rocprofv3 counter
+passes, FETCH_SIZE and WRITE_SIZE, over a child process rendering the same frame
+with the same launch configuration and seed, scaled by device copies of known
+size) / that kernel's average launch time (HIP events on the stream it runs on).
+The bytes the traversal asks of the whole memory hierarchy (32 B ray + 16 B hit +
+128 B per BVH4 node visit + 48 B per triangle test, counted by a counting run of
+the same kernel on the same rays) are reported beside it as `cache_level`.
+`cpu_baseline` renders whole frames of the same workload with the CPU oracle (a
+restatement of the reference's PathtracerMIS with its mt19937 column sampler and
+binary BVH) over the host's cores.  `guided` is the equal-time comparison of the
+learned guide against plain PT on the hard-lighting interior (1280x720, budgets
+10/30/60 s, 65,536-spp FP64 reference).
 """
 import argparse
 import json
@@ -43,7 +49,7 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--grid-blocks", type=int, default=0)
     ap.add_argument("--max-paths", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample duration")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU baseline duration (whole frames)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip cpt_autotune (default launch configuration)")
@@ -68,6 +74,15 @@ def parse():
                          "32 emitters, mixed GGX; device BVH build)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scales HBM counters)")
+    ap.add_argument("--shade-block", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="terrain: HBM traffic from the committed counter measurement instead of two rocprofv3 passes")
+    ap.add_argument("--no-guided", action="store_true", help="terrain: leave out the equal-time guided comparison")
+    ap.add_argument("--guided-res", type=str, default="1280x720", help="terrain line: resolution of the guided comparison")
+    ap.add_argument("--guided-ref-spp-axis", type=int, default=256,
+                    help="terrain line: reference spp per axis of the guided comparison (256 -> 65,536 spp)")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -85,44 +100,43 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_mib": mib, "checksum": float(y[:16].sum())}))
 
 
-def bench_guided(args):
+def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=0, guide_depth=0, scene_kind="interior",
+                      boxes=40000, ref_spp_axis=256, ppg=False, log=None):
     """Equal-time comparison on the hard-lighting interior (one 0.2x0.2 light outside
     a cluttered room, light entering through a 0.3-wide door gap): plain PT-NEE-MIS
     vs guided PT (learned MLP guide trained online, training time included), both
     accumulated one sample per pixel at a time; relMSE = mean((I-R)^2/(R^2+0.01))
-    against an FP64-accumulated plain reference."""
+    against an FP64-accumulated plain reference of ref_spp_axis^2 samples per pixel.
+    Returns the result record (the `guided` object of the bench line)."""
     import numpy as np
 
     import cpt
 
-    W, H = args.width, args.height
-    budgets = [float(b) for b in args.budgets.split(",")]
+    log = log or (lambda m: print(m, file=sys.stderr, flush=True))
     ctx = cpt.Context(0)
-    if args.scene == "grid":
+    t0 = time.perf_counter()
+    if scene_kind == "grid":
         scene = cpt.scenes.interior_grid_scene(res=(W, H))
-        t0 = time.perf_counter()
         # the device build keeps set-up short at 7.7M triangles; the tie-rank table only
         # decides exact-t ties between coincident triangles, which this scene has none of
         ctx.upload(scene, gpu_bvh=True, tie_rank=False)
-        print(f"bench: grid scene {scene.n_triangles} triangles uploaded in {time.perf_counter() - t0:.1f} s "
-              f"(BVH {ctx.bvh_info()['build_ms']:.0f} ms on the device)", file=sys.stderr, flush=True)
     else:
-        scene = cpt.scenes.interior_scene(res=(W, H), n_boxes=args.boxes)
+        scene = cpt.scenes.interior_scene(res=(W, H), n_boxes=boxes)
         ctx.upload(scene)
-    depth = args.depth
+    log(f"bench: guided scene {scene.n_triangles} triangles ready in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     # reference in chunks of at most 64x64 spp (progress lines; distinct seeds per chunk)
-    chunk = min(args.ref_spp_axis, 64 if W * H <= (1 << 21) else 16)
-    n_chunks = max(1, (args.ref_spp_axis // chunk) ** 2)
+    chunk = min(ref_spp_axis, 64 if W * H <= (1 << 21) else 16)
+    n_chunks = max(1, (ref_spp_axis // chunk) ** 2)
     for k in range(n_chunks):
         ctx.render(W, H, chunk, max_depth=depth, seed=99991 + 7919 * k,
                    flags=cpt.CPT_RENDER_ACCUMULATE if k else 0)
-        print(f"bench: reference chunk {k + 1}/{n_chunks}", file=sys.stderr, flush=True)
+        log(f"bench: reference chunk {k + 1}/{n_chunks}")
     _, ref_sums = ctx.read_film(sums=True)
     ref_spp = n_chunks * chunk * chunk
     ref = ref_sums / float(ref_spp)
     ref_s = time.perf_counter() - t0
-    print(f"bench: reference {ref_spp} spp in {ref_s:.1f} s", file=sys.stderr, flush=True)
+    log(f"bench: reference {ref_spp} spp in {ref_s:.1f} s")
 
     def relmse(img):
         return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
@@ -133,10 +147,12 @@ def bench_guided(args):
         e = np.sort(((img - ref) ** 2 / (ref ** 2 + 1e-2)).mean(axis=2).ravel())
         return float(e[:int(0.99 * e.size)].mean())
 
+    tp = train_paths or min(W * H, 1 << 18)
+
     def run(guided):
         if guided:
-            ctx.guide_init(seed=11, lr=3e-3, train_paths=args.train_paths or min(W * H, 1 << 18), train_steps=1,
-                           guide_depth=args.guide_depth, infer_fp8=args.fp8, width=args.guide_width)
+            ctx.guide_init(seed=11, lr=3e-3, train_paths=tp, train_steps=1, guide_depth=guide_depth, infer_fp8=fp8,
+                           width=width)
         flags = (cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN) if guided else 0
         curve = []
         elapsed, spp, k = 0.0, 0, 0
@@ -154,19 +170,18 @@ def bench_guided(args):
                 _, sums = ctx.read_film(sums=True)
                 curve.append((elapsed, spp, relmse(sums / spp), relmse_trim(sums / spp)))
                 next_check = elapsed * 1.25
-                print(f"bench: {'guided' if guided else 'plain'} t={elapsed:.2f}s spp={spp} relMSE={curve[-1][2]:.4f} "
-                      f"trimmed={curve[-1][3]:.5f}" +
-                      (f" training samples/step {gsamples / max(gsteps, 1):.0f}" if guided else ""),
-                      file=sys.stderr, flush=True)
-        return curve
+                log(f"bench: {'guided' if guided else 'plain'} t={elapsed:.2f}s spp={spp} relMSE={curve[-1][2]:.4f} "
+                    f"trimmed={curve[-1][3]:.5f}" +
+                    (f" training samples/step {gsamples / max(gsteps, 1):.0f}" if guided else ""))
+        return curve, gsamples / max(gsteps, 1), gsteps
 
-    plain = run(False)
-    guided = run(True)
+    plain, _, _ = run(False)
+    guided, samples_per_step, adam_steps = run(True)
 
     # SD-tree arm: the reference's PPG schedule (iterations of 1, 2, 4, ... spp, the
     # last one kept; no NEE), sized to each budget from a short calibration run
-    ppg = {}
-    if not args.no_ppg:
+    ppg_res = {}
+    if ppg:
         t = time.perf_counter()
         ctx.render(W, H, 8, max_depth=depth, seed=7, flags=cpt.CPT_RENDER_PPG)
         per_spp = (time.perf_counter() - t) / 63.0
@@ -180,9 +195,8 @@ def bench_guided(args):
             while last * 2 < sa * sa:
                 last *= 2
             _, sums = ctx.read_film(sums=True)
-            ppg[T] = (el, last, relmse(sums / last), relmse_trim(sums / last))
-            print(f"bench: ppg budget {T:.0f}s: {el:.1f}s, last iteration {last} spp, relMSE={ppg[T][2]:.4f}",
-                  file=sys.stderr, flush=True)
+            ppg_res[T] = (el, last, relmse(sums / last), relmse_trim(sums / last))
+            log(f"bench: ppg budget {T:.0f}s: {el:.1f}s, last iteration {last} spp, relMSE={ppg_res[T][2]:.4f}")
 
     # per-kernel time of a 1-spp frame from HIP events (eager launches), with the
     # guide as trained by the guided run
@@ -195,7 +209,7 @@ def bench_guided(args):
         return {k: round(v, 3) for k, v in acc.items()}
 
     kernel_ms = {"plain": frame_ms(0), "guided": frame_ms(cpt.CPT_RENDER_GUIDED)}
-    print(f"bench: per-frame kernel ms {kernel_ms}", file=sys.stderr, flush=True)
+    log(f"bench: per-frame kernel ms {kernel_ms}")
 
     def at(curve, T):
         best = curve[0]
@@ -215,26 +229,48 @@ def bench_guided(args):
                      "time_ratio": (reach / p_[0]) if reach else None,
                      "plain_relmse_trimmed": p_[3], "guided_relmse_trimmed": g_[3],
                      "time_ratio_trimmed": (reach_t / p_[0]) if reach_t else None})
-        if T in ppg:
-            rows[-1].update({"ppg_seconds": ppg[T][0], "ppg_last_spp": ppg[T][1], "ppg_relmse": ppg[T][2],
-                             "ppg_relmse_trimmed": ppg[T][3]})
-        print(f"bench: T={T:.0f}s plain {p_[2]:.4f} ({p_[1]} spp) guided {g_[2]:.4f} ({g_[1]} spp) "
-              f"guided reaches plain's error at {reach}", file=sys.stderr, flush=True)
-    out = {
+        if T in ppg_res:
+            rows[-1].update({"ppg_seconds": ppg_res[T][0], "ppg_last_spp": ppg_res[T][1], "ppg_relmse": ppg_res[T][2],
+                             "ppg_relmse_trimmed": ppg_res[T][3]})
+        log(f"bench: T={T:.0f}s plain {p_[2]:.4f} ({p_[1]} spp) guided {g_[2]:.4f} ({g_[1]} spp) "
+            f"guided reaches plain's error at {reach}")
+    ctx.close()
+    return {
         "metric": "equal-time relMSE guided vs plain PT (1 GPU)",
-        "value": rows[-1]["plain_relmse"] / max(rows[-1]["guided_relmse"], 1e-30),
-        "unit": "relMSE ratio plain/guided at the largest budget",
-        "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "f32 (guide: " + ("fp8 e4m3 MFMA inference, bf16 training" if args.fp8 else "bf16 MFMA") + ", fp32 acc)",
-        "guide": {"width": args.guide_width, "lobes": 8 if args.guide_width == 64 else 16, "infer_fp8": bool(args.fp8),
-                  "train_paths": args.train_paths or min(W * H, 1 << 18)},
-        "config": {"workload": f"{'4x4 grid of hard-lighting interiors, 32 emitters, mixed GGX' if args.scene == 'grid' else 'hard-lighting interior'}, "
-                               f"{scene.n_triangles} triangles, {W}x{H}, max depth {depth}, "
-                               f"reference {ref_spp} spp FP64", "budgets_s": budgets},
+        "workload": f"{'4x4 grid of hard-lighting interiors, 32 emitters, mixed GGX' if scene_kind == 'grid' else 'hard-lighting interior'}, "
+                    f"{scene.n_triangles} triangles, {W}x{H}, max depth {depth}, reference {ref_spp} spp FP64",
+        "budgets_s": budgets,
+        "time_ratio_at_largest_budget": rows[-1]["time_ratio"],
+        "relmse_ratio_plain_over_guided": rows[-1]["plain_relmse"] / max(rows[-1]["guided_relmse"], 1e-30),
+        "guide": {"width": width, "lobes": 8 if width == 64 else 16, "inference": "fp8 e4m3 MFMA" if fp8 else "bf16 MFMA",
+                  "training": "bf16 MFMA, fp32 accumulation and Adam", "train_paths": tp,
+                  "training_samples_per_step": samples_per_step, "adam_steps": adam_steps},
         "budgets": rows,
         "frame_kernel_ms": kernel_ms,
         "reference_seconds": ref_s,
     }
-    ctx.close()
+
+
+def bench_guided(args):
+    """`--workload guided`: the equal-time comparison alone, as its own JSON line."""
+    W, H = args.width, args.height
+    budgets = [float(b) for b in args.budgets.split(",")]
+    g = guided_equal_time(W, H, budgets, depth=args.depth, fp8=args.fp8, width=args.guide_width,
+                          train_paths=args.train_paths, guide_depth=args.guide_depth, scene_kind=args.scene,
+                          boxes=args.boxes, ref_spp_axis=args.ref_spp_axis, ppg=not args.no_ppg)
+    rows = g["budgets"]
+    out = {
+        "metric": g["metric"],
+        "value": g["relmse_ratio_plain_over_guided"],
+        "unit": "relMSE ratio plain/guided at the largest budget",
+        "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+        "dtype": "f32 (guide: " + ("fp8 e4m3 MFMA inference, bf16 training" if args.fp8 else "bf16 MFMA") + ", fp32 acc)",
+        "guide": g["guide"],
+        "config": {"workload": g["workload"], "budgets_s": budgets},
+        "budgets": rows,
+        "frame_kernel_ms": g["frame_kernel_ms"],
+        "reference_seconds": g["reference_seconds"],
+    }
     print(json.dumps(out))
 
 
@@ -359,40 +395,189 @@ def bench_box(args):
         "mrays_per_s": res[8]["mrays_per_s"], "depth32": res[32],
     }
     if not args.no_cpu:
-        cb = cpu_baseline(scene, W, H, 8, args.cpu_seconds)
+        cb = cpu_baseline(scene, W, H, 8, args.cpu_seconds, S)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = out["value"] / cb["value"]
     print(json.dumps(out))
 
 
-def cpu_baseline(scene, W, H, depth, target_s):
-    import numpy as np
+def host_info():
+    """The host's cores as the CPU baseline sees them: the machine's logical CPUs,
+    the ones this process may run on, the OpenMP thread setting, and lscpu's model,
+    sockets and threads per core."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        import subprocess
 
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Model name": "cpu_model", "Socket(s)": "sockets", "Thread(s) per core": "threads_per_core",
+                "Core(s) per socket": "cores_per_socket"}
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except Exception as e:  # lscpu missing: keep the counts
+        info["lscpu_error"] = str(e)
+    return info
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU this process may use, capped by the
+    host's OpenMP thread setting when it has one."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return n
+
+
+def cpu_baseline(scene, W, H, depth, target_s, full_spp_axis):
+    """The oracle's restatement of the reference (PathtracerMIS with its mt19937
+    column sampler and binary BVH) rendering WHOLE frames of the same workload over
+    the host's cores, OpenMP over columns as the reference does.  A 1-spp frame
+    measures the rate; a second frame with spp_axis chosen for about target_s of
+    work follows when the first was short.  samples/s = all paths / all seconds."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
+    t0 = time.perf_counter()
     o = oracle.OracleScene(scene)
-    # spread single rows over the frame; grow the sample until it takes ~target_s
-    rows_done, paths, rays, elapsed = 0, 0, 0, 0.0
-    order = np.random.default_rng(0).permutation(H)
-    while elapsed < target_s and rows_done < 4 * H:
-        r = int(order[rows_done % H])
+    build_s = time.perf_counter() - t0
+    frames, paths, rays, elapsed = [], 0, 0, 0.0
+    sa = 1
+    while True:
         t0 = time.perf_counter()
-        _, _, counts = o.render(W, H, 1, max_depth=depth, sampler="mt", seed=0, rows=(r, r + 1), threads=threads)
-        elapsed += time.perf_counter() - t0
+        _, _, counts = o.render(W, H, sa, max_depth=depth, sampler="mt", seed=0, threads=threads)
+        dt = time.perf_counter() - t0
+        elapsed += dt
         paths += int(counts[0])
         rays += int(counts[1] + counts[2])
-        rows_done += 1
+        frames.append(f"{W}x{H} x {sa * sa} spp in {dt:.2f} s")
+        if elapsed >= 0.5 * target_s or sa >= full_spp_axis or len(frames) >= 2:
+            break
+        sa = max(1, min(full_spp_axis, int(math.sqrt((target_s - elapsed) / max(dt, 1e-3)))))
     o.close()
     return {
         "value": paths / elapsed,
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{rows_done} random rows (cycled) x {W} px x 1 spp of the same {W}x{H} scene, depth {depth}, "
-                  f"mt19937 column sampler, reference binary BVH; {elapsed:.1f} s",
+        "sample": f"whole frames of the same workload ({'; '.join(frames)}), depth {depth}, the oracle's "
+                  f"restatement of PathtracerMIS (mt19937 column sampler, reference binary BVH), OpenMP dynamic "
+                  f"schedule over columns; scene set-up ({build_s:.1f} s) not timed",
         "mrays_per_s": rays / elapsed / 1e6,
+        "seconds": elapsed,
+        "host": host_info(),
     }
+
+
+def traffic_probe(args):
+    """Child mode run under `rocprofv3 --pmc`: one frame of the bench's workload with
+    the given launch configuration and seed 1 (the rays of the parent's timing run),
+    then three 512 MiB device-to-device copies (a transfer of known size that scales
+    the counters)."""
+    import ctypes
+
+    import cpt
+
+    W, H, S, depth = args.width, args.height, args.spp_axis, args.depth
+    gx, gz = (int(v) for v in args.grid.split("x"))
+    scene = cpt.scenes.terrain_scene(res=(W, H), n_spheres=args.spheres, grid=(gx, gz))
+    ctx = cpt.Context(0)
+    ctx.upload(scene)
+    ctx.render(W, H, S, seed=1, max_depth=depth, block_size=args.block, grid_blocks=args.grid_blocks,
+               shade_block=args.shade_block, refill=args.refill, flags=cpt.CPT_RENDER_NO_GRAPH)
+    ctx.close()
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    n = 512 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0 and hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    for _ in range(3):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("traffic probe done", flush=True)
+
+
+# Read bytes from the L2's read requests to the fabric by size (gfx950 counts 32-,
+# 64- and 128-byte requests separately; its FETCH_SIZE formula takes 128-byte
+# requests from TCC_BUBBLE, which reads half of a plain copy's bytes here), write
+# bytes from WRITE_SIZE (32- and 64-byte write requests).
+READ_COUNTERS = {"TCC_EA0_RDREQ_32B_sum": 32, "TCC_EA0_RDREQ_64B_sum": 64, "TCC_EA0_RDREQ_128B_sum": 128}
+WRITE_COUNTERS = {"WRITE_SIZE": 1024}
+
+
+def measure_traffic(args, launch, log):
+    """Bytes per k_extend launch between the L2 and the memory fabric (the Infinity
+    Cache and HBM behind it) from two rocprofv3 counter passes over a child process
+    (traffic_probe): the read requests by size, then WRITE_SIZE.  The child's three
+    512 MiB copies check the scale: each pass must read (write) 1.5 GiB for them;
+    the measured ratio is applied.  Returns a dict, or None with the reason logged
+    (then the committed measurement is used)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    if not shutil.which("rocprofv3"):
+        log("bench: rocprofv3 not found; HBM traffic from the committed measurement")
+        return None
+    child = [sys.executable, os.path.abspath(__file__), "--traffic-probe", "--width", str(args.width),
+             "--height", str(args.height), "--spp-axis", str(args.spp_axis), "--depth", str(args.depth),
+             "--spheres", str(args.spheres), "--grid", args.grid, "--block", str(launch["block_size"]),
+             "--grid-blocks", str(launch["grid_blocks"]), "--shade-block", str(launch["shade_block"]),
+             "--refill", str(launch["refill"])]
+    res = {}
+    copy_bytes = 3 * (512 << 20)
+    env = dict(os.environ, TMPDIR="/tmp")
+    for kind, counters in (("read", READ_COUNTERS), ("write", WRITE_COUNTERS)):
+        d = tempfile.mkdtemp(prefix="cpt_pmc_", dir="/tmp")
+        cmd = ["rocprofv3", "--pmc"] + list(counters) + ["-d", d, "-o", "probe", "--output-format", "csv", "--"] + child
+        t0 = time.perf_counter()
+        try:
+            p = subprocess.run(cmd, capture_output=True, text=True, timeout=240, env=env, cwd="/tmp")
+        except subprocess.TimeoutExpired:
+            log(f"bench: rocprofv3 --pmc ({kind}) timed out; HBM traffic from the committed measurement")
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        ext, cpy = {}, {}
+        for f in files:
+            with open(f, newline="") as fh:
+                for row in csv.DictReader(fh):
+                    name, cn = row.get("Kernel_Name", ""), row.get("Counter_Name", "")
+                    if cn not in counters:
+                        continue
+                    b = float(row.get("Counter_Value", "nan")) * counters[cn]
+                    did = row.get("Dispatch_Id", row.get("Correlation_Id", ""))
+                    if "k_extend<" in name:
+                        ext[did] = ext.get(did, 0.0) + b
+                    elif "copy" in name.lower():
+                        cpy[did] = cpy.get(did, 0.0) + b
+        shutil.rmtree(d, ignore_errors=True)
+        log(f"bench: rocprofv3 --pmc {' '.join(counters)}: rc {p.returncode}, {len(ext)} k_extend and "
+            f"{len(cpy)} copy dispatches in {time.perf_counter() - t0:.1f} s")
+        if not ext or not cpy or "traffic probe done" not in p.stdout:
+            log(f"bench: no usable {kind} counts (stderr tail: {p.stderr[-400:]!r}); "
+                "HBM traffic from the committed measurement")
+            return None
+        per = sum(ext.values()) / len(ext)
+        scale = copy_bytes / sum(cpy.values())
+        res[kind] = {"counters": list(counters), "bytes_per_launch": per, "launches": len(ext),
+                     "copy_bytes_counted": sum(cpy.values()), "copy_bytes": copy_bytes, "scale": scale}
+    hbm = res["read"]["bytes_per_launch"] * res["read"]["scale"] + res["write"]["bytes_per_launch"] * res["write"]["scale"]
+    return {"hbm_bytes_per_launch": hbm, "counters": res,
+            "source": "rocprofv3 --pmc over a child process of this run (same scene, launch configuration and "
+                      "seed): TCC_EA0_RDREQ_{32B,64B,128B} x size and WRITE_SIZE per k_extend dispatch, each "
+                      "pass scaled by its count of 3 x 512 MiB device copies"}
 
 
 def main():
@@ -403,6 +588,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.traffic_probe:
+        traffic_probe(args)
         return
     if args.workload == "views":
         bench_views(args)
@@ -474,39 +662,65 @@ def main():
         "shadow_mrays_per_s": shadow / total / 1e6,
         "setup_s": setup_s,
     }
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if not args.no_roofline:
-        # same seed -> same rays: timing run (HIP events around every launch) and counting run
+        # same seed -> same rays: timing run (HIP events around every launch), counting run,
+        # and the counter passes of measure_traffic
         tm = ctx.render(W, H, S, seed=1, flags=cpt.CPT_RENDER_KERNEL_TIMING, **kw)
         ct = ctx.render(W, H, S, seed=1, flags=cpt.CPT_RENDER_COUNT, **kw)
-        alg_bytes = 48 * ct["extension_rays"] + 128 * ct["node_visits"] + 48 * ct["triangle_tests"]
-        ext_s = tm["ms_extend"] / 1e3
-        achieved = alg_bytes / ext_s / 1e9
-        traffic = None
-        tp = os.path.join(ROOT, "profiles", "extend_traffic.json")
-        if os.path.exists(tp):
-            traffic = json.load(open(tp)).get("hbm_bytes_per_launch")
+        launches = max(1, tm["n_extend"])
+        launch_s = tm["ms_extend"] / 1e3 / launches
+        # bytes the traversal asks of the memory system (every node and triangle fetch
+        # counted, whether L2, Infinity Cache or HBM serves it)
+        cache_bytes = (48 * ct["extension_rays"] + 128 * ct["node_visits"] + 48 * ct["triangle_tests"]) / launches
+        launch = dict(tune) if tune else {"block_size": args.block, "grid_blocks": args.grid_blocks,
+                                          "shade_block": args.shade_block, "refill": args.refill}
+        if args.block:
+            launch["block_size"] = args.block
+        traffic = None if args.no_traffic else measure_traffic(args, launch, log)
+        if traffic is None:
+            tp = os.path.join(ROOT, "profiles", "extend_traffic.json")
+            with open(tp) as fh:
+                rec = json.load(fh)
+            traffic = {"hbm_bytes_per_launch": rec["hbm_bytes_per_launch"],
+                       "source": f"profiles/extend_traffic.json ({rec.get('measured', 'committed measurement')})"}
+        hbm = traffic["hbm_bytes_per_launch"]
+        achieved = hbm / launch_s / 1e9
         out["roofline"] = {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "traffic": hbm,
+            "traffic_source": traffic["source"],
             "kernel": "k_extend (closest-hit BVH4 traversal)",
-            "launches": tm["n_extend"],
-            "avg_launch_us": tm["ms_extend"] / max(1, tm["n_extend"]) * 1e3,
-            "bytes_per_launch": alg_bytes / max(1, tm["n_extend"]),
+            "launches": launches,
+            "avg_launch_us": launch_s * 1e6,
+            "rays_per_launch": ct["extension_rays"] / launches,
             "nodes_per_ray": ct["node_visits"] / max(1, ct["extension_rays"]),
             "tris_per_ray": ct["triangle_tests"] / max(1, ct["extension_rays"]),
-            "ms_shade": tm["ms_shade"],
-            "ms_shadow": tm["ms_shadow"],
+            "cache_level": {"bytes_per_launch": cache_bytes, "gbs": cache_bytes / launch_s / 1e9,
+                            "note": "48 B per ray + 128 B per node visit + 48 B per triangle test, served by "
+                                    "L2/Infinity Cache/HBM together; not an HBM figure"},
             "ms_extend": tm["ms_extend"],
+            "ms_shadow": tm["ms_shadow"],
+            "ms_shade": tm["ms_shade"],
+            "ms_other": tm["ms_other"],
         }
+        if "counters" in traffic:
+            out["roofline"]["counters"] = traffic["counters"]
+    ctx.close()
     if not args.no_cpu:
-        cb = cpu_baseline(scene, W, H, depth, args.cpu_seconds)
+        cb = cpu_baseline(scene, W, H, depth, args.cpu_seconds, S)
         out["cpu_baseline"] = cb
         out["gpu_over_cpu"] = samples_per_s / cb["value"]
-    ctx.close()
+        log(f"bench: cpu baseline {cb['value']:.4g} samples/s on {cb['cores']} threads ({cb['sample']})")
+    del scene
+    if not args.no_guided:
+        gw, gh = (int(v) for v in args.guided_res.split("x"))
+        out["guided"] = guided_equal_time(gw, gh, [10.0, 30.0, 60.0], depth=12, fp8=False, width=64,
+                                          ref_spp_axis=args.guided_ref_spp_axis, log=log)
     print(json.dumps(out))
 
 
