@@ -304,6 +304,10 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
             if (!has && r < endr) {
                 load(r, o, d, best);
                 best_rank = INT_MIN;
+                // a miss reports zeros, not the lane's previous ray's hit
+                h.t = 0.0f;
+                h.beta = 0.0f;
+                h.gamma = 0.0f;
                 h.tri = -1;
                 sp = 0;
                 node = 0;  // root
