@@ -45,6 +45,7 @@ ABI_FUNCTIONS = (
     "cpt_reference_tie_rank", "cpt_guide_num_params", "cpt_guide_shape", "cpt_guide_init", "cpt_guide_get_params",
     "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
     "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export", "cpt_debug_fail_alloc",
+    "cpt_trace_closest_device_noalloc",
 )
 
 
@@ -127,6 +128,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     lib.cpt_trace_any.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     lib.cpt_trace_closest_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    lib.cpt_trace_closest_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     lib.cpt_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.POINTER(RenderStats)]
     lib.cpt_autotune.argtypes = [C.c_void_p, C.POINTER(RenderParams), C.POINTER(TuneResult)]
     lib.cpt_ppg_reset.argtypes = [C.c_void_p]

@@ -23,7 +23,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1156,6 +1158,16 @@ __global__ __launch_bounds__(B) void k_trace_any(DevScene sc, const float4* rays
         [&](unsigned i, const Hit& h) { occ[i] = h.tri >= 0 ? 1 : 0; }, nv, nt);
 }
 
+// Traversal overflow stacks of cpt_trace_closest_device_noalloc, which may not
+// allocate: one file-scope array per GPU, sized for its fixed grid of
+// NOALLOC_BLOCKS x 128 threads x CPT_GSTACK entries (56 MiB), held until the
+// process ends.  Host code never uses this name as an address; it uses the
+// per-device address from hipGetSymbolAddress (noalloc_scratch).
+constexpr int NOALLOC_BLOCKS = 1024;
+constexpr int NOALLOC_B = 128;
+constexpr size_t NOALLOC_SCRATCH_INTS = (size_t)NOALLOC_BLOCKS * NOALLOC_B * CPT_GSTACK;
+static __device__ __align__(256) int scratch[NOALLOC_SCRATCH_INTS];
+
 }  // namespace
 
 // ====================================================================== host side
@@ -1165,6 +1177,8 @@ struct cpt_ctx {
     int cus = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    // device address of this GPU's `scratch` array (cpt_trace_closest_device_noalloc)
+    int* noalloc_scratch = nullptr;
     // fault injection for tests (cpt_debug_fail_alloc): device allocations
     // left before one fails; -1 = off
     int fail_alloc_in = -1;
@@ -1664,6 +1678,20 @@ int check_device(cpt_ctx* c) {
 
 }  // namespace
 
+// Device address of `scratch` on the current device, looked up once per device.
+static int* noalloc_scratch_of(int device, hipError_t* err) {
+    static std::mutex mu;
+    static std::map<int, int*> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void* p = nullptr;
+    *err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (*err != hipSuccess) return nullptr;
+    addr[device] = (int*)p;
+    return (int*)p;
+}
+
 extern "C" {
 
 int cpt_version(void) { return 1000; }
@@ -1683,6 +1711,12 @@ int cpt_create(cpt_ctx** out, int device) {
     if (r) {
         delete c;
         return r;
+    }
+    hipError_t se = hipSuccess;
+    c->noalloc_scratch = noalloc_scratch_of(device, &se);
+    if (!c->noalloc_scratch) {
+        delete c;
+        return CPT_E_HIP;
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&c->counters, 8 * sizeof(uint32_t)) != hipSuccess ||
@@ -2077,6 +2111,22 @@ int cpt_trace_closest_device(cpt_ctx* c, const void* d_rays, int32_t n, void* d_
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     hipLaunchKernelGGL((k_trace_closest<B>), dim3(G), dim3(B), 0, s, dev_scene(c), (const float4*)d_rays, n,
                        (float4*)d_hits, c->gstack, CPT_DEFAULT_REFILL);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, CPT_E_HIP, "trace launch failed: %s", hipGetErrorString(e));
+    return CPT_OK;
+}
+
+int cpt_trace_closest_device_noalloc(cpt_ctx* c, const void* d_rays, int32_t n, void* d_hits, void* stream) {
+    if (!c || !d_rays || !d_hits || n < 0) return CPT_E_INVALID;
+    if (!c->has_scene) return fail(c, CPT_E_NOSCENE, "no scene uploaded");
+    if (3 * c->bvh_depth + 1 > CPT_LDS_SS + CPT_GSTACK)  // upload refuses deeper trees; kept as the scratch's own bound
+        return fail(c, CPT_E_LIMIT, "BVH depth %d needs more traversal stack than the scratch holds", c->bvh_depth);
+    if (n == 0) return CPT_OK;
+    // the grid never exceeds the threads the scratch covers; each wave walks its chunk of the batch
+    const int G = std::min(NOALLOC_BLOCKS, (n + NOALLOC_B - 1) / NOALLOC_B);
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    hipLaunchKernelGGL((k_trace_closest<NOALLOC_B>), dim3(G), dim3(NOALLOC_B), 0, s, dev_scene(c), (const float4*)d_rays,
+                       n, (float4*)d_hits, c->noalloc_scratch, CPT_DEFAULT_REFILL);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(c, CPT_E_HIP, "trace launch failed: %s", hipGetErrorString(e));
     return CPT_OK;

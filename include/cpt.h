@@ -226,6 +226,18 @@ int cpt_trace_any(cpt_ctx* ctx, const float* rays, int32_t n, uint8_t* occluded_
  * hits: n x 4 words t, beta, gamma, triangle id) on stream `stream` (NULL = the
  * context's stream); no host synchronisation. */
 int cpt_trace_closest_device(cpt_ctx* ctx, const void* d_rays, int32_t n, void* d_hits, void* stream);
+/* cpt_trace_closest_device without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernel, same results bit for bit; its traversal overflow stacks live in a
+ * file-scope device array of 56 MiB per GPU (1024 blocks x 128 threads x 112
+ * entries, located with hipGetSymbolAddress when a context is created), so the
+ * launch is capped at 1024 blocks, each wave walking its share of the batch; any
+ * n >= 0 is accepted.  Every call on one GPU uses that GPU's array: calls that
+ * overlap on one GPU (other streams, other contexts) race and must be serialised
+ * by the caller.  CPT_E_LIMIT if the scene's BVH needs deeper stacks than the
+ * array holds (scene upload already refuses such trees). */
+int cpt_trace_closest_device_noalloc(cpt_ctx* ctx, const void* d_rays, int32_t n, void* d_hits, void* stream);
 
 /* Render into the context's device film (FP64 accumulators).  Replaces the
  * pixel loop of PathtracerMIS::render (pathtracerMIS.h:139-160). */
