@@ -58,6 +58,8 @@ def lib():
     L.orc_render.argtypes = [vp, i32, i32, i32, i32, i32, C.c_uint64, i32, i32, i32, i32, vp, vp, vp, i32, vp]
     L.orc_path.argtypes = [vp, i32, i32, i32, i32, C.c_uint64, i32, i32, vp]
     L.orc_set_integrator.argtypes = [vp, i32]
+    L.orc_set_view.argtypes = [vp, C.c_uint32]
+    L.orc_set_view.restype = None
     L.orc_ppg_render.argtypes = [vp, i32, i32, i32, i32, C.c_uint64, C.c_uint32] + [vp] * 5 + [i32, vp, vp, vp, i32]
     L.orc_path.restype = None
     L.orc_diffuse_sample.argtypes = [vp, vp, C.c_float, C.c_float, vp, vp]
@@ -194,7 +196,7 @@ class OracleScene:
         return o, d
 
     def render(self, W, H, spp_axis, max_depth=32, sampler="pcg", seed=1, cols=None, rows=None,
-               threads=0, sumsq=False, integrator=0, corrected=False):
+               threads=0, sumsq=False, integrator=0, corrected=False, view=0):
         """Returns (sums (H,W,3) float64, film (H,W,3) float32, counts [paths, ext, shadow])
         and, with sumsq=True, the per-pixel sums of squares as a fourth element.
         sampler "pcg" = the GPU's per-path stream; "mt" = the reference's per-column
@@ -202,6 +204,7 @@ class OracleScene:
         PathtracerMIS, 1 = Pathtracer (NEE without MIS), 2 = PurePt."""
         lib().orc_set_integrator(self.h, integrator)
         lib().orc_set_corrected(self.h, int(corrected))
+        lib().orc_set_view(self.h, view)
         sums = np.zeros((H, W, 3), np.float64)
         sq = np.zeros((H, W, 3), np.float64)
         film = np.zeros((H, W, 3), np.float32)

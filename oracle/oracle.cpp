@@ -162,6 +162,7 @@ struct Scene {
     // MIS from the pre-roulette pdf, point lights weighted 1 in NEE, survival
     // probability clamped at 1, normal maps at every hit, light tracer aspect ratio
     bool corrected = false;
+    uint32_t view = 0;  // view id in the per-path PCG stream id (multi-view batches)
     // image textures: per-vertex uv, images (w, h, channels, texel offset),
     // per material the image of its albedo, roughness, metal and normal slot (-1: constant)
     std::vector<std::array<float, 2>> UV;
@@ -1357,7 +1358,8 @@ int orc_render(void* sp, int W, int H, int S, int max_depth, int sampler, uint64
                         film_coords(i, j, k, l, S, W, H, r1, r2, &cx, &cy);
                         c = integrate(*s, cx, cy, mt, max_depth, &cnt);
                     } else {
-                        PcgSampler pcg(seed, (uint64_t)(i + (int64_t)j * W) * (uint64_t)(S * S) + (uint64_t)(k * S + l));
+                        PcgSampler pcg(seed, ((uint64_t)s->view << 40) +
+                                                 (uint64_t)(i + (int64_t)j * W) * (uint64_t)(S * S) + (uint64_t)(k * S + l));
                         float r1 = pcg.sample();
                         float r2 = pcg.sample();
                         float cx, cy;
@@ -1460,6 +1462,7 @@ int orc_ppg_render(void* sp, int W, int H, int spp, int max_depth, uint64_t seed
 }
 
 void orc_set_integrator(void* sp, int integrator) { ((Scene*)sp)->integrator = integrator; }
+void orc_set_view(void* sp, uint32_t view) { ((Scene*)sp)->view = view; }
 void orc_set_corrected(void* sp, int corrected) { ((Scene*)sp)->corrected = corrected != 0; }
 
 // Lighttracer::render (lighttracer.h:118-154): W*H*S*S light paths, one per camera

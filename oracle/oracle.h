@@ -93,6 +93,9 @@ int orc_render(void* s, int W, int H, int spp_axis, int max_depth, int sampler, 
 /* Integrator of orc_render / orc_path: 0 PathtracerMIS (pathtracerMIS.h:29-130),
  * 1 Pathtracer, NEE without MIS (pathtracer.h:27-110), 2 PurePt (pure_pt.h:27-67). */
 void orc_set_integrator(void* s, int integrator);
+/* view id of orc_render's per-path PCG ids, (view << 40) + pixel * spp + sample, as
+ * the library numbers the paths of view `view` in a multi-view batch (default 0) */
+void orc_set_view(void* s, uint32_t view);
 
 /* One iteration of PPG (src/renderer/ppg.h:41-143) with a fixed, flattened SD-tree (the
  * layout of cpt_ppg_export; bounds = lo[3], hi[3]): spp whole-pixel samples per pixel, PCG

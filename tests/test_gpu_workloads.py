@@ -1,0 +1,203 @@
+"""The batched, guided and large-scale workloads at their own sizes:
+
+  multi-view batch: 64 views of the box room, resolutions drawn from {256^2,
+    512x384, 1024x768, 1920x1080} and spp from {16, 64, 256}, depth 8 (one batch of
+    about 5 G paths); every view equals its single-view render, and the smallest
+    views equal the oracle path by path (the oracle numbers its paths per view as
+    the batch does)
+  hard-lighting interior: 1.92M triangles, 1280x720, 2^18 recording paths per pass;
+    guided (learned guide trained online) against plain PT, held to a plain-vs-plain
+    control (image and quadrant means, 8x8-block 3-sigma exceedance)
+  4x4 grid of interiors: 7.68M triangles at 3840x2160 with the device-built BVH;
+    closest and any hits of 2^17 rays against the oracle's traversal, the 4K film
+    independent of the launch plan, guided (width 128, 16 lobes, FP8 inference)
+    against plain on a 1920x1080 crop, held to the same control
+"""
+import time
+
+import numpy as np
+import pytest
+
+import cpt
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+GUIDED_TRAIN = cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN
+
+
+def _block_z(gs, gq, n1, ps, pq, n2, b):
+    """z-scores of block means (b x b pixels, per channel) of two renders from their
+    per-pixel sums and sums of squares."""
+    H, W = gs.shape[:2]
+    H, W = H - H % b, W - W % b
+
+    def blk(x):
+        return x[:H, :W].reshape(H // b, b, W // b, b, 3).sum(axis=(1, 3))
+    m1, m2 = n1 * b * b, n2 * b * b
+    a1, q1, a2, q2 = blk(gs) / m1, blk(gq) / m1, blk(ps) / m2, blk(pq) / m2
+    v1 = np.maximum(q1 - a1 * a1, 0) * m1 / (m1 - 1)
+    v2 = np.maximum(q2 - a2 * a2, 0) * m2 / (m2 - 1)
+    se = np.sqrt(v1 / m1 + v2 / m2)
+    ok = se > 0
+    return (a1 - a2)[ok] / se[ok]
+
+
+def _image_z(gs, gq, n1, ps, pq, n2):
+    """Per channel: z-score of the difference of two renders' image means, from the
+    per-pixel sums and sums of squares of n1 and n2 samples per pixel."""
+    N1, N2 = n1 * gs.shape[0] * gs.shape[1], n2 * ps.shape[0] * ps.shape[1]
+    m1, m2 = gs.sum(axis=(0, 1)) / N1, ps.sum(axis=(0, 1)) / N2
+    v1, v2 = gq.sum(axis=(0, 1)) / N1 - m1 * m1, pq.sum(axis=(0, 1)) / N2 - m2 * m2
+    return (m1 - m2) / np.sqrt(v1 / N1 + v2 / N2)
+
+
+def _check_unbiased(label, g, a, b, n, block):
+    """g, a, b: (sums, sums of squares) of a guided render and two independent plain
+    renders of n spp each.  On hard lighting the per-block z-test of two correct
+    estimators is itself far from Gaussian at these sample counts (rare bright paths:
+    a block mean is usually below its expectation and its variance estimate too
+    small), so the guided render is held to what the plain-vs-plain control gives:
+    image means (whole frame and quadrants) within 4 standard errors, and a block
+    3-sigma exceedance no larger than the control's plus three binomial sigmas."""
+    H, W = a[0].shape[:2]
+    regions = [(slice(None), slice(None))] + [(slice(y, y + H // 2), slice(x, x + W // 2))
+                                              for y in (0, H // 2) for x in (0, W // 2)]
+    for r in regions:
+        zg = _image_z(g[0][r], g[1][r], n, a[0][r], a[1][r], n)
+        zc = _image_z(b[0][r], b[1][r], n, a[0][r], a[1][r], n)
+        assert np.all(np.abs(zg) < 4) and np.all(np.abs(zc) < 4), (label, r, zg, zc)
+    zg = _block_z(g[0], g[1], n, a[0], a[1], n, block)
+    zc = _block_z(b[0], b[1], n, a[0], a[1], n, block)
+    fg, fc = float((np.abs(zg) > 3).mean()), float((np.abs(zc) > 3).mean())
+    print(f"{label}: image-mean z guided vs plain {np.round(_image_z(g[0], g[1], n, a[0], a[1], n), 2)}, "
+          f"plain vs plain {np.round(_image_z(b[0], b[1], n, a[0], a[1], n), 2)}; {block}x{block}-block 3-sigma "
+          f"exceedance guided {fg * 100:.3f}%, plain control {fc * 100:.3f}% of {zg.size}", flush=True)
+    assert fg <= fc + 3 * np.sqrt(max(fc, 1e-4) / zg.size)
+
+
+# ------------------------------------------------------------------ multi-view batch
+def _c3_views(seed=7, n=64):
+    rng = np.random.default_rng(seed)
+    sizes = [(256, 256), (512, 384), (1024, 768), (1920, 1080)]
+    cams, whs = [], []
+    for v in range(n):
+        w, h = sizes[rng.integers(4)]
+        sa = int(rng.choice([4, 8, 16]))
+        ang = rng.uniform(-0.35, 0.35)
+        o = (3.9 * np.sin(ang), rng.uniform(-0.25, 0.25), 3.9 * np.cos(ang))
+        cams.append(cpt.scenes.camera_frame(o, (0, 0, 0), (0, 1, 0), 0.35411, w, h))
+        whs.append((w, h, sa))
+    return np.stack(cams), np.array(whs, np.int32)
+
+
+def test_multiview_batch_at_full_size(ctx):
+    scene = cpt.scenes.box_room()
+    cams, whs = _c3_views()
+    ctx.upload(scene)
+    t0 = time.perf_counter()
+    imgs, ms, st = ctx.render_views(cams, whs, max_depth=8, seed=1)
+    batch_s = time.perf_counter() - t0
+    paths = int(np.sum(whs[:, 0].astype(np.int64) * whs[:, 1] * whs[:, 2] ** 2))
+    print(f"64-view batch: {paths / 1e9:.2f} G paths in {batch_s:.2f} s, per-view completion "
+          f"p50 {np.percentile(ms, 50):.0f} ms p99 {np.percentile(ms, 99):.0f} ms", flush=True)
+    assert st["paths"] == paths
+    assert all(np.isfinite(i).all() and i.mean() > 0 for i in imgs)
+    assert np.all(ms > 0) and np.percentile(ms, 99) <= batch_s * 1e3
+    # every view equals its single-view render (the path ids carry the view id; the
+    # film adds each pixel's samples in ascending order whatever the packing)
+    for v in range(len(whs)):
+        w, h, sa = (int(x) for x in whs[v])
+        ctx.upload(scene, camera=cams[v])
+        ctx.render(w, h, sa, max_depth=8, seed=1, view=v)
+        assert np.array_equal(imgs[v], ctx.read_film()), f"view {v}"
+    # the smallest views against the oracle, path by path
+    small = [v for v in range(len(whs)) if whs[v, 0] == 256 and whs[v, 2] == 4][:2]
+    assert small
+    for v in small:
+        o = oracle.OracleScene(scene, camera=cams[v])
+        _, film, _ = o.render(256, 256, 4, max_depth=8, sampler="pcg", seed=1, view=v)
+        o.close()
+        per_px = (np.abs(imgs[v] - film) / np.maximum(np.abs(film), 1e-3)).max(axis=-1)
+        off = int((per_px >= 1e-4).sum())
+        print(f"view {v}: {off} of {per_px.size} pixels differ from the oracle by 1e-4 or more")
+        assert off == 0
+
+
+# ------------------------------------------------------------------ hard-lighting interior
+@pytest.fixture(scope="module")
+def interior():
+    return cpt.scenes.interior_scene(res=(1280, 720), n_boxes=40000)
+
+
+def test_interior_guided_unbiased_at_full_size(ctx, interior):
+    W, H = 1280, 720
+    assert interior.n_triangles > 1_900_000
+    ctx.upload(interior)
+    plain = []
+    for seed in (5, 7):
+        ctx.render(W, H, 16, max_depth=12, seed=seed)
+        plain.append(ctx.read_film(sums=True, sumsq=True)[1:])
+    ctx.guide_init(seed=2, lr=3e-3, train_paths=1 << 18)
+    st = ctx.render(W, H, 16, max_depth=12, seed=6, flags=GUIDED_TRAIN)
+    guided = ctx.read_film(sums=True, sumsq=True)[1:]
+    print(f"interior guided: {st['guide_steps']} Adam steps, {st['guide_samples'] / max(1, st['guide_steps']):.0f} "
+          f"training samples per step", flush=True)
+    assert st["guide_steps"] == 256 and st["guide_samples"] > 256 * 100_000
+    _check_unbiased("interior 1280x720, 256 spp", guided, plain[0], plain[1], 256, 8)
+
+
+# ------------------------------------------------------------------ 4x4 grid of interiors at 4K
+@pytest.fixture(scope="module")
+def grid4k():
+    return cpt.scenes.interior_grid_scene(res=(3840, 2160))
+
+
+def test_grid4k_hits_match_oracle(ctx, grid4k):
+    print(f"grid scene: {grid4k.n_triangles} triangles", flush=True)
+    assert grid4k.n_triangles > 7_500_000
+    ctx.upload(grid4k, gpu_bvh=True)
+    orc = oracle.OracleScene(grid4k)
+    rays = cpt.raybatch.make_rays(grid4k, 1 << 17, seed=51)
+    g_tri, g_t = ctx.trace_closest(rays)
+    r_tri, r_t = orc.trace_closest(rays)
+    diff = (g_tri != r_tri) | ((r_tri >= 0) & (g_t.view(np.int32) != r_t.view(np.int32)))
+    print("closest: rays where GPU and the reference traversal differ:", int(diff.sum()), flush=True)
+    assert diff.mean() < 1e-3
+    if diff.any():  # settled by brute force (the reference's slab test can drop grazing hits)
+        d_tri, d_t = orc.trace_closest(rays[diff], brute=True)
+        assert np.array_equal(g_tri[diff], d_tri)
+        assert np.array_equal(g_t[diff][d_tri >= 0].view(np.int32), d_t[d_tri >= 0].view(np.int32))
+    srays = cpt.raybatch.make_rays(grid4k, 1 << 17, seed=52, shadow=True)
+    g = ctx.trace_any(srays)
+    r = orc.trace_any(srays)
+    sd = g != r
+    print("any: rays where GPU and the reference traversal differ:", int(sd.sum()), flush=True)
+    assert sd.mean() < 1e-3
+    if sd.any():
+        assert np.array_equal(g[sd], orc.trace_any(srays[sd], brute=True))
+    orc.close()
+
+
+def test_grid4k_film_independent_of_plan_and_guided_unbiased(ctx, grid4k):
+    W, H = 3840, 2160
+    ctx.upload(grid4k, gpu_bvh=True, tie_rank=False)
+    ctx.render(W, H, 1, max_depth=12, seed=61)
+    _, a = ctx.read_film(sums=True)
+    assert np.isfinite(a).all() and a.mean() > 0
+    for kw in (dict(max_paths=W * H // 3 + 11), dict(block_size=256, shade_block=128, refill=20)):
+        ctx.render(W, H, 1, max_depth=12, seed=61, **kw)
+        _, b = ctx.read_film(sums=True)
+        assert np.array_equal(a, b), kw
+    # guided (the large configuration's network) against plain on a crop
+    crop = (slice(540, 1620), slice(960, 2880))
+    plain = []
+    for seed in (62, 64):
+        ctx.render(W, H, 8, max_depth=12, seed=seed)
+        plain.append(tuple(x[crop] for x in ctx.read_film(sums=True, sumsq=True)[1:]))
+    ctx.guide_init(seed=3, lr=3e-3, train_paths=1 << 18, width=128, infer_fp8=True)
+    st = ctx.render(W, H, 8, max_depth=12, seed=63, flags=GUIDED_TRAIN)
+    guided = tuple(x[crop] for x in ctx.read_film(sums=True, sumsq=True)[1:])
+    print(f"grid guided: {st['guide_steps']} Adam steps, {st['guide_samples']} training samples", flush=True)
+    assert st["guide_steps"] == 64
+    _check_unbiased("grid 4K, 1920x1080 crop, 64 spp", guided, plain[0], plain[1], 64, 8)
