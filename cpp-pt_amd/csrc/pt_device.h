@@ -4,7 +4,9 @@
 // literal, src/math/math.h:11; Vec3::operator/(T) multiplies by 1.0/a computed
 // in double, math.h:103-105; fabs(float) returns double, ...).  Every helper
 // below reproduces those roundings, and the library is compiled with
-// -ffp-contract=off so that no multiply-add is fused.  Where the reference's
+// -ffp-contract=off so that no multiply-add is fused (the box test's explicit
+// fmaf is the one exception: it is conservative, not a restatement of the
+// reference's arithmetic; see slab_offsets).  Where the reference's
 // double result is provably equal to a float operation (1.0/x and 1.0-x rounded
 // to float, sqrt) the float form is used.
 #pragma once
@@ -165,96 +167,23 @@ __device__ __forceinline__ bool tri_test(const float4& A, const float4& E1, cons
     return !(alpha < 0 || alpha > 1 || beta < 0 || beta > 1 || gamma < 0 || gamma > 1);
 }
 
-template <int BLOCK, bool ANY, bool COUNT>
-__device__ bool traverse(const DevScene& sc, V3 o, V3 d, float tmax, Hit& h, TravStack<BLOCK>& st,
-                         unsigned& nvis, unsigned& ntest) {
-    // 1/d with zero components replaced by a tiny signed value so that slab
-    // products never form 0*inf
-    float dx = fabsf(d.x) < 1e-20f ? copysignf(1e-20f, d.x) : d.x;
-    float dy = fabsf(d.y) < 1e-20f ? copysignf(1e-20f, d.y) : d.y;
-    float dz = fabsf(d.z) < 1e-20f ? copysignf(1e-20f, d.z) : d.z;
-    float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-    float best = tmax;
-    int best_rank = INT_MIN;
-    h.tri = -1;
-    int sp = 0;
-    int next = 0;  // root node
-    while (true) {
-        if (next < 0) {  // leaf
-            int first = next & ((1 << 27) - 1);
-            int cnt = ((next >> 27) & 15) + 1;
-            for (int k = 0; k < cnt; k++) {
-                const float4* tp = sc.tris + 3 * (size_t)(first + k);
-                float4 A = tp[0], E1 = tp[1], E2 = tp[2];
-                float t, b, g;
-                if (COUNT) ntest++;
-                if (tri_test(A, E1, E2, o, d, best, t, b, g)) {
-                    if (ANY) return true;
-                    int rk = __float_as_int(E1.w);
-                    if (t < best || rk > best_rank) {
-                        best = t;
-                        best_rank = rk;
-                        h.t = t;
-                        h.beta = b;
-                        h.gamma = g;
-                        h.tri = __float_as_int(A.w);
-                    }
-                }
-            }
-            if (sp == 0) break;
-            next = st.get(--sp);
-            continue;
-        }
-        if (next == EMPTY_CHILD) {
-            if (sp == 0) break;
-            next = st.get(--sp);
-            continue;
-        }
-        if (COUNT) nvis++;
-        const float4* np = reinterpret_cast<const float4*>(sc.nodes + next);
-        float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-        int4 ch = reinterpret_cast<const int4*>(np)[6];
-        float lim = ANY ? best * CPT_BOX_SLACK : best * CPT_TIE_SLACK;
-        float key[4];
-        int cid[4];
-#define CPT_BOX(c, LX, HX, LY, HY, LZ, HZ, CH)                                   \
-    {                                                                             \
-        float t0 = (LX - o.x) * ix, t1 = (HX - o.x) * ix;                         \
-        float t2 = (LY - o.y) * iy, t3 = (HY - o.y) * iy;                         \
-        float t4 = (LZ - o.z) * iz, t5 = (HZ - o.z) * iz;                         \
-        float tn = fmaxf(fmaxf(fminf(t0, t1), fminf(t2, t3)), fmaxf(fminf(t4, t5), 0.0f)); \
-        float tf = fminf(fminf(fmaxf(t0, t1), fmaxf(t2, t3)), fmaxf(t4, t5)) * CPT_BOX_SLACK; \
-        bool hit = tn <= tf && tn <= lim;                                         \
-        key[c] = hit ? tn : INFINITY;                                             \
-        cid[c] = hit ? CH : EMPTY_CHILD;                                          \
-    }
-        CPT_BOX(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
-        CPT_BOX(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y)
-        CPT_BOX(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z)
-        CPT_BOX(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w)
-#undef CPT_BOX
-#define CPT_CSWAP(a, b)                                  \
-    if (key[b] < key[a]) {                               \
-        float tk = key[a]; key[a] = key[b]; key[b] = tk; \
-        int tc = cid[a]; cid[a] = cid[b]; cid[b] = tc;   \
-    }
-        CPT_CSWAP(0, 1)
-        CPT_CSWAP(2, 3)
-        CPT_CSWAP(0, 2)
-        CPT_CSWAP(1, 3)
-        CPT_CSWAP(1, 2)
-#undef CPT_CSWAP
-        if (cid[0] == EMPTY_CHILD) {
-            if (sp == 0) break;
-            next = st.get(--sp);
-            continue;
-        }
-        if (cid[3] != EMPTY_CHILD) st.put(sp++, cid[3]);
-        if (cid[2] != EMPTY_CHILD) st.put(sp++, cid[2]);
-        if (cid[1] != EMPTY_CHILD) st.put(sp++, cid[1]);
-        next = cid[0];
-    }
-    return ANY ? false : h.tri >= 0;
+// Slab offsets of one ray for the box test.  A box plane P is crossed at
+// t = (P - o) * inv; with c = -(o * inv) the test evaluates fma(P, inv, c), one
+// instruction per plane.  That form has an absolute error of up to one rounding
+// of o * inv (plus the rounding of c), so c is moved by e = 2^-21 |o * inv| (four
+// times that error) toward smaller t for the plane that is nearer along the axis
+// and toward larger t for the other: the computed entry never exceeds the exact
+// entry and the computed exit never falls short of the exact exit, beyond one
+// relative rounding of their own, which CPT_BOX_SLACK and CPT_TIE_SLACK cover.
+// Culling stays conservative; only boxes within e of the ray's slabs are entered
+// that the exact test would skip.
+__device__ __forceinline__ void slab_offsets(float o, float inv, float& c_lo, float& c_hi) {
+    const float oi = o * inv;
+    const float e = fabsf(oi) * 4.76837158203125e-7f;  // 2^-21
+    const float c_near = -oi - e, c_far = -oi + e;
+    const bool pos = inv >= 0.0f;  // the low plane is the near one
+    c_lo = pos ? c_near : c_far;
+    c_hi = pos ? c_far : c_near;
 }
 
 // Chunked persistent traversal.  Wave w of the grid owns rays
@@ -285,6 +214,7 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
     unsigned ray = 0;
     V3 o = {0, 0, 0}, d = {0, 0, 0};
     float ix = 0, iy = 0, iz = 0, best = 0;
+    float cxl = 0, cxh = 0, cyl = 0, cyh = 0, czl = 0, czh = 0;  // slab offsets
     int best_rank = INT_MIN;
     Hit h;
     h.t = 0.0f;
@@ -352,6 +282,9 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
             ix = 1.0f / dx;
             iy = 1.0f / dy;
             iz = 1.0f / dz;
+            slab_offsets(o.x, ix, cxl, cxh);
+            slab_offsets(o.y, iy, cyl, cyh);
+            slab_offsets(o.z, iz, czl, czh);
             fresh = false;
         }
         bool finished = false;
@@ -385,9 +318,9 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
             int cid[4];
 #define CPT_BOX(c, LX, HX, LY, HY, LZ, HZ, CH)                                   \
     {                                                                             \
-        float t0 = (LX - o.x) * ix, t1 = (HX - o.x) * ix;                         \
-        float t2 = (LY - o.y) * iy, t3 = (HY - o.y) * iy;                         \
-        float t4 = (LZ - o.z) * iz, t5 = (HZ - o.z) * iz;                         \
+        float t0 = fmaf(LX, ix, cxl), t1 = fmaf(HX, ix, cxh);                     \
+        float t2 = fmaf(LY, iy, cyl), t3 = fmaf(HY, iy, cyh);                     \
+        float t4 = fmaf(LZ, iz, czl), t5 = fmaf(HZ, iz, czh);                     \
         float tn = fmaxf(fmaxf(fminf(t0, t1), fminf(t2, t3)), fmaxf(fminf(t4, t5), 0.0f)); \
         float tf = fminf(fminf(fmaxf(t0, t1), fmaxf(t2, t3)), fmaxf(t4, t5)) * CPT_BOX_SLACK; \
         bool hit = tn <= tf && tn <= lim;                                         \
