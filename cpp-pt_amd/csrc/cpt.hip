@@ -422,6 +422,39 @@ __device__ __forceinline__ bool guidable(const Mat& m) {
 
 // Per bounce: network inputs of every queued hit, one MLP forward per tile of 64,
 // decoded mixture written per queue entry (6 x lobes values, structure of arrays).
+// The guide's inputs at a queued hit (the hit point, the shading normal the
+// shading kernel will use, -d) without the full hit record: the interpolated
+// normal in float (the features need no reference rounding).  Where the shading
+// kernel re-frames the hit (`frame`: first hits, or every hit in corrected mode)
+// and the material bends the normal (a normal-map image, or a constant texel
+// other than the flat (0.5, 0.5, 1)), the full record and frame are taken.
+__device__ __forceinline__ void guide_hit_features(const DevScene& sc, V3 o, V3 d, const Hit& h, bool frame, float* in) {
+    const float4* sr = sc.shade + 6 * (size_t)h.tri;
+    const float4 r0 = sr[0], r1 = sr[1], r2 = sr[2], r3 = sr[3];
+    const int matid = __float_as_int(r0.w);
+    bool full = false;
+    if (frame) {
+        const float* m = sc.mats + 16 * matid;
+        full = sc.mtex[matid].w >= 0 || (m[12] != 0.0f && !(m[9] == 0.5f && m[10] == 0.5f && m[11] == 1.0f));
+    }
+    V3 p, n;
+    if (full) {
+        Isect it;
+        int mid;
+        fill_isect(sc, o, d, h, it, mid);
+        first_hit_frame(hit_mat(sc, mid, h), it);
+        p = it.hitpoint;
+        n = it.normal;
+    } else {
+        const float al = 1.0f - (h.gamma + h.beta);
+        const V3 nn = {r1.x * al + r1.w * h.beta + r2.z * h.gamma, r1.y * al + r2.x * h.beta + r2.w * h.gamma,
+                       r1.z * al + r2.y * h.beta + r3.x * h.gamma};
+        n = mulf(nn, rsqrtf(dot(nn, nn)));
+        p = add(o, mulf(d, h.t));
+    }
+    guide::features_fast(sc.bmin, sc.bext, p.x, p.y, p.z, n.x, n.y, n.z, -d.x, -d.y, -d.z, in);
+}
+
 // WIDTH: 64 (namespace guide) or 128 (guide128); FP8: the e4m3 network (k_quant8's
 // weights) on FP8 MFMA, else BF16
 template <int WIDTH, bool FP8>
@@ -452,14 +485,10 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
                 h.gamma = HH.z;
                 h.tri = __float_as_int(HH.w);
                 if (h.tri >= 0) {
-                    V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
-                    Isect it;
-                    int matid;
-                    fill_isect(a.sc, o, d, h, it, matid);
-                    int slot = __float_as_int(D.w);
-                    if ((a.meta[slot] & 0xffu) == 0 || a.corrected) first_hit_frame(hit_mat(a.sc, matid, h), it);
-                    guide::features(a.sc.bmin, a.sc.bext, it.hitpoint.x, it.hitpoint.y, it.hitpoint.z, it.normal.x,
-                                    it.normal.y, it.normal.z, -d.x, -d.y, -d.z, in);
+                    const V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
+                    const int slot = __float_as_int(D.w);
+                    const bool frame = (a.meta[slot] & 0xffu) == 0 || a.corrected;
+                    guide_hit_features(a.sc, o, d, h, frame, in);
                 }
             }
         }

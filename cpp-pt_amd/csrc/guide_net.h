@@ -84,21 +84,31 @@ __device__ __forceinline__ void blob_run(float x, int b, float scale, float* v) 
     }
 }
 
-// acc[nt] += A(rows of this wave) x Wt^T, A row-major (lda), Wt n-major (ldb):
-// lane l supplies A[l%16][k0 + 4(l/16) + j] and B[k0 + 4(l/16) + j][n0 + l%16]
-// KU: unroll of the k loop (1 when Wt is in global memory: fully unrolled, the
-// hoisted global loads would not fit the register file)
-template <int K, int NT, int KU = K / 16>
+// acc[nt] += A(rows of this wave) x Wt^T, A row-major (lda), Wt n-major (ldb), on
+// gfx950's v_mfma_f32_16x16x32_bf16: lane l supplies A[l%16][k0 + 8(l/16) + j] and
+// B[k0 + 8(l/16) + j][n0 + l%16], j = 0..7 (layout checked on the device with a
+// one-wave probe of exact small-integer products), each as two 8-byte halves (rows
+// are 8-byte aligned).  KU: unroll of the k loop (1 when Wt is in global memory:
+// fully unrolled, the hoisted global loads would not fit the register file)
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ bf8v ld_bf8(const uint16_t* p) {
+    const s4 lo = *reinterpret_cast<const s4*>(p), hi = *reinterpret_cast<const s4*>(p + 4);
+    const s8v v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_bit_cast(bf8v, v);
+}
+template <int K, int NT, int KU = K / 32>
 __device__ __forceinline__ void gemm_rows_nk(const uint16_t* A, int lda, const uint16_t* Wt, int ldb, f4* acc,
                                              int lane) {
+    static_assert(K % 32 == 0, "K in steps of 32");
 #pragma unroll KU
-    for (int kt = 0; kt < K / 16; kt++) {
-        const int kk = kt * 16 + 4 * (lane >> 4);
-        s4 a = *reinterpret_cast<const s4*>(A + (lane & 15) * lda + kk);
+    for (int kt = 0; kt < K / 32; kt++) {
+        const int kk = kt * 32 + 8 * (lane >> 4);
+        const bf8v a = ld_bf8(A + (lane & 15) * lda + kk);
 #pragma unroll
         for (int nt = 0; nt < NT; nt++) {
-            s4 b = *reinterpret_cast<const s4*>(Wt + (nt * 16 + (lane & 15)) * ldb + kk);
-            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+            const bf8v b = ld_bf8(Wt + (nt * 16 + (lane & 15)) * ldb + kk);
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nt], 0, 0, 0);
         }
     }
 }
@@ -175,6 +185,39 @@ __device__ __forceinline__ void features(const float* bmin, const float* bext, f
     in[6] = (atan2f(oy, ox) + 3.14159265f) * (1.0f / TWO_PI);
 }
 
+// The same inputs with cheaper angles, for per-bounce inference (the recorded
+// training vertices keep features()): acos by Abramowitz and Stegun 4.4.45
+// (reflected for negative arguments; 6.8e-5 rad at most, evaluated in float),
+// atan on [0, 1] by Hastings' odd polynomial of degree 11, extended to atan2 by
+// the octant symmetries (2e-6 rad at most).  Both differences stay below 3e-5 of
+// an input's range, a thousandth of a one-blob bin.
+__device__ __forceinline__ float acos_in(float x) {
+    const float ax = fminf(fabsf(x), 1.0f);
+    const float r = sqrtf(1.0f - ax) * (1.5707288f + ax * (-0.2121144f + ax * (0.0742610f + ax * -0.0187293f)));
+    return x < 0.0f ? 3.14159265f - r : r;
+}
+__device__ __forceinline__ float atan2_in(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float a = mx > 0.0f ? mn / mx : 0.0f;
+    const float a2 = a * a;
+    float r = a * (0.99997726f + a2 * (-0.33262347f + a2 * (0.19354346f + a2 * (-0.11643287f +
+                                                                  a2 * (0.05265332f + a2 * -0.01172120f)))));
+    if (ay > ax) r = 1.57079633f - r;
+    if (x < 0.0f) r = 3.14159265f - r;
+    return y < 0.0f ? -r : r;
+}
+__device__ __forceinline__ void features_fast(const float* bmin, const float* bext, float px, float py, float pz,
+                                              float nx, float ny, float nz, float ox, float oy, float oz, float* in) {
+    in[0] = fminf(fmaxf((px - bmin[0]) / bext[0], 0.0f), 1.0f);
+    in[1] = fminf(fmaxf((py - bmin[1]) / bext[1], 0.0f), 1.0f);
+    in[2] = fminf(fmaxf((pz - bmin[2]) / bext[2], 0.0f), 1.0f);
+    in[3] = fminf(acos_in(nz) * (1.0f / 3.14159265f), 1.0f);
+    in[4] = fminf(fmaxf((atan2_in(ny, nx) + 3.14159265f) * (1.0f / TWO_PI), 0.0f), 1.0f);
+    in[5] = fminf(acos_in(oz) * (1.0f / 3.14159265f), 1.0f);
+    in[6] = fminf(fmaxf((atan2_in(oy, ox) + 3.14159265f) * (1.0f / TWO_PI), 0.0f), 1.0f);
+}
+
 // --------------------------------------------------------------------------
 // Shared-memory tile: weights (transposed bf16), biases, activations.
 struct SmemFwd {
@@ -217,6 +260,39 @@ __device__ __forceinline__ void encode_tile(SmemFwd& sm, const float* in, int n)
     }
 }
 
+// First of the 16 bins an inference encode computes for input x: the bins
+// c - 6 .. c + 5 around c = floor(32 x) (the peak lies in bin c) lie in
+// [w0, w0 + 16), w0 a multiple of 4 in [0, 16].  Outside the window every bin is
+// below e^-15.1 = 2.7e-7: 0 in FP8 (x 448 it rounds to 0), and in BF16 it moves a
+// layer-0 pre-activation by less than 1e-6 of its size.
+__device__ __forceinline__ int blob_window(float x) {
+    const int c = (int)(x * (float)NBINS);
+    return min(max(((c - 6) >> 2) << 2, 0), NBINS - 16);
+}
+
+// Inference encode (BF16): per (sample, input) pair the 32 bins are zeroed and the
+// 16 of blob_window are written, each from its own exponential (samples beyond n
+// are zero).  Training keeps encode_tile, every bin computed.
+__device__ __forceinline__ void encode_tile_window(SmemFwd& sm, const float* in, int n) {
+    static_assert(NBINS == 32 && LD0 % 4 == 0, "8-byte stores of 4 bins");
+    for (int p = threadIdx.x; p < TILE * NIN; p += blockDim.x) {
+        const int s = p / NIN, d = p - s * NIN;
+        uint2* seg = reinterpret_cast<uint2*>(sm.x0 + s * LD0 + d * NBINS);
+#pragma unroll
+        for (int k = 0; k < NBINS / 4; k++) seg[k] = make_uint2(0u, 0u);
+        if (s < n) {
+            const float x = in[s * NIN + d];
+            const int w0 = blob_window(x);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int b = w0 + 4 * k;
+                seg[b / 4] = make_uint2((uint32_t)f2bf(blob(x, b)) | ((uint32_t)f2bf(blob(x, b + 1)) << 16),
+                                        (uint32_t)f2bf(blob(x, b + 2)) | ((uint32_t)f2bf(blob(x, b + 3)) << 16));
+            }
+        }
+    }
+}
+
 // forward through all layers; out (TILE x NOUT fp32, row stride NOUT) receives raw
 // outputs; wt = the transposed BF16 weights (sm.wt, or global memory when !WT_LDS)
 __device__ __forceinline__ void forward_tile(SmemFwd& sm, const uint16_t* wt, float* out) {
@@ -224,7 +300,7 @@ __device__ __forceinline__ void forward_tile(SmemFwd& sm, const uint16_t* wt, fl
     const int r0 = 16 * wave;
     {
         f4 acc[NTH] = {};
-        gemm_rows_nk<K0, NTH, WT_LDS ? K0 / 16 : 1>(sm.x0 + r0 * LD0, LD0, wt + T0, LD0, acc, lane);
+        gemm_rows_nk<K0, NTH, WT_LDS ? K0 / 32 : 1>(sm.x0 + r0 * LD0, LD0, wt + T0, LD0, acc, lane);
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
 #pragma unroll
@@ -239,7 +315,7 @@ __device__ __forceinline__ void forward_tile(SmemFwd& sm, const uint16_t* wt, fl
     for (int l = 1; l < 4; l++) {
         f4 acc[NTH] = {};
         const int toff = l == 1 ? T1 : l == 2 ? T2 : T3;
-        gemm_rows_nk<HID, NTH, WT_LDS ? HID / 16 : 1>(sm.h[l - 1] + r0 * LDH, LDH, wt + toff, LDH, acc, lane);
+        gemm_rows_nk<HID, NTH, WT_LDS ? HID / 32 : 1>(sm.h[l - 1] + r0 * LDH, LDH, wt + toff, LDH, acc, lane);
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
 #pragma unroll
@@ -252,7 +328,7 @@ __device__ __forceinline__ void forward_tile(SmemFwd& sm, const uint16_t* wt, fl
     }
     {
         f4 acc[NTO] = {};
-        gemm_rows_nk<HID, NTO, WT_LDS ? HID / 16 : 1>(sm.h[3] + r0 * LDH, LDH, wt + T4, LDH, acc, lane);
+        gemm_rows_nk<HID, NTO, WT_LDS ? HID / 32 : 1>(sm.h[3] + r0 * LDH, LDH, wt + T4, LDH, acc, lane);
 #pragma unroll
         for (int nt = 0; nt < NTO; nt++)
 #pragma unroll
@@ -318,7 +394,7 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* gwt, const 
     __syncthreads();
     for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
         int nt = min(TILE, n - t * TILE);
-        encode_tile(sm, in + (size_t)t * TILE * NIN, nt);
+        encode_tile_window(sm, in + (size_t)t * TILE * NIN, nt);
         __syncthreads();
         forward_tile(sm, WT_LDS ? sm.wt : gwt, o_tile);
         for (int i = threadIdx.x; i < nt * NOUT; i += blockDim.x) out[(size_t)t * TILE * NOUT + i] = o_tile[i];
@@ -375,34 +451,65 @@ __device__ __forceinline__ void load_weights8(SmemFwd8& sm, const uint8_t* gwt8,
     if (threadIdx.x < 5) sm.inv_w[threadIdx.x] = inv_scale[threadIdx.x];
 }
 
-// one-blob features x 448 as FP8 (8 consecutive bins of one input per thread
-// step, one 8-byte store)
+// one-blob features x 448 as FP8: per (sample, input) pair the 32 bins are zeroed
+// and the 16 of blob_window written (the others are 0 in e4m3), two exponentials
+// for the run (blob_run)
 __device__ __forceinline__ void encode_tile8(SmemFwd8& sm, const float* in, int n) {
-    constexpr int CPS = K0 / 8;  // chunks per sample
-    static_assert(NBINS % 8 == 0 && LQ0 % 8 == 0, "8-byte chunks");
-    for (int c = threadIdx.x; c < TILE * CPS; c += blockDim.x) {
-        const int s = c / CPS, f = (c - s * CPS) * 8;
-        const int d = f / NBINS, b = f - d * NBINS;
-        uint2 w = {0u, 0u};
+    static_assert(NBINS == 32 && LQ0 % 8 == 0, "8-byte zero stores, 4-byte window stores");
+    for (int p = threadIdx.x; p < TILE * NIN; p += blockDim.x) {
+        const int s = p / NIN, d = p - s * NIN;
+        uint8_t* seg = sm.x0 + s * LQ0 + d * NBINS;
+#pragma unroll
+        for (int k = 0; k < NBINS / 8; k++) reinterpret_cast<uint2*>(seg)[k] = make_uint2(0u, 0u);
         if (s < n) {
-            float v[8];
-            blob_run<8>(in[s * NIN + d], b, FP8_MAX, v);
-            w.x = fp8x2(v[0], v[1]) | (fp8x2(v[2], v[3]) << 16);
-            w.y = fp8x2(v[4], v[5]) | (fp8x2(v[6], v[7]) << 16);
+            const float x = in[s * NIN + d];
+            const int w0 = blob_window(x);
+            // two runs of 8, so that no run starts farther than 9.5 bins from its
+            // peak-side end (a longer run from a clamped window start could begin in
+            // FP32 underflow and carry the zero to the peak)
+            float v[16];
+            blob_run<8>(x, w0, FP8_MAX, v);
+            blob_run<8>(x, w0 + 8, FP8_MAX, v + 8);
+            uint32_t* wp = reinterpret_cast<uint32_t*>(seg + w0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) wp[k] = fp8x2(v[4 * k], v[4 * k + 1]) | (fp8x2(v[4 * k + 2], v[4 * k + 3]) << 16);
         }
-        *reinterpret_cast<uint2*>(sm.x0 + s * LQ0 + f) = w;
     }
 }
 
 // acc[nt] += (A(rows of this wave) x Wt^T)^T: the weights are the MFMA's first
 // operand, so lane l holds features nt 16 + 4 (l / 16) + j (j = 0..3) of sample row
-// l % 16, four consecutive outputs of one sample for packed stores
+// l % 16, four consecutive outputs of one sample for packed stores.  Steps of 128
+// use gfx950's v_mfma_scale_f32_16x16x128_f8f6f4 (both operands e4m3, unit E8M0
+// scales; lane l supplies row l % 16, bytes 32 (l / 16) .. 32 (l / 16) + 31 of the
+// step, a layout checked on the device with a one-wave probe), the rest of K steps
+// of 32 on v_mfma_f32_16x16x32_fp8_fp8 (8 bytes per lane at 8 (l / 16)).
+typedef int i8v __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ i8v ld32b(const uint8_t* p) {
+    const int2 q0 = *reinterpret_cast<const int2*>(p), q1 = *reinterpret_cast<const int2*>(p + 8);
+    const int2 q2 = *reinterpret_cast<const int2*>(p + 16), q3 = *reinterpret_cast<const int2*>(p + 24);
+    i8v v;
+    v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y; v[4] = q2.x; v[5] = q2.y; v[6] = q3.x; v[7] = q3.y;
+    return v;
+}
 template <int K, int NT>
 __device__ __forceinline__ void gemm8_rows_nk_t(const uint8_t* A, int lda, const uint8_t* Wt, int ldb, f4* acc,
                                                 int lane) {
+    static_assert(K % 32 == 0, "K in steps of 32");
+    constexpr int K128 = K / 128 * 128;
 #pragma unroll
-    for (int kt = 0; kt < K / 32; kt++) {
-        const int kk = kt * 32 + 8 * (lane >> 4);
+    for (int k0 = 0; k0 < K128; k0 += 128) {
+        const int kk = k0 + 32 * (lane >> 4);
+        const i8v a = ld32b(A + (lane & 15) * lda + kk);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) {
+            const i8v b = ld32b(Wt + (nt * 16 + (lane & 15)) * ldb + kk);
+            acc[nt] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(b, a, acc[nt], 0, 0, 0, 127, 0, 127);
+        }
+    }
+#pragma unroll
+    for (int k0 = K128; k0 < K; k0 += 32) {
+        const int kk = k0 + 8 * (lane >> 4);
         const long a = *reinterpret_cast<const long*>(A + (lane & 15) * lda + kk);
 #pragma unroll
         for (int nt = 0; nt < NT; nt++) {
@@ -440,7 +547,7 @@ __device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
             const float bb[4] = {bv.x, bv.y, bv.z, bv.w};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const float z = fmaxf(acc[nt][j] * sc + bb[j], 0.0f);
+                const float z = fmaxf(fmaf(acc[nt][j], sc, bb[j]), 0.0f);
                 acc[nt][j] = z;
                 mx = fmaxf(mx, z);
             }
@@ -468,8 +575,8 @@ __device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
         for (int nt = 0; nt < NTO; nt++) {
             const float4 bv = *reinterpret_cast<const float4*>(sm.bias + 4 * HID + nt * 16 + n4);
             *reinterpret_cast<float4*>(out + m * NOUT + nt * 16 + n4) =
-                make_float4(acc[nt][0] * sc + bv.x, acc[nt][1] * sc + bv.y, acc[nt][2] * sc + bv.z,
-                            acc[nt][3] * sc + bv.w);
+                make_float4(fmaf(acc[nt][0], sc, bv.x), fmaf(acc[nt][1], sc, bv.y), fmaf(acc[nt][2], sc, bv.z),
+                            fmaf(acc[nt][3], sc, bv.w));
         }
     }
     // out holds this wave's 16 rows; a reader of other waves' rows needs a block
@@ -897,7 +1004,7 @@ __device__ __forceinline__ void infer_tile(unsigned char* smem, const uint16_t* 
         forward_tile8(sm, o_tile);
     } else {
         SmemFwd& sm = *reinterpret_cast<SmemFwd*>(smem);
-        encode_tile(sm, in, nt);
+        encode_tile_window(sm, in, nt);
         __syncthreads();
         forward_tile(sm, WT_LDS ? sm.wt : gwt, o_tile);
     }
