@@ -13,9 +13,9 @@ from ._lib import (ABI_FUNCTIONS, CPT_RENDER_ACCUMULATE, CPT_RENDER_COUNT, CPT_R
                    CPT_RENDER_GUIDE_TRAIN, CPT_RENDER_GUIDED, CPT_RENDER_NEE_ONLY, CPT_RENDER_NO_GRAPH, CPT_RENDER_PPG,
                    CPT_RENDER_PPG_STEP, CPT_RENDER_PURE_PT, CPT_RENDER_LIGHT_TRACE, CPT_RENDER_CORRECTED,
                    CPT_RENDER_STREAM, CPT_E_HIP, CPT_E_INVALID, CPT_E_LIMIT, Context, CptError, LIB_PATH, load_library,
-                   reference_tie_rank)
+                   load_scene_file, reference_tie_rank)
 
-__all__ = ["scenes", "raybatch", "reference_tie_rank", "Context", "CptError", "load_library", "LIB_PATH", "ABI_FUNCTIONS",
+__all__ = ["scenes", "raybatch", "reference_tie_rank", "load_scene_file", "Context", "CptError", "load_library", "LIB_PATH", "ABI_FUNCTIONS",
            "CPT_RENDER_NO_GRAPH", "CPT_RENDER_KERNEL_TIMING", "CPT_RENDER_COUNT", "CPT_RENDER_ACCUMULATE", "CPT_RENDER_STREAM",
            "CPT_RENDER_GUIDED", "CPT_RENDER_GUIDE_TRAIN", "CPT_RENDER_NEE_ONLY", "CPT_RENDER_PURE_PT", "CPT_RENDER_PPG",
            "CPT_RENDER_PPG_STEP", "CPT_RENDER_LIGHT_TRACE", "CPT_RENDER_CORRECTED", "CPT_E_HIP", "CPT_E_INVALID",

@@ -45,7 +45,7 @@ ABI_FUNCTIONS = (
     "cpt_reference_tie_rank", "cpt_guide_num_params", "cpt_guide_shape", "cpt_guide_init", "cpt_guide_get_params",
     "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
     "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export", "cpt_debug_fail_alloc",
-    "cpt_trace_closest_device_noalloc",
+    "cpt_trace_closest_device_noalloc", "cpt_scene_file_load", "cpt_scene_file_free",
 )
 
 
@@ -139,6 +139,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                      C.c_uint64, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.POINTER(RenderStats)]
     lib.cpt_reference_tie_rank.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    lib.cpt_scene_file_load.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+                                        C.POINTER(SceneDesc), C.c_char_p, C.c_int32]
+    lib.cpt_scene_file_free.argtypes = [C.c_void_p]
+    lib.cpt_scene_file_free.restype = None
     lib.cpt_guide_num_params.restype = C.c_int
     lib.cpt_guide_init.argtypes = [C.c_void_p, C.POINTER(GuideParams)]
     lib.cpt_guide_shape.argtypes = [C.c_void_p, C.c_void_p]
@@ -160,6 +164,42 @@ def reference_tie_rank(positions, indices):
     if rc != CPT_OK:
         raise CptError(rc, "cpt_reference_tie_rank")
     return out
+
+
+def load_scene_file(path, res_x, res_y):
+    """Host-only: parse a scene file through cpt_scene_file_load and return copies of
+    the flattened arrays, keyed like scenes.Scene.c_arrays()."""
+    lib = load_library()
+    handle = C.c_void_p()
+    desc = SceneDesc()
+    err = C.create_string_buffer(512)
+    rc = lib.cpt_scene_file_load(os.fsencode(path), int(res_x), int(res_y), C.byref(handle), C.byref(desc), err, 512)
+    if rc != CPT_OK:
+        raise CptError(rc, err.value.decode(errors="replace"))
+    try:
+        def arr(ptr, dtype, n, cols):
+            if n == 0 or not ptr:
+                return np.zeros((0, cols), dtype)
+            ct = C.c_float if dtype == np.float32 else C.c_int32
+            a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(n * cols,))
+            return a.copy().reshape(n, cols)
+        nv, nt, nm = desc.n_vertices, desc.n_triangles, desc.n_materials
+        out = dict(
+            positions=arr(desc.positions, np.float32, nv, 3), normals=arr(desc.normals, np.float32, nv, 3),
+            bitangents=arr(desc.bitangents, np.float32, nv, 3), indices=arr(desc.indices, np.int32, nt, 3),
+            tri_material=arr(desc.tri_material, np.int32, nt, 1).reshape(-1),
+            materials=arr(desc.materials, np.float32, nm, 16),
+            light_tris=arr(desc.light_triangles, np.int32, desc.n_lights, 1).reshape(-1),
+            point_lights=arr(desc.point_lights, np.float32, desc.n_point_lights, 6),
+            camera=np.array(desc.camera[:], np.float32), env=np.array(desc.env[:], np.float32),
+            uvs=arr(desc.uvs, np.float32, nv, 2), textures=arr(desc.textures, np.int32, desc.n_textures, 4),
+            material_textures=arr(desc.material_textures, np.int32, nm, 4),
+        )
+        n_texels = max((int(o) + int(w) * int(h) * int(ch) for w, h, ch, o in out["textures"]), default=0)
+        out["texels"] = arr(desc.texels, np.float32, n_texels, 1).reshape(-1)
+        return out
+    finally:
+        lib.cpt_scene_file_free(handle)
 
 
 class CptError(RuntimeError):

@@ -620,3 +620,50 @@ void RgbImage::save(const std::string& filename) const {
 }
 
 }  // namespace cpppt
+
+// ---------------------------------------------------------------- C ABI scene file
+// A scene file loaded and flattened behind the C ABI, so that a caller with host
+// types of its own (the reference's Scene / SceneData, which share these names)
+// can obtain a cpt_scene_desc without linking against this file's C++ classes.
+struct cpt_scene_file {
+    cpppt::Scene scene;
+    cpppt::SceneData data;
+    cpppt::RenderSettings settings;
+};
+
+static void copy_message(char* err, int32_t err_len, const char* msg) {
+    if (!err || err_len <= 0) return;
+    std::strncpy(err, msg, (size_t)err_len - 1);
+    err[err_len - 1] = 0;
+}
+
+extern "C" int cpt_scene_file_load(const char* path, int32_t res_x, int32_t res_y, cpt_scene_file** file,
+                                   cpt_scene_desc* desc, char* err, int32_t err_len) {
+    copy_message(err, err_len, "");
+    if (!path || !file || !desc || res_x <= 0 || res_y <= 0) {
+        copy_message(err, err_len, "cpt_scene_file_load: null argument or non-positive resolution");
+        return CPT_E_INVALID;
+    }
+    *file = nullptr;
+    cpt_scene_file* f = nullptr;
+    try {
+        f = new cpt_scene_file();
+        f->settings.resX = res_x;
+        f->settings.resY = res_y;
+        f->settings.scene_name = path;
+        cpppt::Loader::load_scene(&f->scene, &f->data, &f->settings, path);
+        *desc = f->scene.desc();
+    } catch (const std::exception& e) {
+        copy_message(err, err_len, e.what());
+        delete f;
+        return CPT_E_INVALID;
+    } catch (...) {
+        copy_message(err, err_len, "cpt_scene_file_load: unknown error");
+        delete f;
+        return CPT_E_INVALID;
+    }
+    *file = f;
+    return CPT_OK;
+}
+
+extern "C" void cpt_scene_file_free(cpt_scene_file* file) { delete file; }

@@ -204,6 +204,22 @@ int cpt_debug_fail_alloc(cpt_ctx* ctx, int32_t after);
  * LightGroup population (loader.h:362-364). */
 int cpt_upload_scene(cpt_ctx* ctx, const cpt_scene_desc* desc);
 
+/* Host only (no device needed): parse a scene file of the reference's format
+ * and flatten it into *desc.  Replaces Loader::load_scene's parsing
+ * (src/loader/loader.h:314-351: camera, obj + mtl and light lines, read by
+ * parse_camera / parse_light / parse_mat / parse_obj, loader.h:117-254) for a caller that has no
+ * flattened scene of its own: the reference's renderer receives only `Scene&`
+ * (renderer.h:30-33), whose camera and materials keep their data private, so
+ * its adapter loads `RenderSettings::scene_name` again here.  The camera is set
+ * up for a res_x x res_y film.  *file owns every array *desc points to; release
+ * it with cpt_scene_file_free after cpt_upload_scene (which copies).  On failure
+ * returns CPT_E_INVALID, sets *file to NULL and writes the reason into err
+ * (err_len bytes including the terminating zero; err may be NULL). */
+typedef struct cpt_scene_file cpt_scene_file;
+int cpt_scene_file_load(const char* path, int32_t res_x, int32_t res_y, cpt_scene_file** file,
+                        cpt_scene_desc* desc, char* err, int32_t err_len);
+void cpt_scene_file_free(cpt_scene_file* file);
+
 /* BVH statistics of the uploaded scene (6 entries): [nodes, leaves, max depth,
  * triangles, build time in microseconds, 1 if built on the device else 0] */
 int cpt_bvh_info(cpt_ctx* ctx, int64_t* info6);

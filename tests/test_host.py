@@ -366,3 +366,38 @@ def test_terrain_scene_size():
     P, I, N = a["positions"], a["indices"], a["normals"]
     g = np.cross(P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]])
     assert (np.sum(g * N[I[:, 0]], axis=1) > 0).mean() > 0.999
+
+
+@pytest.mark.parametrize("make", [lambda: cpt.scenes.box_room(res=(64, 48)), lambda: cpt.scenes.textured_room(res=(32, 32))])
+def test_scene_file_load_c_abi(tmp_path, make):
+    """cpt_scene_file_load: the C-ABI loader a reference-side renderer calls with
+    its settings' scene_name gives the same flattened scene as the host loader's
+    dump, field for field, with the camera set up at the resolution passed in."""
+    s = make()
+    settings = s.write_cpppt(str(tmp_path), spp=2)
+    r = subprocess.run([DUMP, settings, str(tmp_path / "d.bin")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    want, _ = _load_dump(str(tmp_path / "d.bin"))
+    rx, ry = s.resolution
+    got = cpt.load_scene_file(str(tmp_path / "scene.ptscene"), rx, ry)
+    for k in ("positions", "normals", "bitangents", "indices", "tri_material", "materials", "light_tris",
+              "camera", "env", "uvs", "textures", "texels", "material_textures", "point_lights"):
+        assert np.array_equal(got[k].reshape(-1), want[k].reshape(-1)), k
+    assert np.array_equal(got["positions"], s.c_arrays()["positions"])
+    # another film size changes only the camera's aspect and resolution entries
+    other = cpt.load_scene_file(str(tmp_path / "scene.ptscene"), 2 * rx, ry)
+    assert np.array_equal(other["camera"][:13], got["camera"][:13])
+    assert other["camera"][13] == np.float32(2 * rx) / np.float32(ry) and other["camera"][14] == 2 * rx
+
+
+def test_scene_file_load_errors(tmp_path):
+    with pytest.raises(cpt.CptError, match="Cannot open scene file"):
+        cpt.load_scene_file(str(tmp_path / "missing.ptscene"), 8, 8)
+    s = cpt.scenes.box_room(res=(8, 8))
+    s.write_cpppt(str(tmp_path))
+    with open(tmp_path / "scene.ptscene", "a") as fs:
+        fs.write("sphere 0 0 0 1\n")
+    with pytest.raises(cpt.CptError, match="Unknown scene obj: sphere"):
+        cpt.load_scene_file(str(tmp_path / "scene.ptscene"), 8, 8)
+    with pytest.raises(cpt.CptError, match="non-positive resolution"):
+        cpt.load_scene_file(str(tmp_path / "scene.ptscene"), 0, 8)
