@@ -276,7 +276,9 @@ def bench_guided(args):
 
 def bench_views(args):
     """64 camera views of the box room, resolutions from {256^2, 512x384, 1024x768,
-    1920x1080}, spp from {16, 64, 256}, rendered as one streaming batch."""
+    1920x1080}, spp from {16, 64, 256}, rendered as one batch of multi-view passes
+    (views scheduled shortest first); per-view completion times from device
+    timestamps after each view's last pass."""
     import numpy as np
 
     import cpt

@@ -46,6 +46,7 @@ ABI_FUNCTIONS = (
     "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
     "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export", "cpt_debug_fail_alloc",
     "cpt_trace_closest_device_noalloc", "cpt_scene_file_load", "cpt_scene_file_free",
+    "cpt_guide_mixtures",
 )
 
 
@@ -86,7 +87,8 @@ class RenderStats(C.Structure):
         ("ms_other", C.c_double), ("n_extend", C.c_int64), ("n_shade", C.c_int64),
         ("n_shadow", C.c_int64), ("passes", C.c_int32), ("paths_per_pass", C.c_int32),
         ("block_size", C.c_int32), ("grid_blocks", C.c_int32), ("guide_samples", C.c_int64),
-        ("guide_steps", C.c_int32), ("reserved_", C.c_int32), ("ms_guide", C.c_double), ("n_guide", C.c_int64),
+        ("guide_steps", C.c_int32), ("graph_captured", C.c_int32), ("ms_guide", C.c_double), ("n_guide", C.c_int64),
+        ("ms_train", C.c_double), ("n_train", C.c_int64),
     ]
 
     def as_dict(self):
@@ -149,6 +151,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_guide_get_params.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_guide_set_params.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_guide_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    lib.cpt_guide_mixtures.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     lib.cpt_guide_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
     _lib = lib
     return lib
@@ -434,6 +437,16 @@ def _guide_forward(self, inputs):
     return out
 
 
+def _guide_mixtures(self, inputs):
+    """Decoded mixtures of the per-bounce inference code: (6, lobes, n) floats, the
+    blocks weights, kappas, mean x, y, z, weight x vMF normalisation."""
+    x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 7)
+    lobes = self.guide_shape()[1]
+    out = np.zeros((6, lobes, x.shape[0]), np.float32)
+    self._check(self.lib.cpt_guide_mixtures(self.h, _ptr(x), x.shape[0], _ptr(out)), "cpt_guide_mixtures")
+    return out
+
+
 def _guide_train_step(self, samples, apply=True, want_grad=False):
     s = np.ascontiguousarray(samples, np.float32).reshape(-1, 11)
     loss = np.zeros(1, np.float32)
@@ -448,4 +461,5 @@ Context.guide_shape = _guide_shape
 Context.guide_params = _guide_params
 Context.guide_set_params = _guide_set_params
 Context.guide_forward = _guide_forward
+Context.guide_mixtures = _guide_mixtures
 Context.guide_train_step = _guide_train_step

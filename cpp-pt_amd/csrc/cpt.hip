@@ -72,6 +72,7 @@ struct Segment {
 };
 struct PassDesc {
     uint32_t n_seg, n_paths, n_pix, pass_id;
+    uint64_t seed;  // render seed: copied in per pass, so a captured graph serves every seed
     Segment seg[CPT_MAX_SEG];
 };
 
@@ -120,7 +121,8 @@ struct WfArgs {
     size_t lobes_stride;
     float* rec;              // training records: [slot][depth][REC_FLOATS]
     uint8_t* rec_flag;       // per record: 0 empty, 1 vertex recorded, 2 radiance after it known
-    uint32_t n_rec;          // slots that record
+    uint32_t n_rec;          // slots that record: every rec_stride-th slot, n_rec of them
+    uint32_t rec_stride;
     int rec_depth;
     float guide_frac;
     int guide_depth;         // vertices at depth >= this are not guided
@@ -230,6 +232,13 @@ __device__ __forceinline__ unsigned wave_sum(unsigned v) {
 }
 
 // ------------------------------------------------------------------ raygen
+// Training record of path slot `slot`: recording slots are spread evenly over the
+// pass (every rec_stride-th slot), so the samples cover the whole image
+__device__ __forceinline__ bool rec_index(const WfArgs& a, unsigned slot, unsigned& ri) {
+    ri = slot / a.rec_stride;
+    return ri < a.n_rec && ri * a.rec_stride == slot;
+}
+
 __device__ __forceinline__ int find_segment(const PassDesc* pd, unsigned i, bool by_pixel) {
     int lo = 0, hi = (int)pd->n_seg - 1;
     while (lo < hi) {
@@ -261,7 +270,7 @@ __global__ void k_raygen(WfArgs a) {
         unsigned sample = sg.s0 + sl;
         int px = (int)(pixel % (unsigned)vd.W), py = (int)(pixel / (unsigned)vd.W);
         uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)vd.spp + sample;
-        uint64_t rng = pcg_init(a.seed, path_id);
+        uint64_t rng = pcg_init(pd->seed, path_id);
         float r1 = pcg_float(rng);
         float r2 = pcg_float(rng);
         float cx, cy;
@@ -274,8 +283,9 @@ __global__ void k_raygen(WfArgs a) {
         a.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.rng[i] = rng;
         a.meta[i] = 1u << 8;  // depth 0, sampled_delta = true (camera ray)
-        if (i < a.n_rec)
-            for (int k = 0; k < a.rec_depth; k++) a.rec_flag[(size_t)i * a.rec_depth + k] = 0;
+        unsigned ri;
+        if (a.n_rec && rec_index(a, i, ri))
+            for (int k = 0; k < a.rec_depth; k++) a.rec_flag[(size_t)ri * a.rec_depth + k] = 0;
     }
 }
 
@@ -301,7 +311,7 @@ __global__ void k_raygen_lt(WfArgs a) {
         unsigned pixel = sg.p0 + pl;
         unsigned sample = sg.s0 + sl;
         uint64_t path_id = ((uint64_t)vd.id << 40) + (uint64_t)pixel * (uint64_t)vd.spp + sample;
-        uint64_t rng = pcg_init(a.seed, path_id);
+        uint64_t rng = pcg_init(pd->seed, path_id);
         V3 o, d, rad;
         float pdf;
         V3 mulv = {0.0f, 0.0f, 0.0f};
@@ -455,9 +465,25 @@ __device__ __forceinline__ void guide_hit_features(const DevScene& sc, V3 o, V3 
     guide::features_fast(sc.bmin, sc.bext, p.x, p.y, p.z, n.x, n.y, n.z, -d.x, -d.y, -d.z, in);
 }
 
-// WIDTH: 64 (namespace guide) or 128 (guide128); FP8: the e4m3 network (k_quant8's
-// weights) on FP8 MFMA, else BF16
-template <int WIDTH, bool FP8>
+// The guide's inputs for queue entry e (zeros for a miss)
+__device__ __forceinline__ void guide_entry_features(const WfArgs& a, int cur, unsigned e, float* in) {
+    for (int k = 0; k < guide::NIN; k++) in[k] = 0.0f;
+    const float4 HH = a.hits[e];
+    Hit h;
+    h.t = HH.x;
+    h.beta = HH.y;
+    h.gamma = HH.z;
+    h.tri = __float_as_int(HH.w);
+    if (h.tri < 0) return;
+    const float4 O = a.qo[cur][e], D = a.qd[cur][e];
+    const V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
+    const int slot = __float_as_int(D.w);
+    const bool frame = (a.meta[slot] & 0xffu) == 0 || a.corrected;
+    guide_hit_features(a.sc, o, d, h, frame, in);
+}
+
+// Per-bounce BF16 inference.  WIDTH: 64 (namespace guide) or 128 (guide128).
+template <int WIDTH>
 __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     constexpr int NOUT = WIDTH == 128 ? guide128::NOUT : guide::NOUT;
@@ -468,29 +494,16 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
     constexpr unsigned GROUP = 4 * guide::TILE;
     const unsigned count = a.counters[2 * cur];
     if (blockIdx.x * GROUP >= count) return;
-    if constexpr (WIDTH == 128) guide128::infer_load<FP8>(smem_raw, a.gwt, a.gwt8, a.gparams, a.ginv8);
-    else guide::infer_load<FP8>(smem_raw, a.gwt, a.gwt8, a.gparams, a.ginv8);
+    if constexpr (WIDTH == 128) guide128::load_weights(*reinterpret_cast<guide128::SmemFwd*>(smem_raw), a.gwt, a.gparams);
+    else guide::load_weights(*reinterpret_cast<guide::SmemFwd*>(smem_raw), a.gwt, a.gparams);
     for (unsigned g = blockIdx.x; g * GROUP < count; g += gridDim.x) {
         const unsigned eg = g * GROUP;
         const int ng = (int)min(GROUP, count - eg);
         {
             float* in = in_s + threadIdx.x * guide::NIN;
-            for (int k = 0; k < guide::NIN; k++) in[k] = 0.0f;
-            if ((int)threadIdx.x < ng) {
-                unsigned e = eg + threadIdx.x;
-                float4 O = a.qo[cur][e], D = a.qd[cur][e], HH = a.hits[e];
-                Hit h;
-                h.t = HH.x;
-                h.beta = HH.y;
-                h.gamma = HH.z;
-                h.tri = __float_as_int(HH.w);
-                if (h.tri >= 0) {
-                    const V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
-                    const int slot = __float_as_int(D.w);
-                    const bool frame = (a.meta[slot] & 0xffu) == 0 || a.corrected;
-                    guide_hit_features(a.sc, o, d, h, frame, in);
-                }
-            }
+            if ((int)threadIdx.x < ng) guide_entry_features(a, cur, eg + threadIdx.x, in);
+            else
+                for (int k = 0; k < guide::NIN; k++) in[k] = 0.0f;
         }
         __syncthreads();
         for (int q = 0; q * guide::TILE < ng; q++) {
@@ -499,9 +512,36 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
             const float* in = in_s + q * guide::TILE * guide::NIN;
             float* dst = a.lobes + e0;
             if constexpr (WIDTH == 128)
-                guide128::infer_tile<FP8>(smem_raw, a.gwt, o_tile, in, nt, dst, a.lobes_stride);
-            else guide::infer_tile<FP8>(smem_raw, a.gwt, o_tile, in, nt, dst, a.lobes_stride);
+                guide128::infer_tile(*reinterpret_cast<guide128::SmemFwd*>(smem_raw), a.gwt, o_tile, in, nt, dst,
+                                     a.lobes_stride);
+            else guide::infer_tile(*reinterpret_cast<guide::SmemFwd*>(smem_raw), a.gwt, o_tile, in, nt, dst,
+                                   a.lobes_stride);
         }
+    }
+}
+
+// Per-bounce FP8 inference (the e4m3 network of k_quant8): one copy of the weights
+// per block, then every wave on its own chunks of 64 queue entries: one entry's
+// inputs per lane, then four tiles of 16 (N::wave_infer8), with no block barrier.
+template <class N>
+__global__ __launch_bounds__(64 * N::kWaves8) void k_guide_infer8(WfArgs a, int cur) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    auto& sm = *reinterpret_cast<typename N::Smem8*>(smem_raw);
+    const unsigned count = a.counters[2 * cur];
+    constexpr unsigned W8 = N::kWaves8;
+    if (blockIdx.x * W8 * 64 >= count) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto& ws = sm.w[wave];
+    N::load8(sm, a.gwt8, a.gparams, a.ginv8);
+    __syncthreads();
+    for (unsigned c = blockIdx.x * W8 + wave; c * 64 < count; c += gridDim.x * W8) {
+        const unsigned e0 = c * 64;
+        const int nin = (int)min(64u, count - e0);
+        if (lane < nin) guide_entry_features(a, cur, e0 + lane, ws.in + lane * guide::NIN);
+        else
+            for (int k = 0; k < guide::NIN; k++) ws.in[lane * guide::NIN + k] = 0.0f;
+        N::sync8();
+        N::infer8(sm, ws, nin, lane, a.lobes + e0, a.lobes_stride);
     }
 }
 
@@ -509,7 +549,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(WfArgs a, int cur) {
 // sample (inputs, direction, weight = luminance of the radiance that came back
 // along the direction / the pdf it was sampled with).
 __global__ void k_guide_collect(WfArgs a) {
-    const unsigned n_slots = min(a.n_rec, a.pass->n_paths);
+    const unsigned n_slots = min(a.n_rec, (a.pass->n_paths + a.rec_stride - 1) / a.rec_stride);
     const unsigned total = n_slots * (unsigned)a.rec_depth;
     for (unsigned base = (gtid() >> 6) * 64; base < total; base += (gthreads() >> 6) * 64) {
         unsigned idx = base + __lane_id();
@@ -517,7 +557,7 @@ __global__ void k_guide_collect(WfArgs a) {
         float smp[guide::SAMPLE_FLOATS];
         if (idx < total && a.rec_flag[idx] == 2) {
             const float* r = a.rec + (size_t)idx * REC_FLOATS;
-            unsigned slot = idx / (unsigned)a.rec_depth;
+            const unsigned slot = idx / (unsigned)a.rec_depth * a.rec_stride;
             float4 c = a.col[slot];
             // radiance gathered through the sampled direction divided by the
             // throughput after the vertex is L_i; divided by the throughput
@@ -574,9 +614,12 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
             h.tri = __float_as_int(HH.w);
             bool alive = false;
             float* rec = nullptr;
-            if (GUIDED && (unsigned)slot < a.n_rec) {
-                rec = a.rec + (size_t)slot * a.rec_depth * REC_FLOATS;
-                uint8_t* fl = a.rec_flag + (size_t)slot * a.rec_depth;
+            uint8_t* rflag = nullptr;
+            unsigned ri;
+            if (GUIDED && a.n_rec && rec_index(a, (unsigned)slot, ri)) {
+                rec = a.rec + (size_t)ri * a.rec_depth * REC_FLOATS;
+                rflag = a.rec_flag + (size_t)ri * a.rec_depth;
+                uint8_t* fl = rflag;
                 if (depth > 0 && fl[depth - 1] == 1) {
                     float* r = rec + (depth - 1) * REC_FLOATS;
                     r[14] = colv.x;
@@ -692,7 +735,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                                     r[17] = mulv.x;
                                     r[18] = mulv.y;
                                     r[19] = mulv.z;
-                                    a.rec_flag[(size_t)slot * a.rec_depth + depth] = 1;
+                                    rflag[depth] = 1;
                                 }
                             }
                         }
@@ -1519,6 +1562,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.rec = c->g_rec;
     a.rec_flag = c->g_rec_flag;
     a.n_rec = 0;
+    a.rec_stride = 1;
     a.rec_depth = depth;
     a.guide_frac = c->g_cfg.guide_fraction;
     a.guide_depth = c->g_cfg.guide_depth > 0 ? c->g_cfg.guide_depth : 256;
@@ -1643,17 +1687,20 @@ void L_shadow(const LaunchCfg& L, const WfArgs& a, int nxt, hipStream_t s) {
 // Issue one pass worth of kernels on stream s.
 struct Timers {
     std::vector<hipEvent_t> ev;
-    std::vector<int> kind;  // 0 extend, 1 shade, 2 shadow, 3 other
+    std::vector<int> kind;  // 0 extend, 1 shade, 2 shadow, 3 other, 4 guide inference, 5 guide training
+    // interval ending here is of `kind` (-1: not counted)
+    void mark(int k, hipStream_t s) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        ev.push_back(e);
+        kind.push_back(k);
+    }
 };
 
 int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, Timers* tm) {
     auto mark = [&](int kind) {
-        if (!tm) return;
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        (void)hipEventRecord(e, s);
-        tm->ev.push_back(e);
-        tm->kind.push_back(kind);
+        if (tm) tm->mark(kind, s);
     };
     mark(-1);
     if (L.lt) hipLaunchKernelGGL(k_raygen_lt, dim3(L.G), dim3(L.B), 0, s, a);
@@ -1664,20 +1711,20 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
         L_extend(L, a, cur, s);
         mark(0);
         if (L.guided && b < a.guide_depth) {
-            // the width-64 FP8 tile state (77 KB of LDS) lets two blocks share a CU
+            // FP8: one block per CU, as many waves as the LDS holds (16 at width 64, 8 at 128)
             if (a.guide_lobes == 16) {
                 if (a.guide_fp8)
-                    hipLaunchKernelGGL((k_guide_infer<128, true>), dim3(L.infer_blocks), dim3(256),
-                                       sizeof(guide128::SmemFwd8), s, a, cur);
+                    hipLaunchKernelGGL(k_guide_infer8<guide128::Net>, dim3(L.infer_blocks),
+                                       dim3(64 * guide128::Net::kWaves8), guide128::Net::kSmemFwd8, s, a, cur);
                 else
-                    hipLaunchKernelGGL((k_guide_infer<128, false>), dim3(L.infer_blocks), dim3(256),
+                    hipLaunchKernelGGL(k_guide_infer<128>, dim3(L.infer_blocks), dim3(256),
                                        sizeof(guide128::SmemFwd), s, a, cur);
             } else if (a.guide_fp8) {
-                hipLaunchKernelGGL((k_guide_infer<64, true>), dim3(2 * L.infer_blocks), dim3(256),
-                                   sizeof(guide::SmemFwd8), s, a, cur);
+                hipLaunchKernelGGL(k_guide_infer8<guide::Net>, dim3(L.infer_blocks), dim3(64 * guide::Net::kWaves8),
+                                   guide::Net::kSmemFwd8, s, a, cur);
             } else {
-                hipLaunchKernelGGL((k_guide_infer<64, false>), dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd),
-                                   s, a, cur);
+                hipLaunchKernelGGL(k_guide_infer<64>, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a,
+                                   cur);
             }
             mark(4);
         }
@@ -1723,7 +1770,7 @@ static int* noalloc_scratch_of(int device, hipError_t* err) {
 
 extern "C" {
 
-int cpt_version(void) { return 1000; }
+int cpt_version(void) { return 1001; }
 
 int cpt_create(cpt_ctx** out, int device) {
     if (!out) return CPT_E_INVALID;
@@ -2194,7 +2241,15 @@ static void plan_passes(const std::vector<ViewDesc>& vd, int64_t cap, std::vecto
         pd.n_pix += npix;
         last_pass_of_view[v] = (int)passes.size() - 1;
     };
-    for (size_t v = 0; v < vd.size(); v++) {
+    // views in ascending order of path count (shortest job first): small views
+    // complete in the first passes instead of waiting behind large ones, which
+    // minimises the mean completion time; each view's samples still run in
+    // ascending order, so the films do not depend on the order
+    std::vector<size_t> order(vd.size());
+    for (size_t v = 0; v < vd.size(); v++) order[v] = v;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return vd[a].n_paths < vd[b].n_paths; });
+    for (size_t v : order) {
         int64_t P = (int64_t)vd[v].W * vd[v].H;
         int64_t S2 = (int64_t)vd[v].spp;
         if (P <= cap) {
@@ -2320,6 +2375,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     std::vector<PassDesc> passes;
     std::vector<int> last_pass;
     plan_passes(vd, cap, passes, last_pass);
+    for (auto& pd : passes) pd.seed = seed;
     size_t slots = 0;
     for (auto& pd : passes) slots = std::max<size_t>(slots, pd.n_paths);
     int r = ensure_wavefront(c, slots);
@@ -2417,6 +2473,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     }
     WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
     a.n_rec = (uint32_t)nrec;
+    a.rec_stride = nrec ? (uint32_t)std::max<size_t>(1, slots / nrec) : 1u;
     a.refill = (unsigned)refill;
     a.integrator = integrator;
     a.corrected = (flags & CPT_RENDER_CORRECTED) ? 1 : 0;
@@ -2435,12 +2492,13 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     bool timing = (flags & CPT_RENDER_KERNEL_TIMING) != 0;
     bool use_graph = !(flags & (CPT_RENDER_NO_GRAPH | CPT_RENDER_KERNEL_TIMING));
     Timers tm;
+    bool captured = false;
     if (use_graph) {
         char key[512];
         snprintf(key, sizeof(key),
-                 "%d,%llu,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%p,%p,%p,%g,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
-                 max_depth, (unsigned long long)seed, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
-                 (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, (void*)c->g_lobes, (void*)c->g_rec,
+                 "%d,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%u,%p,%p,%p,%g,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
+                 max_depth, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
+                 (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, a.rec_stride, (void*)c->g_lobes, (void*)c->g_rec,
                  (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator,
                  a.guide_lobes * 4 + a.corrected * 2 + a.guide_fp8, a.lobes_stride,
                  (void*)c->sd_snode, (void*)c->sd_dinfo, (void*)c->sd_quad, (void*)c->sd_rect, (void*)c->sd_sum,
@@ -2458,6 +2516,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
             (void)hipGraphDestroy(g);
             if (ei != hipSuccess) return fail(c, CPT_E_HIP, "graph instantiate failed: %s", hipGetErrorString(ei));
             c->graph_key = key;
+            captured = true;
         }
     }
     for (size_t i = 0; i < passes.size(); i++) {
@@ -2470,6 +2529,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         }
         if (L.ppg) hipLaunchKernelGGL(k_sd_splat, dim3(G), dim3(B), 0, c->stream, a, c->h_pass[i].n_paths);
         if (train) {
+            if (timing) tm.mark(-1, c->stream);
             HIPCHK(c, hipMemsetAsync(c->g_count, 0, sizeof(uint32_t), c->stream));
             hipLaunchKernelGGL(k_guide_collect, dim3(G), dim3(B), 0, c->stream, a);
             HIPCHK(c, hipMemcpyAsync(c->h_count, c->g_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -2483,6 +2543,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
                 }
                 c->g_samples_total += n;
             }
+            if (timing) tm.mark(5, c->stream);
         }
     }
     unsigned long long st[ST_N];
@@ -2511,6 +2572,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         stats->grid_blocks = G;
         stats->guide_samples = c->g_samples_total;
         stats->guide_steps = c->g_steps_total;
+        stats->graph_captured = captured ? 1 : 0;
         for (size_t i = 1; i < tm.ev.size(); i++) {
             float ms = 0;
             if (tm.kind[i] >= 0 && hipEventElapsedTime(&ms, tm.ev[i - 1], tm.ev[i]) == hipSuccess) {
@@ -2519,6 +2581,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
                     case 1: stats->ms_shade += ms; stats->n_shade++; break;
                     case 2: stats->ms_shadow += ms; stats->n_shadow++; break;
                     case 4: stats->ms_guide += ms; stats->n_guide++; break;
+                    case 5: stats->ms_train += ms; stats->n_train++; break;
                     default: stats->ms_other += ms; break;
                 }
             }
@@ -3022,9 +3085,13 @@ int guide_alloc(cpt_ctx* c) {
     HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd));
     HIPCHK(c, hipFuncSetAttribute((const void*)N::train_tile, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd));
     HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_forward8, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd8));
-    HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<N::kHid, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<N::kHid>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   N::kSmemFwd));
-    HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<N::kHid, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer8<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  N::kSmemFwd8));
+    HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_mixtures, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  N::kSmemFwd));
+    HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_mixtures8, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   N::kSmemFwd8));
     c->g_width = N::kHid;
     return CPT_OK;
@@ -3077,8 +3144,9 @@ template <class N>
 int guide_forward_impl(cpt_ctx* c, const float* d_in, int n, float* d_out) {
     int G = std::min(c->cus, (n + N::kTile - 1) / N::kTile);
     if (c->g_cfg.infer_fp8)
-        hipLaunchKernelGGL(N::mlp_forward8, dim3(G), dim3(256), N::kSmemFwd8, c->stream, c->g_wt8, c->g_params,
-                           c->g_inv8, d_in, n, d_out);
+        hipLaunchKernelGGL(N::mlp_forward8, dim3(std::min(c->cus, (n + 16 * N::kWaves8 - 1) / (16 * N::kWaves8))),
+                           dim3(64 * N::kWaves8), N::kSmemFwd8, c->stream, c->g_wt8, c->g_params, c->g_inv8, d_in, n,
+                           d_out);
     else
         hipLaunchKernelGGL(N::mlp_forward, dim3(G), dim3(256), N::kSmemFwd, c->stream, c->g_wt, c->g_params, d_in, n,
                            d_out);
@@ -3227,6 +3295,53 @@ int cpt_guide_forward(cpt_ctx* c, const float* inputs, int32_t n, float* outputs
     }
     (void)hipFree(d_in);
     (void)hipFree(d_out);
+    return rc;
+}
+
+int cpt_guide_mixtures(cpt_ctx* c, const float* inputs, int32_t n, float* mixtures) {
+    if (!c || n < 0 || (n > 0 && (!inputs || !mixtures))) return CPT_E_INVALID;
+    if (!c->has_guide) return fail(c, CPT_E_INVALID, "no guide: call cpt_guide_init");
+    if (n == 0) return CPT_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool wide = c->g_width == 128;
+    const size_t nin = guide::NIN, nmix = 6 * (size_t)(wide ? guide128::NLOBES : guide::NLOBES);
+    float *d_in = nullptr, *d_mix = nullptr;
+    HIPCHK(c, hipMalloc((void**)&d_in, sizeof(float) * nin * (size_t)n));
+    if (hipMalloc((void**)&d_mix, sizeof(float) * nmix * (size_t)n) != hipSuccess) {
+        (void)hipFree(d_in);
+        return fail(c, CPT_E_HIP, "hipMalloc failed");
+    }
+    int rc = CPT_OK;
+    if (hipMemcpyAsync(d_in, inputs, sizeof(float) * nin * (size_t)n, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+        rc = fail(c, CPT_E_HIP, "upload failed");
+    if (!rc) {
+        const size_t stride = (size_t)n;
+        if (c->g_cfg.infer_fp8) {
+            const int w8 = wide ? guide128::Net::kWaves8 : guide::Net::kWaves8;
+            const int G = std::min(c->cus, (n + 64 * w8 - 1) / (64 * w8));
+            if (wide)
+                hipLaunchKernelGGL(guide128::Net::mlp_mixtures8, dim3(G), dim3(64 * w8), guide128::Net::kSmemFwd8,
+                                   c->stream, c->g_wt8, c->g_params, c->g_inv8, d_in, n, d_mix, stride);
+            else
+                hipLaunchKernelGGL(guide::Net::mlp_mixtures8, dim3(G), dim3(64 * w8), guide::Net::kSmemFwd8, c->stream,
+                                   c->g_wt8, c->g_params, c->g_inv8, d_in, n, d_mix, stride);
+        } else {
+            const int G = std::min(c->cus, (n + guide::TILE - 1) / guide::TILE);
+            if (wide)
+                hipLaunchKernelGGL(guide128::Net::mlp_mixtures, dim3(G), dim3(256), guide128::Net::kSmemFwd, c->stream,
+                                   c->g_wt, c->g_params, d_in, n, d_mix, stride);
+            else
+                hipLaunchKernelGGL(guide::Net::mlp_mixtures, dim3(G), dim3(256), guide::Net::kSmemFwd, c->stream,
+                                   c->g_wt, c->g_params, d_in, n, d_mix, stride);
+        }
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(mixtures, d_mix, sizeof(float) * nmix * (size_t)n, hipMemcpyDeviceToHost, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = fail(c, CPT_E_HIP, "guide mixtures failed");
+    }
+    (void)hipFree(d_in);
+    (void)hipFree(d_mix);
     return rc;
 }
 

@@ -430,13 +430,42 @@ __device__ __forceinline__ uint32_t fp8x2(float a, float b) {
 }
 __device__ __forceinline__ uint8_t fp8x1(float a) { return (uint8_t)(fp8x2(a, 0.0f) & 0xffu); }
 
+// FP8 inference runs one wave per tile of 16 samples: the block holds one copy
+// of the FP8 weights in LDS and each wave its own tile state, so after the weight
+// load no wave waits for another.  WAVES8 waves per block: as many as the LDS
+// left beside the weights holds (16 at width 64: 140 KB; 8 at width 128: 157 KB).
+constexpr int WAVES8 = HID <= 64 ? 16 : 8;
+constexpr int LPG = NLOBES / 4;           // lobes per 16-lane group of the output tile
+constexpr int PADG = 4 * NTO - 5 * LPG;   // output slots per group beyond its lobes
+static_assert(NLOBES % 4 == 0 && PADG >= 0 && NOUT == 5 * NLOBES + 4 * PADG, "output tile layout");
+
+// Output-layer row order of the FP8 copy.  Lane l of the output MFMA holds
+// features 16 nt + 4 (l / 16) + j (nt < NTO, j < 4) of one sample; in this order the
+// lanes of group g = l / 16 hold the 5 outputs of lobes g LPG .. g LPG + LPG - 1
+// (then PADG unused outputs), so the mixture is decoded in registers.
+// out_orig(f): network output stored at feature f; out_slot: its inverse.
+__host__ __device__ constexpr int out_orig(int f) {
+    const int g = (f % 16) / 4, q = 4 * (f / 16) + f % 4;
+    return q < 5 * LPG ? 5 * (g * LPG + q / 5) + q % 5 : 5 * NLOBES + g * PADG + (q - 5 * LPG);
+}
+__host__ __device__ constexpr int out_slot(int n) {
+    const int g = n < 5 * NLOBES ? n / 5 / LPG : (PADG > 0 ? (n - 5 * NLOBES) / PADG : 0);
+    const int q = n < 5 * NLOBES ? 5 * (n / 5 % LPG) + n % 5 : 5 * LPG + (PADG > 0 ? (n - 5 * NLOBES) % PADG : 0);
+    return 16 * (q / 4) + 4 * g + q % 4;
+}
+
+struct WaveState8 {
+    float in[64 * NIN];   // inputs of up to 64 samples (one per lane when gathered)
+    uint8_t x0[16 * LQ0]; // one-blob features of the current 16 rows, x 448
+    uint8_t h[16 * LQH];  // hidden activations; each layer overwrites its input
+};
 struct SmemFwd8 {
     uint8_t wt[NWQ];
-    float bias[4 * HID + NOUT];
-    float inv_w[8];       // 1 / weight scale per layer (5 used)
-    uint8_t x0[TILE * LQ0];
-    uint8_t h[2][TILE * LQH];
+    float bias[4 * HID + NOUT];  // output-layer biases in feature order (out_orig)
+    float inv_w[8];              // 1 / weight scale per layer (5 used)
+    WaveState8 w[WAVES8];
 };
+static_assert(sizeof(SmemFwd8) <= 160 * 1024, "FP8 inference LDS");
 
 __device__ __forceinline__ void load_weights8(SmemFwd8& sm, const uint8_t* gwt8, const float* params,
                                               const float* inv_scale) {
@@ -445,24 +474,33 @@ __device__ __forceinline__ void load_weights8(SmemFwd8& sm, const uint8_t* gwt8,
     for (int i = threadIdx.x; i < NWQ / 16; i += blockDim.x) dst[i] = src[i];
     for (int i = threadIdx.x; i < 4 * HID + NOUT; i += blockDim.x) {
         int off = i < HID ? B0 + i : i < 2 * HID ? B1 + i - HID : i < 3 * HID ? B2 + i - 2 * HID
-                : i < 4 * HID ? B3 + i - 3 * HID : B4 + i - 4 * HID;
+                : i < 4 * HID ? B3 + i - 3 * HID : B4 + out_orig(i - 4 * HID);
         sm.bias[i] = params[off];
     }
     if (threadIdx.x < 5) sm.inv_w[threadIdx.x] = inv_scale[threadIdx.x];
 }
 
-// one-blob features x 448 as FP8: per (sample, input) pair the 32 bins are zeroed
-// and the 16 of blob_window written (the others are 0 in e4m3), two exponentials
-// for the run (blob_run)
-__device__ __forceinline__ void encode_tile8(SmemFwd8& sm, const float* in, int n) {
+// the lanes of a wave see each other's LDS writes made before this point, and the
+// compiler moves no LDS access across it
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one-blob features x 448 as FP8 for the wave's 16 rows (inputs at in16, rows
+// beyond nrows zero): per (sample, input) pair the 32 bins are zeroed and the 16
+// of blob_window written (the others are 0 in e4m3), two exponentials per run of 8
+// (blob_run)
+__device__ __forceinline__ void encode_rows8(uint8_t* x0, const float* in16, int nrows, int lane) {
     static_assert(NBINS == 32 && LQ0 % 8 == 0, "8-byte zero stores, 4-byte window stores");
-    for (int p = threadIdx.x; p < TILE * NIN; p += blockDim.x) {
+    for (int p = lane; p < 16 * NIN; p += 64) {
         const int s = p / NIN, d = p - s * NIN;
-        uint8_t* seg = sm.x0 + s * LQ0 + d * NBINS;
+        uint8_t* seg = x0 + s * LQ0 + d * NBINS;
 #pragma unroll
         for (int k = 0; k < NBINS / 8; k++) reinterpret_cast<uint2*>(seg)[k] = make_uint2(0u, 0u);
-        if (s < n) {
-            const float x = in[s * NIN + d];
+        if (s < nrows) {
+            const float x = in16[s * NIN + d];
             const int w0 = blob_window(x);
             // two runs of 8, so that no run starts farther than 9.5 bins from its
             // peak-side end (a longer run from a clamped window start could begin in
@@ -526,19 +564,22 @@ __device__ __forceinline__ uint32_t fp8x4_pos(float v0, float v1, float v2, floa
     return (lo & 0xffffu) | (hi << 16);
 }
 
-// forward through all layers with FP8 operands; out as forward_tile
-__device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r0 = 16 * wave;
-    const int m = r0 + (lane & 15);      // this lane's sample row
-    const int n4 = 4 * (lane >> 4);      // its first feature within a 16-wide tile
-    float inv_a = 1.0f / FP8_MAX;  // inverse scale of the current layer's input
+// Forward pass of the wave's 16 rows (inputs at in16, nrows valid) through all
+// layers with FP8 operands.  o: raw outputs in the output MFMA's layout, lane l
+// holding features 16 nt + 4 (l / 16) + j of sample row l % 16 (out_orig order).
+__device__ __forceinline__ void wave_forward8(const SmemFwd8& sm, WaveState8& ws, const float* in16, int nrows,
+                                              int lane, f4 (&o)[NTO]) {
+    encode_rows8(ws.x0, in16, nrows, lane);
+    wave_sync();
+    const int m = lane & 15;         // this lane's sample row
+    const int n4 = 4 * (lane >> 4);  // its first feature within a 16-wide tile
+    float inv_a = 1.0f / FP8_MAX;    // inverse scale of the current layer's input
 #pragma unroll
     for (int l = 0; l < 4; l++) {
         f4 acc[NTH] = {};
-        if (l == 0) gemm8_rows_nk_t<K0, NTH>(sm.x0 + r0 * LQ0, LQ0, sm.wt + Q0, LQ0, acc, lane);
-        else gemm8_rows_nk_t<HID, NTH>(sm.h[(l - 1) & 1] + r0 * LQH, LQH, sm.wt + (l == 1 ? Q1 : l == 2 ? Q2 : Q3),
-                                       LQH, acc, lane);
+        if (l == 0) gemm8_rows_nk_t<K0, NTH>(ws.x0, LQ0, sm.wt + Q0, LQ0, acc, lane);
+        else gemm8_rows_nk_t<HID, NTH>(ws.h, LQH, sm.wt + (l == 1 ? Q1 : l == 2 ? Q2 : Q3), LQH, acc, lane);
+        wave_sync();  // every lane has read its operand rows before any overwrites them
         const float sc = inv_a * sm.inv_w[l];
         float mx = 0.0f;
 #pragma unroll
@@ -552,38 +593,28 @@ __device__ __forceinline__ void forward_tile8(SmemFwd8& sm, float* out) {
                 mx = fmaxf(mx, z);
             }
         }
-        // one scale per wave: its 16 rows are the whole input of its next layer, so
-        // the layers of a tile need no block barrier
+        // one scale per wave (16 rows)
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
         const float s = mx > 0.0f ? FP8_MAX / mx : 1.0f;
         inv_a = mx > 0.0f ? mx / FP8_MAX : 1.0f;
-        uint8_t* hd = sm.h[l & 1] + m * LQH + n4;
+        uint8_t* hd = ws.h + m * LQH + n4;
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
             *reinterpret_cast<uint32_t*>(hd + nt * 16) =
                 fp8x4_pos(acc[nt][0] * s, acc[nt][1] * s, acc[nt][2] * s, acc[nt][3] * s);
-        // the next layer reads what the other lanes of this wave just wrote
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();  // the next layer reads what the other lanes just wrote
     }
-    {
-        f4 acc[NTO] = {};
-        gemm8_rows_nk_t<HID, NTO>(sm.h[1] + r0 * LQH, LQH, sm.wt + Q4, LQH, acc, lane);
-        const float sc = inv_a * sm.inv_w[4];
 #pragma unroll
-        for (int nt = 0; nt < NTO; nt++) {
-            const float4 bv = *reinterpret_cast<const float4*>(sm.bias + 4 * HID + nt * 16 + n4);
-            *reinterpret_cast<float4*>(out + m * NOUT + nt * 16 + n4) =
-                make_float4(fmaf(acc[nt][0], sc, bv.x), fmaf(acc[nt][1], sc, bv.y), fmaf(acc[nt][2], sc, bv.z),
-                            fmaf(acc[nt][3], sc, bv.w));
-        }
+    for (int nt = 0; nt < NTO; nt++) o[nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    gemm8_rows_nk_t<HID, NTO>(ws.h, LQH, sm.wt + Q4, LQH, o, lane);
+    const float sc = inv_a * sm.inv_w[4];
+#pragma unroll
+    for (int nt = 0; nt < NTO; nt++) {
+        const float4 bv = *reinterpret_cast<const float4*>(sm.bias + 4 * HID + nt * 16 + n4);
+        o[nt] = f4{fmaf(o[nt][0], sc, bv.x), fmaf(o[nt][1], sc, bv.y), fmaf(o[nt][2], sc, bv.z),
+                   fmaf(o[nt][3], sc, bv.w)};
     }
-    // out holds this wave's 16 rows; a reader of other waves' rows needs a block
-    // barrier of its own
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();  // the wave's state may be overwritten from here on
 }
 
 // FP8 copy of the weights: one block per layer; inv_scale[l] = max|W_l| / 448
@@ -607,7 +638,7 @@ __global__ __launch_bounds__(256) void k_quant8(const float* params, uint8_t* gw
     if (threadIdx.x == 0) inv_scale[l] = M > 0.0f ? M / FP8_MAX : 1.0f;
     for (int i = threadIdx.x; i < K * N; i += blockDim.x) {
         const int k = i / N, n = i - k * N;
-        gwt8[q + n * ld + k] = fp8x1(params[off + i] * sc);
+        gwt8[q + (l == 4 ? out_slot(n) : n) * ld + k] = fp8x1(params[off + i] * sc);
     }
     for (int i = threadIdx.x; i < N * (ld - K); i += blockDim.x) {
         const int n = i / (ld - K), k = K + (i - n * (ld - K));
@@ -615,22 +646,29 @@ __global__ __launch_bounds__(256) void k_quant8(const float* params, uint8_t* gw
     }
 }
 
-// FP8 inference over n input vectors: raw outputs (n x NOUT)
-__global__ __launch_bounds__(256) void k_mlp_forward8(const uint8_t* gwt8, const float* params, const float* inv_scale,
-                                                      const float* in, int n, float* out) {
+// FP8 inference over n input vectors: raw outputs (n x NOUT, network order), one
+// wave per tile of 16 consecutive inputs
+__global__ __launch_bounds__(64 * WAVES8) void k_mlp_forward8(const uint8_t* gwt8, const float* params,
+                                                              const float* inv_scale, const float* in, int n,
+                                                              float* out) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     SmemFwd8& sm = *reinterpret_cast<SmemFwd8*>(smem_raw);
-    __shared__ __align__(16) float o_tile[TILE * NOUT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    WaveState8& ws = sm.w[wave];
     load_weights8(sm, gwt8, params, inv_scale);
     __syncthreads();
-    for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
-        int nt = min(TILE, n - t * TILE);
-        encode_tile8(sm, in + (size_t)t * TILE * NIN, nt);
-        __syncthreads();
-        forward_tile8(sm, o_tile);
-        __syncthreads();
-        for (int i = threadIdx.x; i < nt * NOUT; i += blockDim.x) out[(size_t)t * TILE * NOUT + i] = o_tile[i];
-        __syncthreads();
+    for (int t = blockIdx.x * WAVES8 + wave; t * 16 < n; t += gridDim.x * WAVES8) {
+        const int nr = min(16, n - t * 16);
+        for (int i = lane; i < 16 * NIN; i += 64) ws.in[i] = i < nr * NIN ? in[(size_t)t * 16 * NIN + i] : 0.0f;
+        wave_sync();
+        f4 o[NTO];
+        wave_forward8(sm, ws, ws.in, nr, lane, o);
+        const int m = lane & 15, n4 = 4 * (lane >> 4);
+        if (m < nr)
+#pragma unroll
+            for (int nt = 0; nt < NTO; nt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) out[((size_t)t * 16 + m) * NOUT + out_orig(nt * 16 + n4 + j)] = o[nt][j];
     }
 }
 
@@ -981,35 +1019,113 @@ __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* 
     }
 }
 
-// The path tracer's per-bounce inference: weights into LDS once per block, then per
-// tile of up to TILE samples (NIN features each at `in`) the forward pass and the
-// decoded mixtures, sample s's lobe k at dst[{k, NLOBES + k, ...} stride + s]
-// (weight, kappa, mean x, y, z, weight x normalisation).  FP8: the e4m3 network of
-// k_quant8.  The decode reads only rows its own wave computed (4 lanes per sample,
-// 16 samples per wave), so after the FP8 forward it needs no block barrier.
-template <bool FP8>
-__device__ __forceinline__ void infer_load(unsigned char* smem, const uint16_t* gwt, const uint8_t* gwt8,
-                                           const float* params, const float* inv8) {
-    if constexpr (FP8) load_weights8(*reinterpret_cast<SmemFwd8*>(smem), gwt8, params, inv8);
-    else load_weights(*reinterpret_cast<SmemFwd*>(smem), gwt, params);
-}
-
-template <bool FP8>
-__device__ __forceinline__ void infer_tile(unsigned char* smem, const uint16_t* gwt, float* o_tile, const float* in,
-                                           int nt, float* dst, size_t stride) {
-    if constexpr (FP8) {
-        SmemFwd8& sm = *reinterpret_cast<SmemFwd8*>(smem);
-        encode_tile8(sm, in, nt);
-        __syncthreads();
-        forward_tile8(sm, o_tile);
-    } else {
-        SmemFwd& sm = *reinterpret_cast<SmemFwd*>(smem);
-        encode_tile_window(sm, in, nt);
-        __syncthreads();
-        forward_tile(sm, WT_LDS ? sm.wt : gwt, o_tile);
-    }
+// The path tracer's per-bounce inference, BF16: weights into LDS once per block,
+// then per tile of up to TILE samples (NIN features each at `in`) the forward pass
+// and the decoded mixtures, sample s's lobe k at dst[{k, NLOBES + k, ...} stride + s]
+// (weight, kappa, mean x, y, z, weight x normalisation).
+__device__ __forceinline__ void infer_tile(SmemFwd& sm, const uint16_t* gwt, float* o_tile, const float* in, int nt,
+                                           float* dst, size_t stride) {
+    encode_tile_window(sm, in, nt);
+    __syncthreads();
+    forward_tile(sm, WT_LDS ? sm.wt : gwt, o_tile);
     decode_tile(o_tile, nt, dst, stride);
     __syncthreads();
+}
+
+// FP8: the mixtures of a wave's 16 rows from wave_forward8's registers, with the
+// same arithmetic as decode_tile: the lanes of group g = lane / 16 hold lobes
+// g LPG .. g LPG + LPG - 1 of sample row lane % 16, and the softmax maximum and sum
+// are combined across the 4 groups (the same pairs as decode_tile's 4 threads).
+__device__ __forceinline__ void wave_decode(const f4 (&o)[NTO], int lane, int nrows, float* dst, size_t stride) {
+    const int g = lane >> 4, row = lane & 15;
+    float v[4 * NTO];
+#pragma unroll
+    for (int nt = 0; nt < NTO; nt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[4 * nt + j] = o[nt][j];
+    float mxv = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < LPG; i++) mxv = fmaxf(mxv, v[5 * i]);
+    mxv = fmaxf(mxv, __shfl_xor(mxv, 16));
+    mxv = fmaxf(mxv, __shfl_xor(mxv, 32));
+    float e[LPG], sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < LPG; i++) {
+        e[i] = __expf(v[5 * i] - mxv);
+        sum += e[i];
+    }
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+    if (row >= nrows) return;
+    const float inv = 1.0f / sum;
+    float* d = dst + row;
+#pragma unroll
+    for (int i = 0; i < LPG; i++) {
+        const int k = g * LPG + i;
+        float x = v[5 * i + 2], y = v[5 * i + 3], z = v[5 * i + 4];
+        float nrm = sqrtf(x * x + y * y + z * z);
+        if (nrm < 1e-12f) {
+            x = 0.0f;
+            y = 0.0f;
+            z = 1.0f;
+            nrm = 1.0f;
+        }
+        const float pi = e[i] * inv;
+        const float kappa = __expf(fminf(fmaxf(v[5 * i + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
+        d[k * stride] = pi;
+        d[(NLOBES + k) * stride] = kappa;
+        d[(2 * NLOBES + k) * stride] = x / nrm;
+        d[(3 * NLOBES + k) * stride] = y / nrm;
+        d[(4 * NLOBES + k) * stride] = z / nrm;
+        d[(5 * NLOBES + k) * stride] = pi * kappa / (TWO_PI * (1.0f - __expf(-2.0f * kappa)));
+    }
+}
+
+// FP8 mixtures of up to 64 samples whose inputs are in ws.in (nin valid), by one
+// wave, four tiles of 16; sample s at dst + s as decode_tile
+__device__ __forceinline__ void wave_infer8(const SmemFwd8& sm, WaveState8& ws, int nin, int lane, float* dst,
+                                            size_t stride) {
+    for (int q = 0; q < 4 && 16 * q < nin; q++) {
+        const int nr = min(16, nin - 16 * q);
+        f4 o[NTO];
+        wave_forward8(sm, ws, ws.in + 16 * q * NIN, nr, lane, o);
+        wave_decode(o, lane, nr, dst + 16 * q, stride);
+    }
+}
+
+// FP8 mixtures of n input vectors (n x NIN) into dst (value v of sample s at
+// dst[v * stride + s]), the code the path tracer runs per bounce after its own
+// gather of the inputs; for tests of the mixtures it produces
+__global__ __launch_bounds__(64 * WAVES8) void k_mlp_mixtures8(const uint8_t* gwt8, const float* params,
+                                                               const float* inv_scale, const float* in, int n,
+                                                               float* dst, size_t stride) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    SmemFwd8& sm = *reinterpret_cast<SmemFwd8*>(smem_raw);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    WaveState8& ws = sm.w[wave];
+    load_weights8(sm, gwt8, params, inv_scale);
+    __syncthreads();
+    for (int c = blockIdx.x * WAVES8 + wave; c * 64 < n; c += gridDim.x * WAVES8) {
+        const int nin = min(64, n - c * 64);
+        for (int i = lane; i < 64 * NIN; i += 64) ws.in[i] = i < nin * NIN ? in[(size_t)c * 64 * NIN + i] : 0.0f;
+        wave_sync();
+        wave_infer8(sm, ws, nin, lane, dst + (size_t)c * 64, stride);
+    }
+}
+
+// BF16 mixtures of n input vectors, as k_mlp_mixtures8 (the per-bounce BF16 path's
+// encode, forward and decode)
+__global__ __launch_bounds__(256) void k_mlp_mixtures(const uint16_t* gwt, const float* params, const float* in, int n,
+                                                      float* dst, size_t stride) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    SmemFwd& sm = *reinterpret_cast<SmemFwd*>(smem_raw);
+    __shared__ __align__(16) float o_tile[TILE * NOUT];
+    load_weights(sm, gwt, params);
+    __syncthreads();
+    for (int t = blockIdx.x; t * TILE < n; t += gridDim.x) {
+        const int nt = min(TILE, n - t * TILE);
+        infer_tile(sm, gwt, o_tile, in + (size_t)t * TILE * NIN, nt, dst + (size_t)t * TILE, stride);
+    }
 }
 
 // What the host code needs of this width, for code templated on the network.
@@ -1018,8 +1134,21 @@ struct Net {
                          kRow = ROW, kTile = TILE, kChunk = CH, kDwGroups = DW_GROUPS;
     static constexpr int kW[5] = {W0, W1, W2, W3, W4}, kB4 = B4;
     static constexpr size_t kSmemFwd = sizeof(SmemFwd), kSmemFwd8 = sizeof(SmemFwd8);
+    static constexpr int kWaves8 = WAVES8;
+    using Smem8 = SmemFwd8;
+    static __device__ __forceinline__ void load8(SmemFwd8& sm, const uint8_t* gwt8, const float* params,
+                                                 const float* inv8) {
+        load_weights8(sm, gwt8, params, inv8);
+    }
+    static __device__ __forceinline__ void sync8() { wave_sync(); }
+    static __device__ __forceinline__ void infer8(const SmemFwd8& sm, WaveState8& ws, int nin, int lane, float* dst,
+                                                  size_t stride) {
+        wave_infer8(sm, ws, nin, lane, dst, stride);
+    }
     static constexpr auto mlp_forward = &k_mlp_forward;
     static constexpr auto mlp_forward8 = &k_mlp_forward8;
+    static constexpr auto mlp_mixtures = &k_mlp_mixtures;
+    static constexpr auto mlp_mixtures8 = &k_mlp_mixtures8;
     static constexpr auto train_tile = &k_train_tile;
     static constexpr auto train_dw = &k_train_dw;
     static constexpr auto reduce_partials = &k_reduce_partials;

@@ -180,9 +180,12 @@ typedef struct {
     int32_t grid_blocks;
     int64_t guide_samples;    /* CPT_RENDER_GUIDE_TRAIN: training samples used */
     int32_t guide_steps;      /* CPT_RENDER_GUIDE_TRAIN: Adam steps taken */
-    int32_t reserved;
+    int32_t graph_captured;   /* 1 if this call captured and instantiated a new pass graph */
     double ms_guide;          /* CPT_RENDER_KERNEL_TIMING, guided renders: guide inference */
     int64_t n_guide;
+    double ms_train;          /* CPT_RENDER_KERNEL_TIMING with GUIDE_TRAIN: sample collection and
+                                 Adam steps between passes (ABI 1.1) */
+    int64_t n_train;          /* passes whose training was timed */
 } cpt_render_stats;
 
 /* Create a context on HIP device `device`.  Fails with CPT_E_NODEVICE when the
@@ -349,6 +352,12 @@ int cpt_guide_set_params(cpt_ctx* ctx, const float* params_in);
 /* Raw network outputs (n x 48, or n x 80 at width 128) for n input vectors (n x 7
  * floats in [0,1]). */
 int cpt_guide_forward(cpt_ctx* ctx, const float* inputs, int32_t n, float* outputs);
+/* Decoded mixtures for n input vectors, computed by the code guided renders run per
+ * bounce once they have gathered a hit's inputs (its encode, forward pass in the
+ * configured precision, and decode).  mixtures_out: 6 x lobes x n floats, value v of
+ * sample s at v * n + s, in blocks of `lobes`: weights (softmax), kappas, mean x,
+ * mean y, mean z, weight x vMF normalisation kappa / (2 pi (1 - exp(-2 kappa))). */
+int cpt_guide_mixtures(cpt_ctx* ctx, const float* inputs, int32_t n, float* mixtures_out);
 /* One training step on n samples (n x 11 floats: 7 inputs, direction xyz, weight =
  * target radiance / sampling pdf).  Loss = sum_i w_i (-log q(dir_i)) / sum_i w_i.
  * loss_out (may be NULL) = that loss; grad_out (may be NULL, one float per

@@ -137,3 +137,39 @@ def test_guide_shape_errors(ctx):
     ctx.guide_init(seed=1)
     assert ctx.guide_shape() == (64, 8, 48, 30000)
     assert ctx.guide_forward(np.zeros((3, 7), np.float32)).shape == (3, 48)
+
+
+@pytest.mark.parametrize("width,fp8", [(64, False), (64, True), (128, False), (128, True)])
+def test_render_mixtures_match_decoded_forward(ctx, width, fp8):
+    """The mixtures guided renders compute per bounce (encode, forward and decode in
+    one kernel; FP8: one wave per 16 samples with the decode in registers) against
+    the decode of cpt_guide_forward's raw outputs: softmax weights, clamped log
+    kappa, normalised means, the stored weight x vMF normalisation, and the mixture
+    pdf against the oracle's at random directions."""
+    ctx.guide_init(seed=3, infer_fp8=fp8, width=width)
+    P = random_params(11, width=width)
+    ctx.guide_set_params(P)
+    rng = np.random.default_rng(5)
+    X = rng.random((4133, 7)).astype(np.float32)   # ragged: not a multiple of 16 or 64
+    mix = ctx.guide_mixtures(X).astype(np.float64)
+    o = ctx.guide_forward(X).astype(np.float64)
+    L = 8 if width == 64 else 16
+    assert mix.shape == (6, L, len(X))
+    lo = o[:, 0:5 * L:5]
+    w = np.exp(lo - lo.max(axis=1, keepdims=True))
+    w /= w.sum(axis=1, keepdims=True)
+    kap = np.exp(np.clip(o[:, 1:5 * L:5], -3.0, 6.0))
+    mu = np.stack([o[:, 2:5 * L:5], o[:, 3:5 * L:5], o[:, 4:5 * L:5]], axis=0)
+    mu /= np.linalg.norm(mu, axis=0, keepdims=True)
+    norm = w * kap / (2 * np.pi * (1 - np.exp(-2 * kap)))
+    for v, ref in enumerate([w, kap, mu[0], mu[1], mu[2], norm]):
+        err = np.abs(mix[v].T - ref).max() / max(np.abs(ref).max(), 1e-30)
+        assert err < 2e-5, (v, err)
+    # the mixture pdf from the stored lobes equals the oracle's pdf of the raw outputs
+    d = rng.normal(size=(16, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    for s in rng.choice(len(X), 12, replace=False):
+        cosang = np.einsum("vk,dv->dk", mix[2:5, :, s], d)
+        q = (mix[5, :, s][None, :] * np.exp(mix[1, :, s][None, :] * (cosang - 1.0))).sum(axis=1)
+        r = oracle.mlp_mixture_pdf(o[s], d, width=width)
+        assert np.allclose(q, r, rtol=1e-4), (s, q, r)

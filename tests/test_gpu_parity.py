@@ -136,6 +136,14 @@ def test_graph_and_eager_identical(ctx):
     ctx.render(48, 48, 3, seed=4, max_paths=1000)
     _, e = ctx.read_film(sums=True)
     assert np.array_equal(a, d) and np.array_equal(a, e)
+    # the seed travels in the pass descriptor: a graph captured for one seed replays
+    # another seed's render exactly as eager launches compute it
+    ctx.render(48, 48, 3, seed=4)  # captures the graph at the default pass size
+    ctx.render(48, 48, 3, seed=9)
+    _, f = ctx.read_film(sums=True)
+    ctx.render(48, 48, 3, seed=9, flags=cpt.CPT_RENDER_NO_GRAPH)
+    _, g = ctx.read_film(sums=True)
+    assert np.array_equal(f, g) and not np.array_equal(a, f)
 
 
 def _zscores(s_a, q_a, n_a, s_b, q_b, n_b):

@@ -1,4 +1,4 @@
-"""Batched multi-view rendering (streaming wavefront) through the C ABI."""
+"""Batched multi-view rendering (multi-view passes) through the C ABI."""
 import numpy as np
 import pytest
 
@@ -45,3 +45,7 @@ def test_views_uneven_batch_completes(ctx):
     assert len(imgs) == 24 and all(np.isfinite(i).all() for i in imgs)
     assert st["extension_rays"] >= st["paths"]
     assert np.all(ms > 0) and np.percentile(ms, 99) <= st["seconds"] * 1e3 + 1.0
+    # shortest job first: a view with fewer paths never completes after one with more
+    work = whs[:, 0].astype(np.int64) * whs[:, 1] * whs[:, 2] ** 2
+    late = [(a, b) for a in range(24) for b in range(24) if work[a] < work[b] and ms[a] > ms[b]]
+    assert not late, late

@@ -20,7 +20,7 @@ def test_abi_exports_every_declared_function():
     lib = cpt.load_library()
     for name in declared:
         assert hasattr(lib, name), name
-    assert lib.cpt_version() == 1000
+    assert lib.cpt_version() == 1001
 
 
 def _load_dump(path):
