@@ -58,13 +58,16 @@ def parse():
                     help="terrain: the headline 1080p line; views: 64-view batch of the box room; "
                          "guided: equal-time relMSE of guided vs plain PT on the hard-lighting interior")
     ap.add_argument("--budgets", type=str, default="10,30,60", help="guided: equal-time budgets in seconds")
+    ap.add_argument("--repeats", type=int, default=1,
+                    help="guided: runs of each arm with distinct seeds; budget rows give medians")
     ap.add_argument("--ref-spp-axis", type=int, default=64, help="guided: reference spp per axis (64 -> 4096 spp)")
     ap.add_argument("--boxes", type=int, default=40000, help="guided: furniture boxes (48 triangles each)")
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--no-ppg", action="store_true", help="guided: skip the SD-tree (PPG) arm")
     ap.add_argument("--fp8", action="store_true", help="guided: run the guide network in FP8 (training stays BF16)")
     ap.add_argument("--train-paths", type=int, default=0,
-                    help="guided: paths per pass that record training samples (0: min(pixels, 2^18))")
+                    help="guided: paths per pass that record training samples, spread over the image "
+                         "(0: 2^18, or 2^21 above 4M pixels)")
     ap.add_argument("--guide-width", type=int, choices=[64, 128], default=64,
                     help="guided: hidden width of the guide network (64: 8 lobes, 128: 16 lobes)")
     ap.add_argument("--guide-depth", type=int, default=0,
@@ -101,13 +104,16 @@ def copy_calibration(mib):
 
 
 def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=0, guide_depth=0, scene_kind="interior",
-                      boxes=40000, ref_spp_axis=256, ppg=False, log=None):
+                      boxes=40000, ref_spp_axis=256, ppg=False, log=None, repeats=1):
     """Equal-time comparison on the hard-lighting interior (one 0.2x0.2 light outside
     a cluttered room, light entering through a 0.3-wide door gap): plain PT-NEE-MIS
     vs guided PT (learned MLP guide trained online, training time included), both
     accumulated one sample per pixel at a time; relMSE = mean((I-R)^2/(R^2+0.01))
     against an FP64-accumulated plain reference of ref_spp_axis^2 samples per pixel.
-    Returns the result record (the `guided` object of the bench line)."""
+    With repeats > 1 both arms run that many times with distinct seeds and the
+    budget rows give medians over the runs (full relMSE at these sample counts is
+    decided by a few fireflies and varies from run to run).  Returns the result
+    record (the `guided` object of the bench line)."""
     import numpy as np
 
     import cpt
@@ -147,12 +153,14 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
         e = np.sort(((img - ref) ** 2 / (ref ** 2 + 1e-2)).mean(axis=2).ravel())
         return float(e[:int(0.99 * e.size)].mean())
 
-    tp = train_paths or min(W * H, 1 << 18)
+    # recording paths per pass (spread over the image): 2^18 at 720p, 2^21 at 4K
+    tp = train_paths or min(W * H, 1 << 21 if W * H > (1 << 22) else 1 << 18)
 
-    def run(guided):
+    def run(guided, rep):
+        seed0 = 1 + 1_000_003 * rep + (500_000 if guided else 0)
         if guided:
-            ctx.guide_init(seed=11, lr=3e-3, train_paths=tp, train_steps=1, guide_depth=guide_depth, infer_fp8=fp8,
-                           width=width)
+            ctx.guide_init(seed=11 + rep, lr=3e-3, train_paths=tp, train_steps=1, guide_depth=guide_depth,
+                           infer_fp8=fp8, width=width)
         flags = (cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN) if guided else 0
         curve = []
         elapsed, spp, k = 0.0, 0, 0
@@ -160,7 +168,8 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
         next_check = 0.25
         while elapsed < budgets[-1]:
             t = time.perf_counter()
-            st = ctx.render(W, H, 1, max_depth=depth, seed=1 + k, flags=flags | (cpt.CPT_RENDER_ACCUMULATE if k else 0))
+            st = ctx.render(W, H, 1, max_depth=depth, seed=seed0 + k,
+                            flags=flags | (cpt.CPT_RENDER_ACCUMULATE if k else 0))
             elapsed += time.perf_counter() - t
             gsamples += st["guide_samples"]
             gsteps += st["guide_steps"]
@@ -170,13 +179,18 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
                 _, sums = ctx.read_film(sums=True)
                 curve.append((elapsed, spp, relmse(sums / spp), relmse_trim(sums / spp)))
                 next_check = elapsed * 1.25
-                log(f"bench: {'guided' if guided else 'plain'} t={elapsed:.2f}s spp={spp} relMSE={curve[-1][2]:.4f} "
+                log(f"bench: run {rep} {'guided' if guided else 'plain'} t={elapsed:.2f}s spp={spp} relMSE={curve[-1][2]:.4f} "
                     f"trimmed={curve[-1][3]:.5f}" +
                     (f" training samples/step {gsamples / max(gsteps, 1):.0f}" if guided else ""))
         return curve, gsamples / max(gsteps, 1), gsteps
 
-    plain, _, _ = run(False)
-    guided, samples_per_step, adam_steps = run(True)
+    plains, guideds, per_step, steps = [], [], [], []
+    for rep in range(repeats):
+        plains.append(run(False, rep)[0])
+        g_, sps, ns = run(True, rep)
+        guideds.append(g_)
+        per_step.append(sps)
+        steps.append(ns)
 
     # SD-tree arm: the reference's PPG schedule (iterations of 1, 2, 4, ... spp, the
     # last one kept; no NEE), sized to each budget from a short calibration run
@@ -204,11 +218,12 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
         acc = {}
         for k in range(4):
             st = ctx.render(W, H, 1, max_depth=depth, seed=900 + k, flags=flags | cpt.CPT_RENDER_KERNEL_TIMING)
-            for key in ("ms_extend", "ms_shade", "ms_shadow", "ms_guide", "ms_other"):
+            for key in ("ms_extend", "ms_shade", "ms_shadow", "ms_guide", "ms_train", "ms_other"):
                 acc[key] = acc.get(key, 0.0) + st[key] / 4
         return {k: round(v, 3) for k, v in acc.items()}
 
-    kernel_ms = {"plain": frame_ms(0), "guided": frame_ms(cpt.CPT_RENDER_GUIDED)}
+    kernel_ms = {"plain": frame_ms(0), "guided": frame_ms(cpt.CPT_RENDER_GUIDED),
+                 "guided_training": frame_ms(cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN)}
     log(f"bench: per-frame kernel ms {kernel_ms}")
 
     def at(curve, T):
@@ -218,22 +233,45 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
                 best = c
         return best
 
-    rows = []
-    for T in budgets:
+    def run_row(plain, guided, T):
         p_ = at(plain, T)
         g_ = at(guided, T)
         reach = next((c[0] for c in guided if c[2] <= p_[2]), None)
         reach_t = next((c[0] for c in guided if c[3] <= p_[3]), None)
-        rows.append({"budget_s": T, "plain_relmse": p_[2], "plain_spp": p_[1], "guided_relmse": g_[2],
-                     "guided_spp": g_[1], "guided_time_to_plain_relmse_s": reach,
-                     "time_ratio": (reach / p_[0]) if reach else None,
-                     "plain_relmse_trimmed": p_[3], "guided_relmse_trimmed": g_[3],
-                     "time_ratio_trimmed": (reach_t / p_[0]) if reach_t else None})
+        return {"budget_s": T, "plain_relmse": p_[2], "plain_spp": p_[1], "guided_relmse": g_[2],
+                "guided_spp": g_[1], "guided_time_to_plain_relmse_s": reach,
+                "time_ratio": (reach / p_[0]) if reach else None,
+                "plain_relmse_trimmed": p_[3], "guided_relmse_trimmed": g_[3],
+                "time_ratio_trimmed": (reach_t / p_[0]) if reach_t else None}
+
+    def median(xs):
+        xs = [x for x in xs if x is not None]
+        return float(np.median(xs)) if xs else None
+
+    runs = [[run_row(plains[r], guideds[r], T) for T in budgets] for r in range(repeats)]
+    rows = []
+    for i, T in enumerate(budgets):
+        if repeats == 1:
+            rows.append(dict(runs[0][i]))
+        else:
+            # a run that never reaches plain's error counts as beyond the budget
+            reach = [runs[r][i]["guided_time_to_plain_relmse_s"] for r in range(repeats)]
+            rows.append({k: median([runs[r][i][k] for r in range(repeats)])
+                         for k in runs[0][i] if k not in ("guided_time_to_plain_relmse_s", "time_ratio")})
+            rows[-1]["budget_s"] = T
+            rows[-1]["runs_reaching_plain_relmse"] = sum(x is not None for x in reach)
+            rows[-1]["guided_time_to_plain_relmse_s"] = (
+                float(np.median([x if x is not None else float("inf") for x in reach]))
+                if sum(x is not None for x in reach) * 2 > repeats else None)
+            rows[-1]["time_ratio"] = (rows[-1]["guided_time_to_plain_relmse_s"] / T
+                                      if rows[-1]["guided_time_to_plain_relmse_s"] is not None else None)
+            rows[-1]["relmse_ratio_per_run"] = [runs[r][i]["plain_relmse"] / max(runs[r][i]["guided_relmse"], 1e-30)
+                                                for r in range(repeats)]
         if T in ppg_res:
             rows[-1].update({"ppg_seconds": ppg_res[T][0], "ppg_last_spp": ppg_res[T][1], "ppg_relmse": ppg_res[T][2],
                              "ppg_relmse_trimmed": ppg_res[T][3]})
-        log(f"bench: T={T:.0f}s plain {p_[2]:.4f} ({p_[1]} spp) guided {g_[2]:.4f} ({g_[1]} spp) "
-            f"guided reaches plain's error at {reach}")
+        log(f"bench: T={T:.0f}s plain {rows[-1]['plain_relmse']:.4f} guided {rows[-1]['guided_relmse']:.4f} "
+            f"guided reaches plain's error at {rows[-1]['guided_time_to_plain_relmse_s']}")
     ctx.close()
     return {
         "metric": "equal-time relMSE guided vs plain PT (1 GPU)",
@@ -241,11 +279,14 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
                     f"{scene.n_triangles} triangles, {W}x{H}, max depth {depth}, reference {ref_spp} spp FP64",
         "budgets_s": budgets,
         "time_ratio_at_largest_budget": rows[-1]["time_ratio"],
-        "relmse_ratio_plain_over_guided": rows[-1]["plain_relmse"] / max(rows[-1]["guided_relmse"], 1e-30),
+        "relmse_ratio_plain_over_guided": (median(rows[-1]["relmse_ratio_per_run"]) if repeats > 1 else
+                                           rows[-1]["plain_relmse"] / max(rows[-1]["guided_relmse"], 1e-30)),
+        "repeats": repeats,
         "guide": {"width": width, "lobes": 8 if width == 64 else 16, "inference": "fp8 e4m3 MFMA" if fp8 else "bf16 MFMA",
                   "training": "bf16 MFMA, fp32 accumulation and Adam", "train_paths": tp,
-                  "training_samples_per_step": samples_per_step, "adam_steps": adam_steps},
+                  "training_samples_per_step": median(per_step), "adam_steps": median(steps)},
         "budgets": rows,
+        "runs": runs if repeats > 1 else None,
         "frame_kernel_ms": kernel_ms,
         "reference_seconds": ref_s,
     }
@@ -257,7 +298,8 @@ def bench_guided(args):
     budgets = [float(b) for b in args.budgets.split(",")]
     g = guided_equal_time(W, H, budgets, depth=args.depth, fp8=args.fp8, width=args.guide_width,
                           train_paths=args.train_paths, guide_depth=args.guide_depth, scene_kind=args.scene,
-                          boxes=args.boxes, ref_spp_axis=args.ref_spp_axis, ppg=not args.no_ppg)
+                          boxes=args.boxes, ref_spp_axis=args.ref_spp_axis, ppg=not args.no_ppg,
+                          repeats=args.repeats)
     rows = g["budgets"]
     out = {
         "metric": g["metric"],
@@ -268,6 +310,8 @@ def bench_guided(args):
         "guide": g["guide"],
         "config": {"workload": g["workload"], "budgets_s": budgets},
         "budgets": rows,
+        "repeats": g["repeats"],
+        "runs": g["runs"],
         "frame_kernel_ms": g["frame_kernel_ms"],
         "reference_seconds": g["reference_seconds"],
     }

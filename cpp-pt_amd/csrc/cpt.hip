@@ -2473,7 +2473,8 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     }
     WfArgs a = make_args(c, 0, 0, 1, max_depth, seed, 0);
     a.n_rec = (uint32_t)nrec;
-    a.rec_stride = nrec ? (uint32_t)std::max<size_t>(1, slots / nrec) : 1u;
+    // rounded up, so the records reach the end of the pass
+    a.rec_stride = nrec ? (uint32_t)((slots + nrec - 1) / nrec) : 1u;
     a.refill = (unsigned)refill;
     a.integrator = integrator;
     a.corrected = (flags & CPT_RENDER_CORRECTED) ? 1 : 0;
