@@ -545,6 +545,31 @@ __global__ __launch_bounds__(64 * N::kWaves8) void k_guide_infer8(WfArgs a, int 
     }
 }
 
+// Per-bounce BF16 inference at width 64 (weights in LDS), structured as
+// k_guide_infer8: one wave per chunk of 64 queue entries, no block barrier after
+// the weight load (N::infer16).
+template <class N>
+__global__ __launch_bounds__(64 * N::kWaves16) void k_guide_infer16(WfArgs a, int cur) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    auto& sm = *reinterpret_cast<typename N::Smem16*>(smem_raw);
+    const unsigned count = a.counters[2 * cur];
+    constexpr unsigned W16 = N::kWaves16;
+    if (blockIdx.x * W16 * 64 >= count) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto& ws = sm.w[wave];
+    N::load16(sm, a.gwt, a.gparams);
+    __syncthreads();
+    for (unsigned c = blockIdx.x * W16 + wave; c * 64 < count; c += gridDim.x * W16) {
+        const unsigned e0 = c * 64;
+        const int nin = (int)min(64u, count - e0);
+        if (lane < nin) guide_entry_features(a, cur, e0 + lane, ws.in + lane * guide::NIN);
+        else
+            for (int k = 0; k < guide::NIN; k++) ws.in[lane * guide::NIN + k] = 0.0f;
+        N::sync8();
+        N::infer16(sm, ws, nin, lane, a.lobes + e0, a.lobes_stride);
+    }
+}
+
 // After a pass: every complete record of a recording slot becomes a training
 // sample (inputs, direction, weight = luminance of the radiance that came back
 // along the direction / the pdf it was sampled with).
@@ -1723,8 +1748,8 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
                 hipLaunchKernelGGL(k_guide_infer8<guide::Net>, dim3(L.infer_blocks), dim3(64 * guide::Net::kWaves8),
                                    guide::Net::kSmemFwd8, s, a, cur);
             } else {
-                hipLaunchKernelGGL(k_guide_infer<64>, dim3(L.infer_blocks), dim3(256), sizeof(guide::SmemFwd), s, a,
-                                   cur);
+                hipLaunchKernelGGL(k_guide_infer16<guide::Net>, dim3(L.infer_blocks),
+                                   dim3(64 * guide::Net::kWaves16), guide::Net::kSmemInf16, s, a, cur);
             }
             mark(4);
         }
@@ -3094,6 +3119,14 @@ int guide_alloc(cpt_ctx* c) {
                                   N::kSmemFwd));
     HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_mixtures8, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   N::kSmemFwd8));
+    if constexpr (N::kWave16) {
+        HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer16<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      N::kSmemInf16));
+        HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_mixtures16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      N::kSmemInf16));
+        HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_forward16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      N::kSmemInf16));
+    }
     c->g_width = N::kHid;
     return CPT_OK;
 }
@@ -3148,6 +3181,9 @@ int guide_forward_impl(cpt_ctx* c, const float* d_in, int n, float* d_out) {
         hipLaunchKernelGGL(N::mlp_forward8, dim3(std::min(c->cus, (n + 16 * N::kWaves8 - 1) / (16 * N::kWaves8))),
                            dim3(64 * N::kWaves8), N::kSmemFwd8, c->stream, c->g_wt8, c->g_params, c->g_inv8, d_in, n,
                            d_out);
+    else if constexpr (N::kWave16)
+        hipLaunchKernelGGL(N::mlp_forward16, dim3(std::min(c->cus, (n + 16 * N::kWaves16 - 1) / (16 * N::kWaves16))),
+                           dim3(64 * N::kWaves16), N::kSmemInf16, c->stream, c->g_wt, c->g_params, d_in, n, d_out);
     else
         hipLaunchKernelGGL(N::mlp_forward, dim3(G), dim3(256), N::kSmemFwd, c->stream, c->g_wt, c->g_params, d_in, n,
                            d_out);
@@ -3326,6 +3362,11 @@ int cpt_guide_mixtures(cpt_ctx* c, const float* inputs, int32_t n, float* mixtur
             else
                 hipLaunchKernelGGL(guide::Net::mlp_mixtures8, dim3(G), dim3(64 * w8), guide::Net::kSmemFwd8, c->stream,
                                    c->g_wt8, c->g_params, c->g_inv8, d_in, n, d_mix, stride);
+        } else if (!wide) {
+            const int w16 = guide::Net::kWaves16;
+            const int G = std::min(c->cus, (n + 64 * w16 - 1) / (64 * w16));
+            hipLaunchKernelGGL(guide::Net::mlp_mixtures16, dim3(G), dim3(64 * w16), guide::Net::kSmemInf16, c->stream,
+                               c->g_wt, c->g_params, d_in, n, d_mix, stride);
         } else {
             const int G = std::min(c->cus, (n + guide::TILE - 1) / guide::TILE);
             if (wide)

@@ -41,6 +41,9 @@ constexpr int T2 = T1 + HID * LDH;
 constexpr int T3 = T2 + HID * LDH;
 constexpr int T4 = T3 + HID * LDH;
 constexpr int NWT = T4 + NOUT * LDH;  // bf16 elements
+// the per-wave inference's copy of the output layer, rows in out_orig order
+constexpr int T4P = NWT;
+constexpr int NWT_ALL = NWT + NOUT * LDH;  // bf16 elements of the device copy
 // BF16 weights resident in LDS (width 64); otherwise read from global memory
 constexpr bool WT_LDS = HID <= 64;
 
@@ -950,13 +953,18 @@ __global__ void k_adam(float* params, float* m, float* v, const float* grad, int
         else if (p >= W1 && p < B1) { off = W1; N = HID; ld = LDH; gwt[T1 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
         else if (p >= W2 && p < B2) { off = W2; N = HID; ld = LDH; gwt[T2 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
         else if (p >= W3 && p < B3) { off = W3; N = HID; ld = LDH; gwt[T3 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
-        else if (p >= W4 && p < B4) { off = W4; N = NOUT; ld = LDH; gwt[T4 + ((p - off) % N) * ld + (p - off) / N] = f2bf(w); }
+        else if (p >= W4 && p < B4) {
+            off = W4; N = NOUT; ld = LDH;
+            const uint16_t wb = f2bf(w);
+            gwt[T4 + ((p - off) % N) * ld + (p - off) / N] = wb;
+            gwt[T4P + out_slot((p - off) % N) * ld + (p - off) / N] = wb;  // inference copy
+        }
     }
 }
 
 // BF16 transposed copy of all weights (after set_weights)
 __global__ void k_pack_weights(const float* params, uint16_t* gwt) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NWT; i += gridDim.x * blockDim.x) gwt[i] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NWT_ALL; i += gridDim.x * blockDim.x) gwt[i] = 0;
     __syncthreads();
 }
 __global__ void k_pack_weights2(const float* params, uint16_t* gwt) {
@@ -966,7 +974,10 @@ __global__ void k_pack_weights2(const float* params, uint16_t* gwt) {
         else if (p >= W1 && p < B1) gwt[T1 + ((p - W1) % HID) * LDH + (p - W1) / HID] = f2bf(w);
         else if (p >= W2 && p < B2) gwt[T2 + ((p - W2) % HID) * LDH + (p - W2) / HID] = f2bf(w);
         else if (p >= W3 && p < B3) gwt[T3 + ((p - W3) % HID) * LDH + (p - W3) / HID] = f2bf(w);
-        else if (p >= W4 && p < B4) gwt[T4 + ((p - W4) % NOUT) * LDH + (p - W4) / NOUT] = f2bf(w);
+        else if (p >= W4 && p < B4) {
+            gwt[T4 + ((p - W4) % NOUT) * LDH + (p - W4) / NOUT] = f2bf(w);
+            gwt[T4P + out_slot((p - W4) % NOUT) * LDH + (p - W4) / NOUT] = f2bf(w);
+        }
     }
 }
 
@@ -1113,6 +1124,164 @@ __global__ __launch_bounds__(64 * WAVES8) void k_mlp_mixtures8(const uint8_t* gw
     }
 }
 
+// --------------------------------------------------------------------------
+// BF16 inference, one wave per 16-sample tile, at the width whose weights fit in
+// LDS (64): the block holds one copy of the transposed BF16 weights, the output
+// layer's rows in out_orig order (from the copy at T4P), and each wave its own tile
+// state.  The one-blob features are formed in registers as the MFMA's second
+// operand (the values encode_tile_window stores: blob() of the 16 bins of
+// blob_window, 0 elsewhere), the weights are the first operand, so a lane holds four
+// consecutive outputs of one sample and the mixture is decoded in registers
+// (wave_decode).  Same arithmetic per output as forward_tile (acc + bias, BF16
+// activations); only the MFMA operand order differs.
+constexpr int WAVES16 = 12;
+struct WaveState16 {
+    float in[64 * NIN];    // inputs of up to 64 samples
+    uint16_t h[16 * LDH];  // hidden activations; each layer overwrites its input
+};
+struct SmemInf16 {
+    uint16_t wt[NWT];            // layers 0-3 as T0..T3, layer 4 from T4P
+    float bias[4 * HID + NOUT];  // output-layer biases in out_orig order
+    WaveState16 w[WAVES16];
+};
+static_assert(!WT_LDS || sizeof(SmemInf16) <= 160 * 1024, "BF16 inference LDS");
+
+__device__ __forceinline__ void load_weights16(SmemInf16& sm, const uint16_t* gwt, const float* params) {
+    static_assert(T4 % 8 == 0 && NWT % 8 == 0 && T4P % 8 == 0, "16-byte copies");
+    const uint4* src = reinterpret_cast<const uint4*>(gwt);
+    uint4* dst = reinterpret_cast<uint4*>(sm.wt);
+    for (int i = threadIdx.x; i < NWT / 8; i += blockDim.x) dst[i] = src[i < T4 / 8 ? i : i - T4 / 8 + T4P / 8];
+    for (int i = threadIdx.x; i < 4 * HID + NOUT; i += blockDim.x) {
+        int off = i < HID ? B0 + i : i < 2 * HID ? B1 + i - HID : i < 3 * HID ? B2 + i - 2 * HID
+                : i < 4 * HID ? B3 + i - 3 * HID : B4 + out_orig(i - 4 * HID);
+        sm.bias[i] = params[off];
+    }
+}
+
+// acc[nt] += Wt (rows nt 16 ..) x X^T over K, X the wave's 16 rows (ldx)
+template <int K, int NT>
+__device__ __forceinline__ void gemm16_rows_nk_t(const uint16_t* X, int ldx, const uint16_t* Wt, int ldw, f4* acc,
+                                                 int lane) {
+#pragma unroll
+    for (int kt = 0; kt < K / 32; kt++) {
+        const int kk = kt * 32 + 8 * (lane >> 4);
+        const bf8v x = ld_bf8(X + (lane & 15) * ldx + kk);
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++)
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld_bf8(Wt + (nt * 16 + (lane & 15)) * ldw + kk), x,
+                                                              acc[nt], 0, 0, 0);
+    }
+}
+
+// ReLU outputs of a hidden layer (bias at b) as BF16 into the wave's rows
+__device__ __forceinline__ void store_hidden16(const f4* acc, const float* b, uint16_t* h, int lane) {
+    const int m = lane & 15, n4 = 4 * (lane >> 4);
+#pragma unroll
+    for (int nt = 0; nt < NTH; nt++) {
+        uint16_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = f2bf(fmaxf(acc[nt][j] + b[nt * 16 + n4 + j], 0.0f));
+        *reinterpret_cast<uint2*>(h + m * LDH + nt * 16 + n4) =
+            make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+    }
+}
+
+// Forward pass of the wave's 16 rows (inputs at in16; rows past the valid ones
+// are computed and ignored); o as wave_forward8
+__device__ __forceinline__ void wave_forward16(const SmemInf16& sm, WaveState16& ws, const float* in16, int lane,
+                                               f4 (&o)[NTO]) {
+    const int m = lane & 15, g = lane >> 4, n4 = 4 * g;
+    f4 acc[NTH];
+#pragma unroll
+    for (int nt = 0; nt < NTH; nt++) acc[nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kt = 0; kt < NIN; kt++) {  // layer 0: one input (32 bins) per step of 32
+        const float x = in16[m * NIN + kt];
+        const int w0 = blob_window(x);
+        s8v xs;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int b = 8 * g + j;
+            xs[j] = (short)((b >= w0 && b < w0 + 16) ? f2bf(blob(x, b)) : (uint16_t)0);
+        }
+        const bf8v xb = __builtin_bit_cast(bf8v, xs);
+        const int kk = 32 * kt + 8 * g;
+#pragma unroll
+        for (int nt = 0; nt < NTH; nt++)
+            acc[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld_bf8(sm.wt + T0 + (nt * 16 + m) * LD0 + kk), xb,
+                                                              acc[nt], 0, 0, 0);
+    }
+    store_hidden16(acc, sm.bias, ws.h, lane);
+    wave_sync();
+#pragma unroll
+    for (int l = 1; l < 4; l++) {
+#pragma unroll
+        for (int nt = 0; nt < NTH; nt++) acc[nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+        gemm16_rows_nk_t<HID, NTH>(ws.h, LDH, sm.wt + (l == 1 ? T1 : l == 2 ? T2 : T3), LDH, acc, lane);
+        wave_sync();  // every lane has read its operand rows before any overwrites them
+        store_hidden16(acc, sm.bias + l * HID, ws.h, lane);
+        wave_sync();
+    }
+#pragma unroll
+    for (int nt = 0; nt < NTO; nt++) o[nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    gemm16_rows_nk_t<HID, NTO>(ws.h, LDH, sm.wt + T4, LDH, o, lane);
+#pragma unroll
+    for (int nt = 0; nt < NTO; nt++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[nt][j] = o[nt][j] + sm.bias[4 * HID + nt * 16 + n4 + j];
+    wave_sync();
+}
+
+// BF16 mixtures of up to 64 samples whose inputs are in ws.in (nin valid)
+__device__ __forceinline__ void wave_infer16(const SmemInf16& sm, WaveState16& ws, int nin, int lane, float* dst,
+                                             size_t stride) {
+    for (int q = 0; q < 4 && 16 * q < nin; q++) {
+        f4 o[NTO];
+        wave_forward16(sm, ws, ws.in + 16 * q * NIN, lane, o);
+        wave_decode(o, lane, min(16, nin - 16 * q), dst + 16 * q, stride);
+    }
+}
+
+__global__ __launch_bounds__(64 * WAVES16) void k_mlp_mixtures16(const uint16_t* gwt, const float* params,
+                                                                 const float* in, int n, float* dst, size_t stride) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    SmemInf16& sm = *reinterpret_cast<SmemInf16*>(smem_raw);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    WaveState16& ws = sm.w[wave];
+    load_weights16(sm, gwt, params);
+    __syncthreads();
+    for (int c = blockIdx.x * WAVES16 + wave; c * 64 < n; c += gridDim.x * WAVES16) {
+        const int nin = min(64, n - c * 64);
+        for (int i = lane; i < 64 * NIN; i += 64) ws.in[i] = i < nin * NIN ? in[(size_t)c * 64 * NIN + i] : 0.0f;
+        wave_sync();
+        wave_infer16(sm, ws, nin, lane, dst + (size_t)c * 64, stride);
+    }
+}
+
+// BF16 raw outputs (n x NOUT, network order) by the per-wave inference code
+__global__ __launch_bounds__(64 * WAVES16) void k_mlp_forward16(const uint16_t* gwt, const float* params,
+                                                                const float* in, int n, float* out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    SmemInf16& sm = *reinterpret_cast<SmemInf16*>(smem_raw);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    WaveState16& ws = sm.w[wave];
+    load_weights16(sm, gwt, params);
+    __syncthreads();
+    for (int t = blockIdx.x * WAVES16 + wave; t * 16 < n; t += gridDim.x * WAVES16) {
+        const int nr = min(16, n - t * 16);
+        for (int i = lane; i < 16 * NIN; i += 64) ws.in[i] = i < nr * NIN ? in[(size_t)t * 16 * NIN + i] : 0.0f;
+        wave_sync();
+        f4 o[NTO];
+        wave_forward16(sm, ws, ws.in, lane, o);
+        const int m = lane & 15, n4 = 4 * (lane >> 4);
+        if (m < nr)
+#pragma unroll
+            for (int nt = 0; nt < NTO; nt++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) out[((size_t)t * 16 + m) * NOUT + out_orig(nt * 16 + n4 + j)] = o[nt][j];
+    }
+}
+
 // BF16 mixtures of n input vectors, as k_mlp_mixtures8 (the per-bounce BF16 path's
 // encode, forward and decode)
 __global__ __launch_bounds__(256) void k_mlp_mixtures(const uint16_t* gwt, const float* params, const float* in, int n,
@@ -1130,11 +1299,25 @@ __global__ __launch_bounds__(256) void k_mlp_mixtures(const uint16_t* gwt, const
 
 // What the host code needs of this width, for code templated on the network.
 struct Net {
-    static constexpr int kHid = HID, kLobes = NLOBES, kOut = NOUT, kParams = NPARAM, kWt = NWT, kWq = NWQ,
+    static constexpr int kHid = HID, kLobes = NLOBES, kOut = NOUT, kParams = NPARAM, kWt = NWT_ALL, kWq = NWQ,
                          kRow = ROW, kTile = TILE, kChunk = CH, kDwGroups = DW_GROUPS;
     static constexpr int kW[5] = {W0, W1, W2, W3, W4}, kB4 = B4;
     static constexpr size_t kSmemFwd = sizeof(SmemFwd), kSmemFwd8 = sizeof(SmemFwd8);
     static constexpr int kWaves8 = WAVES8;
+    // per-wave BF16 inference (width 64: the weights fit in LDS)
+    static constexpr bool kWave16 = WT_LDS;
+    static constexpr int kWaves16 = WAVES16;
+    static constexpr size_t kSmemInf16 = sizeof(SmemInf16);
+    using Smem16 = SmemInf16;
+    static constexpr auto mlp_forward16 = &k_mlp_forward16;
+    static constexpr auto mlp_mixtures16 = &k_mlp_mixtures16;
+    static __device__ __forceinline__ void load16(SmemInf16& sm, const uint16_t* gwt, const float* params) {
+        load_weights16(sm, gwt, params);
+    }
+    static __device__ __forceinline__ void infer16(const SmemInf16& sm, WaveState16& ws, int nin, int lane,
+                                                   float* dst, size_t stride) {
+        wave_infer16(sm, ws, nin, lane, dst, stride);
+    }
     using Smem8 = SmemFwd8;
     static __device__ __forceinline__ void load8(SmemFwd8& sm, const uint8_t* gwt8, const float* params,
                                                  const float* inv8) {
