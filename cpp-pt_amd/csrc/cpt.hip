@@ -3193,21 +3193,25 @@ int guide_forward_impl(cpt_ctx* c, const float* d_in, int n, float* d_out) {
 // One Adam step on device-resident samples (n x 11 floats at c->g_samples).
 template <class N>
 int guide_train_impl(cpt_ctx* c, int n, bool apply) {
-    if (c->g_rows_cap < (size_t)n || !c->g_rows) {
+    // per-sample rows, feature-major over n rounded up to a tile (padding samples zero)
+    const int npad = (n + N::kTile - 1) / N::kTile * N::kTile;
+    if (c->g_rows_cap < (size_t)npad || !c->g_rows) {
         c->g_rows_cap = 0;
-        int r = dalloc(c, c->g_rows, (size_t)n * N::kRow);
+        int r = dalloc(c, c->g_rows, (size_t)npad * N::kRow);
         if (r) return r;
-        c->g_rows_cap = n;
+        c->g_rows_cap = npad;
     }
-    int per_block = ((n + GUIDE_DW_BLOCKS - 1) / GUIDE_DW_BLOCKS + N::kChunk - 1) / N::kChunk * N::kChunk;
+    int per_block = ((npad + GUIDE_DW_BLOCKS - 1) / GUIDE_DW_BLOCKS + N::kChunk - 1) / N::kChunk * N::kChunk;
     hipLaunchKernelGGL(N::weight_partial, dim3(GUIDE_WSUM_BLOCKS), dim3(256), 0, c->stream, c->g_samples, n,
                        c->g_wpart);
     hipLaunchKernelGGL(N::weight_scale, dim3(1), dim3(256), 0, c->stream, c->g_wpart, GUIDE_WSUM_BLOCKS, c->g_wscale);
     hipLaunchKernelGGL(N::train_tile, dim3(GUIDE_TRAIN_BLOCKS), dim3(256), N::kSmemFwd, c->stream, c->g_wt,
-                       c->g_params, c->g_samples, n, c->g_wscale, c->g_rows, c->g_loss_part);
-    hipLaunchKernelGGL(N::train_dw, dim3(GUIDE_DW_BLOCKS, N::kDwGroups), dim3(256), 0, c->stream, c->g_rows, n,
+                       c->g_params, c->g_samples, n, npad, c->g_wscale, c->g_rows, c->g_loss_part);
+    hipLaunchKernelGGL(N::train_dw, dim3(GUIDE_DW_BLOCKS, N::kDwGroups), dim3(256), 0, c->stream, c->g_rows, npad,
                        per_block, c->g_partial);
-    hipLaunchKernelGGL(N::reduce_partials, dim3(64), dim3(256), 0, c->stream, c->g_partial, GUIDE_DW_BLOCKS,
+    // one thread per parameter: each sums its GUIDE_DW_BLOCKS partials in block order
+    hipLaunchKernelGGL(N::reduce_partials, dim3((N::kParams + 255) / 256), dim3(256), 0, c->stream, c->g_partial,
+                       GUIDE_DW_BLOCKS,
                        N::kParams, N::kParams, c->g_grad);
     if (apply) {
         c->g_step++;

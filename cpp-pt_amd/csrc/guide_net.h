@@ -677,8 +677,23 @@ __global__ __launch_bounds__(64 * WAVES8) void k_mlp_forward8(const uint8_t* gwt
 
 // Forward + loss gradient + backward-data for tiles of training samples; writes
 // the per-sample rows used by the weight-gradient GEMMs and each block's loss sum.
+// Per-sample rows for the weight gradients, stored feature-major: value f of
+// sample s at rows[f * npad + s] (npad: n rounded up to TILE, padding samples
+// zero), so a block's samples of one feature are contiguous.  Writes values f0 ..
+// f0 + nf - 1 of the tile's samples from LDS rows src (ld elements apart), two
+// samples per 4-byte store.
+__device__ __forceinline__ void store_rows_t(uint16_t* rows, size_t npad, size_t s0, int nt, int f0, int nf,
+                                             const uint16_t* src, int ld) {
+    for (int i = threadIdx.x; i < nf * (TILE / 2); i += blockDim.x) {
+        const int f = i / (TILE / 2), s = 2 * (i - f * (TILE / 2));
+        const uint32_t lo = s < nt ? src[s * ld + f] : 0u, hi = s + 1 < nt ? src[(s + 1) * ld + f] : 0u;
+        *reinterpret_cast<uint32_t*>(rows + (size_t)(f0 + f) * npad + s0 + s) = lo | (hi << 16);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const float* params, const float* samples,
-                                                    int n, const float* wscale, uint16_t* rows, float* loss_part) {
+                                                    int n, int npad, const float* wscale, uint16_t* rows,
+                                                    float* loss_part) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     SmemFwd& sm = *reinterpret_cast<SmemFwd*>(smem_raw);
     __shared__ float in_s[TILE * NIN];
@@ -705,18 +720,13 @@ __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const f
         __syncthreads();
         encode_tile(sm, in_s, nt);
         __syncthreads();
-        for (int i = threadIdx.x; i < nt * K0; i += blockDim.x) {
-            int s = i / K0, f = i - s * K0;
-            rows[(s0 + s) * ROW + RX0 + f] = sm.x0[s * LD0 + f];
-        }
+        store_rows_t(rows, npad, s0, nt, RX0, K0, sm.x0, LD0);
         __syncthreads();
         float* o_tile = reinterpret_cast<float*>(OT_IN_X0 ? reinterpret_cast<unsigned char*>(sm.x0) : ot_own);
         uint16_t* dz4 = reinterpret_cast<uint16_t*>(o_tile + TILE * NOUT);
         forward_tile(sm, wt, o_tile);
-        for (int i = threadIdx.x; i < nt * 4 * HID; i += blockDim.x) {
-            int s = i / (4 * HID), f = i - s * 4 * HID;
-            rows[(s0 + s) * ROW + RH + f] = sm.h[f / HID][s * LDH + (f % HID)];
-        }
+#pragma unroll
+        for (int l = 0; l < 4; l++) store_rows_t(rows, npad, s0, nt, RH + l * HID, HID, sm.h[l], LDH);
         if (threadIdx.x < TILE) {
             int s = threadIdx.x;
             float g[NOUT];
@@ -731,10 +741,7 @@ __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const f
             for (int k = NOUT; k < LDO; k++) dz4[s * LDO + k] = 0;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < nt * NOUT; i += blockDim.x) {
-            int s = i / NOUT, f = i - s * NOUT;
-            rows[(s0 + s) * ROW + RDZ4 + f] = dz4[s * LDO + f];
-        }
+        store_rows_t(rows, npad, s0, nt, RDZ4, NOUT, dz4, LDO);
         // backward-data: dz_{l-1} = (dz_l W_l^T) .* (h_{l-1} > 0), written over h_{l-1}
         {
             f4 acc[NTH] = {};
@@ -781,10 +788,8 @@ __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const f
             __syncthreads();
         }
         // sm.h[l] now holds dz_l for l = 0..3
-        for (int i = threadIdx.x; i < nt * 4 * HID; i += blockDim.x) {
-            int s = i / (4 * HID), f = i - s * 4 * HID;
-            rows[(s0 + s) * ROW + RDZ + f] = sm.h[f / HID][s * LDH + (f % HID)];
-        }
+#pragma unroll
+        for (int l = 0; l < 4; l++) store_rows_t(rows, npad, s0, nt, RDZ + l * HID, HID, sm.h[l], LDH);
         __syncthreads();
     }
     red[threadIdx.x] = loss_acc;
@@ -802,10 +807,9 @@ __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const f
 // registers), bias gradients by plain sums (blockIdx.y == 0); each block writes
 // its entries of its partial NPARAM vector.
 constexpr int NTILES = (K0 / 16) * NTH + 3 * NTH * NTH + NTH * NTO;  // 116 or 344
-constexpr int DW_GROUPS = HID <= 64 ? 1 : 4;
-constexpr int TPW = (NTILES + 4 * DW_GROUPS - 1) / (4 * DW_GROUPS);  // 29 or 22 tiles per wave
-constexpr int CH = 32;                 // samples staged per step
-constexpr int NBH = 4 * HID / 256;     // hidden-layer biases per thread
+constexpr int DW_GROUPS = HID <= 64 ? 4 : 8;
+constexpr int TPW = (NTILES + 4 * DW_GROUPS - 1) / (4 * DW_GROUPS);  // 8 or 11 tiles per wave
+constexpr int CH = 32;                 // samples per MFMA step of k_train_dw
 
 __device__ __forceinline__ void tile_of(int t, int& layer, int& mt, int& nt) {
     constexpr int TL0 = (K0 / 16) * NTH, TLH = NTH * NTH;
@@ -816,57 +820,38 @@ __device__ __forceinline__ void tile_of(int t, int& layer, int& mt, int& nt) {
     layer = 4; mt = t / NTO; nt = t % NTO;
 }
 
-// grid (blocks, DW_GROUPS) x 256 threads
-__global__ __launch_bounds__(256) void k_train_dw(const uint16_t* rows, int n, int per_block, float* partial) {
-    __shared__ __align__(16) uint16_t chunk[CH * ROW];
+// grid (blocks, DW_GROUPS) x 256 threads.  rows is feature-major (store_rows_t);
+// the block sums samples [blockIdx.x per_block, + per_block) of npad, per_block a
+// multiple of 32.  Each 16x16 output tile takes one v_mfma_f32_16x16x32_bf16 per 32
+// samples: lane l loads its 8 samples of activation row mt 16 + l % 16 and of
+// gradient row nt 16 + l % 16 as one 16-byte load each (samples 8 (l / 16) ..).
+__device__ __forceinline__ bf8v ld_bf8_16(const uint16_t* p) {
+    return __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(p));
+}
+__global__ __launch_bounds__(256) void k_train_dw(const uint16_t* rows, int npad, int per_block, float* partial) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tbase = blockIdx.y * 4 * TPW;
     const bool biases = blockIdx.y == 0;
+    const int begin = min(npad, blockIdx.x * per_block);
+    const int end = min(npad, begin + per_block);
     f4 acc[TPW];
 #pragma unroll
     for (int i = 0; i < TPW; i++) acc[i] = f4{0, 0, 0, 0};
-    float bacc[NBH];  // hidden bias entries threadIdx.x + 256 j (layers 0..3 -> dz_l)
+    for (int k0 = begin; k0 < end; k0 += 32) {
+        const int ks = k0 + 8 * (lane >> 4);
 #pragma unroll
-    for (int j = 0; j < NBH; j++) bacc[j] = 0.0f;
-    float bacc2 = 0.0f;  // output bias entry threadIdx.x (< NOUT) -> dz4
-    const int begin = blockIdx.x * per_block;
-    const int end = min(n, begin + per_block);
-    for (int c0 = begin; c0 < end; c0 += CH) {
-        const int nc = min(CH, end - c0);
-        for (int i = threadIdx.x; i < CH * ROW / 8; i += blockDim.x) {
-            int s = (i * 8) / ROW;
-            uint4 v = s < nc ? reinterpret_cast<const uint4*>(rows + (size_t)c0 * ROW)[i] : uint4{0, 0, 0, 0};
-            reinterpret_cast<uint4*>(chunk)[i] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < CH / 16; ks++) {
-            const int sk = ks * 16 + 4 * (lane >> 4);
-#pragma unroll
-            for (int i = 0; i < TPW; i++) {
-                const int t = tbase + wave + 4 * i;
-                if (t < NTILES) {
-                    int layer, mt, nt;
-                    tile_of(t, layer, mt, nt);
-                    const int aoff = layer == 0 ? RX0 : RH + (layer - 1) * HID;
-                    const int doff = layer == 4 ? RDZ4 : RDZ + layer * HID;
-                    s4 a, b;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        a[j] = (short)chunk[(sk + j) * ROW + aoff + mt * 16 + (lane & 15)];
-                        b[j] = (short)chunk[(sk + j) * ROW + doff + nt * 16 + (lane & 15)];
-                    }
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
-                }
+        for (int i = 0; i < TPW; i++) {
+            const int t = tbase + wave + 4 * i;
+            if (t < NTILES) {
+                int layer, mt, nt;
+                tile_of(t, layer, mt, nt);
+                const int aoff = layer == 0 ? RX0 : RH + (layer - 1) * HID;
+                const int doff = layer == 4 ? RDZ4 : RDZ + layer * HID;
+                const bf8v a = ld_bf8_16(rows + (size_t)(aoff + mt * 16 + (lane & 15)) * npad + ks);
+                const bf8v b = ld_bf8_16(rows + (size_t)(doff + nt * 16 + (lane & 15)) * npad + ks);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
             }
         }
-        if (biases)
-            for (int s = 0; s < nc; s++) {
-#pragma unroll
-                for (int j = 0; j < NBH; j++) bacc[j] += bf2f(chunk[s * ROW + RDZ + threadIdx.x + 256 * j]);
-                if ((int)threadIdx.x < NOUT) bacc2 += bf2f(chunk[s * ROW + RDZ4 + threadIdx.x]);
-            }
-        __syncthreads();
     }
     float* out = partial + (size_t)blockIdx.x * NPARAM;
 #pragma unroll
@@ -885,14 +870,22 @@ __global__ __launch_bounds__(256) void k_train_dw(const uint16_t* rows, int n, i
         }
     }
     if (biases) {
+        // bias gradients: the block's samples of each gradient row, in sample order
+        for (int e = threadIdx.x; e < 4 * HID + NOUT; e += blockDim.x) {
+            const uint16_t* r = rows + (size_t)(RDZ + e) * npad;  // RDZ4 = RDZ + 4 HID
+            float sum = 0.0f;
+            for (int s = begin; s < end; s += 8) {
+                const uint4 v = *reinterpret_cast<const uint4*>(r + s);
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int j = 0; j < NBH; j++) {
-            const int e = threadIdx.x + 256 * j;
+                for (int q = 0; q < 4; q++) {
+                    sum += bf2f((uint16_t)(w[q] & 0xffffu));
+                    sum += bf2f((uint16_t)(w[q] >> 16));
+                }
+            }
             const int l = e / HID, u = e % HID;
-            const int boff = l == 0 ? B0 : l == 1 ? B1 : l == 2 ? B2 : B3;
-            out[boff + u] = bacc[j];
+            out[e < 4 * HID ? (l == 0 ? B0 : l == 1 ? B1 : l == 2 ? B2 : B3) + u : B4 + (e - 4 * HID)] = sum;
         }
-        if ((int)threadIdx.x < NOUT) out[B4 + threadIdx.x] = bacc2;
     }
 }
 
@@ -900,6 +893,7 @@ __global__ __launch_bounds__(256) void k_train_dw(const uint16_t* rows, int n, i
 __global__ void k_reduce_partials(const float* partial, int nblocks, int stride, int count, float* out) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < count; p += gridDim.x * blockDim.x) {
         float s = 0.0f;
+#pragma unroll 8
         for (int b = 0; b < nblocks; b++) s += partial[(size_t)b * stride + p];
         out[p] = s;
     }
