@@ -438,13 +438,17 @@ def _guide_forward(self, inputs):
 
 
 def _guide_mixtures(self, inputs):
-    """Decoded mixtures of the per-bounce inference code: (6, lobes, n) floats, the
-    blocks weights, kappas, mean x, y, z, weight x vMF normalisation."""
+    """Stored mixtures of the per-bounce inference code, unpacked: a dict of
+    (lobes, n) arrays kmu_x, kmu_y, kmu_z, weight (the stored halves as float32)
+    and c (float32); the pdf of w is sum_k exp(kmu_k . w + c_k)."""
     x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 7)
     lobes = self.guide_shape()[1]
-    out = np.zeros((6, lobes, x.shape[0]), np.float32)
-    self._check(self.lib.cpt_guide_mixtures(self.h, _ptr(x), x.shape[0], _ptr(out)), "cpt_guide_mixtures")
-    return out
+    raw = np.zeros((3, lobes, x.shape[0]), np.uint32)
+    self._check(self.lib.cpt_guide_mixtures(self.h, _ptr(x), x.shape[0], _ptr(raw)), "cpt_guide_mixtures")
+    h0 = raw[0].astype("<u4").view(np.float16).reshape(lobes, -1, 2).astype(np.float32)
+    h1 = raw[1].astype("<u4").view(np.float16).reshape(lobes, -1, 2).astype(np.float32)
+    return {"kmu_x": h0[..., 0], "kmu_y": h0[..., 1], "kmu_z": h1[..., 0], "weight": h1[..., 1],
+            "c": raw[2].view(np.float32)}
 
 
 def _guide_train_step(self, samples, apply=True, want_grad=False):

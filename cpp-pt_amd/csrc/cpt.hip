@@ -115,9 +115,9 @@ struct WfArgs {
     double* film;      // per-pixel radiance sums
     double* film2;     // per-pixel sums of squared radiance
     // learned guide (guided renders only)
-    float* lobes;            // decoded mixtures, structure of arrays: value v (0..6 L - 1: pi[L], kappa[L],
-                             // mu.x[L], mu.y[L], mu.z[L], pi kappa / (2 pi (1 - e^-2kappa))[L],
-                             // L = guide_lobes) of queue entry e at lobes[v * lobes_stride + e]
+    float* lobes;            // stored mixtures, structure of arrays: value v (0..3 L - 1: half2 (kappa mu.x,
+                             // kappa mu.y)[L], half2 (kappa mu.z, weight)[L], log term c[L], L = guide_lobes)
+                             // of queue entry e at lobes[v * lobes_stride + e] (see gmix_pdf)
     size_t lobes_stride;
     float* rec;              // training records: [slot][depth][REC_FLOATS]
     uint8_t* rec_flag;       // per record: 0 empty, 1 vertex recorded, 2 radiance after it known
@@ -370,44 +370,55 @@ __global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_extend(WfArgs a, int cur)
 
 // ------------------------------------------------------------------ learned guide in the path
 
-// Decoded mixture of one queue entry (lob = lobes + entry), NL lobes: weight
-// lob[k st], kappa lob[(NL + k) st], mean lob[(2 NL + k) st], lob[(3 NL + k) st],
-// lob[(4 NL + k) st], weight x vMF normalisation lob[(5 NL + k) st]; st = the
-// structure-of-arrays stride, so a wave's loads of one value for consecutive
-// entries are contiguous.
+// Stored mixture of one queue entry (lob = lobes + entry), NL lobes, 3 values per
+// lobe k at lob[(v NL + k) st] (st: the structure-of-arrays stride, so a wave's
+// loads of one value for consecutive entries are contiguous): half2 (kappa mu_x,
+// kappa mu_y), half2 (kappa mu_z, weight w~), and c = log(w~ / S C(kappa')) -
+// kappa' with kappa' = |kappa mu| (the guide network's decoders write them).  The
+// pdf is sum_k exp(kappa mu_k . w + c_k); the sampler below draws from exactly that
+// density (up to float rounding), from the same stored values.
 template <int NL>
 __device__ __forceinline__ float gmix_pdf(const float* lob, size_t st, V3 w) {
     float p = 0.0f;
 #pragma unroll
     for (int k = 0; k < NL; k++) {
-        float c = lob[(2 * NL + k) * st] * w.x + lob[(3 * NL + k) * st] * w.y + lob[(4 * NL + k) * st] * w.z;
-        p += lob[(5 * NL + k) * st] * __expf(lob[(NL + k) * st] * (c - 1.0f));
+        const float2 a = guide::unpack_h2(__float_as_uint(lob[k * st]));
+        const float2 b = guide::unpack_h2(__float_as_uint(lob[(NL + k) * st]));
+        p += __expf(a.x * w.x + a.y * w.y + b.x * w.z + lob[(2 * NL + k) * st]);
     }
     return p;
 }
 
 template <int NL>
 __device__ __forceinline__ V3 gmix_sample(const float* lob, size_t st, float u0, float u1, float u2) {
-    // lobe by its weight, then a vMF sample around its mean (Jakob 2012, "Numerically
-    // stable sampling of the von Mises Fisher distribution"; orthonormal basis after
-    // Duff et al. 2017); values read from memory: a local array indexed by the
-    // chosen lobe would live in scratch
+    // lobe k with probability w~_k / sum w~ (the decoder's S, summed in another
+    // order, differs by rounding only), then a vMF sample of concentration kappa' = |kappa mu_k| about
+    // kappa mu_k / kappa' (Jakob 2012, "Numerically stable sampling of the von
+    // Mises Fisher distribution"; orthonormal basis after Duff et al. 2017); values
+    // read from memory: a local array indexed by the chosen lobe would live in
+    // scratch
+    float S = 0.0f;
+#pragma unroll
+    for (int j = 0; j < NL; j++) S += guide::unpack_h2(__float_as_uint(lob[(NL + j) * st])).y;
+    const float t = u0 * S;
     int k = NL - 1;
     float c = 0.0f;
     for (int j = 0; j < NL; j++) {
-        c += lob[j * st];
-        if (u0 < c) {
+        c += guide::unpack_h2(__float_as_uint(lob[(NL + j) * st])).y;
+        if (t < c) {
             k = j;
             break;
         }
     }
-    float kap = lob[(NL + k) * st];
+    const guide::MixLobe m = guide::mix_lobe(__float_as_uint(lob[k * st]), __float_as_uint(lob[(NL + k) * st]));
+    const float kap = m.kap;
     float w = 1.0f + __logf(u1 + (1.0f - u1) * __expf(-2.0f * kap)) / kap;
     w = fminf(fmaxf(w, -1.0f), 1.0f);
     float r = sqrtf(fmaxf(0.0f, 1.0f - w * w));
     float sn, co;
     __sincosf(guide::TWO_PI * u2, &sn, &co);
-    float mx = lob[(2 * NL + k) * st], my = lob[(3 * NL + k) * st], mz = lob[(4 * NL + k) * st];
+    const float ik = 1.0f / kap;
+    float mx = m.kx * ik, my = m.ky * ik, mz = m.kz * ik;
     float sign = copysignf(1.0f, mz);
     float a = -1.0f / (sign + mz);
     float b = mx * my * a;
@@ -1583,7 +1594,7 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.film = c->film;
     a.film2 = c->film2;
     a.lobes = c->g_lobes;
-    a.lobes_stride = c->g_lobes_cap / (6 * (c->g_width == 128 ? 16 : 8));
+    a.lobes_stride = c->g_lobes_cap / (3 * (c->g_width == 128 ? 16 : 8));
     a.rec = c->g_rec;
     a.rec_flag = c->g_rec_flag;
     a.n_rec = 0;
@@ -2423,7 +2434,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     size_t nrec = 0;
     if (guided) {
         if (!c->has_guide) return fail(c, CPT_E_INVALID, "guided render needs cpt_guide_init");
-        const size_t lfloats = slots * 6 * (c->g_width == 128 ? 16 : 8);
+        const size_t lfloats = slots * 3 * (c->g_width == 128 ? 16 : 8);
         if (c->g_lobes_cap < lfloats || !c->g_lobes) {
             drop_graph(c);
             c->g_lobes_cap = 0;
@@ -3345,7 +3356,7 @@ int cpt_guide_mixtures(cpt_ctx* c, const float* inputs, int32_t n, float* mixtur
     if (n == 0) return CPT_OK;
     HIPCHK(c, hipSetDevice(c->device));
     const bool wide = c->g_width == 128;
-    const size_t nin = guide::NIN, nmix = 6 * (size_t)(wide ? guide128::NLOBES : guide::NLOBES);
+    const size_t nin = guide::NIN, nmix = 3 * (size_t)(wide ? guide128::NLOBES : guide::NLOBES);
     float *d_in = nullptr, *d_mix = nullptr;
     HIPCHK(c, hipMalloc((void**)&d_in, sizeof(float) * nin * (size_t)n));
     if (hipMalloc((void**)&d_mix, sizeof(float) * nmix * (size_t)n) != hipSuccess) {

@@ -137,6 +137,43 @@ __device__ __forceinline__ void gemm_rows_kn(const uint16_t* A, int lda, const u
     }
 }
 
+// Stored mixture of one sample, 3 values per lobe k (structure of arrays: value v
+// of lobe k at dst[(v NLOBES + k) stride]):
+//   v = 0: half2 (kappa mu_x, kappa mu_y)
+//   v = 1: half2 (kappa mu_z, w~), w~ the lobe weight rounded to half
+//   v = 2: c = log(w~ / S C(kappa')) - kappa' (float), S = sum of the w~, kappa' =
+//          |kappa mu| from the stored halves, C(k) = k / (2 pi (1 - exp(-2 k)))
+// so the mixture pdf is sum_k exp(kappa mu_k . w + c_k), and a sampler that picks
+// lobe k with probability w~_k / S and draws a vMF direction of concentration
+// kappa' about (kappa mu) / kappa' (all from the stored values) has exactly that
+// density; rounding the network's mixture to halves changes the guide, not the
+// agreement between sampling and pdf.
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_h2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
+}
+__device__ __forceinline__ float2 unpack_h2(uint32_t u) {
+    const h2v h = __builtin_bit_cast(h2v, u);
+    return make_float2((float)h.x, (float)h.y);
+}
+__device__ __forceinline__ float mix_log_term(float wh, float S, float kap) {
+    return __logf(wh / S * kap / (TWO_PI * (1.0f - __expf(-2.0f * kap)))) - kap;
+}
+// the stored halves of one lobe: kappa mu, the weight w~ and kappa' = |kappa mu|
+struct MixLobe {
+    float kx, ky, kz, w, kap;
+};
+__device__ __forceinline__ MixLobe mix_lobe(uint32_t p0, uint32_t p1) {
+    const float2 a = unpack_h2(p0), b = unpack_h2(p1);
+    MixLobe m;
+    m.kx = a.x;
+    m.ky = a.y;
+    m.kz = b.x;
+    m.w = b.y;
+    m.kap = sqrtf(m.kx * m.kx + m.ky * m.ky + m.kz * m.kz);
+    return m;
+}
+
 struct Lobes {
     float pi[NLOBES], kappa[NLOBES], mx[NLOBES], my[NLOBES], mz[NLOBES];
 };
@@ -977,9 +1014,9 @@ __global__ void k_pack_weights2(const float* params, uint16_t* gwt) {
 
 // Mixtures of a tile's samples from their raw outputs, as decode() but spread over
 // 4 consecutive threads per sample (NLOBES / 4 lobes each; the softmax maximum and
-// sum are combined across the 4 lanes); needs blockDim.x >= 4 TILE.  Value v of
-// sample s goes to dst[v * stride + s] (structure of arrays): weights, kappas, mean
-// x, y, z, then weight x vMF normalisation, NLOBES values each.
+// sum, and the sum S of the stored weights, are combined across the 4 lanes); needs
+// blockDim.x >= 4 TILE.  Stored as the 3 values per lobe described at MixLobe, value
+// v of sample s at dst[v * stride + s] (structure of arrays).
 __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* dst, size_t stride) {
     constexpr int LPT = NLOBES / 4;
     static_assert(NLOBES % 4 == 0, "lobes per thread");
@@ -1000,7 +1037,8 @@ __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* 
     sum += __shfl_xor(sum, 1);
     sum += __shfl_xor(sum, 2);
     const float inv = 1.0f / sum;
-    float* d = dst + s;
+    uint32_t p0[LPT], p1[LPT];
+    float wsum = 0.0f;
 #pragma unroll
     for (int j = 0; j < LPT; j++) {
         const int k = k0 + j;
@@ -1012,22 +1050,29 @@ __device__ __forceinline__ void decode_tile(const float* o_tile, int nt, float* 
             z = 1.0f;
             nrm = 1.0f;
         }
-        const float pi = e[j] * inv;
         const float kappa = __expf(fminf(fmaxf(o[5 * k + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
-        d[k * stride] = pi;
-        d[(NLOBES + k) * stride] = kappa;
-        d[(2 * NLOBES + k) * stride] = x / nrm;
-        d[(3 * NLOBES + k) * stride] = y / nrm;
-        d[(4 * NLOBES + k) * stride] = z / nrm;
-        // the lobe's weight times its vMF normalisation (vmf_pdf without the exponential)
-        d[(5 * NLOBES + k) * stride] = pi * kappa / (TWO_PI * (1.0f - __expf(-2.0f * kappa)));
+        const float kn = kappa / nrm;
+        p0[j] = pack_h2(kn * x, kn * y);
+        p1[j] = pack_h2(kn * z, e[j] * inv);
+        wsum += unpack_h2(p1[j]).y;
+    }
+    wsum += __shfl_xor(wsum, 1);
+    wsum += __shfl_xor(wsum, 2);
+    float* d = dst + s;
+#pragma unroll
+    for (int j = 0; j < LPT; j++) {
+        const int k = k0 + j;
+        const MixLobe m = mix_lobe(p0[j], p1[j]);
+        d[k * stride] = __uint_as_float(p0[j]);
+        d[(NLOBES + k) * stride] = __uint_as_float(p1[j]);
+        d[(2 * NLOBES + k) * stride] = mix_log_term(m.w, wsum, m.kap);
     }
 }
 
-// The path tracer's per-bounce inference, BF16: weights into LDS once per block,
-// then per tile of up to TILE samples (NIN features each at `in`) the forward pass
-// and the decoded mixtures, sample s's lobe k at dst[{k, NLOBES + k, ...} stride + s]
-// (weight, kappa, mean x, y, z, weight x normalisation).
+// The path tracer's per-bounce inference, BF16 block tiles (width 128): weights
+// into LDS once per block (or read through the L2), then per tile of up to TILE
+// samples (NIN features each at `in`) the forward pass and the stored mixtures,
+// sample s's lobe k at dst[{k, NLOBES + k, 2 NLOBES + k} stride + s].
 __device__ __forceinline__ void infer_tile(SmemFwd& sm, const uint16_t* gwt, float* o_tile, const float* in, int nt,
                                            float* dst, size_t stride) {
     encode_tile_window(sm, in, nt);
@@ -1037,10 +1082,11 @@ __device__ __forceinline__ void infer_tile(SmemFwd& sm, const uint16_t* gwt, flo
     __syncthreads();
 }
 
-// FP8: the mixtures of a wave's 16 rows from wave_forward8's registers, with the
-// same arithmetic as decode_tile: the lanes of group g = lane / 16 hold lobes
-// g LPG .. g LPG + LPG - 1 of sample row lane % 16, and the softmax maximum and sum
-// are combined across the 4 groups (the same pairs as decode_tile's 4 threads).
+// The mixtures of a wave's 16 rows from the registers of the per-wave forward
+// passes, with the same arithmetic as decode_tile: the lanes of group g = lane / 16
+// hold lobes g LPG .. g LPG + LPG - 1 of sample row lane % 16, and the softmax
+// maximum and sum and the weight sum S are combined across the 4 groups (the same
+// pairs as decode_tile's 4 threads).
 __device__ __forceinline__ void wave_decode(const f4 (&o)[NTO], int lane, int nrows, float* dst, size_t stride) {
     const int g = lane >> 4, row = lane & 15;
     float v[4 * NTO];
@@ -1061,12 +1107,11 @@ __device__ __forceinline__ void wave_decode(const f4 (&o)[NTO], int lane, int nr
     }
     sum += __shfl_xor(sum, 16);
     sum += __shfl_xor(sum, 32);
-    if (row >= nrows) return;
     const float inv = 1.0f / sum;
-    float* d = dst + row;
+    uint32_t p0[LPG], p1[LPG];
+    float wsum = 0.0f;
 #pragma unroll
     for (int i = 0; i < LPG; i++) {
-        const int k = g * LPG + i;
         float x = v[5 * i + 2], y = v[5 * i + 3], z = v[5 * i + 4];
         float nrm = sqrtf(x * x + y * y + z * z);
         if (nrm < 1e-12f) {
@@ -1075,14 +1120,23 @@ __device__ __forceinline__ void wave_decode(const f4 (&o)[NTO], int lane, int nr
             z = 1.0f;
             nrm = 1.0f;
         }
-        const float pi = e[i] * inv;
         const float kappa = __expf(fminf(fmaxf(v[5 * i + 1], KAPPA_LOG_MIN), KAPPA_LOG_MAX));
-        d[k * stride] = pi;
-        d[(NLOBES + k) * stride] = kappa;
-        d[(2 * NLOBES + k) * stride] = x / nrm;
-        d[(3 * NLOBES + k) * stride] = y / nrm;
-        d[(4 * NLOBES + k) * stride] = z / nrm;
-        d[(5 * NLOBES + k) * stride] = pi * kappa / (TWO_PI * (1.0f - __expf(-2.0f * kappa)));
+        const float kn = kappa / nrm;
+        p0[i] = pack_h2(kn * x, kn * y);
+        p1[i] = pack_h2(kn * z, e[i] * inv);
+        wsum += unpack_h2(p1[i]).y;
+    }
+    wsum += __shfl_xor(wsum, 16);
+    wsum += __shfl_xor(wsum, 32);
+    if (row >= nrows) return;
+    float* d = dst + row;
+#pragma unroll
+    for (int i = 0; i < LPG; i++) {
+        const int k = g * LPG + i;
+        const MixLobe m = mix_lobe(p0[i], p1[i]);
+        d[k * stride] = __uint_as_float(p0[i]);
+        d[(NLOBES + k) * stride] = __uint_as_float(p1[i]);
+        d[(2 * NLOBES + k) * stride] = mix_log_term(m.w, wsum, m.kap);
     }
 }
 

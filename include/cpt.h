@@ -352,11 +352,15 @@ int cpt_guide_set_params(cpt_ctx* ctx, const float* params_in);
 /* Raw network outputs (n x 48, or n x 80 at width 128) for n input vectors (n x 7
  * floats in [0,1]). */
 int cpt_guide_forward(cpt_ctx* ctx, const float* inputs, int32_t n, float* outputs);
-/* Decoded mixtures for n input vectors, computed by the code guided renders run per
+/* Stored mixtures for n input vectors, computed by the code guided renders run per
  * bounce once they have gathered a hit's inputs (its encode, forward pass in the
- * configured precision, and decode).  mixtures_out: 6 x lobes x n floats, value v of
- * sample s at v * n + s, in blocks of `lobes`: weights (softmax), kappas, mean x,
- * mean y, mean z, weight x vMF normalisation kappa / (2 pi (1 - exp(-2 kappa))). */
+ * configured precision, and decode).  mixtures_out: 3 x lobes x n 32-bit words,
+ * value v of sample s at v * n + s, in blocks of `lobes`: for lobe k, half2
+ * (kappa mu_x, kappa mu_y) and half2 (kappa mu_z, w~) as raw bits (half precision,
+ * low half first; kappa = exp(clamped log kappa), mu the unit mean, w~ the softmax
+ * weight), then the float c_k = log(w~_k / S C(kappa'_k)) - kappa'_k with S the sum of
+ * the w~, kappa' = |kappa mu| from the halves and C(k) = k / (2 pi (1 - exp(-2 k))).
+ * The mixture pdf of a direction w is sum_k exp(kappa mu_k . w + c_k). */
 int cpt_guide_mixtures(cpt_ctx* ctx, const float* inputs, int32_t n, float* mixtures_out);
 /* One training step on n samples (n x 11 floats: 7 inputs, direction xyz, weight =
  * target radiance / sampling pdf).  Loss = sum_i w_i (-log q(dir_i)) / sum_i w_i.
