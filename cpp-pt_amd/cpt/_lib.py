@@ -46,7 +46,7 @@ ABI_FUNCTIONS = (
     "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
     "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export", "cpt_debug_fail_alloc",
     "cpt_trace_closest_device_noalloc", "cpt_scene_file_load", "cpt_scene_file_free",
-    "cpt_guide_mixtures",
+    "cpt_guide_mixtures", "cpt_debug_guide_mixtures",
 )
 
 
@@ -152,6 +152,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_guide_set_params.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_guide_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
     lib.cpt_guide_mixtures.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    lib.cpt_debug_guide_mixtures.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
     lib.cpt_guide_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
     _lib = lib
     return lib
@@ -437,6 +438,22 @@ def _guide_forward(self, inputs):
     return out
 
 
+def _unpack_mixtures(raw, lobes):
+    h0 = raw[0].astype("<u4").view(np.float16).reshape(lobes, -1, 2).astype(np.float32)
+    h1 = raw[1].astype("<u4").view(np.float16).reshape(lobes, -1, 2).astype(np.float32)
+    return {"kmu_x": h0[..., 0], "kmu_y": h0[..., 1], "kmu_z": h1[..., 0], "weight": h1[..., 1],
+            "c": raw[2].view(np.float32)}
+
+
+def _debug_guide_mixtures(self, n):
+    """Stored mixtures of the first n queue entries of the last guided bounce
+    (cpt_debug_guide_mixtures), unpacked as guide_mixtures."""
+    lobes = self.guide_shape()[1]
+    raw = np.zeros((3, lobes, n), np.uint32)
+    self._check(self.lib.cpt_debug_guide_mixtures(self.h, n, _ptr(raw)), "cpt_debug_guide_mixtures")
+    return _unpack_mixtures(raw, lobes)
+
+
 def _guide_mixtures(self, inputs):
     """Stored mixtures of the per-bounce inference code, unpacked: a dict of
     (lobes, n) arrays kmu_x, kmu_y, kmu_z, weight (the stored halves as float32)
@@ -445,10 +462,7 @@ def _guide_mixtures(self, inputs):
     lobes = self.guide_shape()[1]
     raw = np.zeros((3, lobes, x.shape[0]), np.uint32)
     self._check(self.lib.cpt_guide_mixtures(self.h, _ptr(x), x.shape[0], _ptr(raw)), "cpt_guide_mixtures")
-    h0 = raw[0].astype("<u4").view(np.float16).reshape(lobes, -1, 2).astype(np.float32)
-    h1 = raw[1].astype("<u4").view(np.float16).reshape(lobes, -1, 2).astype(np.float32)
-    return {"kmu_x": h0[..., 0], "kmu_y": h0[..., 1], "kmu_z": h1[..., 0], "weight": h1[..., 1],
-            "c": raw[2].view(np.float32)}
+    return _unpack_mixtures(raw, lobes)
 
 
 def _guide_train_step(self, samples, apply=True, want_grad=False):
@@ -466,4 +480,5 @@ Context.guide_params = _guide_params
 Context.guide_set_params = _guide_set_params
 Context.guide_forward = _guide_forward
 Context.guide_mixtures = _guide_mixtures
+Context.debug_guide_mixtures = _debug_guide_mixtures
 Context.guide_train_step = _guide_train_step

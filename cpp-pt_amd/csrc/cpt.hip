@@ -3402,6 +3402,20 @@ int cpt_guide_mixtures(cpt_ctx* c, const float* inputs, int32_t n, float* mixtur
     return rc;
 }
 
+int cpt_debug_guide_mixtures(cpt_ctx* c, int32_t n, float* mixtures) {
+    if (!c || n < 0 || (n > 0 && !mixtures)) return CPT_E_INVALID;
+    if (!c->has_guide || !c->g_lobes) return fail(c, CPT_E_INVALID, "no guided render to read back");
+    const int L = c->g_width == 128 ? guide128::NLOBES : guide::NLOBES;
+    const size_t stride = c->g_lobes_cap / (3 * (size_t)L);
+    if ((size_t)n > stride) return fail(c, CPT_E_LIMIT, "only %zu queue entries are held", stride);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int v = 0; v < 3 * L; v++)
+        HIPCHK(c, hipMemcpy(mixtures + (size_t)v * n, c->g_lobes + (size_t)v * stride, sizeof(float) * (size_t)n,
+                            hipMemcpyDeviceToHost));
+    return CPT_OK;
+}
+
 int cpt_guide_train_step(cpt_ctx* c, const float* samples, int32_t n, int32_t apply, float* loss, float* grad) {
     using namespace guide;
     if (!c || n <= 0 || !samples) return CPT_E_INVALID;

@@ -362,6 +362,12 @@ int cpt_guide_forward(cpt_ctx* ctx, const float* inputs, int32_t n, float* outpu
  * the w~, kappa' = |kappa mu| from the halves and C(k) = k / (2 pi (1 - exp(-2 k))).
  * The mixture pdf of a direction w is sum_k exp(kappa mu_k . w + c_k). */
 int cpt_guide_mixtures(cpt_ctx* ctx, const float* inputs, int32_t n, float* mixtures_out);
+/* Test hook, no counterpart in the reference: the stored mixtures of queue entries
+ * 0 .. n-1 of the last guided bounce of the last guided render (after a render of
+ * max_depth 1 and one sample per pixel, entry e is pixel e's camera-ray hit), laid
+ * out as cpt_guide_mixtures' output.  Lets a test check the per-bounce gather of the
+ * hit's inputs as well as the network. */
+int cpt_debug_guide_mixtures(cpt_ctx* ctx, int32_t n, float* mixtures_out);
 /* One training step on n samples (n x 11 floats: 7 inputs, direction xyz, weight =
  * target radiance / sampling pdf).  Loss = sum_i w_i (-log q(dir_i)) / sum_i w_i.
  * loss_out (may be NULL) = that loss; grad_out (may be NULL, one float per
