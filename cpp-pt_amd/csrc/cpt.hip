@@ -156,9 +156,11 @@ struct WfArgs {
 // after it [17, 20)
 #define REC_FLOATS 20
 
-// waves per SIMD the traversal kernels are compiled for (caps their VGPRs)
+// waves per SIMD the traversal kernels are compiled for: caps their VGPRs at 96
+// (k_extend then keeps only the overflow-stack base in scratch, read on the
+// overflow path alone)
 #ifndef CPT_TRAV_WAVES
-#define CPT_TRAV_WAVES 1
+#define CPT_TRAV_WAVES 5
 #endif
 
 // idle lanes (of 64) that make a traversal wave fetch new rays of its chunk
@@ -334,8 +336,8 @@ __global__ void k_raygen_lt(WfArgs a) {
 
 // ------------------------------------------------------------------ extend
 template <int B, bool COUNT>
-__global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_extend(WfArgs a, int cur) {
-    __shared__ int lds[CPT_LDS_SS * B];
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CPT_TRAV_WAVES))) void k_extend(WfArgs a, int cur) {
+    __shared__ int lds[CPT_LDS_ROWS * B];
     const unsigned count = a.counters[2 * cur];
     if (gtid() == 0) {
         a.counters[2 * (cur ^ 1)] = 0;
@@ -1037,8 +1039,8 @@ __global__ void k_sd_splat(WfArgs a, unsigned n_paths) {
 
 // ------------------------------------------------------------------ shadow
 template <int B, bool COUNT>
-__global__ __launch_bounds__(B, CPT_TRAV_WAVES) void k_shadow(WfArgs a, int nxt) {
-    __shared__ int lds[CPT_LDS_SS * B];
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CPT_TRAV_WAVES))) void k_shadow(WfArgs a, int nxt) {
+    __shared__ int lds[CPT_LDS_ROWS * B];
     const unsigned count = a.counters[2 * nxt + 1];
     if (gtid() == 0) atomicAdd(&a.stats[ST_SHADOW], (unsigned long long)count);
     TravStack<B> st{lds, a.gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
@@ -1235,7 +1237,7 @@ __global__ void k_fill_u64(unsigned long long* p, int n) {
 template <int B>
 __global__ __launch_bounds__(B) void k_trace_closest(DevScene sc, const float4* rays, int n, float4* hits, int* gstack,
                                                      unsigned refill) {
-    __shared__ int lds[CPT_LDS_SS * B];
+    __shared__ int lds[CPT_LDS_ROWS * B];
     TravStack<B> st{lds, gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
     unsigned nv = 0, nt = 0;
     trace_chunked<B, false, false>(
@@ -1252,7 +1254,7 @@ __global__ __launch_bounds__(B) void k_trace_closest(DevScene sc, const float4* 
 template <int B>
 __global__ __launch_bounds__(B) void k_trace_any(DevScene sc, const float4* rays, int n, uint8_t* occ, int* gstack,
                                                  unsigned refill) {
-    __shared__ int lds[CPT_LDS_SS * B];
+    __shared__ int lds[CPT_LDS_ROWS * B];
     TravStack<B> st{lds, gstack + gtid(), (size_t)gthreads(), (int)threadIdx.x};
     unsigned nv = 0, nt = 0;
     trace_chunked<B, true, false>(
@@ -1960,6 +1962,8 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
         c->bvh_build_us = (int64_t)std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - tb0).count();
         if (3 * bvh.max_depth + 1 > CPT_LDS_SS + CPT_GSTACK)
             return fail(c, CPT_E_LIMIT, "BVH depth %d exceeds the traversal stack", bvh.max_depth);
+        if (bvh.nodes.size() > CPT_MAX_NODES)  // traversal addresses nodes by 32-bit byte offsets
+            return fail(c, CPT_E_LIMIT, "BVH of %zu nodes exceeds %u", bvh.nodes.size(), CPT_MAX_NODES);
         c->bvh_nodes = (int64_t)bvh.nodes.size();
         c->bvh_leaves = bvh.leaves;
         c->bvh_depth = bvh.max_depth;
@@ -2094,6 +2098,8 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
         c->bvh_build_us = (int64_t)(gb.ms * 1000.0);
         if (3 * gb.max_depth + 1 > CPT_LDS_SS + CPT_GSTACK)
             return fail(c, CPT_E_LIMIT, "BVH depth %d exceeds the traversal stack", gb.max_depth);
+        if (gb.n_nodes > (int64_t)CPT_MAX_NODES)  // traversal addresses nodes by 32-bit byte offsets
+            return fail(c, CPT_E_LIMIT, "BVH of %lld nodes exceeds %u", (long long)gb.n_nodes, CPT_MAX_NODES);
     }
     if (d->n_point_lights > 0) {
         std::vector<float4> pl(2 * (size_t)d->n_point_lights);

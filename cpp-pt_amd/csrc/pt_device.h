@@ -120,6 +120,8 @@ struct Hit {
 #define CPT_LDS_SS 16
 #endif
 #define CPT_GSTACK (128 - CPT_LDS_SS)
+// nodes are addressed by 32-bit byte offsets (index << 7)
+#define CPT_MAX_NODES ((1u << 25) - 1u)
 // conservative culling: a box may be entered up to this relative distance
 // beyond the current best t, so that triangles tying with the best t in other
 // leaves are still tested (ties are decided by rank, not by visit order)
@@ -129,9 +131,13 @@ struct Hit {
 __device__ __forceinline__ unsigned gthreads_() { return gridDim.x * blockDim.x; }
 __device__ __forceinline__ unsigned gtid_() { return blockIdx.x * blockDim.x + threadIdx.x; }
 
+// LDS rows of the traversal stack: CPT_LDS_SS entries plus one row that takes
+// the writes of absent children, so that pushes need no branch.
+#define CPT_LDS_ROWS (CPT_LDS_SS + 1)
+
 template <int BLOCK>
 struct TravStack {
-    int* lds;
+    int* lds;  // CPT_LDS_ROWS x BLOCK
     int* gmem;
     size_t gstride;
     int tid;
@@ -172,18 +178,26 @@ __device__ __forceinline__ bool tri_test(const float4& A, const float4& E1, cons
 // instruction per plane.  That form has an absolute error of up to one rounding
 // of o * inv (plus the rounding of c), so c is moved by e = 2^-21 |o * inv| (four
 // times that error) toward smaller t for the plane that is nearer along the axis
-// and toward larger t for the other: the computed entry never exceeds the exact
-// entry and the computed exit never falls short of the exact exit, beyond one
-// relative rounding of their own, which CPT_BOX_SLACK and CPT_TIE_SLACK cover.
-// Culling stays conservative; only boxes within e of the ray's slabs are entered
-// that the exact test would skip.
-__device__ __forceinline__ void slab_offsets(float o, float inv, float& c_lo, float& c_hi) {
+// (c_near) and toward larger t for the other (c_far): the computed entry never
+// exceeds the exact entry and the computed exit never falls short of the exact
+// exit, beyond one relative rounding of their own, which CPT_BOX_SLACK and
+// CPT_TIE_SLACK cover.  Culling stays conservative; only boxes within e of the
+// ray's slabs are entered that the exact test would skip.
+__device__ __forceinline__ void slab_offsets(float o, float inv, float& c_near, float& c_far) {
     const float oi = o * inv;
     const float e = fabsf(oi) * 4.76837158203125e-7f;  // 2^-21
-    const float c_near = -oi - e, c_far = -oi + e;
-    const bool pos = inv >= 0.0f;  // the low plane is the near one
-    c_lo = pos ? c_near : c_far;
-    c_hi = pos ? c_far : c_near;
+    c_near = -oi - e;
+    c_far = -oi + e;
+}
+
+// Byte offset inside a Node4 of the child planes a ray meets first along an
+// axis: the low planes (lox at 0, loy at 32, loz at 64) when the direction's
+// sign bit is clear (its reciprocal, with zeros replaced by +-1e-20 of the same
+// sign, is then >= 0), the high planes (16 bytes later) otherwise; the far
+// planes are the other ones (offset ^ 16).  Loading the near and far planes by
+// these per-lane offsets replaces a min and a max per axis and child.
+__device__ __forceinline__ uint32_t near_plane_offset(float dir, uint32_t axis_base) {
+    return axis_base + ((__float_as_uint(dir) >> 31) << 4);
 }
 
 // Chunked persistent traversal.  Wave w of the grid owns rays
@@ -194,7 +208,9 @@ __device__ __forceinline__ void slab_offsets(float o, float inv, float& c_lo, fl
 // (four slab tests, near-first push) and one triangle test of a pending leaf:
 // a leaf met during the node step is parked in `leaf` until the previous leaf's
 // triangles are done.  Culling stays conservative and ties are decided by rank,
-// so the result does not depend on this visiting order.
+// so the result does not depend on this visiting order.  Pushes and pops use the LDS rows directly while every lane of the
+// wave stays within them; a wave with a lane beyond them takes the slower path
+// through the global overflow region.
 //   load(i, o, d, tmax)   fetches ray i of the queue
 //   done(i, hit)          receives the closest hit (ANY: h.tri >= 0 iff occluded)
 // no node pending.  As a leaf reference -2 would be 16 triangles from index
@@ -210,11 +226,13 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
     unsigned nextr = min(count, w * per);
     const unsigned endr = min(count, nextr + per);
     const unsigned long long lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+    const char* nbase = reinterpret_cast<const char*>(sc.nodes);
     bool has = false, fresh = false;
     unsigned ray = 0;
     V3 o = {0, 0, 0}, d = {0, 0, 0};
     float ix = 0, iy = 0, iz = 0, best = 0;
-    float cxl = 0, cxh = 0, cyl = 0, cyh = 0, czl = 0, czh = 0;  // slab offsets
+    float cxn = 0, cxf = 0, cyn = 0, cyf = 0, czn = 0, czf = 0;  // slab offsets
+    uint32_t oxn = 0, oyn = 32, ozn = 64;                        // near-plane byte offsets
     int best_rank = INT_MIN;
     Hit h;
     h.t = 0.0f;
@@ -246,6 +264,9 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
                 ray = r;
                 has = true;
                 fresh = true;
+                oxn = near_plane_offset(d.x, 0u);
+                oyn = near_plane_offset(d.y, 32u);
+                ozn = near_plane_offset(d.z, 64u);
             }
             nextr = min(endr, nextr + nidle);
         }
@@ -255,7 +276,7 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
         // the node), then the triangle is tested and the node's boxes culled
         // against the possibly improved best.
         const bool do_tri = tcur < tend, do_node = node >= 0;
-        float4 A, E1, E2, lx, hx, ly, hy, lz, hz;
+        float4 A, E1, E2, nx, fx, ny, fy, nz, fz;
         int4 ch;
         if (do_tri) {
             const float4* tp = sc.tris + 3 * (size_t)tcur;
@@ -264,14 +285,14 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
             E2 = tp[2];
         }
         if (do_node) {
-            const float4* np = reinterpret_cast<const float4*>(sc.nodes + node);
-            lx = np[0];
-            hx = np[1];
-            ly = np[2];
-            hy = np[3];
-            lz = np[4];
-            hz = np[5];
-            ch = reinterpret_cast<const int4*>(np)[6];
+            const uint32_t no = (uint32_t)node << 7;
+            ch = *reinterpret_cast<const int4*>(nbase + (no + 96u));
+            nx = *reinterpret_cast<const float4*>(nbase + (no + oxn));
+            fx = *reinterpret_cast<const float4*>(nbase + (no + (oxn ^ 16u)));
+            ny = *reinterpret_cast<const float4*>(nbase + (no + oyn));
+            fy = *reinterpret_cast<const float4*>(nbase + (no + (oyn ^ 16u)));
+            nz = *reinterpret_cast<const float4*>(nbase + (no + ozn));
+            fz = *reinterpret_cast<const float4*>(nbase + (no + (ozn ^ 16u)));
         }
         if (fresh) {
             // 1/d with zero components replaced by a tiny signed value so that
@@ -282,9 +303,9 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
             ix = 1.0f / dx;
             iy = 1.0f / dy;
             iz = 1.0f / dz;
-            slab_offsets(o.x, ix, cxl, cxh);
-            slab_offsets(o.y, iy, cyl, cyh);
-            slab_offsets(o.z, iz, czl, czh);
+            slab_offsets(o.x, ix, cxn, cxf);
+            slab_offsets(o.y, iy, cyn, cyf);
+            slab_offsets(o.z, iz, czn, czf);
             fresh = false;
         }
         bool finished = false;
@@ -316,21 +337,20 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
             float lim = ANY ? best * CPT_BOX_SLACK : best * CPT_TIE_SLACK;
             float key[4];
             int cid[4];
-#define CPT_BOX(c, LX, HX, LY, HY, LZ, HZ, CH)                                   \
+            // near planes with the c_near offsets, far planes with c_far: the
+            // same values as min and max of the low/high plane pair
+#define CPT_BOX(c, NX, FX, NY, FY, NZ, FZ, CH)                                    \
     {                                                                             \
-        float t0 = fmaf(LX, ix, cxl), t1 = fmaf(HX, ix, cxh);                     \
-        float t2 = fmaf(LY, iy, cyl), t3 = fmaf(HY, iy, cyh);                     \
-        float t4 = fmaf(LZ, iz, czl), t5 = fmaf(HZ, iz, czh);                     \
-        float tn = fmaxf(fmaxf(fminf(t0, t1), fminf(t2, t3)), fmaxf(fminf(t4, t5), 0.0f)); \
-        float tf = fminf(fminf(fmaxf(t0, t1), fmaxf(t2, t3)), fmaxf(t4, t5)) * CPT_BOX_SLACK; \
+        float tn = fmaxf(fmaxf(fmaf(NX, ix, cxn), fmaf(NY, iy, cyn)), fmaxf(fmaf(NZ, iz, czn), 0.0f)); \
+        float tf = fminf(fminf(fmaf(FX, ix, cxf), fmaf(FY, iy, cyf)), fmaf(FZ, iz, czf)) * CPT_BOX_SLACK; \
         bool hit = tn <= tf && tn <= lim;                                         \
         key[c] = hit ? tn : INFINITY;                                             \
         cid[c] = hit ? CH : EMPTY_CHILD;                                          \
     }
-            CPT_BOX(0, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x)
-            CPT_BOX(1, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y)
-            CPT_BOX(2, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z)
-            CPT_BOX(3, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w)
+            CPT_BOX(0, nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, ch.x)
+            CPT_BOX(1, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, ch.y)
+            CPT_BOX(2, nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, ch.z)
+            CPT_BOX(3, nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, ch.w)
 #undef CPT_BOX
 #define CPT_CSWAP(a, b)                                  \
     if (key[b] < key[a]) {                               \
@@ -348,9 +368,20 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
                 CPT_CSWAP(1, 2)
             }
 #undef CPT_CSWAP
-            if (cid[3] != EMPTY_CHILD) st.put(sp++, cid[3]);
-            if (cid[2] != EMPTY_CHILD) st.put(sp++, cid[2]);
-            if (cid[1] != EMPTY_CHILD) st.put(sp++, cid[1]);
+            // the children present are a prefix of cid (absent ones sort last):
+            // push cid[3], cid[2], cid[1] of those present, deepest first
+            const int v3 = cid[3] != EMPTY_CHILD, v2 = cid[2] != EMPTY_CHILD, v1 = cid[1] != EMPTY_CHILD;
+            if (__ballot(sp + 3 > CPT_LDS_SS) == 0ull) {
+                const int s2 = sp + v3, s1 = s2 + v2;
+                st.lds[(v3 ? sp : CPT_LDS_SS) * BLOCK + st.tid] = cid[3];
+                st.lds[(v2 ? s2 : CPT_LDS_SS) * BLOCK + st.tid] = cid[2];
+                st.lds[(v1 ? s1 : CPT_LDS_SS) * BLOCK + st.tid] = cid[1];
+                sp = s1 + v1;
+            } else {
+                if (v3) st.put(sp++, cid[3]);
+                if (v2) st.put(sp++, cid[2]);
+                if (v1) st.put(sp++, cid[1]);
+            }
             node = cid[0] != EMPTY_CHILD ? cid[0] : CPT_NONE;
         }
         if (!finished) {
@@ -365,7 +396,13 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
                 tend = tcur + ((leaf >> 27) & 15) + 1;
                 leaf = CPT_NONE;
             }
-            if (node == CPT_NONE && sp > 0) node = st.get(--sp);
+            const bool pop = node == CPT_NONE && sp > 0;
+            if (__ballot(pop && sp > CPT_LDS_SS) == 0ull) {
+                if (pop) node = st.lds[(sp - 1) * BLOCK + st.tid];
+            } else if (pop) {
+                node = st.get(sp - 1);
+            }
+            sp -= pop ? 1 : 0;
             finished = tcur >= tend && leaf == CPT_NONE && node == CPT_NONE;
         }
         if (finished) {
