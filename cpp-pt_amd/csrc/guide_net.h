@@ -446,10 +446,11 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* gwt, const 
 // FP8 inference (e4m3, OCP format of gfx950) on v_mfma_f32_16x16x32_fp8_fp8.
 // Weights: per layer scaled by 448 / max|W_l| and rounded to e4m3 (k_quant8);
 // one-blob features scaled by 448; each hidden layer's ReLU outputs scaled per
-// 16-row group (one wave's rows of the tile) by 448 / (their maximum).  FP32
-// accumulation; the accumulator is rescaled by the two inverse scales before the
-// bias.  The conversion does not saturate (it returns NaN above 448), so every
-// value is clamped first.  Lane layout of the 16x16x32 MFMA: lane l supplies
+// 16-row group (one wave's rows of the tile) by the largest power of two that
+// keeps their maximum <= 448.  FP32 accumulation; the accumulator is rescaled by
+// the two inverse scales before the bias.  The conversions do not saturate (they
+// return NaN above 448), so weights and features are clamped first and the
+// activation scale never exceeds the range.  Lane layout of the 16x16x32 MFMA: lane l supplies
 // A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j = 0..7, as 8 bytes.
 // Training keeps the BF16 path; only inference uses these weights.
 constexpr int PAD8 = 8;             // byte padding of FP8 LDS rows
@@ -597,11 +598,26 @@ __device__ __forceinline__ void gemm8_rows_nk_t(const uint8_t* A, int lda, const
     }
 }
 
-// four non-negative values (<= 448 up to rounding) as packed e4m3
-__device__ __forceinline__ uint32_t fp8x4_pos(float v0, float v1, float v2, float v3) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(fminf(v0, FP8_MAX), fminf(v1, FP8_MAX), 0, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(fminf(v2, FP8_MAX), fminf(v3, FP8_MAX), 0, false);
-    return (lo & 0xffffu) | (hi << 16);
+// Four values as packed e4m3 after division by rs (gfx950's
+// v_cvt_scalef32_pk_fp8_f32: round to nearest even, no saturation, checked on the
+// device with an in-bounds probe); the callers' rs keep every quotient <= 448.
+typedef short fp8pk_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t fp8x4_scaled(float v0, float v1, float v2, float v3, float rs) {
+    fp8pk_t p = {0, 0};
+    p = __builtin_amdgcn_cvt_scalef32_pk_fp8_f32(p, v0, v1, rs, false);
+    p = __builtin_amdgcn_cvt_scalef32_pk_fp8_f32(p, v2, v3, rs, true);
+    return __builtin_bit_cast(uint32_t, p);
+}
+
+// Exponent k of the power-of-two activation scale 2^k for a group maximum mx > 0:
+// the largest k with mx 2^k <= 448 (mx = m 2^e, m in [0.5, 1): 448 / m lies in
+// (448, 896], so k = 8 - e when m > 0.875, else 9 - e), capped at 126 so that 2^-k
+// stays a normal float.  Scaling by a power of two is exact, so the scaled values
+// need no clamp and the inverse scale is exact.
+__device__ __forceinline__ int act_scale_exp(float mx) {
+    const float m = __builtin_amdgcn_frexp_mantf(mx);
+    const int e = __builtin_amdgcn_frexp_expf(mx);
+    return min((m > 0.875f ? 8 : 9) - e, 126);
 }
 
 // Forward pass of the wave's 16 rows (inputs at in16, nrows valid) through all
@@ -633,15 +649,13 @@ __device__ __forceinline__ void wave_forward8(const SmemFwd8& sm, WaveState8& ws
                 mx = fmaxf(mx, z);
             }
         }
-        // one scale per wave (16 rows)
+        // one power-of-two scale per wave (16 rows)
         for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-        const float s = mx > 0.0f ? FP8_MAX / mx : 1.0f;
-        inv_a = mx > 0.0f ? mx / FP8_MAX : 1.0f;
+        inv_a = mx > 0.0f ? __builtin_amdgcn_ldexpf(1.0f, -act_scale_exp(mx)) : 1.0f;
         uint8_t* hd = ws.h + m * LQH + n4;
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
-            *reinterpret_cast<uint32_t*>(hd + nt * 16) =
-                fp8x4_pos(acc[nt][0] * s, acc[nt][1] * s, acc[nt][2] * s, acc[nt][3] * s);
+            *reinterpret_cast<uint32_t*>(hd + nt * 16) = fp8x4_scaled(acc[nt][0], acc[nt][1], acc[nt][2], acc[nt][3], inv_a);
         wave_sync();  // the next layer reads what the other lanes just wrote
     }
 #pragma unroll

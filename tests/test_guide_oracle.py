@@ -104,12 +104,35 @@ def e4m3(x):
     return np.sign(x) * np.round(a / step) * step
 
 
+def act_scale(M):
+    """Power-of-two scale 2^k of a group of activations with maximum M: the largest
+    k with M 2^k <= 448, capped at 126 (M = m 2^e, m in [0.5, 1): k = 8 - e when
+    m > 0.875, else 9 - e); 1 for M = 0."""
+    M = np.asarray(M, np.float64)
+    m, e = np.frexp(np.where(M > 0, M, 1.0))
+    k = np.minimum(np.where(m > 0.875, 8, 9) - e, 126)
+    return np.where(M > 0, np.ldexp(1.0, k), 1.0)
+
+
+def test_act_scale_is_the_largest_power_of_two_within_range():
+    rng = np.random.default_rng(3)
+    M = np.concatenate([np.exp(rng.uniform(-80, 80, 20000)), [448.0, 448.0 * (1 + 2.0 ** -52), 224.0, 1.0, 0.875, 1e-38]])
+    s = act_scale(M)
+    assert (np.log2(s) == np.round(np.log2(s))).all()       # powers of two
+    assert (M * s <= 448.0).all()                          # within range, no clamp needed
+    big = np.log2(s) < 126
+    assert (M[big] * s[big] * 2 > 448.0).all()             # and the largest such
+    assert act_scale(448.0) == 1.0 and act_scale(449.0) == 0.5 and act_scale(1.0) == 256.0
+    assert act_scale(0.0) == 1.0 and act_scale(1e-40) == 2.0 ** 126
+
+
 def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64, width=64, group=16):
     """FP8 inference pipeline of the guide network restated in numpy: weights
     scaled per layer by 448 / max|W|, one-blob features by 448, each hidden
-    layer's ReLU outputs per group of 16 rows of a 64-sample tile by 448 /
-    (group maximum), the padding rows of a partial tile taking part in that
-    maximum as zero-feature samples.  quant=identity gives the exact network."""
+    layer's ReLU outputs per group of 16 rows of a 64-sample tile by the largest
+    power of two 2^k with (group maximum) 2^k <= 448 (k capped at 126), the padding
+    rows of a partial tile taking part in that maximum as zero-feature samples.
+    quant=identity gives the exact network."""
     K, N, _, _ = net_shape(width)
     P = np.asarray(params, np.float64)
     Ws, bs, o = [], [], 0
@@ -137,8 +160,8 @@ def mlp_forward_fp8(params, inputs, quant=e4m3, tile=64, width=64, group=16):
         for l in range(4):
             z = np.maximum(h @ Wq[l] * (inv_a * inv_w[l]) + bs[l], 0.0)
             M = np.repeat(z.reshape(tile // group, -1).max(axis=1), group)[:, None]
-            s = np.where(M > 0, 448.0 / np.where(M > 0, M, 1.0), 1.0)
-            inv_a = np.where(M > 0, M / 448.0, 1.0)
+            s = act_scale(M)
+            inv_a = 1.0 / s
             h = quant(z * s)
         out[t0:t0 + len(xs)] = (h @ Wq[4] * (inv_a * inv_w[4]) + bs[4])[:len(xs)]
     return out
