@@ -1749,7 +1749,7 @@ int issue_pass(cpt_ctx* c, const LaunchCfg& L, const WfArgs& a, hipStream_t s, T
         L_extend(L, a, cur, s);
         mark(0);
         if (L.guided && b < a.guide_depth) {
-            // FP8: one block per CU, as many waves as the LDS holds (16 at width 64, 8 at 128)
+            // FP8: one block per CU (16 waves at width 64, 12 at 128)
             if (a.guide_lobes == 16) {
                 if (a.guide_fp8)
                     hipLaunchKernelGGL(k_guide_infer8<guide128::Net>, dim3(L.infer_blocks),

@@ -454,14 +454,16 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* gwt, const 
 // A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j = 0..7, as 8 bytes.
 // Training keeps the BF16 path; only inference uses these weights.
 constexpr int PAD8 = 8;             // byte padding of FP8 LDS rows
-constexpr int LQ0 = K0 + PAD8;      // 232
+constexpr int K0P = 256;            // layer-0 K padded to two MFMA steps of 128 (zero weights beyond K0)
+static_assert(K0 <= K0P && K0P == 8 * NBINS, "layer-0 steps: 4 inputs of 32 bins each");
+constexpr int LQ0 = K0P + PAD8;     // 264
 constexpr int LQH = HID + PAD8;     // 72 or 136
 constexpr int Q0 = 0;
 constexpr int Q1 = Q0 + HID * LQ0;
 constexpr int Q2 = Q1 + HID * LQH;
 constexpr int Q3 = Q2 + HID * LQH;
 constexpr int Q4 = Q3 + HID * LQH;
-constexpr int NWQ = Q4 + NOUT * LQH;  // 32128 or 92800 bytes
+constexpr int NWQ = Q4 + NOUT * LQH;  // 34176 or 96896 bytes
 constexpr float FP8_MAX = 448.0f;
 
 __device__ __forceinline__ uint32_t fp8x2(float a, float b) {
@@ -472,10 +474,15 @@ __device__ __forceinline__ uint32_t fp8x2(float a, float b) {
 __device__ __forceinline__ uint8_t fp8x1(float a) { return (uint8_t)(fp8x2(a, 0.0f) & 0xffu); }
 
 // FP8 inference runs one wave per tile of 16 samples: the block holds one copy
-// of the FP8 weights in LDS and each wave its own tile state, so after the weight
-// load no wave waits for another.  WAVES8 waves per block: as many as the LDS
-// left beside the weights holds (16 at width 64: 140 KB; 8 at width 128: 157 KB).
-constexpr int WAVES8 = HID <= 64 ? 16 : 8;
+// of the FP8 weights in LDS and each wave its own tile state (inputs and one
+// activation tile; the one-blob features are formed in registers), so after the
+// weight load no wave waits for another.  Waves per block: 16 at width 64 (four
+// per SIMD, 83 KB of LDS); 12 at width 128 (three per SIMD, 147 KB), whose
+// forward pass and 16-lobe decode need more than the 128 registers of four.
+#ifndef CPT_WAVES8_128
+#define CPT_WAVES8_128 12
+#endif
+constexpr int WAVES8 = HID <= 64 ? 16 : CPT_WAVES8_128;
 constexpr int LPG = NLOBES / 4;           // lobes per 16-lane group of the output tile
 constexpr int PADG = 4 * NTO - 5 * LPG;   // output slots per group beyond its lobes
 static_assert(NLOBES % 4 == 0 && PADG >= 0 && NOUT == 5 * NLOBES + 4 * PADG, "output tile layout");
@@ -497,7 +504,6 @@ __host__ __device__ constexpr int out_slot(int n) {
 
 struct WaveState8 {
     float in[64 * NIN];   // inputs of up to 64 samples (one per lane when gathered)
-    uint8_t x0[16 * LQ0]; // one-blob features of the current 16 rows, x 448
     uint8_t h[16 * LQH];  // hidden activations; each layer overwrites its input
 };
 struct SmemFwd8 {
@@ -529,31 +535,35 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// one-blob features x 448 as FP8 for the wave's 16 rows (inputs at in16, rows
-// beyond nrows zero): per (sample, input) pair the 32 bins are zeroed and the 16
-// of blob_window written (the others are 0 in e4m3), two exponentials per run of 8
-// (blob_run)
-__device__ __forceinline__ void encode_rows8(uint8_t* x0, const float* in16, int nrows, int lane) {
-    static_assert(NBINS == 32 && LQ0 % 8 == 0, "8-byte zero stores, 4-byte window stores");
-    for (int p = lane; p < 16 * NIN; p += 64) {
-        const int s = p / NIN, d = p - s * NIN;
-        uint8_t* seg = x0 + s * LQ0 + d * NBINS;
+// Layer-0 operand of one lane for the K step of 128 that covers inputs
+// 4 step .. 4 step + 3: lane l supplies row l % 16, bytes 32 (l / 16) .. + 31 of
+// the step, i.e. the 32 one-blob bins (x 448, e4m3) of input 4 step + l / 16 of
+// sample row l % 16.  The 16 bins of blob_window come from two runs of 8
+// (blob_run: two exponentials each; a longer run from a clamped window start
+// could begin in FP32 underflow and carry the zero to the peak), every other bin
+// is 0 (below e^-15.1, which e4m3 rounds to 0), and so are the padding input 7
+// and rows at or beyond nrows.
+typedef int i8v __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ i8v blob_operand8(const float* in16, int nrows, int step, int lane) {
+    const int m = lane & 15, d = 4 * step + (lane >> 4);
+    i8v a = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (m < nrows && d < NIN) {
+        const float x = in16[m * NIN + d];
+        const int w0 = blob_window(x);
+        float v[16];
+        blob_run<8>(x, w0, FP8_MAX, v);
+        blob_run<8>(x, w0 + 8, FP8_MAX, v + 8);
+        int wp[4];
 #pragma unroll
-        for (int k = 0; k < NBINS / 8; k++) reinterpret_cast<uint2*>(seg)[k] = make_uint2(0u, 0u);
-        if (s < nrows) {
-            const float x = in16[s * NIN + d];
-            const int w0 = blob_window(x);
-            // two runs of 8, so that no run starts farther than 9.5 bins from its
-            // peak-side end (a longer run from a clamped window start could begin in
-            // FP32 underflow and carry the zero to the peak)
-            float v[16];
-            blob_run<8>(x, w0, FP8_MAX, v);
-            blob_run<8>(x, w0 + 8, FP8_MAX, v + 8);
-            uint32_t* wp = reinterpret_cast<uint32_t*>(seg + w0);
+        for (int k = 0; k < 4; k++) wp[k] = (int)(fp8x2(v[4 * k], v[4 * k + 1]) | (fp8x2(v[4 * k + 2], v[4 * k + 3]) << 16));
+        const int q0 = w0 >> 2;  // first 4-bin word of the window (0..4)
 #pragma unroll
-            for (int k = 0; k < 4; k++) wp[k] = fp8x2(v[4 * k], v[4 * k + 1]) | (fp8x2(v[4 * k + 2], v[4 * k + 3]) << 16);
+        for (int i = 0; i < 8; i++) {
+            const int k = i - q0;
+            a[i] = k == 0 ? wp[0] : k == 1 ? wp[1] : k == 2 ? wp[2] : k == 3 ? wp[3] : 0;
         }
     }
+    return a;
 }
 
 // acc[nt] += (A(rows of this wave) x Wt^T)^T: the weights are the MFMA's first
@@ -563,7 +573,6 @@ __device__ __forceinline__ void encode_rows8(uint8_t* x0, const float* in16, int
 // scales; lane l supplies row l % 16, bytes 32 (l / 16) .. 32 (l / 16) + 31 of the
 // step, a layout checked on the device with a one-wave probe), the rest of K steps
 // of 32 on v_mfma_f32_16x16x32_fp8_fp8 (8 bytes per lane at 8 (l / 16)).
-typedef int i8v __attribute__((ext_vector_type(8)));
 __device__ __forceinline__ i8v ld32b(const uint8_t* p) {
     const int2 q0 = *reinterpret_cast<const int2*>(p), q1 = *reinterpret_cast<const int2*>(p + 8);
     const int2 q2 = *reinterpret_cast<const int2*>(p + 16), q3 = *reinterpret_cast<const int2*>(p + 24);
@@ -625,16 +634,26 @@ __device__ __forceinline__ int act_scale_exp(float mx) {
 // holding features 16 nt + 4 (l / 16) + j of sample row l % 16 (out_orig order).
 __device__ __forceinline__ void wave_forward8(const SmemFwd8& sm, WaveState8& ws, const float* in16, int nrows,
                                               int lane, f4 (&o)[NTO]) {
-    encode_rows8(ws.x0, in16, nrows, lane);
-    wave_sync();
     const int m = lane & 15;         // this lane's sample row
     const int n4 = 4 * (lane >> 4);  // its first feature within a 16-wide tile
     float inv_a = 1.0f / FP8_MAX;    // inverse scale of the current layer's input
 #pragma unroll
     for (int l = 0; l < 4; l++) {
         f4 acc[NTH] = {};
-        if (l == 0) gemm8_rows_nk_t<K0, NTH>(ws.x0, LQ0, sm.wt + Q0, LQ0, acc, lane);
-        else gemm8_rows_nk_t<HID, NTH>(ws.h, LQH, sm.wt + (l == 1 ? Q1 : l == 2 ? Q2 : Q3), LQH, acc, lane);
+        if (l == 0) {
+#pragma unroll
+            for (int step = 0; step < K0P / 128; step++) {
+                const i8v a = blob_operand8(in16, nrows, step, lane);
+                const int kk = 128 * step + 32 * (lane >> 4);
+#pragma unroll
+                for (int nt = 0; nt < NTH; nt++) {
+                    const i8v b = ld32b(sm.wt + Q0 + (nt * 16 + m) * LQ0 + kk);
+                    acc[nt] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(b, a, acc[nt], 0, 0, 0, 127, 0, 127);
+                }
+            }
+        } else {
+            gemm8_rows_nk_t<HID, NTH>(ws.h, LQH, sm.wt + (l == 1 ? Q1 : l == 2 ? Q2 : Q3), LQH, acc, lane);
+        }
         wave_sync();  // every lane has read its operand rows before any overwrites them
         const float sc = inv_a * sm.inv_w[l];
         float mx = 0.0f;
