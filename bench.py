@@ -54,9 +54,10 @@ def parse():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip cpt_autotune (default launch configuration)")
     ap.add_argument("--gpu-bvh", action="store_true", help="build the BVH on the device (CPT_SCENE_GPU_BVH)")
-    ap.add_argument("--workload", choices=["terrain", "views", "guided", "bvh", "box"], default="terrain",
+    ap.add_argument("--workload", choices=["terrain", "views", "guided", "bvh", "box", "mixtures"], default="terrain",
                     help="terrain: the headline 1080p line; views: 64-view batch of the box room; "
-                         "guided: equal-time relMSE of guided vs plain PT on the hard-lighting interior")
+                         "guided: equal-time relMSE of guided vs plain PT on the hard-lighting interior; "
+                         "mixtures: the guide's per-bounce inference (cpt_guide_mixtures) on random inputs")
     ap.add_argument("--budgets", type=str, default="10,30,60", help="guided: equal-time budgets in seconds")
     ap.add_argument("--repeats", type=int, default=1,
                     help="guided: runs of each arm with distinct seeds; budget rows give medians")
@@ -365,6 +366,33 @@ def bench_views(args):
     print(json.dumps(out))
 
 
+def bench_mixtures(args, n=1 << 22, calls=3):
+    """`--workload mixtures`: the guide network's inference with mixture decode
+    (cpt_guide_mixtures, the device code of the per-bounce inference after its hit
+    gather) on n random inputs, --guide-width / --fp8.  The call's time includes
+    the host copies; the kernel's own time comes from a rocprofv3 kernel trace of
+    this command."""
+    import numpy as np
+    import cpt
+
+    X = np.random.default_rng(1).random((n, 7)).astype(np.float32)
+    with cpt.Context(0) as ctx:
+        ctx.guide_init(seed=3, infer_fp8=args.fp8, width=args.guide_width)
+        ctx.guide_mixtures(X[:4096])
+        times = []
+        for _ in range(calls):
+            t0 = time.perf_counter()
+            ctx.guide_mixtures(X)
+            times.append(time.perf_counter() - t0)
+    t = min(times)
+    print(json.dumps({
+        "metric": "guide mixtures per second (host copies included)", "value": n / t, "unit": "samples/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "fp8 e4m3" if args.fp8 else "bf16", "data": "synthetic",
+        "config": {"workload": f"cpt_guide_mixtures on {n} random inputs, width {args.guide_width}",
+                   "calls": calls, "best_call_s": t},
+    }))
+
+
 def bench_bvh(args):
     """Host binned-SAH build vs the device PLOC build on the terrain workload:
     build time, BVH shape, and the traversal kernels' time and node visits over
@@ -646,6 +674,9 @@ def main():
         return
     if args.workload == "bvh":
         bench_bvh(args)
+        return
+    if args.workload == "mixtures":
+        bench_mixtures(args)
         return
     if args.workload == "box":
         bench_box(args)
