@@ -71,6 +71,8 @@ def parse():
                          "(0: 2^18, or 2^21 above 4M pixels)")
     ap.add_argument("--guide-width", type=int, choices=[64, 128], default=64,
                     help="guided: hidden width of the guide network (64: 8 lobes, 128: 16 lobes)")
+    ap.add_argument("--fraction-ramp", type=int, default=100,
+                    help="guided: Adam steps over which the guide fraction rises from 0 to 0.5 (0: constant)")
     ap.add_argument("--guide-depth", type=int, default=0,
                     help="guided: guide the vertices below this depth only (0: every vertex)")
     ap.add_argument("--scene", choices=["interior", "grid"], default="interior",
@@ -105,7 +107,7 @@ def copy_calibration(mib):
 
 
 def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=0, guide_depth=0, scene_kind="interior",
-                      boxes=40000, ref_spp_axis=256, ppg=False, log=None, repeats=1):
+                      boxes=40000, ref_spp_axis=256, ppg=False, log=None, repeats=1, fraction_ramp=0):
     """Equal-time comparison on the hard-lighting interior (one 0.2x0.2 light outside
     a cluttered room, light entering through a 0.3-wide door gap): plain PT-NEE-MIS
     vs guided PT (learned MLP guide trained online, training time included), both
@@ -161,7 +163,7 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
         seed0 = 1 + 1_000_003 * rep + (500_000 if guided else 0)
         if guided:
             ctx.guide_init(seed=11 + rep, lr=3e-3, train_paths=tp, train_steps=1, guide_depth=guide_depth,
-                           infer_fp8=fp8, width=width)
+                           infer_fp8=fp8, width=width, fraction_ramp=fraction_ramp)
         flags = (cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN) if guided else 0
         curve = []
         elapsed, spp, k = 0.0, 0, 0
@@ -285,7 +287,8 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
         "repeats": repeats,
         "guide": {"width": width, "lobes": 8 if width == 64 else 16, "inference": "fp8 e4m3 MFMA" if fp8 else "bf16 MFMA",
                   "training": "bf16 MFMA, fp32 accumulation and Adam", "train_paths": tp,
-                  "training_samples_per_step": median(per_step), "adam_steps": median(steps)},
+                  "training_samples_per_step": median(per_step), "adam_steps": median(steps),
+                  "guide_fraction": 0.5, "fraction_ramp_steps": fraction_ramp},
         "budgets": rows,
         "runs": runs if repeats > 1 else None,
         "frame_kernel_ms": kernel_ms,
@@ -300,7 +303,7 @@ def bench_guided(args):
     g = guided_equal_time(W, H, budgets, depth=args.depth, fp8=args.fp8, width=args.guide_width,
                           train_paths=args.train_paths, guide_depth=args.guide_depth, scene_kind=args.scene,
                           boxes=args.boxes, ref_spp_axis=args.ref_spp_axis, ppg=not args.no_ppg,
-                          repeats=args.repeats)
+                          repeats=args.repeats, fraction_ramp=args.fraction_ramp)
     rows = g["budgets"]
     out = {
         "metric": g["metric"],
@@ -797,7 +800,8 @@ def main():
     if not args.no_guided:
         gw, gh = (int(v) for v in args.guided_res.split("x"))
         out["guided"] = guided_equal_time(gw, gh, [10.0, 30.0, 60.0], depth=12, fp8=False, width=64,
-                                          ref_spp_axis=args.guided_ref_spp_axis, log=log)
+                                          ref_spp_axis=args.guided_ref_spp_axis, log=log,
+                                          fraction_ramp=args.fraction_ramp)
     print(json.dumps(out))
 
 

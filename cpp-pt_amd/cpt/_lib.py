@@ -54,7 +54,7 @@ class GuideParams(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("seed", C.c_uint64), ("guide_fraction", C.c_float), ("train_paths", C.c_int32),
                 ("train_steps", C.c_int32), ("guide_depth", C.c_int32), ("guide_target", C.c_int32),
-                ("infer_fp8", C.c_int32), ("width", C.c_int32)]
+                ("infer_fp8", C.c_int32), ("width", C.c_int32), ("fraction_ramp", C.c_int32)]
 
 
 class SceneDesc(C.Structure):
@@ -400,7 +400,7 @@ Context.render_views = _render_views
 
 
 def _guide_init(self, lr=1e-3, seed=1, guide_fraction=0.5, train_paths=0, train_steps=1, beta1=0.9, beta2=0.99,
-                eps=1e-8, guide_depth=0, guide_target=0, infer_fp8=False, width=64):
+                eps=1e-8, guide_depth=0, guide_target=0, infer_fp8=False, width=64, fraction_ramp=0):
     gp = GuideParams()
     gp.lr, gp.beta1, gp.beta2, gp.eps = lr, beta1, beta2, eps
     gp.seed, gp.guide_fraction, gp.train_paths, gp.train_steps = seed, guide_fraction, train_paths, train_steps
@@ -408,6 +408,7 @@ def _guide_init(self, lr=1e-3, seed=1, guide_fraction=0.5, train_paths=0, train_
     gp.guide_target = guide_target
     gp.infer_fp8 = 1 if infer_fp8 else 0
     gp.width = width
+    gp.fraction_ramp = fraction_ramp
     self._check(self.lib.cpt_guide_init(self.h, C.byref(gp)), "cpt_guide_init")
 
 

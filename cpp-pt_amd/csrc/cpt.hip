@@ -72,7 +72,9 @@ struct Segment {
 };
 struct PassDesc {
     uint32_t n_seg, n_paths, n_pix, pass_id;
-    uint64_t seed;  // render seed: copied in per pass, so a captured graph serves every seed
+    uint64_t seed;     // render seed: copied in per pass, so a captured graph serves every seed
+    float guide_frac;  // guided renders: probability of sampling the guide in this pass
+    uint32_t pad;
     Segment seg[CPT_MAX_SEG];
 };
 
@@ -124,7 +126,6 @@ struct WfArgs {
     uint32_t n_rec;          // slots that record: every rec_stride-th slot, n_rec of them
     uint32_t rec_stride;
     int rec_depth;
-    float guide_frac;
     int guide_depth;         // vertices at depth >= this are not guided
     int guide_target;        // 0: train on L_i f cos, 1: on L_i
     unsigned refill;         // idle lanes that make a traversal wave fetch new rays
@@ -695,7 +696,7 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                     sampled_delta = false;
                     const float* lob = GUIDED ? a.lobes + i : nullptr;
                     const bool gv = GUIDED && depth < a.guide_depth && guidable(m);
-                    const float alpha = a.guide_frac;
+                    const float alpha = a.pass->guide_frac;
                     DSample ds;
                     if (gv) {
                         // one-sample MIS of guide and BSDF (balance heuristic, ppg.h:74-103)
@@ -1602,7 +1603,6 @@ WfArgs make_args(cpt_ctx* c, int W, int H, int S, int depth, uint64_t seed, uint
     a.n_rec = 0;
     a.rec_stride = 1;
     a.rec_depth = depth;
-    a.guide_frac = c->g_cfg.guide_fraction;
     a.guide_depth = c->g_cfg.guide_depth > 0 ? c->g_cfg.guide_depth : 256;
     a.guide_target = c->g_cfg.guide_target;
     a.sdt.snode = c->sd_snode;
@@ -2256,6 +2256,14 @@ static int render_views_stream(cpt_ctx* c, int32_t n_views, const float* cameras
                                cpt_render_stats* stats);
 static int guide_train_device(cpt_ctx* c, int n, bool apply);
 
+// Probability of sampling the guide after `step` Adam steps of the network: the
+// configured fraction, reached linearly over the first fraction_ramp steps (0: at
+// once).  Any value in [0, 1) keeps the one-sample MIS estimator unbiased.
+static float guide_fraction_at(const cpt_guide_params& g, int step) {
+    if (g.fraction_ramp <= 0 || step >= g.fraction_ramp) return g.guide_fraction;
+    return g.guide_fraction * (float)step / (float)g.fraction_ramp;
+}
+
 // Pack views into passes of at most `cap` paths (see PassDesc).
 static void plan_passes(const std::vector<ViewDesc>& vd, int64_t cap, std::vector<PassDesc>& passes,
                         std::vector<int>& last_pass_of_view) {
@@ -2539,10 +2547,10 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
     if (use_graph) {
         char key[512];
         snprintf(key, sizeof(key),
-                 "%d,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%u,%p,%p,%p,%g,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
+                 "%d,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%u,%p,%p,%p,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
                  max_depth, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, a.rec_stride, (void*)c->g_lobes, (void*)c->g_rec,
-                 (void*)c->g_samples, (double)a.guide_frac, a.guide_depth * 2 + a.guide_target, refill, integrator,
+                 (void*)c->g_samples, a.guide_depth * 2 + a.guide_target, refill, integrator,
                  a.guide_lobes * 4 + a.corrected * 2 + a.guide_fp8, a.lobes_stride,
                  (void*)c->sd_snode, (void*)c->sd_dinfo, (void*)c->sd_quad, (void*)c->sd_rect, (void*)c->sd_sum,
                  (void*)c->ppg_vtx, (void*)c->ppg_nv, a.sdt.bmin[0], a.sdt.bmin[1], a.sdt.bmin[2], a.sdt.bmax[0],
@@ -2563,6 +2571,9 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         }
     }
     for (size_t i = 0; i < passes.size(); i++) {
+        // the guide fraction of this pass (the ramp follows the network's Adam
+        // steps; training below synchronises the stream before the next pass)
+        c->h_pass[i].guide_frac = guided ? guide_fraction_at(c->g_cfg, c->g_step) : 0.0f;
         HIPCHK(c, hipMemcpyAsync(c->d_pass, &c->h_pass[i], sizeof(PassDesc), hipMemcpyHostToDevice, c->stream));
         if (use_graph) {
             HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));

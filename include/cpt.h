@@ -339,6 +339,9 @@ typedef struct {
     int32_t width;         /* hidden width: 64 (8 lobes; 0 = 64) or 128 (16 lobes, the large-scene
                               network: BF16 inference reads its weights through the L2, FP8
                               inference keeps them in LDS) */
+    int32_t fraction_ramp; /* guided renders: the guide fraction rises linearly from 0 to
+                              guide_fraction over the network's first this many Adam steps, so
+                              that an untrained guide is sampled rarely (0 = constant) */
 } cpt_guide_params;
 
 int cpt_guide_num_params(void);  /* of the width-64 network */

@@ -49,6 +49,26 @@ def test_guided_is_unbiased(ctx, fp8, width):
     assert frac < 0.01 and abs(z.mean()) < 0.06
 
 
+def test_guided_with_fraction_ramp_is_unbiased(ctx):
+    """The guide fraction rising over the first Adam steps (0 on the first pass,
+    0.5 from the 8th of the 256 one-sample passes on) changes the sampling density from pass to pass; each pass's
+    one-sample MIS estimator stays unbiased, so the film still agrees with plain PT."""
+    W = H = 48
+    s = cpt.scenes.box_room(res=(W, H))
+    ctx.upload(s)
+    ctx.render(W, H, 16, max_depth=16, seed=5)
+    _, ps, pq = ctx.read_film(sums=True, sumsq=True)
+    ctx.guide_init(seed=2, lr=3e-3, train_paths=8192, fraction_ramp=8)
+    st = ctx.render(W, H, 16, max_depth=16, seed=6, flags=cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN)
+    _, gs, gq = ctx.read_film(sums=True, sumsq=True)
+    z = _z(gs, gq, 256, ps, pq, 256)
+    frac = (np.abs(z) > 3).mean()
+    print(f"ramped guide vs plain: 3-sigma exceedance {frac * 100:.2f}%, mean z {z.mean():+.4f}, "
+          f"{st['guide_steps']} steps")
+    assert st["guide_steps"] == 256  # one pass and one Adam step per sample per pixel
+    assert frac < 0.01 and abs(z.mean()) < 0.06
+
+
 def _errors(imgs, ref):
     """Mean over renders of (relMSE without the worst 1% of pixels, mean relative
     L1 error).  Full relMSE at these sample counts is decided by a few fireflies
