@@ -54,10 +54,12 @@ def parse():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip cpt_autotune (default launch configuration)")
     ap.add_argument("--gpu-bvh", action="store_true", help="build the BVH on the device (CPT_SCENE_GPU_BVH)")
-    ap.add_argument("--workload", choices=["terrain", "views", "guided", "bvh", "box", "mixtures"], default="terrain",
+    ap.add_argument("--workload", choices=["terrain", "views", "guided", "bvh", "box", "mixtures", "train"],
+                    default="terrain",
                     help="terrain: the headline 1080p line; views: 64-view batch of the box room; "
                          "guided: equal-time relMSE of guided vs plain PT on the hard-lighting interior; "
-                         "mixtures: the guide's per-bounce inference (cpt_guide_mixtures) on random inputs")
+                         "mixtures: the guide's per-bounce inference (cpt_guide_mixtures) on random inputs; "
+                         "train: Adam steps of the guide (cpt_guide_train_step) on random samples")
     ap.add_argument("--budgets", type=str, default="10,30,60", help="guided: equal-time budgets in seconds")
     ap.add_argument("--repeats", type=int, default=1,
                     help="guided: runs of each arm with distinct seeds; budget rows give medians")
@@ -396,6 +398,39 @@ def bench_mixtures(args, n=1 << 22, calls=3):
     }))
 
 
+def bench_train(args, n=204800, steps=8):
+    """`--workload train`: Adam steps of the guide network (cpt_guide_train_step:
+    forward, loss gradient, backward, weight gradients, Adam, the FP8 copy when
+    --fp8) on n random training samples (the interior configuration's about 200k
+    samples per step at width 64), --guide-width.  The call's time includes the
+    samples' host copy; the kernels' own times come from a rocprofv3 kernel trace
+    of this command."""
+    import numpy as np
+    import cpt
+
+    rng = np.random.default_rng(2)
+    S = np.zeros((n, 11), np.float32)
+    S[:, :7] = rng.random((n, 7))
+    d = rng.standard_normal((n, 3))
+    S[:, 7:10] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    S[:, 10] = rng.exponential(1.0, n)
+    with cpt.Context(0) as ctx:
+        ctx.guide_init(seed=3, lr=3e-3, infer_fp8=args.fp8, width=args.guide_width)
+        ctx.guide_train_step(S[:4096])
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            ctx.guide_train_step(S)
+            times.append(time.perf_counter() - t0)
+    t = min(times)
+    print(json.dumps({
+        "metric": "guide training samples per second (host copies included)", "value": n / t, "unit": "samples/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 (fp32 accumulation and Adam)", "data": "synthetic",
+        "config": {"workload": f"cpt_guide_train_step on {n} random samples, width {args.guide_width}",
+                   "steps": steps, "best_step_s": t},
+    }))
+
+
 def bench_bvh(args):
     """Host binned-SAH build vs the device PLOC build on the terrain workload:
     build time, BVH shape, and the traversal kernels' time and node visits over
@@ -680,6 +715,9 @@ def main():
         return
     if args.workload == "mixtures":
         bench_mixtures(args)
+        return
+    if args.workload == "train":
+        bench_train(args)
         return
     if args.workload == "box":
         bench_box(args)
