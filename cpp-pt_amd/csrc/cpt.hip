@@ -164,6 +164,13 @@ struct WfArgs {
 #define CPT_TRAV_WAVES 5
 #endif
 
+// waves per SIMD the unguided shading kernel is compiled for (caps its VGPRs at
+// 128: 7% faster on the 1080p terrain; the guided variant spills under that
+// cap and runs uncapped)
+#ifndef CPT_SHADE_WAVES
+#define CPT_SHADE_WAVES 4
+#endif
+
 // idle lanes (of 64) that make a traversal wave fetch new rays of its chunk
 #define CPT_DEFAULT_REFILL 32u
 
@@ -392,6 +399,22 @@ __device__ __forceinline__ float gmix_pdf(const float* lob, size_t st, V3 w) {
     return p;
 }
 
+// The mixture pdf at two directions in one pass over the stored lobes (each sum
+// formed exactly as gmix_pdf forms it)
+template <int NL>
+__device__ __forceinline__ void gmix_pdf2(const float* lob, size_t st, V3 w1, V3 w2, float& p1, float& p2) {
+    p1 = 0.0f;
+    p2 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NL; k++) {
+        const float2 a = guide::unpack_h2(__float_as_uint(lob[k * st]));
+        const float2 b = guide::unpack_h2(__float_as_uint(lob[(NL + k) * st]));
+        const float c = lob[(2 * NL + k) * st];
+        p1 += __expf(a.x * w1.x + a.y * w1.y + b.x * w1.z + c);
+        p2 += __expf(a.x * w2.x + a.y * w2.y + b.x * w2.z + c);
+    }
+}
+
 template <int NL>
 __device__ __forceinline__ V3 gmix_sample(const float* lob, size_t st, float u0, float u1, float u2) {
     // lobe k with probability w~_k / sum w~ (the decoder's S, summed in another
@@ -435,6 +458,10 @@ __device__ __forceinline__ V3 gmix_sample(const float* lob, size_t st, float u0,
 // the guide's lobe count is 8 (width-64 network) or 16 (width 128)
 __device__ __forceinline__ float gmix_pdf_n(const float* lob, size_t st, int nl, V3 w) {
     return nl == 16 ? gmix_pdf<16>(lob, st, w) : gmix_pdf<8>(lob, st, w);
+}
+__device__ __forceinline__ void gmix_pdf2_n(const float* lob, size_t st, int nl, V3 w1, V3 w2, float& p1, float& p2) {
+    if (nl == 16) gmix_pdf2<16>(lob, st, w1, w2, p1, p2);
+    else gmix_pdf2<8>(lob, st, w1, w2, p1, p2);
 }
 __device__ __forceinline__ V3 gmix_sample_n(const float* lob, size_t st, int nl, float u0, float u1, float u2) {
     return nl == 16 ? gmix_sample<16>(lob, st, u0, u1, u2) : gmix_sample<8>(lob, st, u0, u1, u2);
@@ -622,7 +649,8 @@ __global__ void k_guide_collect(WfArgs a) {
 // ------------------------------------------------------------------ shade
 // pathtracerMIS.h:37-126, one loop iteration per queue entry.
 template <int B, bool STREAM, bool GUIDED>
-__global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(GUIDED ? 1 : CPT_SHADE_WAVES))) void k_shade(
+    WfArgs a, int cur, int depth_cap) {
     __shared__ unsigned qsh[2][2 * (B / 64) + 2];
     const unsigned count = a.counters[2 * cur];
     unsigned long long* const out_ctr = reinterpret_cast<unsigned long long*>(a.counters + 2 * (cur ^ 1));
@@ -710,12 +738,13 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                         } else {
                             ds = bsdf_sample(m, it, wo, rng);
                         }
-                        ds.pdf = alpha * gmix_pdf_n(lob, a.lobes_stride, a.guide_lobes, ds.wi) + (1.0f - alpha) * ds.pdf;
                     } else {
                         ds = bsdf_sample(m, it, wo, rng);
                     }
-                    p = ds.pdf;
-                    const float p_sampled = p;
+                    // A guided vertex evaluates the mixture at the sampled direction and
+                    // at NEE's light direction in one pass over its lobes (below).
+                    float gq_wi = 0.0f;
+                    bool have_gq = false;
                     if (!ds.delta) {
                         if (a.sc.n_lights > 0 && a.integrator != 2) {
                             LSample ls = light_sample(a.sc, rng, it.hitpoint);
@@ -724,8 +753,12 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                             sdir = divs(sdir, len);
                             if (bsdf_non_zero(m, it, wo, sdir)) {
                                 float pnb = bsdf_pdf(m, it, wo, sdir);
-                                if (gv)
-                                    pnb = alpha * gmix_pdf_n(lob, a.lobes_stride, a.guide_lobes, sdir) + (1.0f - alpha) * pnb;
+                                if (gv) {
+                                    float gq_l;
+                                    gmix_pdf2_n(lob, a.lobes_stride, a.guide_lobes, ds.wi, sdir, gq_wi, gq_l);
+                                    have_gq = true;
+                                    pnb = alpha * gq_l + (1.0f - alpha) * pnb;
+                                }
                                 float ct = (float)(fabs((double)dot(ls.normal, sdir)) + 1e-4);
                                 float pnn = ls.pdf * len * len / ct;
                                 float w = pnn / (pnb + pnn);
@@ -742,6 +775,12 @@ __global__ __launch_bounds__(B) void k_shade(WfArgs a, int cur, int depth_cap) {
                     } else {
                         sampled_delta = true;
                     }
+                    if (gv) {
+                        if (!have_gq) gq_wi = gmix_pdf_n(lob, a.lobes_stride, a.guide_lobes, ds.wi);
+                        ds.pdf = alpha * gq_wi + (1.0f - alpha) * ds.pdf;
+                    }
+                    p = ds.pdf;
+                    const float p_sampled = p;
                     if (bsdf_non_zero(m, it, wo, ds.wi)) {
                         V3 ev = bsdf_eval(m, it, wo, ds.wi);
                         bool survive = true;
