@@ -218,14 +218,21 @@ def guided_equal_time(W, H, budgets, depth=12, fp8=False, width=64, train_paths=
             log(f"bench: ppg budget {T:.0f}s: {el:.1f}s, last iteration {last} spp, relMSE={ppg_res[T][2]:.4f}")
 
     # per-kernel time of a 1-spp frame from HIP events (eager launches), with the
-    # guide as trained by the guided run
+    # guide as trained by the guided run; guide inference also per launch (one
+    # launch per bounce)
     def frame_ms(flags):
         acc = {}
+        n_guide = 0
         for k in range(4):
             st = ctx.render(W, H, 1, max_depth=depth, seed=900 + k, flags=flags | cpt.CPT_RENDER_KERNEL_TIMING)
             for key in ("ms_extend", "ms_shade", "ms_shadow", "ms_guide", "ms_train", "ms_other"):
                 acc[key] = acc.get(key, 0.0) + st[key] / 4
-        return {k: round(v, 3) for k, v in acc.items()}
+            n_guide += st["n_guide"]
+        res = {k: round(v, 3) for k, v in acc.items()}
+        if n_guide:
+            res["guide_launches"] = n_guide / 4
+            res["ms_guide_per_launch"] = round(4 * acc["ms_guide"] / n_guide, 4)
+        return res
 
     kernel_ms = {"plain": frame_ms(0), "guided": frame_ms(cpt.CPT_RENDER_GUIDED),
                  "guided_training": frame_ms(cpt.CPT_RENDER_GUIDED | cpt.CPT_RENDER_GUIDE_TRAIN)}
