@@ -3141,7 +3141,8 @@ int cpt_render_views(cpt_ctx* c, int32_t n_views, const float* cameras, const in
 }
 
 // ------------------------------------------------------------------ learned guide
-static const int GUIDE_TRAIN_BLOCKS = 256;
+// two per CU (the training kernel's block uses 75 KB of LDS at width 64)
+static const int GUIDE_TRAIN_BLOCKS = 512;
 static const int GUIDE_DW_BLOCKS = 256;
 static const int GUIDE_WSUM_BLOCKS = 128;
 
@@ -3176,7 +3177,7 @@ int guide_alloc(cpt_ctx* c) {
     r |= dalloc(c, c->g_partial, (size_t)GUIDE_DW_BLOCKS * N::kParams);
     if (r) return CPT_E_HIP;
     HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_forward, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd));
-    HIPCHK(c, hipFuncSetAttribute((const void*)N::train_tile, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd));
+    HIPCHK(c, hipFuncSetAttribute((const void*)N::train_tile, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemTrain));
     HIPCHK(c, hipFuncSetAttribute((const void*)N::mlp_forward8, hipFuncAttributeMaxDynamicSharedMemorySize, N::kSmemFwd8));
     HIPCHK(c, hipFuncSetAttribute((const void*)k_guide_infer<N::kHid>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   N::kSmemFwd));
@@ -3272,7 +3273,7 @@ int guide_train_impl(cpt_ctx* c, int n, bool apply) {
     hipLaunchKernelGGL(N::weight_partial, dim3(GUIDE_WSUM_BLOCKS), dim3(256), 0, c->stream, c->g_samples, n,
                        c->g_wpart);
     hipLaunchKernelGGL(N::weight_scale, dim3(1), dim3(256), 0, c->stream, c->g_wpart, GUIDE_WSUM_BLOCKS, c->g_wscale);
-    hipLaunchKernelGGL(N::train_tile, dim3(GUIDE_TRAIN_BLOCKS), dim3(256), N::kSmemFwd, c->stream, c->g_wt,
+    hipLaunchKernelGGL(N::train_tile, dim3(GUIDE_TRAIN_BLOCKS), dim3(256), N::kSmemTrain, c->stream, c->g_wt,
                        c->g_params, c->g_samples, n, npad, c->g_wscale, c->g_rows, c->g_loss_part);
     hipLaunchKernelGGL(N::train_dw, dim3(GUIDE_DW_BLOCKS, N::kDwGroups), dim3(256), 0, c->stream, c->g_rows, npad,
                        per_block, c->g_partial);

@@ -2,9 +2,9 @@
 // per width inside its own namespace (no include guard on purpose).  Before each
 // inclusion guide_mlp.h defines CPT_GUIDE_HID (hidden width: 64 or 128) and
 // CPT_GUIDE_LOBES (von Mises-Fisher lobes: 8 or 16).  Width 64 keeps its BF16
-// weights resident in LDS for inference and training; width 128 (186 KB of BF16
-// weights, more than a CU's 160 KB of LDS) runs inference in FP8 only (90 KB of
-// e4m3 weights in LDS) and trains with the BF16 weights read through the L2.
+// weights resident in LDS for inference; width 128 (186 KB of BF16 weights, more
+// than a CU's 160 KB of LDS) runs inference in FP8 only (90 KB of e4m3 weights in
+// LDS).  Training reads the BF16 weights through the caches at both widths.
 
 constexpr int NIN = 7;          // pos.xyz, normal (theta, phi), wo (theta, phi), all in [0,1]
 constexpr int NBINS = 32;
@@ -46,6 +46,11 @@ constexpr int T4P = NWT;
 constexpr int NWT_ALL = NWT + NOUT * LDH;  // bf16 elements of the device copy
 // BF16 weights resident in LDS (width 64); otherwise read from global memory
 constexpr bool WT_LDS = HID <= 64;
+// The training kernel reads them through the caches at every width: its block
+// also holds a 64-sample tile's features and four hidden layers, and without the
+// weights (66 KB at width 64) two blocks fit a CU's LDS instead of one (width 64:
+// 331 -> 214 us per step of 204,800 samples)
+constexpr bool WT_LDS_TRAIN = false;
 
 constexpr float KAPPA_LOG_MIN = -3.0f, KAPPA_LOG_MAX = 6.0f;
 constexpr float TWO_PI = 6.28318530717958647692f;
@@ -260,17 +265,23 @@ __device__ __forceinline__ void features_fast(const float* bmin, const float* be
 
 // --------------------------------------------------------------------------
 // Shared-memory tile: weights (transposed bf16), biases, activations.
-struct SmemFwd {
-    uint16_t wt[WT_LDS ? NWT : 8];
+// WL: the weights are in the struct (else the kernels read them from global memory)
+template <bool WL>
+struct SmemFwdT {
+    static constexpr bool kWL = WL;
+    uint16_t wt[WL ? NWT : 8];
     float bias[4 * HID + NOUT];
     uint16_t x0[TILE * LD0];
     uint16_t h[4][TILE * LDH];
 };
+using SmemFwd = SmemFwdT<WT_LDS>;
+using SmemTrain = SmemFwdT<WT_LDS_TRAIN>;
 
-__device__ __forceinline__ void load_weights(SmemFwd& sm, const uint16_t* gwt, const float* params) {
+template <class SM>
+__device__ __forceinline__ void load_weights(SM& sm, const uint16_t* gwt, const float* params) {
     const uint4* src = reinterpret_cast<const uint4*>(gwt);
     uint4* dst = reinterpret_cast<uint4*>(sm.wt);
-    if constexpr (WT_LDS)
+    if constexpr (SM::kWL)
         for (int i = threadIdx.x; i < NWT / 8; i += blockDim.x) dst[i] = src[i];
     for (int i = threadIdx.x; i < 4 * HID + NOUT; i += blockDim.x) {
         int off = i < HID ? B0 + i : i < 2 * HID ? B1 + i - HID : i < 3 * HID ? B2 + i - 2 * HID
@@ -281,7 +292,8 @@ __device__ __forceinline__ void load_weights(SmemFwd& sm, const uint16_t* gwt, c
 
 // one-blob features of the tile's samples into sm.x0 (samples beyond n are zero)
 // (4 consecutive bins of one input per thread step, one 8-byte store)
-__device__ __forceinline__ void encode_tile(SmemFwd& sm, const float* in, int n) {
+template <class SM>
+__device__ __forceinline__ void encode_tile(SM& sm, const float* in, int n) {
     constexpr int CPS = K0 / 4;  // chunks per sample
     static_assert(NBINS % 4 == 0 && LD0 % 4 == 0, "8-byte chunks");
     for (int c = threadIdx.x; c < TILE * CPS; c += blockDim.x) {
@@ -334,13 +346,15 @@ __device__ __forceinline__ void encode_tile_window(SmemFwd& sm, const float* in,
 }
 
 // forward through all layers; out (TILE x NOUT fp32, row stride NOUT) receives raw
-// outputs; wt = the transposed BF16 weights (sm.wt, or global memory when !WT_LDS)
-__device__ __forceinline__ void forward_tile(SmemFwd& sm, const uint16_t* wt, float* out) {
+// outputs; wt = the transposed BF16 weights (sm.wt, or global memory when !SM::kWL)
+template <class SM>
+__device__ __forceinline__ void forward_tile(SM& sm, const uint16_t* wt, float* out) {
+    constexpr bool WL = SM::kWL;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = 16 * wave;
     {
         f4 acc[NTH] = {};
-        gemm_rows_nk<K0, NTH, WT_LDS ? K0 / 32 : 1>(sm.x0 + r0 * LD0, LD0, wt + T0, LD0, acc, lane);
+        gemm_rows_nk<K0, NTH, WL ? K0 / 32 : 1>(sm.x0 + r0 * LD0, LD0, wt + T0, LD0, acc, lane);
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
 #pragma unroll
@@ -355,7 +369,7 @@ __device__ __forceinline__ void forward_tile(SmemFwd& sm, const uint16_t* wt, fl
     for (int l = 1; l < 4; l++) {
         f4 acc[NTH] = {};
         const int toff = l == 1 ? T1 : l == 2 ? T2 : T3;
-        gemm_rows_nk<HID, NTH, WT_LDS ? HID / 32 : 1>(sm.h[l - 1] + r0 * LDH, LDH, wt + toff, LDH, acc, lane);
+        gemm_rows_nk<HID, NTH, WL ? HID / 32 : 1>(sm.h[l - 1] + r0 * LDH, LDH, wt + toff, LDH, acc, lane);
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
 #pragma unroll
@@ -368,7 +382,7 @@ __device__ __forceinline__ void forward_tile(SmemFwd& sm, const uint16_t* wt, fl
     }
     {
         f4 acc[NTO] = {};
-        gemm_rows_nk<HID, NTO, WT_LDS ? HID / 32 : 1>(sm.h[3] + r0 * LDH, LDH, wt + T4, LDH, acc, lane);
+        gemm_rows_nk<HID, NTO, WL ? HID / 32 : 1>(sm.h[3] + r0 * LDH, LDH, wt + T4, LDH, acc, lane);
 #pragma unroll
         for (int nt = 0; nt < NTO; nt++)
 #pragma unroll
@@ -765,7 +779,7 @@ __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const f
                                                     int n, int npad, const float* wscale, uint16_t* rows,
                                                     float* loss_part) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    SmemFwd& sm = *reinterpret_cast<SmemFwd*>(smem_raw);
+    SmemTrain& sm = *reinterpret_cast<SmemTrain*>(smem_raw);
     __shared__ float in_s[TILE * NIN];
     __shared__ float dir_s[TILE * 4];
     __shared__ float red[256];
@@ -775,7 +789,7 @@ __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const f
     __shared__ __align__(16) unsigned char ot_own[OT_IN_X0 ? 16 : OT_BYTES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r0 = 16 * wave;
     load_weights(sm, gwt, params);
-    const uint16_t* wt = WT_LDS ? sm.wt : gwt;
+    const uint16_t* wt = WT_LDS_TRAIN ? sm.wt : gwt;
     const float ws = *wscale;
     float loss_acc = 0.0f;
     __syncthreads();
@@ -815,7 +829,7 @@ __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const f
         // backward-data: dz_{l-1} = (dz_l W_l^T) .* (h_{l-1} > 0), written over h_{l-1}
         {
             f4 acc[NTH] = {};
-            gemm_rows_kn<NOUT, NTH, WT_LDS ? NOUT / 16 : 1>(dz4 + r0 * LDO, LDO, wt + T4, LDH, acc, lane);
+            gemm_rows_kn<NOUT, NTH, WT_LDS_TRAIN ? NOUT / 16 : 1>(dz4 + r0 * LDO, LDO, wt + T4, LDH, acc, lane);
 #pragma unroll
             for (int ntl = 0; ntl < NTH; ntl++)
 #pragma unroll
@@ -838,7 +852,7 @@ __global__ __launch_bounds__(256) void k_train_tile(const uint16_t* gwt, const f
         for (int l = 3; l >= 1; l--) {
             const int toff = l == 1 ? T1 : l == 2 ? T2 : T3;
             f4 acc[NTH] = {};
-            gemm_rows_kn<HID, NTH, WT_LDS ? HID / 16 : 1>(sm.h[l] + r0 * LDH, LDH, wt + toff, LDH, acc, lane);
+            gemm_rows_kn<HID, NTH, WT_LDS_TRAIN ? HID / 16 : 1>(sm.h[l] + r0 * LDH, LDH, wt + toff, LDH, acc, lane);
 #pragma unroll
             for (int ntl = 0; ntl < NTH; ntl++)
 #pragma unroll
@@ -1383,7 +1397,7 @@ struct Net {
     static constexpr int kHid = HID, kLobes = NLOBES, kOut = NOUT, kParams = NPARAM, kWt = NWT_ALL, kWq = NWQ,
                          kRow = ROW, kTile = TILE, kChunk = CH, kDwGroups = DW_GROUPS;
     static constexpr int kW[5] = {W0, W1, W2, W3, W4}, kB4 = B4;
-    static constexpr size_t kSmemFwd = sizeof(SmemFwd), kSmemFwd8 = sizeof(SmemFwd8);
+    static constexpr size_t kSmemFwd = sizeof(SmemFwd), kSmemFwd8 = sizeof(SmemFwd8), kSmemTrain = sizeof(SmemTrain);
     static constexpr int kWaves8 = WAVES8;
     // per-wave BF16 inference (width 64: the weights fit in LDS)
     static constexpr bool kWave16 = WT_LDS;
