@@ -3,7 +3,7 @@
 // inclusion guide_mlp.h defines CPT_GUIDE_HID (hidden width: 64 or 128) and
 // CPT_GUIDE_LOBES (von Mises-Fisher lobes: 8 or 16).  Width 64 keeps its BF16
 // weights resident in LDS for inference; width 128 (186 KB of BF16 weights, more
-// than a CU's 160 KB of LDS) runs inference in FP8 only (90 KB of e4m3 weights in
+// than a CU's 160 KB of LDS) runs inference in FP8 only (102 KB of e4m3 weights in
 // LDS).  Training reads the BF16 weights through the caches at both widths.
 
 constexpr int NIN = 7;          // pos.xyz, normal (theta, phi), wo (theta, phi), all in [0,1]
@@ -467,17 +467,23 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* gwt, const 
 // activation scale never exceeds the range.  Lane layout of the 16x16x32 MFMA: lane l supplies
 // A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j = 0..7, as 8 bytes.
 // Training keeps the BF16 path; only inference uses these weights.
-constexpr int PAD8 = 8;             // byte padding of FP8 LDS rows
+// Byte padding of FP8 LDS rows.  Width 128: 16 keeps every row 16-byte aligned,
+// so an MFMA operand is two 16-byte LDS reads whose offsets fit the instruction
+// (no address arithmetic per tile) and the 16 rows of a read (68 dwords apart)
+// fall in distinct banks: 4M-sample inference 1078 -> 1020 us.  Width 64 keeps
+// 8 (its operands as four 8-byte reads; 16 costs it registers: 575 -> 581 us).
+constexpr int PAD8 = HID > 64 ? 16 : 8;
 constexpr int K0P = 256;            // layer-0 K padded to two MFMA steps of 128 (zero weights beyond K0)
 static_assert(K0 <= K0P && K0P == 8 * NBINS, "layer-0 steps: 4 inputs of 32 bins each");
-constexpr int LQ0 = K0P + PAD8;     // 264
-constexpr int LQH = HID + PAD8;     // 72 or 136
+constexpr int LQ0 = K0P + PAD8;     // 264 or 272
+constexpr int LQH = HID + PAD8;     // 72 or 144
 constexpr int Q0 = 0;
 constexpr int Q1 = Q0 + HID * LQ0;
 constexpr int Q2 = Q1 + HID * LQH;
 constexpr int Q3 = Q2 + HID * LQH;
 constexpr int Q4 = Q3 + HID * LQH;
-constexpr int NWQ = Q4 + NOUT * LQH;  // 34176 or 96896 bytes
+constexpr int NWQ = Q4 + NOUT * LQH;  // 34176 or 101632 bytes
+static_assert(PAD8 % 16 != 0 || (LQ0 % 16 == 0 && LQH % 16 == 0 && NWQ % 16 == 0), "16-byte rows");
 constexpr float FP8_MAX = 448.0f;
 
 __device__ __forceinline__ uint32_t fp8x2(float a, float b) {
@@ -491,7 +497,7 @@ __device__ __forceinline__ uint8_t fp8x1(float a) { return (uint8_t)(fp8x2(a, 0.
 // of the FP8 weights in LDS and each wave its own tile state (inputs and one
 // activation tile; the one-blob features are formed in registers), so after the
 // weight load no wave waits for another.  Waves per block: 16 at width 64 (four
-// per SIMD, 83 KB of LDS); 12 at width 128 (three per SIMD, 147 KB), whose
+// per SIMD, 83 KB of LDS); 12 at width 128 (three per SIMD, 153 KB), whose
 // forward pass and 16-lobe decode need more than the 128 registers of four.
 #ifndef CPT_WAVES8_128
 #define CPT_WAVES8_128 12
@@ -527,6 +533,9 @@ struct SmemFwd8 {
     WaveState8 w[WAVES8];
 };
 static_assert(sizeof(SmemFwd8) <= 160 * 1024, "FP8 inference LDS");
+static_assert(PAD8 % 16 != 0 || (offsetof(SmemFwd8, w) % 16 == 0 && sizeof(WaveState8) % 16 == 0 &&
+                                 offsetof(WaveState8, h) % 16 == 0),
+              "16-byte aligned activation rows");
 
 __device__ __forceinline__ void load_weights8(SmemFwd8& sm, const uint8_t* gwt8, const float* params,
                                               const float* inv_scale) {
@@ -588,10 +597,15 @@ __device__ __forceinline__ i8v blob_operand8(const float* in16, int nrows, int s
 // step, a layout checked on the device with a one-wave probe), the rest of K steps
 // of 32 on v_mfma_f32_16x16x32_fp8_fp8 (8 bytes per lane at 8 (l / 16)).
 __device__ __forceinline__ i8v ld32b(const uint8_t* p) {
-    const int2 q0 = *reinterpret_cast<const int2*>(p), q1 = *reinterpret_cast<const int2*>(p + 8);
-    const int2 q2 = *reinterpret_cast<const int2*>(p + 16), q3 = *reinterpret_cast<const int2*>(p + 24);
     i8v v;
-    v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y; v[4] = q2.x; v[5] = q2.y; v[6] = q3.x; v[7] = q3.y;
+    if constexpr (PAD8 % 16 == 0) {  // rows 16-byte aligned
+        const int4 q0 = *reinterpret_cast<const int4*>(p), q1 = *reinterpret_cast<const int4*>(p + 16);
+        v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w; v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
+    } else {
+        const int2 q0 = *reinterpret_cast<const int2*>(p), q1 = *reinterpret_cast<const int2*>(p + 8);
+        const int2 q2 = *reinterpret_cast<const int2*>(p + 16), q3 = *reinterpret_cast<const int2*>(p + 24);
+        v[0] = q0.x; v[1] = q0.y; v[2] = q1.x; v[3] = q1.y; v[4] = q2.x; v[5] = q2.y; v[6] = q3.x; v[7] = q3.y;
+    }
     return v;
 }
 template <int K, int NT>
