@@ -1310,7 +1310,7 @@ __global__ __launch_bounds__(B) void k_trace_any(DevScene sc, const float4* rays
 
 // Traversal overflow stacks of cpt_trace_closest_device_noalloc, which may not
 // allocate: one file-scope array per GPU, sized for its fixed grid of
-// NOALLOC_BLOCKS x 128 threads x CPT_GSTACK entries (56 MiB), held until the
+// NOALLOC_BLOCKS x 128 threads x CPT_GSTACK entries (52 MiB), held until the
 // process ends.  Host code never uses this name as an address; it uses the
 // per-device address from hipGetSymbolAddress (noalloc_scratch).
 constexpr int NOALLOC_BLOCKS = 1024;

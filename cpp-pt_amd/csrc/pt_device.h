@@ -116,8 +116,11 @@ struct Hit {
 // Ordered BVH4 traversal, one ray per lane.  Stack: LDS_SS entries in LDS
 // (laid out entry-major so that lanes hit distinct banks) then a per-thread
 // overflow region in global memory.
+// 24 LDS entries (a wave takes the overflow path when any lane's stack is deeper):
+// 1-2% less closest-hit time than 16 on the 1080p terrain and the 4K interior grid,
+// and ten 128-thread blocks per CU still fit the LDS
 #ifndef CPT_LDS_SS
-#define CPT_LDS_SS 16
+#define CPT_LDS_SS 24
 #endif
 #define CPT_GSTACK (128 - CPT_LDS_SS)
 // nodes are addressed by 32-bit byte offsets (index << 7)

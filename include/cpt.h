@@ -249,7 +249,7 @@ int cpt_trace_closest_device(cpt_ctx* ctx, const void* d_rays, int32_t n, void* 
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
  * kernel, same results bit for bit; its traversal overflow stacks live in a
- * file-scope device array of 56 MiB per GPU (1024 blocks x 128 threads x 112
+ * file-scope device array of 52 MiB per GPU (1024 blocks x 128 threads x 104
  * entries, located with hipGetSymbolAddress when a context is created), so the
  * launch is capped at 1024 blocks, each wave walking its share of the batch; any
  * n >= 0 is accepted.  Every call on one GPU uses that GPU's array: calls that
