@@ -260,6 +260,13 @@ __device__ __forceinline__ int find_segment(const PassDesc* pd, unsigned i, bool
     return lo;
 }
 
+// The tmax of k_raygen's camera rays: the largest finite float, which culls and
+// accepts exactly as an infinite one (every t and box distance is finite), and marks
+// a queue entry whose path state is the initial one: throughput 1, pdf 1, radiance
+// 0, depth 0, camera ray.  Raygen writes only the path's random state; the shading
+// kernels take the rest from this mark instead of reading 36 bytes per path.
+#define CPT_FRESH_TMAX 3.40282347e+38f
+
 __global__ void k_raygen(WfArgs a) {
     const PassDesc* pd = a.pass;
     const unsigned n_paths = pd->n_paths;
@@ -287,12 +294,9 @@ __global__ void k_raygen(WfArgs a) {
         view_coords(vd, px, py, sample, r1, r2, cx, cy);
         V3 o, d;
         camera_ray(vd.cam, cx, cy, o, d);
-        a.qo[0][i] = make_float4(o.x, o.y, o.z, INFINITY);
+        a.qo[0][i] = make_float4(o.x, o.y, o.z, CPT_FRESH_TMAX);
         a.qd[0][i] = make_float4(d.x, d.y, d.z, __int_as_float((int)i));
-        a.mul_p[i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-        a.col[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         a.rng[i] = rng;
-        a.meta[i] = 1u << 8;  // depth 0, sampled_delta = true (camera ray)
         unsigned ri;
         if (a.n_rec && rec_index(a, i, ri))
             for (int k = 0; k < a.rec_depth; k++) a.rec_flag[(size_t)ri * a.rec_depth + k] = 0;
@@ -519,7 +523,7 @@ __device__ __forceinline__ void guide_entry_features(const WfArgs& a, int cur, u
     const float4 O = a.qo[cur][e], D = a.qd[cur][e];
     const V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
     const int slot = __float_as_int(D.w);
-    const bool frame = (a.meta[slot] & 0xffu) == 0 || a.corrected;
+    const bool frame = O.w == CPT_FRESH_TMAX || (a.meta[slot] & 0xffu) == 0 || a.corrected;
     guide_hit_features(a.sc, o, d, h, frame, in);
 }
 
@@ -667,9 +671,15 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(GUIDED ? 1 : 
             float4 O = a.qo[cur][i], D = a.qd[cur][i], HH = a.hits[i];
             slot = __float_as_int(D.w);
             V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
-            float4 MP = a.mul_p[slot], C = a.col[slot];
+            // a camera ray's path state is the initial one (depth 0, sampled_delta)
+            float4 MP = make_float4(1.0f, 1.0f, 1.0f, 1.0f), C = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t meta = 1u << 8;
+            if (O.w != CPT_FRESH_TMAX) {
+                MP = a.mul_p[slot];
+                C = a.col[slot];
+                meta = a.meta[slot];
+            }
             uint64_t rng = a.rng[slot];
-            uint32_t meta = a.meta[slot];
             int depth = (int)(meta & 0xffu);
             bool sampled_delta = (meta >> 8) & 1u;
             V3 mulv = v3(MP.x, MP.y, MP.z), colv = v3(C.x, C.y, C.z);
@@ -965,9 +975,11 @@ __global__ __launch_bounds__(B) void k_shade_ppg(WfArgs a, int cur, int depth_ca
             float4 O = a.qo[cur][i], D = a.qd[cur][i], HH = a.hits[i];
             slot = __float_as_int(D.w);
             V3 o = v3(O.x, O.y, O.z), d = v3(D.x, D.y, D.z);
-            float4 MP = a.mul_p[slot], C = a.col[slot];
+            const bool fresh = O.w == CPT_FRESH_TMAX;  // a camera ray: initial path state
+            const float4 MP = fresh ? make_float4(1.0f, 1.0f, 1.0f, 1.0f) : a.mul_p[slot];
+            const float4 C = fresh ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : a.col[slot];
             uint64_t rng = a.rng[slot];
-            const int depth = (int)(a.meta[slot] & 0xffu);
+            const int depth = fresh ? 0 : (int)(a.meta[slot] & 0xffu);
             V3 mulv = v3(MP.x, MP.y, MP.z), colv = v3(C.x, C.y, C.z);
             Hit h;
             h.t = HH.x;
