@@ -1238,11 +1238,11 @@ __global__ __launch_bounds__(64 * WAVES8) void k_mlp_mixtures8(const uint8_t* gw
 // LDS (64): the block holds one copy of the transposed BF16 weights, the output
 // layer's rows in out_orig order (from the copy at T4P), and each wave its own tile
 // state.  The one-blob features are formed in registers as the MFMA's second
-// operand (the values encode_tile_window stores: blob() of the 16 bins of
-// blob_window, 0 elsewhere), the weights are the first operand, so a lane holds four
-// consecutive outputs of one sample and the mixture is decoded in registers
-// (wave_decode).  Same arithmetic per output as forward_tile (acc + bias, BF16
-// activations); only the MFMA operand order differs.
+// operand (the 16 bins of blob_window by the ratio recurrence, within a few ulp of
+// blob() before their BF16 rounding; 0 elsewhere), the weights are the first
+// operand, so a lane holds four consecutive outputs of one sample and the mixture is
+// decoded in registers (wave_decode).  Per output the arithmetic of forward_tile
+// (acc + bias, BF16 activations); only the MFMA operand order differs.
 constexpr int WAVES16 = 12;
 struct WaveState16 {
     float in[64 * NIN];    // inputs of up to 64 samples
@@ -1282,17 +1282,45 @@ __device__ __forceinline__ void gemm16_rows_nk_t(const uint16_t* X, int ldx, con
     }
 }
 
+// Two values as packed BF16, rounded to nearest even (gfx950's v_cvt_pk_bf16_f32:
+// the rounding of f2bf, one instruction per pair)
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16x2(float a, float b) {
+    const bf2v v = {(__bf16)a, (__bf16)b};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
 // ReLU outputs of a hidden layer (bias at b) as BF16 into the wave's rows
 __device__ __forceinline__ void store_hidden16(const f4* acc, const float* b, uint16_t* h, int lane) {
     const int m = lane & 15, n4 = 4 * (lane >> 4);
 #pragma unroll
     for (int nt = 0; nt < NTH; nt++) {
-        uint16_t v[4];
+        float v[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) v[j] = f2bf(fmaxf(acc[nt][j] + b[nt * 16 + n4 + j], 0.0f));
-        *reinterpret_cast<uint2*>(h + m * LDH + nt * 16 + n4) =
-            make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+        for (int j = 0; j < 4; j++) v[j] = fmaxf(acc[nt][j] + b[nt * 16 + n4 + j], 0.0f);
+        *reinterpret_cast<uint2*>(h + m * LDH + nt * 16 + n4) = make_uint2(bf16x2(v[0], v[1]), bf16x2(v[2], v[3]));
     }
+}
+
+// Layer-0 operand of lane group g for input x (the MFMA's 8 values of bins 8 g ..
+// 8 g + 7): the BF16 one-blob values of the bins in blob_window, 0 elsewhere, with
+// two exponentials (blob_run from the first window bin the lane holds: the window's
+// bounds are multiples of 4, so that bin is 8 g or 8 g + 4, and the run starts
+// within the window, where no value underflows)
+__device__ __forceinline__ bf8v blob_operand16(float x, int g) {
+    const int w0 = blob_window(x), s = 8 * g, st = max(s, w0);
+    float v[8];
+    blob_run<8>(x, st, 1.0f, v);
+    float u[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int b = s + j;
+        const float val = st == s ? v[j] : (j >= 4 ? v[j - 4] : 0.0f);  // bin b is run value b - st
+        u[j] = (b >= w0 && b < w0 + 16) ? val : 0.0f;
+    }
+    typedef unsigned u4v __attribute__((ext_vector_type(4)));
+    const u4v w = {bf16x2(u[0], u[1]), bf16x2(u[2], u[3]), bf16x2(u[4], u[5]), bf16x2(u[6], u[7])};
+    return __builtin_bit_cast(bf8v, w);
 }
 
 // Forward pass of the wave's 16 rows (inputs at in16; rows past the valid ones
@@ -1305,15 +1333,7 @@ __device__ __forceinline__ void wave_forward16(const SmemInf16& sm, WaveState16&
     for (int nt = 0; nt < NTH; nt++) acc[nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kt = 0; kt < NIN; kt++) {  // layer 0: one input (32 bins) per step of 32
-        const float x = in16[m * NIN + kt];
-        const int w0 = blob_window(x);
-        s8v xs;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int b = 8 * g + j;
-            xs[j] = (short)((b >= w0 && b < w0 + 16) ? f2bf(blob(x, b)) : (uint16_t)0);
-        }
-        const bf8v xb = __builtin_bit_cast(bf8v, xs);
+        const bf8v xb = blob_operand16(in16[m * NIN + kt], g);
         const int kk = 32 * kt + 8 * g;
 #pragma unroll
         for (int nt = 0; nt < NTH; nt++)
