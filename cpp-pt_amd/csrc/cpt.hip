@@ -157,11 +157,15 @@ struct WfArgs {
 // after it [17, 20)
 #define REC_FLOATS 20
 
-// waves per SIMD the traversal kernels are compiled for: caps their VGPRs at 96
-// (k_extend then keeps only the overflow-stack base in scratch, read on the
-// overflow path alone)
+// waves per SIMD the closest-hit traversal is compiled for: caps its VGPRs at 96
+// (its loop over quantized nodes needs 90)
 #ifndef CPT_TRAV_WAVES
 #define CPT_TRAV_WAVES 5
+#endif
+// ... and the any-hit kernel, whose loop needs fewer registers: 6 (80 VGPRs, 2
+// spilled) is 7% faster for shadow rays than 5, while closest hit loses at 6
+#ifndef CPT_SHADOW_WAVES
+#define CPT_SHADOW_WAVES 6
 #endif
 
 // waves per SIMD the unguided shading kernel is compiled for (caps its VGPRs at
@@ -258,6 +262,12 @@ __device__ __forceinline__ int find_segment(const PassDesc* pd, unsigned i, bool
         else hi = mid - 1;
     }
     return lo;
+}
+
+// The traversal's 64-byte copy of the tree, one node per thread
+__global__ void k_quantize_nodes(const Node4* nodes, int64_t n, QNode* q) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        quantize_node(nodes[i], q[i]);
 }
 
 // The tmax of k_raygen's camera rays: the largest finite float, which culls and
@@ -1091,7 +1101,7 @@ __global__ void k_sd_splat(WfArgs a, unsigned n_paths) {
 
 // ------------------------------------------------------------------ shadow
 template <int B, bool COUNT>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CPT_TRAV_WAVES))) void k_shadow(WfArgs a, int nxt) {
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CPT_SHADOW_WAVES))) void k_shadow(WfArgs a, int nxt) {
     __shared__ int lds[CPT_LDS_ROWS * B];
     const unsigned count = a.counters[2 * nxt + 1];
     if (gtid() == 0) atomicAdd(&a.stats[ST_SHADOW], (unsigned long long)count);
@@ -1361,6 +1371,7 @@ struct cpt_ctx {
     bool bvh_gpu = false;
     std::vector<int32_t> rank;
     Node4* d_nodes = nullptr;
+    QNode* d_qnodes = nullptr;  // the traversal's quantized tree (d_nodes: the build's float nodes, freed after)
     float4* d_tris = nullptr;
     int4* d_tri_info = nullptr;
     float4* d_shade = nullptr;
@@ -1480,6 +1491,7 @@ void dfree(T*& p) {
 
 void free_scene(cpt_ctx* c) {
     dfree(c->d_nodes);
+    dfree(c->d_qnodes);
     dfree(c->d_tris);
     dfree(c->d_tri_info);
     dfree(c->d_shade);
@@ -1523,7 +1535,7 @@ void drop_graph(cpt_ctx* c) {
 
 DevScene dev_scene(const cpt_ctx* c) {
     DevScene s;
-    s.nodes = c->d_nodes;
+    s.qnodes = c->d_qnodes;
     s.tris = c->d_tris;
     s.tri_info = c->d_tri_info;
     s.shade = c->d_shade;
@@ -2161,6 +2173,17 @@ int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
         }
         HIPCHK(c, hipMemcpy(c->d_plights, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice));
     }
+    // the traversal's quantized nodes (their scales cover extents up to 255 x 2^60)
+    for (int k = 0; k < 3; k++)
+        if ((double)c->bext[k] > QNODE_MAX_EXTENT)
+            return fail(c, CPT_E_LIMIT, "scene extent %g exceeds the quantized BVH's range", (double)c->bext[k]);
+    if (c->bvh_nodes < 1) return fail(c, CPT_E_INVALID, "empty BVH");
+    if (dalloc(c, c->d_qnodes, (size_t)c->bvh_nodes)) return CPT_E_HIP;
+    hipLaunchKernelGGL(k_quantize_nodes, dim3((unsigned)std::min<int64_t>((c->bvh_nodes + 255) / 256, 4096)), dim3(256),
+                       0, c->stream, c->d_nodes, c->bvh_nodes, c->d_qnodes);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    dfree(c->d_nodes);  // the float nodes were only the conversion's input
     c->has_scene = true;
     c->scene_gen++;
     return CPT_OK;
@@ -2599,7 +2622,7 @@ static int render_batch(cpt_ctx* c, std::vector<ViewDesc>& vd, int max_depth, ui
         char key[512];
         snprintf(key, sizeof(key),
                  "%d,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%d,%zu,%u,%p,%p,%p,%d,%d,%d,%d,%zu,%p,%p,%p,%p,%p,%p,%p,%g,%g,%g,%g,%g,%g",
-                 max_depth, B, G, SB, (int)L.count, c->cap, (void*)c->d_nodes, (void*)c->film,
+                 max_depth, B, G, SB, (int)L.count, c->cap, (void*)c->d_qnodes, (void*)c->film,
                  (void*)c->d_views, (void*)c->pass_tend, (int)guided, nrec, a.rec_stride, (void*)c->g_lobes, (void*)c->g_rec,
                  (void*)c->g_samples, a.guide_depth * 2 + a.guide_target, refill, integrator,
                  a.guide_lobes * 4 + a.corrected * 2 + a.guide_fp8, a.lobes_stride,

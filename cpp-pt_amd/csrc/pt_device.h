@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "bvh_build.h"
+#include "bvh_quant.h"
 
 namespace cpt {
 
@@ -86,7 +87,7 @@ __device__ __forceinline__ float pcg_float(uint64_t& s) { return (float)(pcg_nex
 
 // ---------------------------------------------------------------- scene
 struct DevScene {
-    const Node4* nodes;
+    const QNode* qnodes;     // the BVH4 as 64-byte quantized nodes
     const float4* tris;      // 3 per triangle, leaf order (TriRec)
     const int4* tri_info;    // v0, v1, v2, material (by triangle id)
     const float4* shade;     // 6 per triangle id: g_normal + material, then n0 n1 n2 b0 b1 b2 packed
@@ -123,7 +124,7 @@ struct Hit {
 #define CPT_LDS_SS 24
 #endif
 #define CPT_GSTACK (128 - CPT_LDS_SS)
-// nodes are addressed by 32-bit byte offsets (index << 7)
+// nodes are addressed by 32-bit byte offsets (index << 6 for 64-byte nodes)
 #define CPT_MAX_NODES ((1u << 25) - 1u)
 // conservative culling: a box may be entered up to this relative distance
 // beyond the current best t, so that triangles tying with the best t in other
@@ -177,8 +178,8 @@ __device__ __forceinline__ bool tri_test(const float4& A, const float4& E1, cons
 }
 
 // Slab offsets of one ray for the box test.  A box plane P is crossed at
-// t = (P - o) * inv; with c = -(o * inv) the test evaluates fma(P, inv, c), one
-// instruction per plane.  That form has an absolute error of up to one rounding
+// t = (P - o) * inv; with c = -(o * inv) a test can evaluate fma(P, inv, c), one
+// instruction per plane (the quantized node test adds its own offset on top).  That form has an absolute error of up to one rounding
 // of o * inv (plus the rounding of c), so c is moved by e = 2^-21 |o * inv| (four
 // times that error) toward smaller t for the plane that is nearer along the axis
 // (c_near) and toward larger t for the other (c_far): the computed entry never
@@ -193,22 +194,13 @@ __device__ __forceinline__ void slab_offsets(float o, float inv, float& c_near, 
     c_far = -oi + e;
 }
 
-// Byte offset inside a Node4 of the child planes a ray meets first along an
-// axis: the low planes (lox at 0, loy at 32, loz at 64) when the direction's
-// sign bit is clear (its reciprocal, with zeros replaced by +-1e-20 of the same
-// sign, is then >= 0), the high planes (16 bytes later) otherwise; the far
-// planes are the other ones (offset ^ 16).  Loading the near and far planes by
-// these per-lane offsets replaces a min and a max per axis and child.
-__device__ __forceinline__ uint32_t near_plane_offset(float dir, uint32_t axis_base) {
-    return axis_base + ((__float_as_uint(dir) >> 31) << 4);
-}
-
 // Chunked persistent traversal.  Wave w of the grid owns rays
 // [w*per, (w+1)*per) of the queue and keeps its 64 lanes busy: whenever at
 // least `refill` lanes (or all of them) are idle it hands them the next rays of
 // its chunk (refill: a launch parameter), so a wave no longer waits for its slowest ray before starting new
 // ones, and no atomics are needed.  Each iteration a lane may do one node step
-// (four slab tests, near-first push) and one triangle test of a pending leaf:
+// (four slab tests on a 64-byte quantized node, near-first push) and one
+// triangle test of a pending leaf:
 // a leaf met during the node step is parked in `leaf` until the previous leaf's
 // triangles are done.  Culling stays conservative and ties are decided by rank,
 // so the result does not depend on this visiting order.  Pushes and pops use the LDS rows directly while every lane of the
@@ -229,13 +221,13 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
     unsigned nextr = min(count, w * per);
     const unsigned endr = min(count, nextr + per);
     const unsigned long long lt = lane ? ((~0ull) >> (64 - lane)) : 0ull;
-    const char* nbase = reinterpret_cast<const char*>(sc.nodes);
+    const char* nbase = reinterpret_cast<const char*>(sc.qnodes);
     bool has = false, fresh = false;
     unsigned ray = 0;
     V3 o = {0, 0, 0}, d = {0, 0, 0};
     float ix = 0, iy = 0, iz = 0, best = 0;
     float cxn = 0, cxf = 0, cyn = 0, cyf = 0, czn = 0, czf = 0;  // slab offsets
-    uint32_t oxn = 0, oyn = 32, ozn = 64;                        // near-plane byte offsets
+    bool sgx = false, sgy = false, sgz = false;                  // direction signs: near planes high
     int best_rank = INT_MIN;
     Hit h;
     h.t = 0.0f;
@@ -267,9 +259,9 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
                 ray = r;
                 has = true;
                 fresh = true;
-                oxn = near_plane_offset(d.x, 0u);
-                oyn = near_plane_offset(d.y, 32u);
-                ozn = near_plane_offset(d.z, 64u);
+                sgx = (__float_as_uint(d.x) >> 31) != 0;
+                sgy = (__float_as_uint(d.y) >> 31) != 0;
+                sgz = (__float_as_uint(d.z) >> 31) != 0;
             }
             nextr = min(endr, nextr + nidle);
         }
@@ -279,8 +271,11 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
         // the node), then the triangle is tested and the node's boxes culled
         // against the possibly improved best.
         const bool do_tri = tcur < tend, do_node = node >= 0;
-        float4 A, E1, E2, nx, fx, ny, fy, nz, fz;
+        float4 A, E1, E2;
         int4 ch;
+        float4 qo;  // quantized node: low corner and scale exponents
+        uint4 qa;   // its x and y plane bytes
+        uint2 qb;   // its z plane bytes
         if (do_tri) {
             const float4* tp = sc.tris + 3 * (size_t)tcur;
             A = tp[0];
@@ -288,14 +283,11 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
             E2 = tp[2];
         }
         if (do_node) {
-            const uint32_t no = (uint32_t)node << 7;
-            ch = *reinterpret_cast<const int4*>(nbase + (no + 96u));
-            nx = *reinterpret_cast<const float4*>(nbase + (no + oxn));
-            fx = *reinterpret_cast<const float4*>(nbase + (no + (oxn ^ 16u)));
-            ny = *reinterpret_cast<const float4*>(nbase + (no + oyn));
-            fy = *reinterpret_cast<const float4*>(nbase + (no + (oyn ^ 16u)));
-            nz = *reinterpret_cast<const float4*>(nbase + (no + ozn));
-            fz = *reinterpret_cast<const float4*>(nbase + (no + (ozn ^ 16u)));
+            const uint32_t no = (uint32_t)node << 6;
+            qo = *reinterpret_cast<const float4*>(nbase + no);
+            ch = *reinterpret_cast<const int4*>(nbase + (no + 16u));
+            qa = *reinterpret_cast<const uint4*>(nbase + (no + 32u));
+            qb = *reinterpret_cast<const uint2*>(nbase + (no + 48u));
         }
         if (fresh) {
             // 1/d with zero components replaced by a tiny signed value so that
@@ -340,21 +332,50 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
             float lim = ANY ? best * CPT_BOX_SLACK : best * CPT_TIE_SLACK;
             float key[4];
             int cid[4];
-            // near planes with the c_near offsets, far planes with c_far: the
-            // same values as min and max of the low/high plane pair
-#define CPT_BOX(c, NX, FX, NY, FY, NZ, FZ, CH)                                    \
-    {                                                                             \
-        float tn = fmaxf(fmaxf(fmaf(NX, ix, cxn), fmaf(NY, iy, cyn)), fmaxf(fmaf(NZ, iz, czn), 0.0f)); \
-        float tf = fminf(fminf(fmaf(FX, ix, cxf), fmaf(FY, iy, cyf)), fmaf(FZ, iz, czf)) * CPT_BOX_SLACK; \
-        bool hit = tn <= tf && tn <= lim;                                         \
-        key[c] = hit ? tn : INFINITY;                                             \
-        cid[c] = hit ? CH : EMPTY_CHILD;                                          \
+            {
+                // A plane of axis a is P = o_a + q 2^k_a (q the child's byte), crossed at
+                // t = q (2^k_a inv_a) + (o_a inv_a + c): the first term is exact (a power
+                // of two times inv, a normal float for the stored k), the node's offset
+                // takes one rounding, so it is moved outward by 2^-21 of its size
+                // (below the near planes, above the far ones; the near and far planes
+                // are the low or high bytes by the direction's sign, as for float
+                // nodes).  Quantized planes lie outside the exact ones, so culling stays
+                // conservative.
+                const uint32_t ex = __float_as_uint(qo.w);
+                const float sx = __uint_as_float((ex & 0xffu) << 23) * ix;
+                const float sy = __uint_as_float(((ex >> 8) & 0xffu) << 23) * iy;
+                const float sz = __uint_as_float(((ex >> 16) & 0xffu) << 23) * iz;
+                float bxn = fmaf(qo.x, ix, cxn), bxf = fmaf(qo.x, ix, cxf);
+                float byn = fmaf(qo.y, iy, cyn), byf = fmaf(qo.y, iy, cyf);
+                float bzn = fmaf(qo.z, iz, czn), bzf = fmaf(qo.z, iz, czf);
+                constexpr float R = 4.76837158203125e-7f;  // 2^-21
+                bxn = fmaf(-fabsf(bxn), R, bxn);
+                byn = fmaf(-fabsf(byn), R, byn);
+                bzn = fmaf(-fabsf(bzn), R, bzn);
+                bxf = fmaf(fabsf(bxf), R, bxf);
+                byf = fmaf(fabsf(byf), R, byf);
+                bzf = fmaf(fabsf(bzf), R, bzf);
+                const uint32_t nqx = sgx ? qa.y : qa.x, fqx = sgx ? qa.x : qa.y;
+                const uint32_t nqy = sgy ? qa.w : qa.z, fqy = sgy ? qa.z : qa.w;
+                const uint32_t nqz = sgz ? qb.y : qb.x, fqz = sgz ? qb.x : qb.y;
+#define CPT_QB(v, c) ((float)(((v) >> (8 * (c))) & 0xffu))
+#define CPT_QBOX(c, CH)                                                                                      \
+    {                                                                                                         \
+        float tn = fmaxf(fmaxf(fmaf(CPT_QB(nqx, c), sx, bxn), fmaf(CPT_QB(nqy, c), sy, byn)),                 \
+                         fmaxf(fmaf(CPT_QB(nqz, c), sz, bzn), 0.0f));                                           \
+        float tf = fminf(fminf(fmaf(CPT_QB(fqx, c), sx, bxf), fmaf(CPT_QB(fqy, c), sy, byf)),                 \
+                         fmaf(CPT_QB(fqz, c), sz, bzf)) * CPT_BOX_SLACK;                                        \
+        bool hit = tn <= tf && tn <= lim;                                                                     \
+        key[c] = hit ? tn : INFINITY;                                                                         \
+        cid[c] = hit ? CH : EMPTY_CHILD;                                                                      \
     }
-            CPT_BOX(0, nx.x, fx.x, ny.x, fy.x, nz.x, fz.x, ch.x)
-            CPT_BOX(1, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y, ch.y)
-            CPT_BOX(2, nx.z, fx.z, ny.z, fy.z, nz.z, fz.z, ch.z)
-            CPT_BOX(3, nx.w, fx.w, ny.w, fy.w, nz.w, fz.w, ch.w)
-#undef CPT_BOX
+                CPT_QBOX(0, ch.x)
+                CPT_QBOX(1, ch.y)
+                CPT_QBOX(2, ch.z)
+                CPT_QBOX(3, ch.w)
+#undef CPT_QBOX
+#undef CPT_QB
+            }
 #define CPT_CSWAP(a, b)                                  \
     if (key[b] < key[a]) {                               \
         float tk = key[a]; key[a] = key[b]; key[b] = tk; \
@@ -371,8 +392,9 @@ __device__ __forceinline__ void trace_chunked(const DevScene& sc, unsigned count
                 CPT_CSWAP(1, 2)
             }
 #undef CPT_CSWAP
-            // the children present are a prefix of cid (absent ones sort last):
-            // push cid[3], cid[2], cid[1] of those present, deepest first
+            // push cid[3], cid[2], cid[1] of the children present, deepest first
+            // (absent ones sort last; an absent child of a quantized node on a flat
+            // axis can pass the slab test but is still not pushed)
             const int v3 = cid[3] != EMPTY_CHILD, v2 = cid[2] != EMPTY_CHILD, v1 = cid[1] != EMPTY_CHILD;
             if (__ballot(sp + 3 > CPT_LDS_SS) == 0ull) {
                 const int s2 = sp + v3, s1 = s2 + v2;
