@@ -46,7 +46,7 @@ ABI_FUNCTIONS = (
     "cpt_guide_set_params", "cpt_guide_forward", "cpt_guide_train_step", "cpt_autotune",
     "cpt_ppg_reset", "cpt_ppg_update", "cpt_ppg_export", "cpt_debug_fail_alloc",
     "cpt_trace_closest_device_noalloc", "cpt_scene_file_load", "cpt_scene_file_free",
-    "cpt_guide_mixtures", "cpt_debug_guide_mixtures",
+    "cpt_guide_mixtures", "cpt_debug_guide_mixtures", "cpt_debug_quantize_node",
 )
 
 
@@ -124,6 +124,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.cpt_last_error.argtypes = [C.c_void_p]
     lib.cpt_last_error.restype = C.c_char_p
     lib.cpt_debug_fail_alloc.argtypes = [C.c_void_p, C.c_int32]
+    lib.cpt_debug_quantize_node.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_upload_scene.argtypes = [C.c_void_p, C.POINTER(SceneDesc)]
     lib.cpt_bvh_info.argtypes = [C.c_void_p, C.c_void_p]
     lib.cpt_tie_rank.argtypes = [C.c_void_p, C.c_void_p]

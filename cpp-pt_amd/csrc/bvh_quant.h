@@ -26,6 +26,16 @@ constexpr int QNODE_EMIN = -100, QNODE_EMAX = 60;
 // extents beyond 255 x 2^QNODE_EMAX are refused at upload
 constexpr double QNODE_MAX_EXTENT = 255.0 * 1152921504606846976.0;  // 255 x 2^60
 
+// x = (p - o) / s for a power of two s; true when the double difference is exact
+// (Knuth's TwoSum error of p + (-o) is 0, so x is exact too)
+__host__ __device__ inline bool scaled_offset(float p, float o, double s, double& x) {
+    const double a = (double)p, b = -(double)o;
+    const double sum = a + b, bv = sum - a;
+    const double err = (a - (sum - bv)) + (b - bv);
+    x = sum / s;
+    return err == 0.0;
+}
+
 __host__ __device__ inline void quantize_node(const Node4& n, QNode& qn) {
     const float* lo[3] = {n.lox, n.loy, n.loz};
     const float* hi[3] = {n.hix, n.hiy, n.hiz};
@@ -54,10 +64,12 @@ __host__ __device__ inline void quantize_node(const Node4& n, QNode& qn) {
         for (int c = 0; c < 4; c++) {
             uint32_t bl = 255, bh = 0;  // an absent child: an empty box
             if (n.child[c] != EMPTY_CHILD) {
-                // (plane - o) / s in units of the scale, moved outward by far more than
-                // the double rounding of the difference before floor/ceil
-                const double xl = ((double)lo[a][c] - (double)o) / s, xh = ((double)hi[a][c] - (double)o) / s;
-                double fl = floor(xl - 1e-7 * (1.0 + xl)), fh = ceil(xh + 1e-7 * (1.0 + xh));
+                // (plane - o) / s in units of the scale; where the double difference
+                // is not exact (its TwoSum error is not 0), moved outward by far more
+                // than that error before floor/ceil
+                double xl, xh;
+                const bool el = scaled_offset(lo[a][c], o, s, xl), eh = scaled_offset(hi[a][c], o, s, xh);
+                double fl = floor(el ? xl : xl - 1e-7 * (1.0 + xl)), fh = ceil(eh ? xh : xh + 1e-7 * (1.0 + xh));
                 fl = fl < 0.0 ? 0.0 : fl;  // lo >= o: plane 0 is o itself
                 fh = fh > 255.0 ? 255.0 : fh < 0.0 ? 0.0 : fh;
                 bl = (uint32_t)fl;

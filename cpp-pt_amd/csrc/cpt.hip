@@ -1960,6 +1960,16 @@ int cpt_debug_fail_alloc(cpt_ctx* c, int32_t after) {
     return CPT_OK;
 }
 
+int cpt_debug_quantize_node(const uint32_t* node4, uint32_t* qnode_out) {
+    if (!node4 || !qnode_out) return CPT_E_INVALID;
+    Node4 n;
+    QNode q;
+    std::memcpy(&n, node4, sizeof(n));
+    quantize_node(n, q);
+    std::memcpy(qnode_out, &q, sizeof(q));
+    return CPT_OK;
+}
+
 int cpt_upload_scene(cpt_ctx* c, const cpt_scene_desc* d) {
     if (!c || !d) return CPT_E_INVALID;
     if (d->n_triangles < 0 || d->n_vertices < 0 || d->n_materials <= 0 || d->n_lights < 0)

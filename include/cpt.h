@@ -200,6 +200,14 @@ int cpt_version(void);
  * hipMalloc had failed (after < 0: off).  Lets a test check that a call after a
  * failed allocation reallocates instead of using a stale buffer. */
 int cpt_debug_fail_alloc(cpt_ctx* ctx, int32_t after);
+/* Test hook; no counterpart in the reference.  Converts one 4-wide node as the
+ * traversal stores it: node4 = 32 words (low x, high x, low y, high y, low z,
+ * high z planes of the 4 children as floats, then 4 child references, 0x7fffffff
+ * for an absent child, then 4 unused), qnode_out = 16 words (low corner x y z as
+ * floats, the three scale exponents + 127 in bytes 0-2, the 4 child references,
+ * then per axis the 4 children's low and high plane bytes, 2 unused).  Host code
+ * only: needs no context and no GPU. */
+int cpt_debug_quantize_node(const uint32_t* node4, uint32_t* qnode_out);
 
 /* Upload a scene, build the 4-wide BVH and (unless CPT_SCENE_NO_TIE_RANK) the
  * reference test-order table.  Replaces Loader::load_scene's BVH construction
