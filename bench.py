@@ -8,11 +8,11 @@ finished (it synchronises its stream).
 
 Prints one JSON line.  `roofline` describes the dominant kernel (closest-hit
 traversal, `k_extend`): achieved = HBM bytes per launch (two rocprofv3 counter
-passes, FETCH_SIZE and WRITE_SIZE, over a child process rendering the same frame
+passes, TCC_EA0_RDREQ_{32B,64B,128B} x size and WRITE_SIZE, over a child process rendering the same frame
 with the same launch configuration and seed, scaled by device copies of known
 size) / that kernel's average launch time (HIP events on the stream it runs on).
 The bytes the traversal asks of the whole memory hierarchy (32 B ray + 16 B hit +
-128 B per BVH4 node visit + 48 B per triangle test, counted by a counting run of
+64 B per quantized BVH4 node visit + 48 B per triangle test, counted by a counting run of
 the same kernel on the same rays) are reported beside it as `cache_level`.
 `cpu_baseline` renders whole frames of the same workload with the CPU oracle (a
 restatement of the reference's PathtracerMIS with its mt19937 column sampler and
@@ -797,7 +797,7 @@ def main():
         launch_s = tm["ms_extend"] / 1e3 / launches
         # bytes the traversal asks of the memory system (every node and triangle fetch
         # counted, whether L2, Infinity Cache or HBM serves it)
-        cache_bytes = (48 * ct["extension_rays"] + 128 * ct["node_visits"] + 48 * ct["triangle_tests"]) / launches
+        cache_bytes = (48 * ct["extension_rays"] + 64 * ct["node_visits"] + 48 * ct["triangle_tests"]) / launches
         launch = dict(tune) if tune else {"block_size": args.block, "grid_blocks": args.grid_blocks,
                                           "shade_block": args.shade_block, "refill": args.refill}
         if args.block:
@@ -826,7 +826,7 @@ def main():
             "nodes_per_ray": ct["node_visits"] / max(1, ct["extension_rays"]),
             "tris_per_ray": ct["triangle_tests"] / max(1, ct["extension_rays"]),
             "cache_level": {"bytes_per_launch": cache_bytes, "gbs": cache_bytes / launch_s / 1e9,
-                            "note": "48 B per ray + 128 B per node visit + 48 B per triangle test, served by "
+                            "note": "48 B per ray + 64 B per node visit + 48 B per triangle test, served by "
                                     "L2/Infinity Cache/HBM together; not an HBM figure"},
             "ms_extend": tm["ms_extend"],
             "ms_shadow": tm["ms_shadow"],
